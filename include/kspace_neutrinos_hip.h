@@ -1,0 +1,258 @@
+/* kspace_neutrinos_hip.h -- C ABI of the MI355X implementation of the
+ * kspace-neutrinos hot path (libkspace_neutrinos_hip.so).
+ *
+ * The entry points below keep the reference library's C signatures so the
+ * library drops into a Gadget-style PM step unchanged; each declaration cites
+ * the reference declaration it replaces.  Behind them, host C code drives
+ * hand-written HIP kernels for gfx950.  Additive entry points (prefix kn_)
+ * cover device-resident grids, the adjoints and the batched solver; they never
+ * replace the reference entry points.
+ *
+ * Conventions kept from the reference:
+ *  - FFTW2's fftw_complex must be double precision ({double re, im}).
+ *  - Errors are fatal: the library calls the host's terminate(code, fmt, ...)
+ *    with the reference's error codes (2001..2024, 2, 5); GPU failures use
+ *    codes 3001..3099.  kn_ entry points that return int return 0 on success.
+ *  - Host services (gadget_defines.h:11-32) are imported: hubble_function,
+ *    terminate, message, mymalloc_fullinfo, myfree_fullinfo.  Each is a weak
+ *    reference; when the host does not define one, a built-in fallback is used
+ *    (hubble_function has none: register one with kn_set_hubble_function).
+ *  - One rank, one GPU: the MPI communicator arguments are accepted for ABI
+ *    compatibility and not used.
+ */
+#ifndef KSPACE_NEUTRINOS_HIP_H
+#define KSPACE_NEUTRINOS_HIP_H
+
+#include <stddef.h>
+#include <mpi.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- types */
+
+/* FFTW2 complex (powerspectrum.h:6-14 with DOUBLEPRECISION_FFTW).  Include
+ * dfftw.h before this header to use FFTW's own definition. */
+#ifndef FFTW_H
+typedef double fftw_real;
+typedef struct { fftw_real re, im; } fftw_complex;
+#endif
+
+#define NUSPECIES 3
+
+/* omega_nu_single.h:21-28; interp/acc hold this library's spline state. */
+struct _rho_nu_single {
+    double *loga;
+    double *rhonu;
+    void *interp;
+    void *acc;
+    double mnu;
+};
+typedef struct _rho_nu_single _rho_nu_single;
+
+/* omega_nu_single.h:51-63 */
+struct _hybrid_nu {
+    int enabled;
+    double nufrac_low[NUSPECIES];
+    double nu_crit_time;
+    double vcrit;
+};
+typedef struct _hybrid_nu _hybrid_nu;
+
+/* omega_nu_single.h:94-107 */
+struct _omega_nu {
+    _rho_nu_single *RhoNuTab[NUSPECIES];
+    int nu_degeneracies[NUSPECIES];
+    double rhocrit;
+    double kBtnu;
+    double tcmb0;
+    _hybrid_nu hybnu;
+};
+typedef struct _omega_nu _omega_nu;
+
+/* transfer_init.h:12-17 */
+struct _transfer_init_table {
+    int NPowerTable;
+    double *logk;
+    double *T_nu;
+};
+typedef struct _transfer_init_table _transfer_init_table;
+
+/* delta_tot_table.h:17-58: field order and types unchanged; the history block
+ * is scalefact[namax] followed by nk_allocated rows delta_tot[k][namax]. */
+struct _delta_tot_table {
+    int nk;
+    int nk_allocated;
+    int namax;
+    int ia;
+    int ThisTask;
+    double delta_nu_prefac;
+    int delta_tot_init_done;
+    int debug;
+    double **delta_tot;
+    double *scalefact;
+    double *delta_nu_init;
+    double *delta_nu_last;
+    double *wavenum;
+    const _omega_nu *omnu;
+    double Omeganonu;
+    double light;
+    double TimeTransfer;
+};
+typedef struct _delta_tot_table _delta_tot_table;
+
+/* delta_pow.h:10-17; spline/acc hold this library's interpolation state. */
+struct _delta_pow {
+    double *logkk;
+    double *delta_ratio;
+    void *spline;
+    void *acc;
+    int nbins;
+    double norm;
+};
+typedef struct _delta_pow _delta_pow;
+
+/* ------------------------------------------ host services (imports) */
+
+/* gadget_defines.h:11 and :17-32.  Weak imports, see above. */
+double hubble_function(double a);
+void terminate(int ierr, const char *fmt, ...);
+void message(int ierr, const char *fmt, ...);
+void *mymalloc_fullinfo(const char *string, size_t size, const char *func, const char *file, int line);
+void myfree_fullinfo(void *ptr, const char *func, const char *file, int line);
+
+/* ------------------------------------------------ (a) k-shell binning */
+
+/* powerspectrum.h:30.  outfield may be device memory (the fast path, no
+ * copy) or host memory (staged through HBM).  Outputs are host arrays. */
+int total_powerspectrum(const int dims, fftw_complex *outfield, const int nrbins, const int startslab,
+                        const int nslab, double *power, long long int *count, double *keffs,
+                        MPI_Comm MYMPI_COMM_WORLD);
+
+/* --------------------------------------- (b) free-streaming integrator */
+
+/* delta_tot_table.h:70 */
+void allocate_delta_tot_table(_delta_tot_table *d_tot, const int nk_in, const double TimeTransfer,
+                              const double TimeMax, const double Omega0, const _omega_nu *const omnu,
+                              const double UnitTime_in_s, const double UnitLength_in_cm, int debug);
+/* delta_tot_table.h:73 */
+void free_delta_tot_table(_delta_tot_table *d_tot);
+/* delta_tot_table.h:85 */
+void delta_tot_init(_delta_tot_table *const d_tot, const int nk_in, const double wavenum[],
+                    const double delta_cdm_curr[], const _transfer_init_table *const t_init, const double Time);
+/* delta_tot_table.h:90 */
+void update_delta_tot(_delta_tot_table *const d_tot, const double a, const double delta_cdm_curr[],
+                      const double delta_nu_curr[], const int overwrite);
+/* delta_tot_table.h:104 */
+void get_delta_nu_update(_delta_tot_table *const d_tot, const double a, const int nk_in, const double keff[],
+                         const double P_cdm_curr[], double delta_nu_curr[], _transfer_init_table *transfer_init);
+/* delta_tot_table.h:113 */
+void get_delta_nu(const _delta_tot_table *const d_tot, const double a, const double wavenum[],
+                  double delta_nu_curr[], const double mnu);
+/* delta_tot_table.h:117 */
+void get_delta_nu_combined(const _delta_tot_table *const d_tot, const double a, const double wavenum[],
+                           double delta_nu_curr[]);
+/* delta_tot_table.h:138 */
+double specialJ(const double x, const double vcmnubylight, const double nufrac_low);
+/* delta_tot_table.h:149 */
+double fslength(const double logai, const double logaf, const double light);
+/* delta_tot_table.h:155 */
+double get_delta_tot(const double delta_nu_curr, const double delta_cdm_curr, const double OmegaNua3,
+                     const double Omeganonu, const double Omeganu1, const double partnu);
+
+/* ---------------------------------------- Omega_nu (host, no kernel) */
+
+/* omega_nu_single.h:116, :119, :122 */
+void init_omega_nu(_omega_nu *const omnu, const double MNu[], const double a0, const double HubbleParam,
+                   const double tcmb0);
+double get_omega_nu(const _omega_nu *const omnu, const double a);
+double get_omega_nu_nopart(const _omega_nu *const omnu, const double a);
+/* omega_nu_single.h:125, :131 */
+double get_omegag(const _omega_nu *const omnu, const double a);
+double omega_nu_single(const _omega_nu *const rho_nu_tab, const double a, const int i);
+/* omega_nu_single.h:37, :45 */
+void rho_nu_init(_rho_nu_single *rho_nu_tab, double a0, const double mnu, const double HubbleParam,
+                 const double kBtnu);
+double rho_nu(_rho_nu_single *rho_nu_tab, const double a, const double kT);
+/* omega_nu_single.h:75, :84, :87 */
+void init_hybrid_nu(_hybrid_nu *const hybnu, const double mnu[], const double vcrit, const double light,
+                    const double nu_crit_time, const double kBtnu);
+double particle_nu_fraction(const _hybrid_nu *const hybnu, const double a, int i);
+double nufrac_low(const double qc);
+
+/* transfer_init.h:28, :31 */
+void allocate_transfer_init_table(_transfer_init_table *t_init, const double BoxSize, const double UnitLength_in_cm,
+                                  const double InputSpectrum_UnitLength_in_cm, const char *KspaceTransferFunction);
+void free_transfer_init_table(_transfer_init_table *t_init);
+
+/* -------------------------------------------- (c) ratio + rescale */
+
+/* delta_pow.h:30, :39, :42 */
+void init_delta_pow(_delta_pow *d_pow, double logkk[], double delta_ratio[], int nbins, double norm);
+double get_dnudcdm_powerspec(_delta_pow *d_pow, double kk);
+void free_d_pow(_delta_pow *d_pow);
+
+/* interface_common.h:12-30 parameters and the module-level entry points */
+extern struct __kspace_params {
+    char KspaceTransferFunction[500];
+    double TimeTransfer;
+    double InputSpectrum_UnitLength_in_cm;
+    double MNu[NUSPECIES];
+    int hybrid_neutrinos_on;
+    double vcrit;
+    double nu_crit_time;
+} kspace_params;
+
+/* interface_common.h:37, :44, :46, :66 */
+double OmegaNu(double a);
+double OmegaNu_nopart(double a);
+void InitOmegaNu(const double HubbleParam, const double tcmb0, MPI_Comm MYMPI_COMM_WORLD);
+void allocate_kspace_memory(const int nk_in, const int ThisTask, const double BoxSize, const double UnitTime_in_s,
+                            const double UnitLength_in_cm, const double Omega0, char *snapdir, const double TimeMax,
+                            MPI_Comm MYMPI_COMM_WORLD);
+/* interface_common.h:80 (code-agnostic entry: pre-binned P(k)) */
+_delta_pow compute_neutrino_power_from_cdm(const double Time, const double keff_in[], const double P_cdm[],
+                                           const long int Nmodes[], const int nk_in, MPI_Comm MYMPI_COMM_WORLD);
+/* interface_common.h:97, :108 */
+void get_nu_state(double **scalefact, double **delta_tot, size_t *nk, size_t *ia);
+void set_nu_state(double *scalefact, double *delta_tot, const size_t nk, const size_t ia, MPI_Comm MYMPI_COMM_WORLD);
+/* interface_common.h:121 */
+int particle_nu_active(double a);
+
+/* interface_gadget.h:38: bin (a) -> integrate (b) -> rescale (c) of the
+ * Fourier grid in place.  fft_of_rhogrid may be device or host memory. */
+void add_nu_power_to_rhogrid(const double Time, const double BoxSize, fftw_complex *fft_of_rhogrid, const int pmgrid,
+                             int slabstart_y, int nslab_y, MPI_Comm MYMPI_COMM_WORLD);
+
+/* ================================================ additive entry points */
+
+/* Register the Hubble function H(a) in internal time units when the host does
+ * not define hubble_function() at link time (e.g. ctypes callers). */
+void kn_set_hubble_function(double (*fn)(double a));
+
+/* Synthetic Gaussian random field on the device (counter-based RNG keyed by
+ * seed and mode index): layout [nslab][dims][dims/2+1], mode k gets
+ * Re, Im = sqrt(P(|k| 2 pi/L)/2) N(0,1) with P = A k^ns T_BBKS(k/Gamma)^2 and
+ * F[0] = (dims^3, 0).  d_field is device memory. */
+int kn_synth_grid(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize, double amplitude,
+                  double ns, double gamma, unsigned long long seed);
+
+/* (a) on a device grid with device-side outputs left in HBM: power[nrbins]
+ * holds the unnormalised bin sums, count/keffs the geometry sums, total_mass2
+ * |F0|^2.  stream is a hipStream_t (NULL: default stream).  No host sync. */
+int kn_bin_power_device(const fftw_complex *d_field, int dims, int nrbins, int startslab, int nslab,
+                        double *d_power, double *d_total_mass2, void *stream);
+
+/* (c) rescale of a device grid by 1 + norm * lininterp(log k) of the ratio
+ * table (device arrays logkk/ratio of length nbins).  No host sync. */
+int kn_rescale_device(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
+                      const double *d_logkk, const double *d_ratio, int nbins, double norm, void *stream);
+
+/* Last GPU error text (empty if none). */
+const char *kn_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

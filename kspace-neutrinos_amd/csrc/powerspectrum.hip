@@ -1,0 +1,427 @@
+// powerspectrum.hip -- (a) k-shell binning of the r2c PM grid into P(k).
+//
+// Replaces total_powerspectrum (powerspectrum.c:33-117, decl powerspectrum.h:30).
+// Semantics kept from the reference:
+//  - bin of a mode = floor(binsperunit * log(|k|)), binsperunit =
+//    (nrbins-1)/log(sqrt(3) dims/2) (powerspectrum.c:40, :67);
+//  - weight multiplicity * |F|^2 * W^-4, W^-1 the per-axis CiC inverse window
+//    pi k/(n sin(pi k/n)) (powerspectrum.c:8-24, squared twice at :68);
+//  - multiplicity 1 on the kz = 0 and kz = N/2 planes, 2 elsewhere; the zero
+//    mode is skipped (powerspectrum.c:63-87);
+//  - power /= |F0|^2, then /= count; keffs /= count; empty bins compacted
+//    (powerspectrum.c:97-116).
+//
+// MI355X design: the pass streams the grid once from HBM (16 B per mode,
+// coalesced dwordx4 loads, several loads in flight per lane).  The bin index
+// is integer-exact: the host evaluates the reference expression once per
+// integer |k|^2 and stores, per bin, the smallest |k|^2 that reaches it; the
+// kernel takes an fp32 estimate of the bin and corrects it against that
+// threshold table in LDS, so counts are bit-exact with the reference's libm.
+// Each wavefront accumulates into its own LDS histogram (ds_add_f64), blocks
+// write partial histograms and a second kernel sums them in fixed order, so
+// results are deterministic run to run.  Mode counts and <|k|> depend only on
+// the geometry; they are computed once per (dims, nrbins, slab range) by a
+// read-free kernel and cached.
+#include <math.h>
+#include <string.h>
+#include <vector>
+
+#include "kn_hip.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kUnroll = 4;
+
+struct BinGeom {
+    int dims, nrbins, startslab, nslab;
+    int *d_thr;       // nrbins+1 thresholds on |k|^2
+    double *d_iw;     // per-axis inverse window by storage index
+    float half_bpu;   // 0.5 * binsperunit, for the fp32 estimate
+    std::vector<long long> count;
+    std::vector<double> keff_sum;
+    int grid;
+    double *d_partial;      // grid * nrbins
+    unsigned *d_cpartial;   // grid * nrbins (geometry pass)
+    double *d_sum;          // nrbins + 1 (last: |F0|^2)
+};
+
+std::vector<BinGeom *> g_geoms;
+
+// Per-bin smallest |k|^2 reaching the bin, from the reference expression.
+void build_thresholds(int dims, int nrbins, std::vector<int> &thr)
+{
+    const double binsperunit = (nrbins - 1) / log(sqrt(3) * dims / 2.0);
+    const long long kmax2 = 3LL * (dims / 2) * (dims / 2);
+    thr.assign(nrbins + 1, 0x7fffffff);
+    thr[0] = 0;
+    int prev = 0;
+    for (long long k2 = 1; k2 <= kmax2; k2++) {
+        const int b = (int)floor(binsperunit * log(sqrt((double)k2)));
+        if (b < prev) kn_terminate(KN_ERR_RANGE, "bin index not monotone in |k|^2 at %lld\n", k2);
+        if (b >= nrbins) kn_terminate(KN_ERR_RANGE, "bin index %d beyond %d bins\n", b, nrbins);
+        while (prev < b) thr[++prev] = (int)k2;
+    }
+}
+
+__device__ __forceinline__ int find_bin(int k2, float half_bpu, const int *__restrict__ thr, int nrbins)
+{
+    int b = (int)(__logf((float)k2) * half_bpu);
+    b = b < 0 ? 0 : (b > nrbins - 1 ? nrbins - 1 : b);
+    while (b > 0 && k2 < thr[b]) b--;
+    while (b < nrbins - 1 && k2 >= thr[b + 1]) b++;
+    return b;
+}
+
+// Position (slab i, row j, z) of flat mode offset within the slab range.
+struct Pos {
+    int i, j, z;
+};
+
+__device__ __forceinline__ void advance(Pos &p, int step, int rowlen, int dims)
+{
+    p.z += step;
+    while (p.z >= rowlen) {
+        p.z -= rowlen;
+        if (++p.j == dims) {
+            p.j = 0;
+            p.i++;
+        }
+    }
+}
+
+// Power pass: one HBM sweep, per-wave LDS histograms, per-block partials.
+__global__ __launch_bounds__(kBlock) void k_bin_power(const double2 *__restrict__ F, int dims, int startslab,
+                                                       long long rows, int nrbins, const int *__restrict__ thr_g,
+                                                       const double *__restrict__ iw_g, float half_bpu,
+                                                       double *__restrict__ partial)
+{
+    extern __shared__ double smem[];
+    double *hist = smem;                                   // kWaves * nrbins
+    double *iw = hist + (size_t)kWaves * nrbins;           // dims
+    int *thr = (int *)(iw + dims);                         // nrbins + 1
+    const int tid = threadIdx.x, wave = tid >> 6;
+    for (int t = tid; t < kWaves * nrbins; t += kBlock) hist[t] = 0.0;
+    for (int t = tid; t < dims; t += kBlock) iw[t] = iw_g[t];
+    for (int t = tid; t <= nrbins; t += kBlock) thr[t] = thr_g[t];
+    __syncthreads();
+
+    const int rowlen = dims / 2 + 1, half = dims / 2;
+    const long long r0 = rows * blockIdx.x / gridDim.x, r1 = rows * (blockIdx.x + 1) / gridDim.x;
+    const long long m1 = r1 * rowlen;
+    long long m = r0 * rowlen + tid;
+    Pos p;
+    {
+        const long long rl = r0 + tid / rowlen;
+        p.z = tid % rowlen;
+        p.j = (int)(rl % dims);
+        p.i = (int)(rl / dims);
+    }
+    double *my = hist + (size_t)wave * nrbins;
+    for (; m < m1; m += (long long)kBlock * kUnroll) {
+        double2 f[kUnroll];
+        Pos q[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            q[u] = p;
+            const long long mu = m + (long long)kBlock * u;
+            f[u] = mu < m1 ? F[mu] : make_double2(0.0, 0.0);
+            advance(p, kBlock, rowlen, dims);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const long long mu = m + (long long)kBlock * u;
+            if (mu >= m1) break;
+            const int kx = kn_kval(startslab + q[u].i, dims), ky = kn_kval(q[u].j, dims), kz = q[u].z;
+            const int k2 = kx * kx + ky * ky + kz * kz;
+            if (k2 == 0) continue;
+            const int b = find_bin(k2, half_bpu, thr, nrbins);
+            // reference order: (|F|^2) * ((iwx iwy iwz)^2)^2, times 2 off the kz = 0, N/2 planes
+            const double w = __dmul_rn(__dmul_rn(iw[startslab + q[u].i], iw[q[u].j]), iw[kz]);
+            const double w2 = __dmul_rn(w, w);
+            const double w4 = __dmul_rn(w2, w2);
+            const double p2 = __dadd_rn(__dmul_rn(f[u].x, f[u].x), __dmul_rn(f[u].y, f[u].y));
+            double v = __dmul_rn(p2, w4);
+            if (kz != 0 && kz != half) v *= 2.0;
+            atomicAdd(&my[b], v);
+        }
+    }
+    __syncthreads();
+    for (int b = tid; b < nrbins; b += kBlock) {
+        double s = hist[b];
+        for (int w = 1; w < kWaves; w++) s += hist[(size_t)w * nrbins + b];
+        partial[(size_t)blockIdx.x * nrbins + b] = s;
+    }
+}
+
+// Geometry pass (no grid reads): multiplicity counts and sum of multiplicity*|k|.
+__global__ __launch_bounds__(kBlock) void k_bin_geometry(int dims, int startslab, long long rows, int nrbins,
+                                                          const int *__restrict__ thr_g, float half_bpu,
+                                                          unsigned *__restrict__ cpartial,
+                                                          double *__restrict__ kpartial)
+{
+    extern __shared__ double smem[];
+    double *khist = smem;                                      // kWaves * nrbins
+    unsigned *chist = (unsigned *)(khist + (size_t)kWaves * nrbins);  // kWaves * nrbins
+    int *thr = (int *)(chist + (size_t)kWaves * nrbins);
+    const int tid = threadIdx.x, wave = tid >> 6;
+    for (int t = tid; t < kWaves * nrbins; t += kBlock) {
+        khist[t] = 0.0;
+        chist[t] = 0u;
+    }
+    for (int t = tid; t <= nrbins; t += kBlock) thr[t] = thr_g[t];
+    __syncthreads();
+    const int rowlen = dims / 2 + 1, half = dims / 2;
+    const long long r0 = rows * blockIdx.x / gridDim.x, r1 = rows * (blockIdx.x + 1) / gridDim.x;
+    const long long m1 = r1 * rowlen;
+    long long m = r0 * rowlen + tid;
+    Pos p;
+    {
+        const long long rl = r0 + tid / rowlen;
+        p.z = tid % rowlen;
+        p.j = (int)(rl % dims);
+        p.i = (int)(rl / dims);
+    }
+    for (; m < m1; m += kBlock) {
+        const int kx = kn_kval(startslab + p.i, dims), ky = kn_kval(p.j, dims), kz = p.z;
+        const int k2 = kx * kx + ky * ky + kz * kz;
+        if (k2 != 0) {
+            const int b = find_bin(k2, half_bpu, thr, nrbins);
+            const unsigned c = (kz != 0 && kz != half) ? 2u : 1u;
+            const double kk = sqrt((double)k2);
+            atomicAdd(&chist[(size_t)wave * nrbins + b], c);
+            atomicAdd(&khist[(size_t)wave * nrbins + b], c == 2u ? 2.0 * kk : kk);
+        }
+        advance(p, kBlock, rowlen, dims);
+    }
+    __syncthreads();
+    for (int b = tid; b < nrbins; b += kBlock) {
+        double s = khist[b];
+        unsigned c = chist[b];
+        for (int w = 1; w < kWaves; w++) {
+            s += khist[(size_t)w * nrbins + b];
+            c += chist[(size_t)w * nrbins + b];
+        }
+        kpartial[(size_t)blockIdx.x * nrbins + b] = s;
+        cpartial[(size_t)blockIdx.x * nrbins + b] = c;
+    }
+}
+
+// Fixed-order sum of per-block partial histograms: 64 bins per block across
+// lanes, 16 waves split the blocks, then a fixed-order combine in LDS.
+constexpr int kRedWaves = 16;
+__global__ __launch_bounds__(64 * kRedWaves) void k_reduce_partials(const double *__restrict__ partial, int nblk,
+                                                                     int nrbins, double *__restrict__ out)
+{
+    __shared__ double acc[kRedWaves][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = blockIdx.x * 64 + lane;
+    const int g0 = nblk * wave / kRedWaves, g1 = nblk * (wave + 1) / kRedWaves;
+    double s = 0.0;
+    if (b < nrbins)
+        for (int g = g0; g < g1; g++) s += partial[(size_t)g * nrbins + b];
+    acc[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && b < nrbins) {
+        double t = acc[0][lane];
+        for (int w = 1; w < kRedWaves; w++) t += acc[w][lane];
+        out[b] = t;
+    }
+}
+
+__global__ void k_reduce_counts(const unsigned *__restrict__ cpartial, const double *__restrict__ kpartial,
+                                int nblk, int nrbins, unsigned long long *__restrict__ count,
+                                double *__restrict__ ksum)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nrbins) return;
+    unsigned long long c = 0;
+    double s = 0.0;
+    for (int g = 0; g < nblk; g++) {
+        c += cpartial[(size_t)g * nrbins + b];
+        s += kpartial[(size_t)g * nrbins + b];
+    }
+    count[b] = c;
+    ksum[b] = s;
+}
+
+// |F0|^2 from the rank holding slab 0 (powerspectrum.c:45-47), else 0.
+__global__ void k_total_mass2(const double2 *__restrict__ F, int has_zero_mode, double *__restrict__ out)
+{
+    if (has_zero_mode) {
+        const double2 f = F[0];
+        *out = f.x * f.x + f.y * f.y;
+    } else {
+        *out = 0.0;
+    }
+}
+
+int num_cus()
+{
+    static int n = 0;
+    if (!n) {
+        int dev = 0;
+        KN_CHECK(hipGetDevice(&dev));
+        KN_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    return n;
+}
+
+size_t power_smem(int dims, int nrbins)
+{
+    return sizeof(double) * ((size_t)kWaves * nrbins + dims) + sizeof(int) * (nrbins + 1);
+}
+
+BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
+{
+    for (BinGeom *g : g_geoms)
+        if (g->dims == dims && g->nrbins == nrbins && g->startslab == startslab && g->nslab == nslab) return g;
+    if (dims < 2 || (dims & 1) || nrbins < 1 || nslab < 1 || startslab < 0 || startslab + nslab > dims)
+        kn_terminate(KN_ERR_ARG, "total_powerspectrum: bad geometry dims=%d nrbins=%d slabs=[%d,%d)\n", dims,
+                     nrbins, startslab, startslab + nslab);
+    if (power_smem(dims, nrbins) > 64 * 1024)
+        kn_terminate(KN_ERR_ARG, "total_powerspectrum: nrbins=%d dims=%d exceed the LDS budget\n", nrbins, dims);
+    BinGeom *g = new BinGeom();
+    g->dims = dims;
+    g->nrbins = nrbins;
+    g->startslab = startslab;
+    g->nslab = nslab;
+    std::vector<int> thr;
+    build_thresholds(dims, nrbins, thr);
+    std::vector<double> iw(dims);
+    for (int n = 0; n < dims; n++) {
+        const int kx = kn_kval(n, dims);
+        iw[n] = kx ? M_PI * kx / (dims * sin(M_PI * kx / (double)dims)) : 1.0;  // powerspectrum.c:11
+    }
+    g->half_bpu = (float)(0.5 * (nrbins - 1) / log(sqrt(3) * dims / 2.0));
+    const long long rows = (long long)nslab * dims;
+    g->grid = (int)(rows < 4LL * num_cus() ? rows : 4LL * num_cus());
+    KN_CHECK(hipMalloc(&g->d_thr, sizeof(int) * (nrbins + 1)));
+    KN_CHECK(hipMalloc(&g->d_iw, sizeof(double) * dims));
+    KN_CHECK(hipMalloc(&g->d_partial, sizeof(double) * (size_t)g->grid * nrbins));
+    KN_CHECK(hipMalloc(&g->d_cpartial, sizeof(unsigned) * (size_t)g->grid * nrbins));
+    KN_CHECK(hipMalloc(&g->d_sum, sizeof(double) * (nrbins + 1)));
+    KN_CHECK(hipMemcpy(g->d_thr, thr.data(), sizeof(int) * (nrbins + 1), hipMemcpyHostToDevice));
+    KN_CHECK(hipMemcpy(g->d_iw, iw.data(), sizeof(double) * dims, hipMemcpyHostToDevice));
+    // geometry pass, once per geometry
+    unsigned long long *d_count;
+    double *d_ksum;
+    KN_CHECK(hipMalloc(&d_count, sizeof(unsigned long long) * nrbins));
+    KN_CHECK(hipMalloc(&d_ksum, sizeof(double) * nrbins));
+    const size_t gsmem = (sizeof(double) + sizeof(unsigned)) * (size_t)kWaves * nrbins + sizeof(int) * (nrbins + 1);
+    hipLaunchKernelGGL(k_bin_geometry, dim3(g->grid), dim3(kBlock), gsmem, 0, dims, startslab, rows, nrbins,
+                       g->d_thr, g->half_bpu, g->d_cpartial, g->d_partial);
+    KN_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_counts, dim3((nrbins + 255) / 256), dim3(256), 0, 0, g->d_cpartial, g->d_partial,
+                       g->grid, nrbins, d_count, d_ksum);
+    KN_CHECK(hipGetLastError());
+    std::vector<unsigned long long> c(nrbins);
+    g->count.resize(nrbins);
+    g->keff_sum.resize(nrbins);
+    KN_CHECK(hipMemcpy(c.data(), d_count, sizeof(unsigned long long) * nrbins, hipMemcpyDeviceToHost));
+    KN_CHECK(hipMemcpy(g->keff_sum.data(), d_ksum, sizeof(double) * nrbins, hipMemcpyDeviceToHost));
+    for (int b = 0; b < nrbins; b++) g->count[b] = (long long)c[b];
+    KN_CHECK(hipFree(d_count));
+    KN_CHECK(hipFree(d_ksum));
+    g_geoms.push_back(g);
+    return g;
+}
+
+void launch_power(BinGeom *g, const double2 *F, hipStream_t st)
+{
+    const long long rows = (long long)g->nslab * g->dims;
+    hipLaunchKernelGGL(k_bin_power, dim3(g->grid), dim3(kBlock), power_smem(g->dims, g->nrbins), st, F, g->dims,
+                       g->startslab, rows, g->nrbins, g->d_thr, g->d_iw, g->half_bpu, g->d_partial);
+    KN_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_partials, dim3((g->nrbins + 63) / 64), dim3(64 * kRedWaves), 0, st, g->d_partial,
+                       g->grid, g->nrbins, g->d_sum);
+    KN_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_total_mass2, dim3(1), dim3(1), 0, st, F, g->startslab == 0 ? 1 : 0, g->d_sum + g->nrbins);
+    KN_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" int kn_bin_power_device(const fftw_complex *d_field, int dims, int nrbins, int startslab, int nslab,
+                                   double *d_power, double *d_total_mass2, void *stream)
+{
+    BinGeom *g = get_geom(dims, nrbins, startslab, nslab);
+    hipStream_t st = (hipStream_t)stream;
+    launch_power(g, (const double2 *)d_field, st);
+    if (d_power)
+        KN_CHECK(hipMemcpyAsync(d_power, g->d_sum, sizeof(double) * nrbins, hipMemcpyDeviceToDevice, st));
+    if (d_total_mass2)
+        KN_CHECK(hipMemcpyAsync(d_total_mass2, g->d_sum + nrbins, sizeof(double), hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
+// Geometry sums for callers that already hold the power sums (add_nu_power_to_rhogrid).
+extern "C" void kn_bin_geometry_host(int dims, int nrbins, int startslab, int nslab, long long *count,
+                                     double *keff_sum)
+{
+    BinGeom *g = get_geom(dims, nrbins, startslab, nslab);
+    memcpy(count, g->count.data(), sizeof(long long) * nrbins);
+    memcpy(keff_sum, g->keff_sum.data(), sizeof(double) * nrbins);
+}
+
+// Normalise and compact as powerspectrum.c:97-116.
+extern "C" int kn_finish_powerspectrum(int nrbins, double total_mass2, double *power, long long *count,
+                                       double *keffs)
+{
+    int nonzero = 0;
+    kn_message(0, "Total powerspectrum mass: %g\n", sqrt(total_mass2));
+    for (int i = 0; i < nrbins; i++) {
+        power[i] /= total_mass2;
+        if (count[i]) {
+            keffs[i] /= count[i];
+            power[i] /= count[i];
+        }
+    }
+    for (int i = 0; i < nrbins; i++) {
+        if (count[i]) {
+            if (nonzero < i) {
+                power[nonzero] = power[i];
+                keffs[nonzero] = keffs[i];
+                count[nonzero] = count[i];
+            }
+            nonzero++;
+        }
+    }
+    return nonzero;
+}
+
+extern "C" void kn_bin_power_host_sums(const fftw_complex *d_field, int dims, int nrbins, int startslab, int nslab,
+                                       double *power, double *total_mass2)
+{
+    BinGeom *g = get_geom(dims, nrbins, startslab, nslab);
+    std::vector<double> sums(nrbins + 1);
+    launch_power(g, (const double2 *)d_field, 0);
+    KN_CHECK(hipMemcpy(sums.data(), g->d_sum, sizeof(double) * (nrbins + 1), hipMemcpyDeviceToHost));
+    memcpy(power, sums.data(), sizeof(double) * nrbins);
+    *total_mass2 = sums[nrbins];
+}
+
+extern "C" int total_powerspectrum(const int dims, fftw_complex *outfield, const int nrbins, const int startslab,
+                                   const int nslab, double *power, long long int *count, double *keffs,
+                                   MPI_Comm comm)
+{
+    (void)comm;
+    BinGeom *g = get_geom(dims, nrbins, startslab, nslab);
+    const double2 *F = (const double2 *)outfield;
+    double2 *staged = nullptr;
+    if (!kn_is_device_ptr(outfield)) {
+        const size_t bytes = sizeof(double2) * (size_t)nslab * dims * (dims / 2 + 1);
+        KN_CHECK(hipMalloc(&staged, bytes));
+        KN_CHECK(hipMemcpy(staged, outfield, bytes, hipMemcpyHostToDevice));
+        F = staged;
+    }
+    launch_power(g, F, 0);
+    std::vector<double> sums(nrbins + 1);
+    KN_CHECK(hipMemcpy(sums.data(), g->d_sum, sizeof(double) * (nrbins + 1), hipMemcpyDeviceToHost));
+    if (staged) KN_CHECK(hipFree(staged));
+    memcpy(power, sums.data(), sizeof(double) * nrbins);
+    memcpy(count, g->count.data(), sizeof(long long) * nrbins);
+    memcpy(keffs, g->keff_sum.data(), sizeof(double) * nrbins);
+    return kn_finish_powerspectrum(nrbins, sums[nrbins], power, count, keffs);
+}

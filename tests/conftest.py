@@ -1,0 +1,78 @@
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "kspace-neutrinos_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+TESTHOST = os.path.join(ROOT, "tests", "host", "_build", "libkn_testhost.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+REFDATA = os.path.join(GOLDEN, "ref_testdata")
+
+UNIT_LENGTH = 3.085678e21          # kpc/h in cm (delta_tot_table_test.c:90)
+UNIT_TIME = UNIT_LENGTH / 1e5
+T_CMB0 = 2.7255
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the C ABI on the device)")
+
+
+_host = None
+
+
+def testhost():
+    """The N-body host services library, loaded RTLD_GLOBAL before the product."""
+    global _host
+    if _host is None:
+        if not os.path.exists(TESTHOST):
+            raise RuntimeError("test host library missing: run __graft_entry__.build()")
+        h = C.CDLL(TESTHOST, mode=C.RTLD_GLOBAL)
+        import knhip
+        h.kn_test_init_hubble.argtypes = [C.POINTER(knhip.OmegaNu), C.c_double, C.c_double]
+        h.kn_test_hubble_calls.restype = C.c_longlong
+        h.kn_test_messages.restype = C.c_longlong
+        _host = h
+    return _host
+
+
+@pytest.fixture(scope="session")
+def host():
+    return testhost()
+
+
+@pytest.fixture(scope="session")
+def kn(host):
+    import knhip
+    return knhip.lib()
+
+
+@pytest.fixture(scope="session")
+def orc():
+    import oracle_py
+    return oracle_py.lib()
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))

@@ -1,0 +1,196 @@
+"""ctypes view of the oracle (oracle/_build/libkn_oracle.so).
+
+Test infrastructure: the oracle is the checker, never the thing measured.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libkn_oracle.so")
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+llp = C.POINTER(C.c_longlong)
+
+
+class OrcRhoNu(C.Structure):
+    _fields_ = [("have_table", C.c_int), ("mnu", C.c_double), ("loga", C.c_double * 200),
+                ("rhonu", C.c_double * 200), ("c2", C.c_double * 200)]
+
+
+class OrcHybrid(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("nufrac_low", C.c_double * 3), ("nu_crit_time", C.c_double),
+                ("vcrit", C.c_double)]
+
+
+class OrcOmegaNu(C.Structure):
+    _fields_ = [("tab", OrcRhoNu * 3), ("deg", C.c_int * 3), ("rhocrit", C.c_double), ("kBtnu", C.c_double),
+                ("tcmb0", C.c_double), ("hybnu", OrcHybrid)]
+
+
+class OrcTransfer(C.Structure):
+    _fields_ = [("NPowerTable", C.c_int), ("logk", dp), ("T_nu", dp)]
+
+
+class OrcDtt(C.Structure):
+    _fields_ = [("nk", C.c_int), ("nk_allocated", C.c_int), ("namax", C.c_int), ("ia", C.c_int),
+                ("delta_nu_prefac", C.c_double), ("delta_tot_init_done", C.c_int),
+                ("scalefact", dp), ("block_delta_tot", dp), ("delta_nu_init", dp), ("delta_nu_last", dp),
+                ("wavenum", dp), ("omnu", C.POINTER(OrcOmegaNu)), ("Omeganonu", C.c_double),
+                ("light", C.c_double), ("TimeTransfer", C.c_double), ("mode", C.c_int),
+                ("last_error", C.c_int), ("qag_max_intervals", C.c_int)]
+
+
+class OrcDeltaPow(C.Structure):
+    _fields_ = [("logkk", dp), ("ratio", dp), ("nbins", C.c_int), ("norm", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "orc_init_omega_nu": (None, [C.POINTER(OrcOmegaNu), dp, C.c_double, C.c_double, C.c_double]),
+            "orc_get_omega_nu": (C.c_double, [C.POINTER(OrcOmegaNu), C.c_double]),
+            "orc_get_omega_nu_nopart": (C.c_double, [C.POINTER(OrcOmegaNu), C.c_double]),
+            "orc_get_omegag": (C.c_double, [C.POINTER(OrcOmegaNu), C.c_double]),
+            "orc_omega_nu_single": (C.c_double, [C.POINTER(OrcOmegaNu), C.c_double, C.c_int]),
+            "orc_nufrac_low": (C.c_double, [C.c_double]),
+            "orc_init_hybrid_nu": (None, [C.POINTER(OrcHybrid), dp, C.c_double, C.c_double, C.c_double, C.c_double]),
+            "orc_particle_nu_fraction": (C.c_double, [C.POINTER(OrcHybrid), C.c_double, C.c_int]),
+            "orc_rho_nu_exact": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+            "orc_init_hubble": (None, [C.POINTER(OrcOmegaNu), C.c_double, C.c_double]),
+            "orc_hubble": (C.c_double, [C.c_double]),
+            "orc_specialJ": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+            "orc_fslength": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+            "orc_get_delta_tot": (C.c_double, [C.c_double] * 6),
+            "orc_bessel_j0": (C.c_double, [C.c_double]),
+            "orc_transfer_load": (C.c_int, [C.POINTER(OrcTransfer), C.c_double, C.c_double, C.c_double, C.c_char_p]),
+            "orc_transfer_free": (None, [C.POINTER(OrcTransfer)]),
+            "orc_dtt_allocate": (None, [C.POINTER(OrcDtt), C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.POINTER(OrcOmegaNu), C.c_double, C.c_double]),
+            "orc_dtt_free": (None, [C.POINTER(OrcDtt)]),
+            "orc_delta_tot_init": (C.c_int, [C.POINTER(OrcDtt), C.c_int, dp, dp, C.POINTER(OrcTransfer), C.c_double]),
+            "orc_update_delta_tot": (None, [C.POINTER(OrcDtt), C.c_double, dp, dp, C.c_int]),
+            "orc_get_delta_nu": (C.c_int, [C.POINTER(OrcDtt), C.c_double, dp, dp, C.c_double]),
+            "orc_get_delta_nu_combined": (C.c_int, [C.POINTER(OrcDtt), C.c_double, dp, dp]),
+            "orc_get_delta_nu_update": (C.c_int, [C.POINTER(OrcDtt), C.c_double, C.c_int, dp, dp, dp,
+                                                  C.POINTER(OrcTransfer)]),
+            "orc_total_powerspectrum": (C.c_int, [C.c_int, dp, C.c_int, C.c_int, C.c_int, dp, llp, dp]),
+            "orc_psindex": (C.c_int, [C.c_longlong, C.c_int, C.c_int]),
+            "orc_get_dnudcdm": (C.c_double, [C.POINTER(OrcDeltaPow), C.c_double]),
+            "orc_rescale": (C.c_int, [C.c_int, dp, C.c_int, C.c_int, C.c_double, C.POINTER(OrcDeltaPow)]),
+            "orc_kspace_prefac": (C.c_double, [C.POINTER(OrcDtt), C.c_double]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    """Pointer to a contiguous float64 numpy array."""
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(dp)
+
+
+def omega_nu(masses, a0=0.01, h=0.7, tcmb=2.7255):
+    om = OrcOmegaNu()
+    m = (C.c_double * 3)(*masses)
+    lib().orc_init_omega_nu(C.byref(om), m, a0, h, tcmb)
+    return om
+
+
+def total_powerspectrum(field_c, nrbins, startslab=0, nslab=None):
+    """field_c: complex128 array [nslab][N][N/2+1]."""
+    dims = field_c.shape[1]
+    if nslab is None:
+        nslab = field_c.shape[0]
+    f = np.ascontiguousarray(field_c).view(np.float64).ravel()
+    power = np.zeros(nrbins)
+    count = np.zeros(nrbins, dtype=np.int64)
+    keffs = np.zeros(nrbins)
+    n = lib().orc_total_powerspectrum(dims, ptr(f), nrbins, startslab, nslab, ptr(power),
+                                      count.ctypes.data_as(llp), ptr(keffs))
+    return n, power, count, keffs
+
+
+def rescale(field_c, boxsize, logkk, ratio, norm, startslab=0):
+    dims = field_c.shape[1]
+    f = np.ascontiguousarray(field_c.copy())
+    fv = f.view(np.float64).ravel()
+    lk = np.ascontiguousarray(logkk, dtype=np.float64)
+    rt = np.ascontiguousarray(ratio, dtype=np.float64)
+    dpow = OrcDeltaPow(ptr(lk), ptr(rt), len(lk), norm)
+    rc = lib().orc_rescale(dims, ptr(fv), startslab, f.shape[0], boxsize, C.byref(dpow))
+    return rc, f
+
+
+def get_dnudcdm(logkk, ratio, norm, x):
+    lk = np.ascontiguousarray(logkk, dtype=np.float64)
+    rt = np.ascontiguousarray(ratio, dtype=np.float64)
+    dpow = OrcDeltaPow(ptr(lk), ptr(rt), len(lk), norm)
+    return lib().orc_get_dnudcdm(C.byref(dpow), x)
+
+
+class Table:
+    """Oracle integrator state with numpy views of its history block."""
+
+    def __init__(self, om, nk, time_transfer, time_max, omega0, unit_time, unit_length, mode=0):
+        self.om = om
+        self.d = OrcDtt()
+        lib().orc_dtt_allocate(C.byref(self.d), nk, time_transfer, time_max, omega0, C.byref(om),
+                               unit_time, unit_length)
+        self.d.mode = mode
+        namax = self.d.namax
+        self.scalefact = np.ctypeslib.as_array(self.d.scalefact, shape=(namax,))
+        self.delta_tot = np.ctypeslib.as_array(self.d.block_delta_tot, shape=(nk, namax))
+        self.delta_nu_init = np.ctypeslib.as_array(self.d.delta_nu_init, shape=(nk,))
+        self.delta_nu_last = np.ctypeslib.as_array(self.d.delta_nu_last, shape=(nk,))
+        self.wavenum = np.ctypeslib.as_array(self.d.wavenum, shape=(nk,))
+
+    def __del__(self):
+        try:
+            lib().orc_dtt_free(C.byref(self.d))
+        except Exception:
+            pass
+
+    def get_delta_nu(self, a, wavenum, mnu):
+        out = np.zeros(self.d.nk)
+        w = np.ascontiguousarray(wavenum, dtype=np.float64)
+        rc = lib().orc_get_delta_nu(C.byref(self.d), a, ptr(w), ptr(out), mnu)
+        return rc, out
+
+    def get_delta_nu_combined(self, a, wavenum):
+        out = np.zeros(self.d.nk)
+        w = np.ascontiguousarray(wavenum, dtype=np.float64)
+        rc = lib().orc_get_delta_nu_combined(C.byref(self.d), a, ptr(w), ptr(out))
+        return rc, out
+
+    def delta_tot_init(self, wavenum, delta_cdm, transfer, time):
+        w = np.ascontiguousarray(wavenum, dtype=np.float64)
+        c = np.ascontiguousarray(delta_cdm, dtype=np.float64)
+        return lib().orc_delta_tot_init(C.byref(self.d), len(w), ptr(w), ptr(c), C.byref(transfer), time)
+
+    def get_delta_nu_update(self, a, keff, delta_cdm, transfer):
+        k = np.ascontiguousarray(keff, dtype=np.float64)
+        c = np.ascontiguousarray(delta_cdm, dtype=np.float64)
+        out = np.zeros(len(k))
+        rc = lib().orc_get_delta_nu_update(C.byref(self.d), a, len(k), ptr(k), ptr(c), ptr(out),
+                                           C.byref(transfer))
+        return rc, out
+
+
+def load_transfer(path, boxsize, unit_length, input_unit_length):
+    t = OrcTransfer()
+    rc = lib().orc_transfer_load(C.byref(t), boxsize, unit_length, input_unit_length, path.encode())
+    return rc, t
