@@ -79,7 +79,7 @@ def main():
         sys.exit(2)
     import torch
     import conftest
-    conftest.testhost()
+    conftest.load_testhost()
     import knhip
     kn = knhip.lib()
 

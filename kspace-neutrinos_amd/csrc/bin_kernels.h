@@ -1,0 +1,269 @@
+// bin_kernels.h -- device kernels of the k-shell binning sweep (a).
+//
+// Included by powerspectrum.hip.  The kernels are templates over the launch
+// shape so the shape can be tuned without touching the arithmetic; the
+// arithmetic itself follows powerspectrum.c:56-89 (see powerspectrum.hip).
+#ifndef KN_BIN_KERNELS_H
+#define KN_BIN_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <vector>
+
+namespace kn_bin {
+
+// KVAL of powerspectrum.c:27
+__host__ __device__ inline int kval(int n, int dims) { return n <= dims / 2 ? n : n - dims; }
+
+// Per-bin smallest |k|^2 reaching the bin under the reference expression
+// floor(binsperunit * log(sqrt(k2))) (powerspectrum.c:40, :67).  Returns 0,
+// or the first k2 where the expression is not monotone / exceeds nrbins-1.
+inline long long build_thresholds(int dims, int nrbins, std::vector<int> &thr)
+{
+    const double binsperunit = (nrbins - 1) / log(sqrt(3) * dims / 2.0);
+    const long long kmax2 = 3LL * (dims / 2) * (dims / 2);
+    thr.assign(nrbins + 1, 0x7fffffff);
+    thr[0] = 0;
+    int prev = 0;
+    for (long long k2 = 1; k2 <= kmax2; k2++) {
+        const int b = (int)floor(binsperunit * log(sqrt((double)k2)));
+        if (b < prev || b >= nrbins) return k2;
+        while (prev < b) thr[++prev] = (int)k2;
+    }
+    return 0;
+}
+
+// Per-axis inverse CiC window by storage index (powerspectrum.c:8-12).
+inline void build_invwindow(int dims, std::vector<double> &iw)
+{
+    iw.resize(dims);
+    for (int n = 0; n < dims; n++) {
+        const int kx = kval(n, dims);
+        iw[n] = kx ? M_PI * kx / (dims * sin(M_PI * kx / (double)dims)) : 1.0;
+    }
+}
+
+__device__ __forceinline__ int find_bin(int k2, float half_bpu, const int *__restrict__ thr, int nrbins)
+{
+    int b = (int)(__logf((float)k2) * half_bpu);
+    b = b < 0 ? 0 : (b > nrbins - 1 ? nrbins - 1 : b);
+    while (b > 0 && k2 < thr[b]) b--;
+    while (b < nrbins - 1 && k2 >= thr[b + 1]) b++;
+    return b;
+}
+
+struct Pos {
+    int i, j, z;  // slab offset, row, z of a flat mode offset
+};
+
+__device__ __forceinline__ void advance(Pos &p, int step, int rowlen, int dims)
+{
+    p.z += step;
+    while (p.z >= rowlen) {
+        p.z -= rowlen;
+        if (++p.j == dims) {
+            p.j = 0;
+            p.i++;
+        }
+    }
+}
+
+__device__ __forceinline__ Pos pos_of(long long off, long long row0, int rowlen, int dims)
+{
+    Pos p;
+    const long long rl = row0 + off / rowlen;
+    p.z = (int)(off % rowlen);
+    p.j = (int)(rl % dims);
+    p.i = (int)(rl / dims);
+    return p;
+}
+
+// multiplicity * |F|^2 * ((iwx iwy iwz)^2)^2 in the reference's rounding order
+__device__ __forceinline__ double mode_weight(double2 f, double iwxy, double iwz, int kz, int half)
+{
+    const double w = __dmul_rn(iwxy, iwz);
+    const double w2 = __dmul_rn(w, w);
+    const double w4 = __dmul_rn(w2, w2);
+    const double p2 = __dadd_rn(__dmul_rn(f.x, f.x), __dmul_rn(f.y, f.y));
+    double v = __dmul_rn(p2, w4);
+    if (kz != 0 && kz != half) v *= 2.0;
+    return v;
+}
+
+struct BinArgs {
+    const double2 *F;
+    int dims, startslab, nrbins;
+    long long rows;
+    const int *thr;
+    const double *iw;
+    float half_bpu;
+    double *partial;  // gridDim.x * nrbins
+};
+
+template <int WAVES>
+__host__ __device__ constexpr size_t smem_bytes(int dims, int nrbins)
+{
+    return sizeof(double) * ((size_t)WAVES * nrbins + dims) + sizeof(int) * (nrbins + 1);
+}
+
+template <int WAVES>
+__device__ __forceinline__ void load_tables(const BinArgs &a, double *&hist, double *&iw, int *&thr)
+{
+    extern __shared__ double smem[];
+    hist = smem;
+    iw = hist + (size_t)WAVES * a.nrbins;
+    thr = (int *)(iw + a.dims);
+    for (int t = threadIdx.x; t < WAVES * a.nrbins; t += 64 * WAVES) hist[t] = 0.0;
+    for (int t = threadIdx.x; t < a.dims; t += 64 * WAVES) iw[t] = a.iw[t];
+    for (int t = threadIdx.x; t <= a.nrbins; t += 64 * WAVES) thr[t] = a.thr[t];
+    __syncthreads();
+}
+
+template <int WAVES>
+__device__ __forceinline__ void store_partial(const BinArgs &a, const double *hist)
+{
+    __syncthreads();
+    for (int b = threadIdx.x; b < a.nrbins; b += 64 * WAVES) {
+        double s = hist[b];
+        for (int w = 1; w < WAVES; w++) s += hist[(size_t)w * a.nrbins + b];
+        a.partial[(size_t)blockIdx.x * a.nrbins + b] = s;
+    }
+}
+
+// Each lane streams RUN consecutive modes of its strip (stride RUN*16 B across
+// lanes, so a wave's RUN loads cover one contiguous 1 KB*RUN span); bins are
+// non-decreasing along a row, so equal-bin runs are summed in registers and
+// flushed to the wave's LDS histogram once per run.
+template <int WAVES, int RUN>
+__global__ __launch_bounds__(64 * WAVES) void k_bin_power_runs(BinArgs a)
+{
+    double *hist, *iw;
+    int *thr;
+    load_tables<WAVES>(a, hist, iw, thr);
+    constexpr int B = 64 * WAVES;
+    const int rowlen = a.dims / 2 + 1, half = a.dims / 2;
+    const long long r0 = a.rows * blockIdx.x / gridDim.x, r1 = a.rows * (blockIdx.x + 1) / gridDim.x;
+    const long long m0 = r0 * rowlen, m1 = r1 * rowlen;
+    double *my = hist + (size_t)(threadIdx.x >> 6) * a.nrbins;
+    Pos p = pos_of((long long)threadIdx.x * RUN, r0, rowlen, a.dims);
+    for (long long m = m0 + (long long)threadIdx.x * RUN; m < m1; m += (long long)B * RUN) {
+        double2 f[RUN];
+#pragma unroll
+        for (int r = 0; r < RUN; r++) f[r] = (m + r < m1) ? a.F[m + r] : make_double2(0.0, 0.0);
+        Pos q = p;
+        int cur = -1;
+        double acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < RUN; r++) {
+            if (m + r >= m1) break;
+            if (r) advance(q, 1, rowlen, a.dims);
+            const int kx = kval(a.startslab + q.i, a.dims), ky = kval(q.j, a.dims);
+            const int k2 = kx * kx + ky * ky + q.z * q.z;
+            if (k2 == 0) continue;
+            const int b = find_bin(k2, a.half_bpu, thr, a.nrbins);
+            const double v = mode_weight(f[r], __dmul_rn(iw[a.startslab + q.i], iw[q.j]), iw[q.z], q.z, half);
+            if (b != cur) {
+                if (cur >= 0) atomicAdd(&my[cur], acc);
+                cur = b;
+                acc = v;
+            } else {
+                acc += v;
+            }
+        }
+        if (cur >= 0) atomicAdd(&my[cur], acc);
+        advance(p, B * RUN, rowlen, a.dims);
+    }
+    store_partial<WAVES>(a, hist);
+}
+
+// Coalesced variant: lane l of the block takes modes l, l+B, l+2B, ... with
+// UNROLL loads in flight.
+template <int WAVES, int UNROLL>
+__global__ __launch_bounds__(64 * WAVES) void k_bin_power_coalesced(BinArgs a)
+{
+    double *hist, *iw;
+    int *thr;
+    load_tables<WAVES>(a, hist, iw, thr);
+    constexpr int B = 64 * WAVES;
+    const int rowlen = a.dims / 2 + 1, half = a.dims / 2;
+    const long long r0 = a.rows * blockIdx.x / gridDim.x, r1 = a.rows * (blockIdx.x + 1) / gridDim.x;
+    const long long m1 = r1 * rowlen;
+    double *my = hist + (size_t)(threadIdx.x >> 6) * a.nrbins;
+    Pos p = pos_of(threadIdx.x, r0, rowlen, a.dims);
+    for (long long m = r0 * rowlen + threadIdx.x; m < m1; m += (long long)B * UNROLL) {
+        double2 f[UNROLL];
+        Pos q[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            q[u] = p;
+            const long long mu = m + (long long)B * u;
+            f[u] = mu < m1 ? a.F[mu] : make_double2(0.0, 0.0);
+            advance(p, B, rowlen, a.dims);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            if (m + (long long)B * u >= m1) break;
+            const int kx = kval(a.startslab + q[u].i, a.dims), ky = kval(q[u].j, a.dims), kz = q[u].z;
+            const int k2 = kx * kx + ky * ky + kz * kz;
+            if (k2 == 0) continue;
+            const int b = find_bin(k2, a.half_bpu, thr, a.nrbins);
+            atomicAdd(&my[b], mode_weight(f[u], __dmul_rn(iw[a.startslab + q[u].i], iw[q[u].j]), iw[kz], kz, half));
+        }
+    }
+    store_partial<WAVES>(a, hist);
+}
+
+// Software-pipelined runs variant: the next strip's loads are issued before
+// the current strip is binned, doubling the bytes in flight per wave.
+template <int WAVES, int RUN>
+__global__ __launch_bounds__(64 * WAVES) void k_bin_power_runs_pf(BinArgs a)
+{
+    double *hist, *iw;
+    int *thr;
+    load_tables<WAVES>(a, hist, iw, thr);
+    constexpr int B = 64 * WAVES;
+    const int rowlen = a.dims / 2 + 1, half = a.dims / 2;
+    const long long r0 = a.rows * blockIdx.x / gridDim.x, r1 = a.rows * (blockIdx.x + 1) / gridDim.x;
+    const long long m0 = r0 * rowlen, m1 = r1 * rowlen;
+    const long long S = (long long)B * RUN;
+    double *my = hist + (size_t)(threadIdx.x >> 6) * a.nrbins;
+    Pos p = pos_of((long long)threadIdx.x * RUN, r0, rowlen, a.dims);
+    long long m = m0 + (long long)threadIdx.x * RUN;
+    double2 cur[RUN];
+#pragma unroll
+    for (int r = 0; r < RUN; r++) cur[r] = (m + r < m1) ? a.F[m + r] : make_double2(0.0, 0.0);
+    for (; m < m1; m += S) {
+        double2 nxt[RUN];
+#pragma unroll
+        for (int r = 0; r < RUN; r++) nxt[r] = (m + S + r < m1) ? a.F[m + S + r] : make_double2(0.0, 0.0);
+        Pos q = p;
+        int curb = -1;
+        double acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < RUN; r++) {
+            if (m + r >= m1) break;
+            if (r) advance(q, 1, rowlen, a.dims);
+            const int kx = kval(a.startslab + q.i, a.dims), ky = kval(q.j, a.dims);
+            const int k2 = kx * kx + ky * ky + q.z * q.z;
+            if (k2 == 0) continue;
+            const int b = find_bin(k2, a.half_bpu, thr, a.nrbins);
+            const double v = mode_weight(cur[r], __dmul_rn(iw[a.startslab + q.i], iw[q.j]), iw[q.z], q.z, half);
+            if (b != curb) {
+                if (curb >= 0) atomicAdd(&my[curb], acc);
+                curb = b;
+                acc = v;
+            } else {
+                acc += v;
+            }
+        }
+        if (curb >= 0) atomicAdd(&my[curb], acc);
+#pragma unroll
+        for (int r = 0; r < RUN; r++) cur[r] = nxt[r];
+        advance(p, B * RUN, rowlen, a.dims);
+    }
+    store_partial<WAVES>(a, hist);
+}
+
+}  // namespace kn_bin
+
+#endif

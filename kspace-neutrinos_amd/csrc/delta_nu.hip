@@ -1,7 +1,424 @@
-// delta_nu.hip -- placeholder until the integrator kernels land.
+// delta_nu.hip -- (b) the free-streaming integral get_delta_nu on the GPU.
+//
+// Replaces the integral of get_delta_nu (delta_tot_table.c:551-605) for every
+// species at once.  The host keeps what the reference computes once per call
+// with the host's own hubble_function (the initial-condition piece,
+// fslength(T0, a), delta_tot_table.c:522-535) and the per-species parameters
+// (:539-548); the device computes, in order:
+//   k_fs_nodes   the 16 Na free-streaming lengths fslength(x_i, log a) on the
+//                reference's uniform log-a grid, each a QAG-GK61 integral of
+//                1/(a^2 H) (relerr 1e-6), one wavefront per node;
+//   k_spline     natural cubic spline coefficients of that table and of every
+//                delta_tot[k][0..Na-1] row (windowed parallel tridiagonal solve);
+//   k_bucket     a uniform bucket index over the stored log a values;
+//   k_dnu_qag    one wavefront per (species, k): QAG-GK61 of the integrand
+//                fsl/(a H) * specialJ * S_k over [log T0, log a] at the
+//                species' tolerance, as the reference does per k.
+// H(a) on the device is a Chebyshev interpolant of ln H(ln a) sampled from the
+// host's hubble_function on fixed panels (built once, extended on demand).
+#include <math.h>
+#include <string.h>
+#include <unordered_map>
+#include <vector>
+
+#include "dnu_device.h"
 #include "kn_hip.h"
-extern "C" void kn_dnu_solve(const _delta_tot_table *, double, const double *, int, const kn_species *, double *)
+
+namespace {
+
+using namespace kn_dnu;
+
+constexpr int kWavesPerBlock = 4;
+constexpr int kChunk = 32;  // spline rows per thread (window chunk)
+
+// ---------------------------------------------------------------- H table
+
+struct HostHTab {
+    double x0 = 0.0;             // ln a of panel 0; a panel edge falls on ln TimeTransfer
+    double width = 1.0 / 256.0;  // panel width in ln a
+    int npanel = 0;
+    std::vector<double> coef;
+    std::vector<unsigned char> built;
+    double *d_coef = nullptr;
+    size_t d_cap = 0;
+    bool dirty = false;
+};
+HostHTab g_h;
+
+// ln a covered: [origin - 20, origin + 8]
+void h_reset(double origin)
 {
-    kn_terminate(KN_ERR_ARG, "get_delta_nu: GPU integrator not built\n");
+    g_h.x0 = origin - 20.0;
+    g_h.npanel = (int)(28.0 / g_h.width);
+    g_h.coef.assign((size_t)g_h.npanel * kHnc, 0.0);
+    g_h.built.assign(g_h.npanel, 0);
+    g_h.dirty = true;
 }
-extern "C" void kn_dnu_release(const _delta_tot_table *) {}
+
+void h_build_panel(int p)
+{
+    const double lo = g_h.x0 + p * g_h.width, half = 0.5 * g_h.width, mid = lo + half;
+    double g[kHnc];
+    for (int j = 0; j < kHnc; j++) {
+        const double t = cos(M_PI * (j + 0.5) / kHnc);
+        g[j] = log(kn_hubble(exp(mid + half * t)));
+    }
+    for (int k = 0; k < kHnc; k++) {
+        double s = 0.0;
+        for (int j = 0; j < kHnc; j++) s += g[j] * cos(M_PI * k * (j + 0.5) / kHnc);
+        g_h.coef[(size_t)p * kHnc + k] = (k == 0 ? 1.0 : 2.0) * s / kHnc;
+    }
+    g_h.built[p] = 1;
+}
+
+HTab device_htab()
+{
+    HTab H;
+    H.coef = g_h.d_coef;
+    H.x0 = g_h.x0;
+    H.inv_w = 1.0 / g_h.width;
+    H.npanel = g_h.npanel;
+    return H;
+}
+
+// ---------------------------------------------------------------- kernels
+
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_fs_nodes(HTab H, int nfs, double x0, double xa,
+                                                                   double light, double *__restrict__ fx,
+                                                                   double *__restrict__ fy, int *__restrict__ status)
+{
+    __shared__ QagWs ws[kWavesPerBlock];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int node = blockIdx.x * kWavesPerBlock + wave;
+    if (node >= nfs) return;
+    // delta_tot_table.c:582: log T0 + i (log a - log T0) / (Nfs - 1.)
+    const double xi = x0 + node * (xa - x0) / (nfs - 1.);
+    double v = 0.0;
+    if (xi < xa) {  // fslength returns 0 when logai >= logaf (:402)
+        FslIntegrand f;
+        f.H = H;
+        double r;
+        int ni;
+        const int rc = wave_qag61(f, xi, xa, 1e-6, kLimit, ws[wave], &r, &ni);
+        if (rc && lane == 0) atomicOr(status, 1);
+        v = light * r;
+    }
+    if (lane == 0) {
+        fx[node] = xi;
+        fy[node] = v;
+    }
+}
+
+// rows of natural-spline coefficients: row r uses x[0..n), y + r*ystride, c + r*cstride
+__global__ void k_spline(const double *__restrict__ x, const double *__restrict__ y, double *__restrict__ c, int n,
+                         int nrows, long long ystride, long long cstride)
+{
+    const int nchunk = (n + kChunk - 1) / kChunk;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nchunk * nrows) return;
+    const int r = (int)(t / nchunk), ch = (int)(t % nchunk);
+    const int lo = ch * kChunk, hi = lo + kChunk < n ? lo + kChunk : n;
+    spline_window(x, y + r * ystride, n, lo, hi, c + r * cstride);
+}
+
+// bucket[q] = largest i <= n-2 with x[i] <= x[0] + q / inv_bw
+__global__ void k_bucket(const double *__restrict__ x, int n, int nb, double inv_bw, int *__restrict__ bucket)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > nb) return;
+    const double edge = x[0] + q / inv_bw;
+    int lo = 0, hi = n - 1;
+    while (hi > lo + 1) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] > edge) hi = mid;
+        else lo = mid;
+    }
+    bucket[q] = lo;
+}
+
+struct SpeciesDev {
+    double mnubykT, relerr;
+    JPar J;
+};
+
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_dnu_qag(
+    HTab H, const double *__restrict__ fx, const double *__restrict__ fy, const double *__restrict__ fc, int nfs,
+    const double *__restrict__ sx, const double *__restrict__ Y, const double *__restrict__ C, const int *__restrict__ sb,
+    int ns, int nsb, double sinv_bw, const double *__restrict__ wavenum, int nk, const SpeciesDev *__restrict__ sp,
+    const int *__restrict__ item_species, int nitems, double x0, double xa, double *__restrict__ out,
+    int *__restrict__ nint_out, int *__restrict__ status)
+{
+    __shared__ QagWs ws[kWavesPerBlock];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int item = blockIdx.x * kWavesPerBlock + wave;
+    if (item >= nitems) return;
+    const int s = item_species[item / nk], k = item % nk;
+    DnuIntegrand f;
+    f.H = H;
+    f.fx = fx;
+    f.fy = fy;
+    f.fc = fc;
+    f.nfs = nfs;
+    f.fx0 = fx[0];
+    f.finv_h = (nfs - 1) / (fx[nfs - 1] - fx[0]);
+    f.sx = sx;
+    f.sy = Y + (size_t)k * ns;
+    f.sc = C + (size_t)k * ns;
+    f.sbucket = sb;
+    f.ns = ns;
+    f.nsb = nsb;
+    f.linear = ns <= 2;
+    f.sx0 = sx[0];
+    f.sinv_bw = sinv_bw;
+    f.k = wavenum[k];
+    f.mnubykT = sp[s].mnubykT;
+    f.J = sp[s].J;
+    f.edom = status + 1;
+    double r;
+    int ni;
+    const int rc = wave_qag61(f, x0, xa, sp[s].relerr, kLimit, ws[wave], &r, &ni);
+    if (lane == 0) {
+        out[item] = r;
+        nint_out[item] = ni;
+        if (rc) atomicOr(status, 2);
+    }
+}
+
+// ---------------------------------------------------------------- state
+
+struct DnuDev {
+    size_t cap_fs = 0, cap_y = 0, cap_x = 0, cap_k = 0, cap_items = 0, cap_b = 0;
+    double *fx = nullptr, *fy = nullptr, *fc = nullptr;
+    double *sx = nullptr, *Y = nullptr, *C = nullptr, *wavenum = nullptr;
+    int *bucket = nullptr;
+    double *out = nullptr;
+    int *nint = nullptr, *item_species = nullptr;
+    SpeciesDev *sp = nullptr;
+    int *status = nullptr;
+};
+std::unordered_map<const void *, DnuDev *> g_dev;
+int g_last_max_intervals = 0;
+
+template <typename T>
+void grow(T *&p, size_t &cap, size_t need)
+{
+    if (need <= cap) return;
+    if (p) KN_CHECK(hipFree(p));
+    KN_CHECK(hipMalloc(&p, sizeof(T) * need));
+    cap = need;
+}
+
+bool g_consts_ready = false;
+
+void upload_constants()
+{
+    if (g_consts_ready) return;
+    KN_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_xgk), kn_xgk61, sizeof(kn_xgk61)));
+    KN_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_wgk), kn_wgk61, sizeof(kn_wgk61)));
+    KN_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_wg), kn_wg30, sizeof(kn_wg30)));
+    g_consts_ready = true;
+}
+
+int blocks_for(long long waves) { return (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock); }
+
+}  // namespace
+
+// ------------------------------------------------------------ host API
+
+// Host part: build the panels covering [xlo, xhi] (no device work).  The
+// table is anchored so that a panel edge falls exactly on origin (the
+// integration's lower limit ln TimeTransfer): discontinuities of the host's
+// H below that limit never enter a panel that the integrals use.
+extern "C" void kn_htable_build_host(double origin, double xlo, double xhi)
+{
+    if (g_h.coef.empty() || fabs(origin - (g_h.x0 + 20.0)) > 0.0) h_reset(origin);
+    const int p0 = (int)floor((xlo - g_h.x0) / g_h.width), p1 = (int)floor((xhi - g_h.x0) / g_h.width);
+    if (p0 < 0 || p1 >= g_h.npanel)
+        kn_terminate(KN_ERR_RANGE, "scale factor range [%g, %g] outside the Hubble table\n", exp(xlo), exp(xhi));
+    for (int p = p0; p <= p1; p++)
+        if (!g_h.built[p]) {
+            h_build_panel(p);
+            g_h.dirty = true;
+        }
+}
+
+extern "C" double kn_htable_eval_host(double x);
+
+// Drop the table when the host's H no longer matches it (new cosmology).
+static void h_validate(double xlo, double xhi)
+{
+    const double xs[3] = {xlo, 0.5 * (xlo + xhi), xhi};
+    for (double x : xs) {
+        const int p = (int)floor((x - g_h.x0) / g_h.width);
+        if (p < 0 || p >= g_h.npanel || !g_h.built[p]) continue;
+        const double h = kn_hubble(exp(x));
+        if (fabs(kn_htable_eval_host(x) / h - 1) > 1e-11) {
+            h_reset(g_h.x0 + 20.0);
+            return;
+        }
+    }
+}
+
+extern "C" void kn_htable_ensure(double origin, double xlo, double xhi)
+{
+    if (!g_h.coef.empty() && fabs(origin - (g_h.x0 + 20.0)) == 0.0) h_validate(xlo, xhi);
+    kn_htable_build_host(origin, xlo, xhi);
+    if (g_h.d_cap < g_h.coef.size()) {
+        if (g_h.d_coef) KN_CHECK(hipFree(g_h.d_coef));
+        KN_CHECK(hipMalloc(&g_h.d_coef, sizeof(double) * g_h.coef.size()));
+        g_h.d_cap = g_h.coef.size();
+        g_h.dirty = true;
+    }
+    if (g_h.dirty) {
+        KN_CHECK(hipMemcpy(g_h.d_coef, g_h.coef.data(), sizeof(double) * g_h.coef.size(), hipMemcpyHostToDevice));
+        g_h.dirty = false;
+    }
+}
+
+extern "C" double kn_htable_eval_host(double x)
+{
+    const double u = (x - g_h.x0) / g_h.width;
+    int p = (int)floor(u);
+    p = p < 0 ? 0 : (p >= g_h.npanel ? g_h.npanel - 1 : p);
+    const double t = 2.0 * (u - p) - 1.0;
+    const double *c = g_h.coef.data() + (size_t)p * kHnc;
+    double b1 = 0.0, b2 = 0.0;
+    for (int k = kHnc - 1; k >= 1; k--) {
+        const double b0 = fma(2.0 * t, b1, c[k] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return exp(fma(t, b1, c[0] - b2));
+}
+
+extern "C" int kn_dnu_last_max_intervals(void) { return g_last_max_intervals; }
+
+extern "C" void kn_dnu_release(const _delta_tot_table *d)
+{
+    auto it = g_dev.find(d);
+    if (it == g_dev.end()) return;
+    DnuDev *v = it->second;
+    void *ptrs[] = {v->fx, v->fy, v->fc, v->sx, v->Y, v->C, v->wavenum, v->bucket, v->out, v->nint, v->item_species,
+                    v->sp, v->status};
+    for (void *p : ptrs)
+        if (p) KN_CHECK(hipFree(p));
+    delete v;
+    g_dev.erase(it);
+}
+
+extern "C" void kn_dnu_solve(const _delta_tot_table *d, double a, const double *wavenum, int nspecies,
+                             const kn_species *sp, double *out)
+{
+    const int Na = d->ia, nk = d->nk;
+    const double x0 = log(d->TimeTransfer), xa = log(a);
+    const double nfl0 = d->omnu->hybnu.nufrac_low[0];
+    // initial-condition piece, on the host exactly as delta_tot_table.c:522-535
+    const double fsl_A0a = fslength(x0, xa, d->light);
+    const double deriv_prefac = kn_deriv_prefac(d);
+    for (int s = 0; s < nspecies; s++)
+        for (int k = 0; k < nk; k++) {
+            const double m = sp[s].mnubykT > 0 ? sp[s].mnubykT : 1;
+            const double specJ = specialJ(wavenum[k] * fsl_A0a / m, 0, nfl0);
+            out[(size_t)s * nk + k] = specJ * d->delta_nu_init[k] * (1. + deriv_prefac * fsl_A0a);
+        }
+    std::vector<int> need;
+    for (int s = 0; s < nspecies; s++)
+        if (!sp[s].ic_only) need.push_back(s);
+    if (need.empty()) return;
+    if (Na < 2) return;
+
+    upload_constants();
+    kn_htable_ensure(x0, x0, xa);
+    DnuDev *&dev = g_dev[d];
+    if (!dev) dev = new DnuDev();
+    const int nfs = 16 * Na;
+    const int nsb = 4 * Na;
+    const int nitems = (int)need.size() * nk;
+    if (dev->cap_fs < (size_t)nfs) {  // fx, fy, fc
+        for (double **p : {&dev->fx, &dev->fy, &dev->fc}) {
+            if (*p) KN_CHECK(hipFree(*p));
+            KN_CHECK(hipMalloc(p, sizeof(double) * nfs));
+        }
+        dev->cap_fs = nfs;
+    }
+    grow(dev->sx, dev->cap_x, (size_t)Na);
+    if (dev->cap_y < (size_t)nk * Na) {  // Y, C
+        for (double **p : {&dev->Y, &dev->C}) {
+            if (*p) KN_CHECK(hipFree(*p));
+            KN_CHECK(hipMalloc(p, sizeof(double) * (size_t)nk * Na));
+        }
+        dev->cap_y = (size_t)nk * Na;
+    }
+    grow(dev->wavenum, dev->cap_k, (size_t)nk);
+    grow(dev->bucket, dev->cap_b, (size_t)nsb + 1);
+    if (dev->cap_items < (size_t)nitems) {
+        if (dev->out) KN_CHECK(hipFree(dev->out));
+        if (dev->nint) KN_CHECK(hipFree(dev->nint));
+        KN_CHECK(hipMalloc(&dev->out, sizeof(double) * nitems));
+        KN_CHECK(hipMalloc(&dev->nint, sizeof(int) * nitems));
+        dev->cap_items = nitems;
+    }
+    if (!dev->sp) {
+        KN_CHECK(hipMalloc(&dev->sp, sizeof(SpeciesDev) * NUSPECIES));
+        KN_CHECK(hipMalloc(&dev->item_species, sizeof(int) * NUSPECIES));
+        KN_CHECK(hipMalloc(&dev->status, sizeof(int) * 2));
+    }
+    // history rows packed with stride Na (host rows have stride namax)
+    std::vector<double> Yh((size_t)nk * Na);
+    for (int k = 0; k < nk; k++) memcpy(&Yh[(size_t)k * Na], d->delta_tot[k], sizeof(double) * Na);
+    KN_CHECK(hipMemcpy(dev->Y, Yh.data(), sizeof(double) * Yh.size(), hipMemcpyHostToDevice));
+    KN_CHECK(hipMemcpy(dev->sx, d->scalefact, sizeof(double) * Na, hipMemcpyHostToDevice));
+    KN_CHECK(hipMemcpy(dev->wavenum, wavenum, sizeof(double) * nk, hipMemcpyHostToDevice));
+    SpeciesDev sd[NUSPECIES];
+    int ispec[NUSPECIES];
+    for (size_t i = 0; i < need.size(); i++) {
+        const kn_species &q = sp[need[i]];
+        sd[i].mnubykT = q.mnubykT;
+        sd[i].relerr = q.relerr;
+        sd[i].J.qc = q.qc;
+        sd[i].J.nufrac_low = nfl0;
+        sd[i].J.en[0] = 1.0;
+        for (int n = 1; n < 20; n++) sd[i].J.en[n] = exp(-n * q.qc);
+        ispec[i] = (int)i;
+    }
+    KN_CHECK(hipMemcpy(dev->sp, sd, sizeof(SpeciesDev) * need.size(), hipMemcpyHostToDevice));
+    KN_CHECK(hipMemcpy(dev->item_species, ispec, sizeof(int) * need.size(), hipMemcpyHostToDevice));
+    KN_CHECK(hipMemset(dev->status, 0, sizeof(int) * 2));
+
+    const HTab H = device_htab();
+    hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, d->light,
+                       dev->fx, dev->fy, dev->status);
+    KN_CHECK(hipGetLastError());
+    {
+        const long long t = (long long)((nfs + kChunk - 1) / kChunk);
+        hipLaunchKernelGGL(k_spline, dim3((int)((t + 127) / 128)), dim3(128), 0, 0, dev->fx, dev->fy, dev->fc, nfs, 1,
+                           0LL, 0LL);
+        KN_CHECK(hipGetLastError());
+    }
+    if (Na > 2) {
+        const long long t = (long long)((Na + kChunk - 1) / kChunk) * nk;
+        hipLaunchKernelGGL(k_spline, dim3((int)((t + 127) / 128)), dim3(128), 0, 0, dev->sx, dev->Y, dev->C, Na, nk,
+                           (long long)Na, (long long)Na);
+        KN_CHECK(hipGetLastError());
+    }
+    const double sinv_bw = nsb / (d->scalefact[Na - 1] - d->scalefact[0]);
+    hipLaunchKernelGGL(k_bucket, dim3((nsb + 256) / 256), dim3(256), 0, 0, dev->sx, Na, nsb, sinv_bw, dev->bucket);
+    KN_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nitems)), dim3(64 * kWavesPerBlock), 0, 0, H, dev->fx, dev->fy,
+                       dev->fc, nfs, dev->sx, dev->Y, dev->C, dev->bucket, Na, nsb, sinv_bw, dev->wavenum, nk, dev->sp,
+                       dev->item_species, nitems, x0, xa, dev->out, dev->nint, dev->status);
+    KN_CHECK(hipGetLastError());
+    std::vector<double> res(nitems);
+    std::vector<int> nint(nitems);
+    int status[2];
+    KN_CHECK(hipMemcpy(res.data(), dev->out, sizeof(double) * nitems, hipMemcpyDeviceToHost));
+    KN_CHECK(hipMemcpy(nint.data(), dev->nint, sizeof(int) * nitems, hipMemcpyDeviceToHost));
+    KN_CHECK(hipMemcpy(status, dev->status, sizeof(int) * 2, hipMemcpyDeviceToHost));
+    if (status[1]) kn_terminate(2001, "GSL_ERROR-equivalent: interpolation outside the stored table (GSL_EDOM)\n");
+    if (status[0]) kn_terminate(2001, "GSL_ERROR-equivalent: QAG failed in get_delta_nu (status %d)\n", status[0]);
+    g_last_max_intervals = 0;
+    for (int v : nint) g_last_max_intervals = v > g_last_max_intervals ? v : g_last_max_intervals;
+    for (size_t i = 0; i < need.size(); i++)
+        for (int k = 0; k < nk; k++)
+            out[(size_t)need[i] * nk + k] += d->delta_nu_prefac * res[i * nk + k];
+}

@@ -1,0 +1,386 @@
+// dnu_device.h -- device building blocks of the free-streaming integral (b).
+//
+// Included by delta_nu.hip.  The integrand, interpolants and adaptive rule
+// follow delta_tot_table.c:375-611 of the reference:
+//   delta_nu(k) = IC(k) + delta_nu_prefac * int_{log T0}^{log a} I_k(x) dx,
+//   I_k(x) = fsl(x) / (e^x H(e^x)) * specialJ(k fsl(x) / (m/kT), qc, nufrac_low) * S_k(x),
+// with fsl a natural cubic spline through 16 Na fslength values on a uniform
+// log-a grid and S_k a natural cubic spline (linear for Na = 2) through the
+// stored delta_tot[k][.], integrated by QUADPACK QAG with the 61-point
+// Gauss-Kronrod rule (relative tolerance 1e-6, at most 200 subintervals).
+// One wavefront runs one QAG: its lanes evaluate the 61 nodes of both halves
+// of the bisected interval in one pass; the interval list lives in LDS.
+#ifndef KN_DNU_DEVICE_H
+#define KN_DNU_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <math.h>
+
+#include "kn_gk61.h"
+
+namespace kn_dnu {
+
+constexpr int kLimit = 200;   // QAG subinterval limit (GSL_VAL)
+constexpr int kHnc = 10;      // Chebyshev coefficients per ln H panel
+
+__constant__ double c_xgk[31];
+__constant__ double c_wgk[31];
+__constant__ double c_wg[15];
+
+// ---- ln H(x), x = ln a: Chebyshev panels of width 1/inv_w from origin x0 ----
+struct HTab {
+    const double *coef;  // npanel * kHnc
+    double x0, inv_w;
+    int npanel;
+};
+
+__device__ __forceinline__ double ln_hubble(const HTab &H, double x)
+{
+    double u = (x - H.x0) * H.inv_w;
+    int p = (int)floor(u);
+    p = p < 0 ? 0 : (p >= H.npanel ? H.npanel - 1 : p);
+    const double t = 2.0 * (u - p) - 1.0;
+    const double *c = H.coef + (size_t)p * kHnc;
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int k = kHnc - 1; k >= 1; k--) {
+        const double b0 = fma(2.0 * t, b1, c[k] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return fma(t, b1, c[0] - b2);
+}
+
+// ---- special functions (delta_tot_table.c:417-463) ----
+struct JPar {
+    double qc, nufrac_low;
+    double en[20];  // exp(-n qc) for n = 1..19, from the host
+};
+
+__device__ __forceinline__ double sph_j0(double x)
+{
+    if (fabs(x) < 0.5) {
+        const double y = x * x;
+        return 1.0 + y * (-1.0 / 6.0 + y * (1.0 / 120.0 + y * (-1.0 / 5040.0 + y * (1.0 / 362880.0 +
+                         y * (-1.0 / 39916800.0 + y * (1.0 / 6227020800.0))))));
+    }
+    return sin(x) / x;
+}
+
+__device__ __forceinline__ double specialJ(double x, const JPar &P)
+{
+    if (P.qc > 0) {
+        const double qc = P.qc;
+        double s, c;
+        sincos(qc * x, &s, &c);
+        const double j0 = fabs(qc * x) < 0.5 ? sph_j0(qc * x) : s / (qc * x);
+        double integ = 0.0;
+        for (int n = 1; n < 20; n++) {
+            const double nn = (double)(n * n);
+            const double ii = (n * n + n * n * n * qc + n * qc * x * x - x * x) * qc * j0 + (2 * n + n * n * qc + qc * x * x) * c;
+            const double sgn = (n & 1) ? 1.0 : -1.0;   // -1 * pow(-1, n)
+            integ += sgn * P.en[n] / (nn + x * x) / (nn + x * x) * ii;
+        }
+        return integ / (1.5 * 1.202056903159594 * (1 - P.nufrac_low));
+    }
+    if (x <= 0.) return 1.;
+    const double x2 = x * x, x4 = x2 * x2, x8 = x4 * x4;
+    return (1. + 0.0168 * x2 + 0.0407 * x4) / (1. + 2.1734 * x2 + 1.6787 * exp(4.1811 * log(x)) + 0.1467 * x8);
+}
+
+// ---- natural cubic spline evaluation, GSL cspline/linear semantics ----
+__device__ __forceinline__ double cspline_at(const double *xa, const double *ya, const double *c, int i, double x)
+{
+    const double xl = xa[i], h = xa[i + 1] - xl;
+    const double yl = ya[i], dy = ya[i + 1] - yl;
+    const double dx = x - xl;
+    const double ci = c[i], ci1 = c[i + 1];
+    const double b = (dy / h) - h * (ci1 + 2.0 * ci) / 3.0;
+    const double d = (ci1 - ci) / (3.0 * h);
+    return yl + dx * (b + dx * (ci + dx * d));
+}
+
+__device__ __forceinline__ double linear_at(const double *xa, const double *ya, int i, double x)
+{
+    const double xl = xa[i], h = xa[i + 1] - xl;
+    if (h > 0.0) return ya[i] + (x - xl) / h * (ya[i + 1] - ya[i]);
+    return ya[i];
+}
+
+// Largest i <= n-2 with xa[i] <= x, starting from a guess (GSL bsearch result).
+__device__ __forceinline__ int settle(const double *xa, int n, int i, double x)
+{
+    i = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
+    while (i > 0 && xa[i] > x) i--;
+    while (i < n - 2 && xa[i + 1] <= x) i++;
+    return i;
+}
+
+// ---- integrands ----
+struct FslIntegrand {  // delta_tot_table.c:378-383: 1 / a / (a H(a))
+    HTab H;
+    __device__ __forceinline__ double operator()(double x) const
+    {
+        const double a = exp(x);
+        return 1. / a / (a * exp(ln_hubble(H, x)));
+    }
+};
+
+struct DnuIntegrand {  // delta_tot_table.c:492-500
+    HTab H;
+    const double *fx, *fy, *fc;  // fslength spline, uniform nodes
+    int nfs;
+    double fx0, finv_h;
+    const double *sx, *sy, *sc;  // delta_tot spline of this k
+    const int *sbucket;
+    int ns, nsb, linear;
+    double sx0, sinv_bw;
+    double k, mnubykT;
+    JPar J;
+    int *edom;
+    __device__ __forceinline__ double operator()(double x) const
+    {
+        if (x < fx[0] || x > fx[nfs - 1] || x < sx[0] || x > sx[ns - 1]) {
+            atomicOr(edom, 1);  // GSL_EDOM in gsl_interp_eval -> terminate(2001)
+            return 0.0;
+        }
+        const int fi = settle(fx, nfs, (int)((x - fx0) * finv_h), x);
+        const double fsl = cspline_at(fx, fy, fc, fi, x);
+        int q = (int)((x - sx0) * sinv_bw);
+        q = q < 0 ? 0 : (q > nsb ? nsb : q);
+        const int si = settle(sx, ns, sbucket[q], x);
+        const double dtot = linear ? linear_at(sx, sy, si, x) : cspline_at(sx, sy, sc, si, x);
+        const double specJ = specialJ(k * fsl / mnubykT, J);
+        const double ai = exp(x);
+        return fsl / (ai * exp(ln_hubble(H, x))) * specJ * dtot;
+    }
+};
+
+// ---- wave reductions over 32-lane halves ----
+__device__ __forceinline__ double half_sum(double v)
+{
+#pragma unroll
+    for (int m = 16; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__device__ __forceinline__ double gk_rescale(double err, double resabs, double resasc)
+{
+    err = fabs(err);
+    if (resasc != 0 && err != 0) {
+        const double s = pow(200 * err / resasc, 1.5);
+        err = s < 1 ? resasc * s : resasc;
+    }
+    if (resabs > DBL_MIN / (50 * DBL_EPSILON)) {
+        const double floor_err = 50 * DBL_EPSILON * resabs;
+        if (floor_err > err) err = floor_err;
+    }
+    return err;
+}
+
+struct GK {
+    double res, err, resabs, resasc;
+};
+
+// 61-point Gauss-Kronrod on [a1,b1] (lanes 0-31) and [a2,b2] (lanes 32-63).
+template <class F>
+__device__ __forceinline__ void gk61_pair(const F &f, double a1, double b1, double a2, double b2, GK &g1, GK &g2)
+{
+    const int lane = threadIdx.x & 63, hf = lane >> 5, j = lane & 31;
+    const double a = hf ? a2 : a1, b = hf ? b2 : b1;
+    const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+    double f1 = 0.0, f2 = 0.0;
+    if (j < 30) {
+        const double dx = h * c_xgk[j];
+        f1 = f(c - dx);
+        f2 = f(c + dx);
+    } else if (j == 30) {
+        f1 = f(c);
+    }
+    const double wk = j <= 30 ? c_wgk[j] : 0.0;
+    const double wgv = (j < 30 && (j & 1)) ? c_wg[j >> 1] : 0.0;
+    const double sk = half_sum(j == 30 ? wk * f1 : wk * (f1 + f2));
+    const double sg = half_sum(wgv * (f1 + f2));
+    const double sa = half_sum(j == 30 ? wk * fabs(f1) : wk * (fabs(f1) + fabs(f2)));
+    const double mean = sk * 0.5;
+    const double sasc = half_sum(j == 30 ? wk * fabs(f1 - mean) : (j < 30 ? wk * (fabs(f1 - mean) + fabs(f2 - mean)) : 0.0));
+    GK r;
+    r.res = sk * h;
+    r.resabs = sa * fabs(h);
+    r.resasc = sasc * fabs(h);
+    r.err = gk_rescale((sk - sg) * h, r.resabs, r.resasc);
+    // broadcast each half's result to every lane
+    g1.res = __shfl(r.res, 0, 64);
+    g1.err = __shfl(r.err, 0, 64);
+    g1.resabs = __shfl(r.resabs, 0, 64);
+    g1.resasc = __shfl(r.resasc, 0, 64);
+    g2.res = __shfl(r.res, 32, 64);
+    g2.err = __shfl(r.err, 32, 64);
+    g2.resabs = __shfl(r.resabs, 32, 64);
+    g2.resasc = __shfl(r.resasc, 32, 64);
+}
+
+// Interval list of one wavefront (QUADPACK workspace), in LDS.
+struct QagWs {
+    double lo[kLimit], hi[kLimit], r[kLimit], e[kLimit];
+    int ord[kLimit], lev[kLimit];
+};
+
+// Keep ord[] partially sorted by error, as QUADPACK's dqpsrt.  All lanes run
+// it on identical values.
+__device__ __forceinline__ void qag_sort(QagWs &w, int n, int limit, int &nrmax, int &imax)
+{
+    const int last = n - 1;
+    int nr = nrmax, im = w.ord[nr], top, i, k;
+    if (last < 2) {
+        w.ord[0] = 0;
+        w.ord[1] = 1;
+        imax = im;
+        return;
+    }
+    const double emax = w.e[im];
+    while (nr > 0 && emax > w.e[w.ord[nr - 1]]) {
+        w.ord[nr] = w.ord[nr - 1];
+        nr--;
+    }
+    top = last < limit / 2 + 2 ? last : limit - last + 1;
+    for (i = nr + 1; i < top && emax < w.e[w.ord[i]]; i++) w.ord[i - 1] = w.ord[i];
+    w.ord[i - 1] = im;
+    const double emin = w.e[last];
+    for (k = top - 1; k > i - 2 && emin >= w.e[w.ord[k]]; k--) w.ord[k + 1] = w.ord[k];
+    w.ord[k + 1] = last;
+    imax = w.ord[nr];
+    nrmax = nr;
+}
+
+// QAG (QUADPACK dqage semantics, GSL's qag.c control flow) by one wavefront.
+// Returns 0 on success or the GSL errno (11 iteration limit, 18 roundoff,
+// 21 bad integrand); *result is the sum over the final subintervals.
+template <class F>
+__device__ int wave_qag61(const F &f, double a, double b, double epsrel, int limit, QagWs &w, double *result,
+                          int *nint)
+{
+    GK g0, gx;
+    gk61_pair(f, a, b, a, b, g0, gx);
+    *result = g0.res;
+    *nint = 1;
+    w.lo[0] = a;
+    w.hi[0] = b;
+    w.r[0] = g0.res;
+    w.e[0] = g0.err;
+    w.ord[0] = 0;
+    w.lev[0] = 0;
+    double tol = epsrel * fabs(g0.res);
+    if (g0.err <= 50 * DBL_EPSILON * g0.resabs && g0.err > tol) return 18;
+    if ((g0.err <= tol && g0.err != g0.resasc) || g0.err == 0.0) return 0;
+    if (limit == 1) return 11;
+    double area = g0.res, errsum = g0.err;
+    int n = 1, nrmax = 0, imax = 0, it = 1, rt1 = 0, rt2 = 0, etype = 0;
+    do {
+        const int im = imax;
+        const double ai = w.lo[im], bi = w.hi[im], ri = w.r[im], ei = w.e[im];
+        const double a1 = ai, b1 = 0.5 * (ai + bi), a2 = b1, b2 = bi;
+        GK g1, g2;
+        gk61_pair(f, a1, b1, a2, b2, g1, g2);
+        const double r12 = g1.res + g2.res, e12 = g1.err + g2.err;
+        errsum += e12 - ei;
+        area += r12 - ri;
+        if (g1.resasc != g1.err && g2.resasc != g2.err) {
+            if (fabs(ri - r12) <= 1.0e-5 * fabs(r12) && e12 >= 0.99 * ei) rt1++;
+            if (it >= 10 && e12 > ei) rt2++;
+        }
+        tol = epsrel * fabs(area);
+        if (errsum > tol) {
+            const double t = (1 + 100 * DBL_EPSILON) * (fabs(a2) + 1000 * DBL_MIN);
+            if (rt1 >= 6 || rt2 >= 20) etype = 2;
+            if (fabs(a1) <= t && fabs(b2) <= t) etype = 3;
+        }
+        const int lv = w.lev[im] + 1;
+        if (g2.err > g1.err) {
+            w.lo[im] = a2;
+            w.r[im] = g2.res;
+            w.e[im] = g2.err;
+            w.lo[n] = a1;
+            w.hi[n] = b1;
+            w.r[n] = g1.res;
+            w.e[n] = g1.err;
+        } else {
+            w.hi[im] = b1;
+            w.r[im] = g1.res;
+            w.e[im] = g1.err;
+            w.lo[n] = a2;
+            w.hi[n] = b2;
+            w.r[n] = g2.res;
+            w.e[n] = g2.err;
+        }
+        w.lev[im] = lv;
+        w.lev[n] = lv;
+        n++;
+        qag_sort(w, n, limit, nrmax, imax);
+        it++;
+    } while (it < limit && !etype && errsum > tol);
+    double s = 0.0;
+    for (int k = 0; k < n; k++) s += w.r[k];
+    *result = s;
+    *nint = n;
+    if (errsum <= tol) return 0;
+    if (etype == 2) return 18;
+    if (etype == 3) return 21;
+    if (it == limit) return 11;
+    return 5;
+}
+
+// ---- natural cubic spline coefficients, windowed parallel solve ----
+// The interior system h_{i-1} c_{i-1} + 2(h_{i-1}+h_i) c_i + h_i c_{i+1} = g_i is
+// 2:1 diagonally dominant, so the influence of a row decays at least as
+// 0.268^distance.  A thread solves rows [lo-halo, hi+halo) with zero
+// boundary values at the window edges and keeps rows [lo, hi): the edge error
+// reaching the kept rows is below 0.268^48 ~ 1e-27 relative.
+constexpr int kHalo = 48;
+
+__device__ inline void spline_window(const double *x, const double *y, int n, int lo, int hi, double *c)
+{
+    // unknowns c[1..n-2]; c[0] = c[n-1] = 0
+    if (n < 3) {
+        for (int i = lo; i < hi; i++) c[i] = 0.0;
+        return;
+    }
+    int a = lo - kHalo, b = hi + kHalo;  // window of interior unknowns [a, b)
+    if (a < 1) a = 1;
+    if (b > n - 1) b = n - 1;
+    const int m = b - a;
+    // forward elimination (L D L^T of the window), then back substitution into c
+    double gam_prev = 0.0, z_prev = 0.0, alp = 0.0;
+    // store gamma and z per row in c[] temporarily is not possible (c is shared); keep a small ring:
+    // the window is at most hi-lo+2*kHalo rows, solved twice: forward for z and gamma, backward for c.
+    double gam[64 + 2 * kHalo + 2], zz[64 + 2 * kHalo + 2];
+    for (int r = 0; r < m; r++) {
+        const int i = a + r;  // unknown c[i], row couples c[i-1], c[i], c[i+1]
+        const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
+        const double gl = hl != 0.0 ? 1.0 / hl : 0.0, gr = hr != 0.0 ? 1.0 / hr : 0.0;
+        const double diag = 2.0 * (hr + hl);
+        const double rhs = 3.0 * ((y[i + 1] - y[i]) * gr - (y[i] - y[i - 1]) * gl);
+        alp = r == 0 ? diag : diag - hl * gam_prev;
+        const double g = hr / alp;
+        const double z = r == 0 ? rhs : rhs - gam_prev * z_prev;
+        gam[r] = g;
+        zz[r] = z / alp;
+        gam_prev = g;
+        z_prev = z;
+    }
+    double cnext = 0.0;
+    for (int r = m - 1; r >= 0; r--) {
+        const double ci = zz[r] - (r == m - 1 ? 0.0 : gam[r] * cnext);
+        const int i = a + r;
+        if (i >= lo && i < hi) c[i] = ci;
+        cnext = ci;
+    }
+    if (lo == 0) c[0] = 0.0;
+    if (hi >= n) c[n - 1] = 0.0;
+    if (lo <= n - 1 && n - 1 < hi) c[n - 1] = 0.0;
+}
+
+}  // namespace kn_dnu
+
+#endif

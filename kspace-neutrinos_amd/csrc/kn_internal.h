@@ -51,8 +51,10 @@ double kn_linear_eval(const double *x, const double *y, int n, double xv, int *e
 size_t kn_bsearch(const double *xa, double x, size_t lo, size_t hi);
 
 /* ---- device Hubble table (delta_nu.hip) ---- */
-/* Make sure the device interpolant of ln H covers [xlo, xhi] (x = ln a). */
-void kn_htable_ensure(double xlo, double xhi);
+/* Make sure the device interpolant of ln H covers [xlo, xhi] (x = ln a), with a
+ * panel edge at origin (the integration's lower limit). */
+void kn_htable_ensure(double origin, double xlo, double xhi);
+void kn_htable_build_host(double origin, double xlo, double xhi);
 /* Host evaluation of the same interpolant (tests). */
 double kn_htable_eval_host(double x);
 

@@ -28,11 +28,19 @@
 
 #include "kn_hip.h"
 
+#include "bin_kernels.h"
+
 namespace {
 
-constexpr int kBlock = 256;
+using kn_bin::Pos;
+using kn_bin::advance;
+using kn_bin::find_bin;
+using kn_bin::kval;
+
+constexpr int kBlock = 256;  // geometry pass
 constexpr int kWaves = kBlock / 64;
-constexpr int kUnroll = 4;
+constexpr int kPowWaves = 8;  // power pass: 8 waves, 2 modes per lane per step, next step prefetched
+constexpr int kPowRun = 2;
 
 struct BinGeom {
     int dims, nrbins, startslab, nslab;
@@ -41,119 +49,13 @@ struct BinGeom {
     float half_bpu;   // 0.5 * binsperunit, for the fp32 estimate
     std::vector<long long> count;
     std::vector<double> keff_sum;
-    int grid;
-    double *d_partial;      // grid * nrbins
-    unsigned *d_cpartial;   // grid * nrbins (geometry pass)
+    int grid, ggrid;        // power-pass and geometry-pass grids
+    double *d_partial;      // max(grid, ggrid) * nrbins
+    unsigned *d_cpartial;   // ggrid * nrbins (geometry pass)
     double *d_sum;          // nrbins + 1 (last: |F0|^2)
 };
 
 std::vector<BinGeom *> g_geoms;
-
-// Per-bin smallest |k|^2 reaching the bin, from the reference expression.
-void build_thresholds(int dims, int nrbins, std::vector<int> &thr)
-{
-    const double binsperunit = (nrbins - 1) / log(sqrt(3) * dims / 2.0);
-    const long long kmax2 = 3LL * (dims / 2) * (dims / 2);
-    thr.assign(nrbins + 1, 0x7fffffff);
-    thr[0] = 0;
-    int prev = 0;
-    for (long long k2 = 1; k2 <= kmax2; k2++) {
-        const int b = (int)floor(binsperunit * log(sqrt((double)k2)));
-        if (b < prev) kn_terminate(KN_ERR_RANGE, "bin index not monotone in |k|^2 at %lld\n", k2);
-        if (b >= nrbins) kn_terminate(KN_ERR_RANGE, "bin index %d beyond %d bins\n", b, nrbins);
-        while (prev < b) thr[++prev] = (int)k2;
-    }
-}
-
-__device__ __forceinline__ int find_bin(int k2, float half_bpu, const int *__restrict__ thr, int nrbins)
-{
-    int b = (int)(__logf((float)k2) * half_bpu);
-    b = b < 0 ? 0 : (b > nrbins - 1 ? nrbins - 1 : b);
-    while (b > 0 && k2 < thr[b]) b--;
-    while (b < nrbins - 1 && k2 >= thr[b + 1]) b++;
-    return b;
-}
-
-// Position (slab i, row j, z) of flat mode offset within the slab range.
-struct Pos {
-    int i, j, z;
-};
-
-__device__ __forceinline__ void advance(Pos &p, int step, int rowlen, int dims)
-{
-    p.z += step;
-    while (p.z >= rowlen) {
-        p.z -= rowlen;
-        if (++p.j == dims) {
-            p.j = 0;
-            p.i++;
-        }
-    }
-}
-
-// Power pass: one HBM sweep, per-wave LDS histograms, per-block partials.
-__global__ __launch_bounds__(kBlock) void k_bin_power(const double2 *__restrict__ F, int dims, int startslab,
-                                                       long long rows, int nrbins, const int *__restrict__ thr_g,
-                                                       const double *__restrict__ iw_g, float half_bpu,
-                                                       double *__restrict__ partial)
-{
-    extern __shared__ double smem[];
-    double *hist = smem;                                   // kWaves * nrbins
-    double *iw = hist + (size_t)kWaves * nrbins;           // dims
-    int *thr = (int *)(iw + dims);                         // nrbins + 1
-    const int tid = threadIdx.x, wave = tid >> 6;
-    for (int t = tid; t < kWaves * nrbins; t += kBlock) hist[t] = 0.0;
-    for (int t = tid; t < dims; t += kBlock) iw[t] = iw_g[t];
-    for (int t = tid; t <= nrbins; t += kBlock) thr[t] = thr_g[t];
-    __syncthreads();
-
-    const int rowlen = dims / 2 + 1, half = dims / 2;
-    const long long r0 = rows * blockIdx.x / gridDim.x, r1 = rows * (blockIdx.x + 1) / gridDim.x;
-    const long long m1 = r1 * rowlen;
-    long long m = r0 * rowlen + tid;
-    Pos p;
-    {
-        const long long rl = r0 + tid / rowlen;
-        p.z = tid % rowlen;
-        p.j = (int)(rl % dims);
-        p.i = (int)(rl / dims);
-    }
-    double *my = hist + (size_t)wave * nrbins;
-    for (; m < m1; m += (long long)kBlock * kUnroll) {
-        double2 f[kUnroll];
-        Pos q[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            q[u] = p;
-            const long long mu = m + (long long)kBlock * u;
-            f[u] = mu < m1 ? F[mu] : make_double2(0.0, 0.0);
-            advance(p, kBlock, rowlen, dims);
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const long long mu = m + (long long)kBlock * u;
-            if (mu >= m1) break;
-            const int kx = kn_kval(startslab + q[u].i, dims), ky = kn_kval(q[u].j, dims), kz = q[u].z;
-            const int k2 = kx * kx + ky * ky + kz * kz;
-            if (k2 == 0) continue;
-            const int b = find_bin(k2, half_bpu, thr, nrbins);
-            // reference order: (|F|^2) * ((iwx iwy iwz)^2)^2, times 2 off the kz = 0, N/2 planes
-            const double w = __dmul_rn(__dmul_rn(iw[startslab + q[u].i], iw[q[u].j]), iw[kz]);
-            const double w2 = __dmul_rn(w, w);
-            const double w4 = __dmul_rn(w2, w2);
-            const double p2 = __dadd_rn(__dmul_rn(f[u].x, f[u].x), __dmul_rn(f[u].y, f[u].y));
-            double v = __dmul_rn(p2, w4);
-            if (kz != 0 && kz != half) v *= 2.0;
-            atomicAdd(&my[b], v);
-        }
-    }
-    __syncthreads();
-    for (int b = tid; b < nrbins; b += kBlock) {
-        double s = hist[b];
-        for (int w = 1; w < kWaves; w++) s += hist[(size_t)w * nrbins + b];
-        partial[(size_t)blockIdx.x * nrbins + b] = s;
-    }
-}
 
 // Geometry pass (no grid reads): multiplicity counts and sum of multiplicity*|k|.
 __global__ __launch_bounds__(kBlock) void k_bin_geometry(int dims, int startslab, long long rows, int nrbins,
@@ -184,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_geometry(int dims, int startslab
         p.i = (int)(rl / dims);
     }
     for (; m < m1; m += kBlock) {
-        const int kx = kn_kval(startslab + p.i, dims), ky = kn_kval(p.j, dims), kz = p.z;
+        const int kx = kval(startslab + p.i, dims), ky = kval(p.j, dims), kz = p.z;
         const int k2 = kx * kx + ky * ky + kz * kz;
         if (k2 != 0) {
             const int b = find_bin(k2, half_bpu, thr, nrbins);
@@ -268,9 +170,17 @@ int num_cus()
     return n;
 }
 
-size_t power_smem(int dims, int nrbins)
+size_t power_smem(int dims, int nrbins) { return kn_bin::smem_bytes<kPowWaves>(dims, nrbins); }
+
+// Resident blocks per CU of a kernel at a given dynamic LDS size.
+template <typename K>
+int resident_blocks(K kernel, int threads, size_t smem)
 {
-    return sizeof(double) * ((size_t)kWaves * nrbins + dims) + sizeof(int) * (nrbins + 1);
+    int n = 0;
+    if (smem > 64 * 1024)
+        KN_CHECK(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    KN_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, smem));
+    return n > 0 ? n : 1;
 }
 
 BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
@@ -280,7 +190,7 @@ BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
     if (dims < 2 || (dims & 1) || nrbins < 1 || nslab < 1 || startslab < 0 || startslab + nslab > dims)
         kn_terminate(KN_ERR_ARG, "total_powerspectrum: bad geometry dims=%d nrbins=%d slabs=[%d,%d)\n", dims,
                      nrbins, startslab, startslab + nslab);
-    if (power_smem(dims, nrbins) > 64 * 1024)
+    if (power_smem(dims, nrbins) > 160 * 1024)
         kn_terminate(KN_ERR_ARG, "total_powerspectrum: nrbins=%d dims=%d exceed the LDS budget\n", nrbins, dims);
     BinGeom *g = new BinGeom();
     g->dims = dims;
@@ -288,19 +198,22 @@ BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
     g->startslab = startslab;
     g->nslab = nslab;
     std::vector<int> thr;
-    build_thresholds(dims, nrbins, thr);
-    std::vector<double> iw(dims);
-    for (int n = 0; n < dims; n++) {
-        const int kx = kn_kval(n, dims);
-        iw[n] = kx ? M_PI * kx / (dims * sin(M_PI * kx / (double)dims)) : 1.0;  // powerspectrum.c:11
-    }
+    const long long bad = kn_bin::build_thresholds(dims, nrbins, thr);
+    if (bad) kn_terminate(KN_ERR_RANGE, "bin index of |k|^2 = %lld is not monotone or beyond %d bins\n", bad, nrbins);
+    std::vector<double> iw;
+    kn_bin::build_invwindow(dims, iw);
     g->half_bpu = (float)(0.5 * (nrbins - 1) / log(sqrt(3) * dims / 2.0));
     const long long rows = (long long)nslab * dims;
-    g->grid = (int)(rows < 4LL * num_cus() ? rows : 4LL * num_cus());
+    // one resident wave of row-range blocks: no tail of late blocks
+    const long long pgrid = (long long)resident_blocks(kn_bin::k_bin_power_runs_pf<kPowWaves, kPowRun>, 64 * kPowWaves,
+                                                       power_smem(dims, nrbins)) * num_cus();
+    g->grid = (int)(rows < pgrid ? rows : pgrid);
+    g->ggrid = (int)(rows < 4LL * num_cus() ? rows : 4LL * num_cus());
+    const int pmax = g->grid > g->ggrid ? g->grid : g->ggrid;
     KN_CHECK(hipMalloc(&g->d_thr, sizeof(int) * (nrbins + 1)));
     KN_CHECK(hipMalloc(&g->d_iw, sizeof(double) * dims));
-    KN_CHECK(hipMalloc(&g->d_partial, sizeof(double) * (size_t)g->grid * nrbins));
-    KN_CHECK(hipMalloc(&g->d_cpartial, sizeof(unsigned) * (size_t)g->grid * nrbins));
+    KN_CHECK(hipMalloc(&g->d_partial, sizeof(double) * (size_t)pmax * nrbins));
+    KN_CHECK(hipMalloc(&g->d_cpartial, sizeof(unsigned) * (size_t)g->ggrid * nrbins));
     KN_CHECK(hipMalloc(&g->d_sum, sizeof(double) * (nrbins + 1)));
     KN_CHECK(hipMemcpy(g->d_thr, thr.data(), sizeof(int) * (nrbins + 1), hipMemcpyHostToDevice));
     KN_CHECK(hipMemcpy(g->d_iw, iw.data(), sizeof(double) * dims, hipMemcpyHostToDevice));
@@ -310,11 +223,11 @@ BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
     KN_CHECK(hipMalloc(&d_count, sizeof(unsigned long long) * nrbins));
     KN_CHECK(hipMalloc(&d_ksum, sizeof(double) * nrbins));
     const size_t gsmem = (sizeof(double) + sizeof(unsigned)) * (size_t)kWaves * nrbins + sizeof(int) * (nrbins + 1);
-    hipLaunchKernelGGL(k_bin_geometry, dim3(g->grid), dim3(kBlock), gsmem, 0, dims, startslab, rows, nrbins,
+    hipLaunchKernelGGL(k_bin_geometry, dim3(g->ggrid), dim3(kBlock), gsmem, 0, dims, startslab, rows, nrbins,
                        g->d_thr, g->half_bpu, g->d_cpartial, g->d_partial);
     KN_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_reduce_counts, dim3((nrbins + 255) / 256), dim3(256), 0, 0, g->d_cpartial, g->d_partial,
-                       g->grid, nrbins, d_count, d_ksum);
+                       g->ggrid, nrbins, d_count, d_ksum);
     KN_CHECK(hipGetLastError());
     std::vector<unsigned long long> c(nrbins);
     g->count.resize(nrbins);
@@ -330,9 +243,18 @@ BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
 
 void launch_power(BinGeom *g, const double2 *F, hipStream_t st)
 {
-    const long long rows = (long long)g->nslab * g->dims;
-    hipLaunchKernelGGL(k_bin_power, dim3(g->grid), dim3(kBlock), power_smem(g->dims, g->nrbins), st, F, g->dims,
-                       g->startslab, rows, g->nrbins, g->d_thr, g->d_iw, g->half_bpu, g->d_partial);
+    kn_bin::BinArgs a;
+    a.F = F;
+    a.dims = g->dims;
+    a.startslab = g->startslab;
+    a.nrbins = g->nrbins;
+    a.rows = (long long)g->nslab * g->dims;
+    a.thr = g->d_thr;
+    a.iw = g->d_iw;
+    a.half_bpu = g->half_bpu;
+    a.partial = g->d_partial;
+    hipLaunchKernelGGL((kn_bin::k_bin_power_runs_pf<kPowWaves, kPowRun>), dim3(g->grid), dim3(64 * kPowWaves),
+                       power_smem(g->dims, g->nrbins), st, a);
     KN_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_reduce_partials, dim3((g->nrbins + 63) / 64), dim3(64 * kRedWaves), 0, st, g->d_partial,
                        g->grid, g->nrbins, g->d_sum);

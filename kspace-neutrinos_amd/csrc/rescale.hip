@@ -16,98 +16,41 @@
 
 #include "kn_hip.h"
 
+#include "rescale_kernels.h"
+
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kUnroll = 4;
-constexpr int kBucketsPerBin = 4;
+constexpr int kBlock = 256;   // synthetic-field generator
+constexpr int kResWaves = 8;  // rescale: 8 waves per block, 2 modes per lane per step, next step prefetched
+constexpr int kResUnroll = 2;
 
-__device__ int g_nan_flag;
+int *g_nan_flag = nullptr;    // device flag raised by the rescale on a NaN factor
 
-__global__ __launch_bounds__(kBlock) void k_rescale(double2 *__restrict__ F, int dims, int startslab,
-                                                     long long rows, double boxsize,
-                                                     const double *__restrict__ logkk_g,
-                                                     const double *__restrict__ ratio_g, int nbins, double norm,
-                                                     int nbuckets)
+int *nan_flag_ptr()
 {
-    extern __shared__ double sm[];
-    double *lk = sm;               // nbins
-    double *rt = lk + nbins;       // nbins
-    int *bucket = (int *)(rt + nbins);  // nbuckets + 1
-    const int tid = threadIdx.x;
-    for (int t = tid; t < nbins; t += kBlock) {
-        lk[t] = logkk_g[t];
-        rt[t] = ratio_g[t];
+    if (!g_nan_flag) {
+        KN_CHECK(hipMalloc(&g_nan_flag, sizeof(int)));
+        KN_CHECK(hipMemset(g_nan_flag, 0, sizeof(int)));
     }
-    __syncthreads();
-    const double x0 = lk[0], x1 = lk[nbins - 1];
-    const double inv_bw = nbins > 1 ? nbuckets / (x1 - x0) : 0.0;
-    // bucket[q] = last segment start index i <= nbins-2 with lk[i] <= left edge of bucket q
-    for (int q = tid; q <= nbuckets; q += kBlock) {
-        const double edge = x0 + q / inv_bw;
-        int lo = 0, hi = nbins - 1;
-        while (hi > lo + 1) {
-            const int mid = (lo + hi) >> 1;
-            if (lk[mid] > edge) hi = mid;
-            else lo = mid;
-        }
-        bucket[q] = lo;
+    return g_nan_flag;
+}
+
+int res_grid(long long rows, size_t smem)
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        KN_CHECK(hipGetDevice(&dev));
+        KN_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
-    __syncthreads();
-    const double lower_clamp = x0 - log(2.0);
-    const int rowlen = dims / 2 + 1;
-    const long long total = rows * rowlen;
-    const long long stride = (long long)gridDim.x * kBlock;
-    bool saw_nan = false;
-    for (long long m0 = (long long)blockIdx.x * kBlock + tid; m0 < total; m0 += stride * kUnroll) {
-        double2 f[kUnroll];
-        double smth[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const long long m = m0 + stride * u;
-            if (m < total) f[u] = F[m];
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const long long m = m0 + stride * u;
-            smth[u] = 1.0;
-            if (m >= total) continue;
-            const long long row = m / rowlen;
-            const int z = (int)(m - row * rowlen);
-            const int x = (int)(row % dims), y = startslab + (int)(row / dims);
-            const int kx = kn_kval(x, dims), ky = kn_kval(y, dims);
-            const double k2 = (double)(kx * kx + ky * ky + z * z);
-            if (k2 <= 0) continue;
-            double kk = log(sqrt(k2) * 2 * M_PI / boxsize);
-            // delta_pow.c:25-34 clamps
-            if (kk < x0 && kk > lower_clamp) kk = x0;
-            if (kk < lower_clamp) kk = x0;
-            if (kk > x1) kk = x1;
-            double interp;
-            if (nbins == 1) {
-                interp = rt[0];
-            } else {
-                int q = (int)((kk - x0) * inv_bw);
-                q = q < 0 ? 0 : (q > nbuckets ? nbuckets : q);
-                int i = bucket[q];
-                while (i < nbins - 2 && lk[i + 1] <= kk) i++;
-                const double xl = lk[i], h = lk[i + 1] - xl;
-                interp = h > 0.0 ? rt[i] + (kk - xl) / h * (rt[i + 1] - rt[i]) : rt[i];
-            }
-            smth[u] = 1 + norm * interp;
-            if (isnan(smth[u])) saw_nan = true;
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const long long m = m0 + stride * u;
-            if (m < total && smth[u] != 1.0) {
-                f[u].x *= smth[u];
-                f[u].y *= smth[u];
-                F[m] = f[u];
-            }
-        }
-    }
-    if (saw_nan) atomicOr(&g_nan_flag, 1);
+    int per_cu = 0;
+    if (smem > 64 * 1024)
+        KN_CHECK(hipFuncSetAttribute((const void *)kn_res::k_rescale_rows_pf<kResWaves, kResUnroll>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    KN_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kn_res::k_rescale_rows_pf<kResWaves, kResUnroll>,
+                                                          64 * kResWaves, smem));
+    const long long g = (long long)(per_cu > 0 ? per_cu : 1) * cus;
+    return (int)(rows < g ? rows : g);
 }
 
 // ---- synthetic Gaussian random field -----------------------------------
@@ -141,7 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_synth(double2 *__restrict__ F, int d
         const long long row = m / rowlen;
         const int z = (int)(m - row * rowlen);
         const int x = (int)(row % dims), y = startslab + (int)(row / dims);
-        const int kx = kn_kval(x, dims), ky = kn_kval(y, dims);
+        const int kx = kn_res::kval(x, dims), ky = kn_res::kval(y, dims);
         const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)z * z;
         const long long gidx = ((long long)y * dims + x) * rowlen + z;  // index in the full grid
         double2 out;
@@ -174,24 +117,34 @@ int launch_grid(long long work)
 extern "C" int kn_rescale_device(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
                                  const double *d_logkk, const double *d_ratio, int nbins, double norm, void *stream)
 {
-    if (nbins < 1 || dims < 2 || nslab < 1) return KN_ERR_ARG;
-    const int nbuckets = kBucketsPerBin * nbins;
-    const size_t smem = sizeof(double) * 2 * nbins + sizeof(int) * (nbuckets + 1);
-    if (smem > 64 * 1024) return KN_ERR_ARG;
-    const long long rows = (long long)nslab * dims;
-    const int grid = launch_grid(rows * (dims / 2 + 1) / kUnroll);
-    hipLaunchKernelGGL(k_rescale, dim3(grid), dim3(kBlock), smem, (hipStream_t)stream, (double2 *)d_field, dims,
-                       startslab, rows, boxsize, d_logkk, d_ratio, nbins, norm, nbuckets);
+    if (nbins < 1 || dims < 2 || nslab < 1 || (dims & 1)) return KN_ERR_ARG;
+    const int nbuckets = kn_res::kBucketsPerBin * nbins;
+    const size_t smem = kn_res::smem_bytes(nbins, nbuckets);
+    if (smem > 160 * 1024) return KN_ERR_ARG;
+    kn_res::ResArgs a;
+    a.F = (double2 *)d_field;
+    a.dims = dims;
+    a.startslab = startslab;
+    a.nbins = nbins;
+    a.nbuckets = nbuckets;
+    a.rows = (long long)nslab * dims;
+    a.boxsize = boxsize;
+    a.norm = norm;
+    a.logkk = d_logkk;
+    a.ratio = d_ratio;
+    a.nan_flag = nan_flag_ptr();
+    hipLaunchKernelGGL((kn_res::k_rescale_rows_pf<kResWaves, kResUnroll>), dim3(res_grid(a.rows, smem)),
+                       dim3(64 * kResWaves), smem, (hipStream_t)stream, a);
     KN_CHECK(hipGetLastError());
     return 0;
 }
 
-// Host-side check of the NaN flag raised by k_rescale (clears it).
+// Host-side check of the NaN flag raised by the rescale (clears it).
 extern "C" int kn_rescale_nan_flag(void)
 {
     int v = 0, z = 0;
-    KN_CHECK(hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_nan_flag), sizeof(int)));
-    if (v) KN_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_nan_flag), &z, sizeof(int)));
+    KN_CHECK(hipMemcpy(&v, nan_flag_ptr(), sizeof(int), hipMemcpyDeviceToHost));
+    if (v) KN_CHECK(hipMemcpy(nan_flag_ptr(), &z, sizeof(int), hipMemcpyHostToDevice));
     return v;
 }
 

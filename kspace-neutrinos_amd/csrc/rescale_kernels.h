@@ -1,0 +1,253 @@
+// rescale_kernels.h -- device kernels of the per-mode rescale sweep (c).
+//
+// Included by rescale.hip.  Arithmetic follows interface_gadget.c:163-188 and
+// delta_pow.c:19-37 (see rescale.hip); the templates only vary the launch
+// shape and the lane-to-mode mapping.
+#ifndef KN_RESCALE_KERNELS_H
+#define KN_RESCALE_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace kn_res {
+
+__host__ __device__ inline int kval(int n, int dims) { return n <= dims / 2 ? n : n - dims; }
+
+struct ResArgs {
+    double2 *F;
+    int dims, startslab, nbins, nbuckets;
+    long long rows;
+    double boxsize, norm;
+    const double *logkk, *ratio;
+    int *nan_flag;
+};
+
+constexpr int kBucketsPerBin = 4;
+
+__host__ __device__ inline size_t smem_bytes(int nbins, int nbuckets)
+{
+    return sizeof(double) * 2 * (size_t)nbins + sizeof(int) * (size_t)(nbuckets + 1);
+}
+
+struct Tab {
+    const double *lk, *rt;
+    const int *bucket;
+    int nbins, nbuckets;
+    double x0, x1, inv_bw, lower_clamp, norm, log_kunit;
+};
+
+// Ratio table and bucket index into LDS (all threads of the block).
+__device__ __forceinline__ Tab load_tab(const ResArgs &a, int nthreads)
+{
+    extern __shared__ double sm[];
+    double *lk = sm, *rt = sm + a.nbins;
+    int *bucket = (int *)(rt + a.nbins);
+    for (int t = threadIdx.x; t < a.nbins; t += nthreads) {
+        lk[t] = a.logkk[t];
+        rt[t] = a.ratio[t];
+    }
+    __syncthreads();
+    Tab T;
+    T.lk = lk;
+    T.rt = rt;
+    T.bucket = bucket;
+    T.nbins = a.nbins;
+    T.nbuckets = a.nbuckets;
+    T.x0 = lk[0];
+    T.x1 = lk[a.nbins - 1];
+    T.inv_bw = a.nbins > 1 ? a.nbuckets / (T.x1 - T.x0) : 0.0;
+    T.lower_clamp = T.x0 - log(2.0);
+    T.norm = a.norm;
+    T.log_kunit = log(2 * M_PI / a.boxsize);
+    // bucket[q]: the GSL segment index (largest i <= n-2 with lk[i] <= x) at the bucket's left edge
+    for (int q = threadIdx.x; q <= a.nbuckets; q += nthreads) {
+        const double edge = T.x0 + q / T.inv_bw;
+        int lo = 0, hi = a.nbins - 1;
+        while (hi > lo + 1) {
+            const int mid = (lo + hi) >> 1;
+            if (lk[mid] > edge) hi = mid;
+            else lo = mid;
+        }
+        bucket[q] = lo;
+    }
+    __syncthreads();
+    return T;
+}
+
+// 1 + get_dnudcdm_powerspec(log(|k|)) for k2 > 0, with |k| = sqrt(k2) 2 pi / BoxSize.
+// log|k| is evaluated as 0.5 log(k2) + log(2 pi / BoxSize): the same value as
+// the reference's log(sqrt(k2)*2*M_PI/BoxSize) to about one ulp, with one
+// transcendental per mode instead of sqrt, two divides and a log.
+__device__ __forceinline__ double smth_of(int k2i, double boxsize, const Tab &T)
+{
+    (void)boxsize;
+    double kk = 0.5 * log((double)k2i) + T.log_kunit;
+    if (kk < T.x0 && kk > T.lower_clamp) kk = T.x0;
+    if (kk < T.lower_clamp) kk = T.x0;
+    if (kk > T.x1) kk = T.x1;
+    double interp;
+    if (T.nbins == 1) {
+        interp = T.rt[0];
+    } else {
+        int q = (int)((kk - T.x0) * T.inv_bw);
+        q = q < 0 ? 0 : (q > T.nbuckets ? T.nbuckets : q);
+        int i = T.bucket[q];
+        while (i < T.nbins - 2 && T.lk[i + 1] <= kk) i++;
+        const double xl = T.lk[i], h = T.lk[i + 1] - xl;
+        interp = h > 0.0 ? T.rt[i] + (kk - xl) / h * (T.rt[i + 1] - T.rt[i]) : T.rt[i];
+    }
+    return 1 + T.norm * interp;
+}
+
+struct Pos {
+    int i, j, z;
+};
+
+__device__ __forceinline__ void advance(Pos &p, int step, int rowlen, int dims)
+{
+    p.z += step;
+    while (p.z >= rowlen) {
+        p.z -= rowlen;
+        if (++p.j == dims) {
+            p.j = 0;
+            p.i++;
+        }
+    }
+}
+
+__device__ __forceinline__ Pos pos_of(long long off, long long row0, int rowlen, int dims)
+{
+    Pos p;
+    const long long rl = row0 + off / rowlen;
+    p.z = (int)(off % rowlen);
+    p.j = (int)(rl % dims);
+    p.i = (int)(rl / dims);
+    return p;
+}
+
+// Coalesced sweep over a contiguous row range per block, UNROLL loads in flight.
+template <int WAVES, int UNROLL>
+__global__ __launch_bounds__(64 * WAVES) void k_rescale_rows(ResArgs a)
+{
+    constexpr int B = 64 * WAVES;
+    const Tab T = load_tab(a, B);
+    const int rowlen = a.dims / 2 + 1;
+    const long long r0 = a.rows * blockIdx.x / gridDim.x, r1 = a.rows * (blockIdx.x + 1) / gridDim.x;
+    const long long m1 = r1 * rowlen;
+    Pos p = pos_of(threadIdx.x, r0, rowlen, a.dims);
+    bool saw_nan = false;
+    for (long long m = r0 * rowlen + threadIdx.x; m < m1; m += (long long)B * UNROLL) {
+        double2 f[UNROLL];
+        Pos q[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            q[u] = p;
+            const long long mu = m + (long long)B * u;
+            if (mu < m1) f[u] = a.F[mu];
+            advance(p, B, rowlen, a.dims);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const long long mu = m + (long long)B * u;
+            if (mu >= m1) break;
+            const int kx = kval(a.startslab + q[u].i, a.dims), ky = kval(q[u].j, a.dims);
+            const int k2 = kx * kx + ky * ky + q[u].z * q[u].z;
+            if (k2 <= 0) continue;
+            const double s = smth_of(k2, a.boxsize, T);
+            saw_nan |= isnan(s);
+            f[u].x *= s;
+            f[u].y *= s;
+            a.F[mu] = f[u];
+        }
+    }
+    if (saw_nan) atomicOr(a.nan_flag, 1);
+}
+
+// Each lane rescales RUN consecutive modes (one row mostly); the wave's RUN
+// accesses cover one contiguous span.
+template <int WAVES, int RUN>
+__global__ __launch_bounds__(64 * WAVES) void k_rescale_runs(ResArgs a)
+{
+    constexpr int B = 64 * WAVES;
+    const Tab T = load_tab(a, B);
+    const int rowlen = a.dims / 2 + 1;
+    const long long r0 = a.rows * blockIdx.x / gridDim.x, r1 = a.rows * (blockIdx.x + 1) / gridDim.x;
+    const long long m1 = r1 * rowlen;
+    Pos p = pos_of((long long)threadIdx.x * RUN, r0, rowlen, a.dims);
+    bool saw_nan = false;
+    for (long long m = r0 * rowlen + (long long)threadIdx.x * RUN; m < m1; m += (long long)B * RUN) {
+        double2 f[RUN];
+#pragma unroll
+        for (int r = 0; r < RUN; r++)
+            if (m + r < m1) f[r] = a.F[m + r];
+        Pos q = p;
+#pragma unroll
+        for (int r = 0; r < RUN; r++) {
+            if (m + r >= m1) break;
+            if (r) advance(q, 1, rowlen, a.dims);
+            const int kx = kval(a.startslab + q.i, a.dims), ky = kval(q.j, a.dims);
+            const int k2 = kx * kx + ky * ky + q.z * q.z;
+            if (k2 <= 0) continue;
+            const double s = smth_of(k2, a.boxsize, T);
+            saw_nan |= isnan(s);
+            f[r].x *= s;
+            f[r].y *= s;
+            a.F[m + r] = f[r];
+        }
+        advance(p, B * RUN, rowlen, a.dims);
+    }
+    if (saw_nan) atomicOr(a.nan_flag, 1);
+}
+
+// Software-pipelined coalesced sweep: the next UNROLL modes are loaded
+// before the current ones are rescaled and stored.
+template <int WAVES, int UNROLL>
+__global__ __launch_bounds__(64 * WAVES) void k_rescale_rows_pf(ResArgs a)
+{
+    constexpr int B = 64 * WAVES;
+    const Tab T = load_tab(a, B);
+    const int rowlen = a.dims / 2 + 1;
+    const long long r0 = a.rows * blockIdx.x / gridDim.x, r1 = a.rows * (blockIdx.x + 1) / gridDim.x;
+    const long long m1 = r1 * rowlen;
+    const long long S = (long long)B * UNROLL;
+    Pos p = pos_of(threadIdx.x, r0, rowlen, a.dims);
+    bool saw_nan = false;
+    long long m = r0 * rowlen + threadIdx.x;
+    double2 cur[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) {
+        const long long mu = m + (long long)B * u;
+        cur[u] = mu < m1 ? a.F[mu] : make_double2(0.0, 0.0);
+    }
+    for (; m < m1; m += S) {
+        double2 nxt[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const long long mu = m + S + (long long)B * u;
+            nxt[u] = mu < m1 ? a.F[mu] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const long long mu = m + (long long)B * u;
+            const Pos q = p;
+            advance(p, B, rowlen, a.dims);
+            if (mu >= m1) continue;
+            const int kx = kval(a.startslab + q.i, a.dims), ky = kval(q.j, a.dims);
+            const int k2 = kx * kx + ky * ky + q.z * q.z;
+            if (k2 <= 0) continue;
+            const double s = smth_of(k2, a.boxsize, T);
+            saw_nan |= isnan(s);
+            double2 f = cur[u];
+            f.x *= s;
+            f.y *= s;
+            a.F[mu] = f;
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) cur[u] = nxt[u];
+    }
+    if (saw_nan) atomicOr(a.nan_flag, 1);
+}
+
+}  // namespace kn_res
+
+#endif

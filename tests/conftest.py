@@ -25,7 +25,7 @@ def pytest_configure(config):
 _host = None
 
 
-def testhost():
+def load_testhost():
     """The N-body host services library, loaded RTLD_GLOBAL before the product."""
     global _host
     if _host is None:
@@ -42,7 +42,7 @@ def testhost():
 
 @pytest.fixture(scope="session")
 def host():
-    return testhost()
+    return load_testhost()
 
 
 @pytest.fixture(scope="session")

@@ -123,7 +123,7 @@ def _run_child(code):
 
 def test_errors_reach_host_terminate_with_reference_codes():
     prelude = ("import sys, ctypes as C; sys.path[:0]=['kspace-neutrinos_amd','tests'];"
-               "import conftest; h=conftest.testhost(); import knhip; L=knhip.lib();")
+               "import conftest; h=conftest.load_testhost(); import knhip; L=knhip.lib();")
     # transfer file missing -> terminate(2019) (transfer_init.c:219-221)
     r = _run_child(prelude + "t=knhip.TransferInitTable(); L.allocate_transfer_init_table(C.byref(t),1.,1.,1.,b'/nonexistent')")
     assert "terminate(2019)" in r.stderr and r.returncode == 2019 % 256, r.stderr
