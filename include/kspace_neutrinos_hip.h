@@ -252,6 +252,79 @@ int kn_rescale_device(fftw_complex *d_field, int dims, int startslab, int nslab,
 /* Last GPU error text (empty if none). */
 const char *kn_last_error(void);
 
+/* ---- (b) batched: many get_delta_nu integrals in one launch set ----
+ * One history is what one get_delta_nu call integrates
+ * (delta_tot_table.c:507-611): its stored log a values and delta_tot rows,
+ * its own H(a), mass and tolerance.  All per-history state lives in device
+ * memory; na grows on the device by kn_batch_append, so a captured graph
+ * (append + solve [+ adjoint]) can be replayed step after step. */
+typedef struct kn_batch_history {
+    int nk;                 /* k bins */
+    int na;                 /* stored steps now (>= 1) */
+    int na_max;             /* capacity */
+    double a;               /* current scale factor: upper limit log a */
+    double a_max;           /* largest a the history will reach (H table range) */
+    double time_transfer;   /* TimeTransfer: lower limit log T0 */
+    double mnu, kBtnu;      /* neutrino mass (eV) and k_B T_nu (eV) */
+    double light;           /* speed of light in internal units */
+    double delta_nu_prefac; /* 1.5 Omega0 H0^2 / c */
+    double qc, relerr;      /* hybrid critical momentum (0: off) and QAG tolerance */
+    double nufrac_low;      /* hybnu.nufrac_low[0] */
+    double vcrit;           /* hybnu.vcrit (d qc / d (m/kT)) */
+    int ic_only;            /* 1: only the initial-condition piece */
+    double (*hubble)(double a, void *ctx);
+    void *hubble_ctx;
+} kn_batch_history;
+
+typedef struct kn_batch kn_batch;
+
+/* Returns 3005 (switched off) in this build: the batched kernels have not
+ * completed a device run on gfx950 and stay disabled until they do; they are
+ * exercised in a CPU SIMT emulation build only. */
+int kn_batch_create(kn_batch **out, int nhist, const kn_batch_history *hist);
+void kn_batch_destroy(kn_batch *b);
+/* Device arrays of the batch (any pointer may be NULL): scalefact (log a) per
+ * history [na_max], delta_tot per history [nk][na_max], wavenum/
+ * delta_nu_init/delta_nu/gbar per item, dtot_bar like delta_tot, init_bar per
+ * item, mnu_bar per history. */
+int kn_batch_views(kn_batch *b, double **scalefact, double **delta_tot, double **wavenum, double **delta_nu_init,
+                   double **delta_nu, double **gbar, double **dtot_bar, double **init_bar, double **mnu_bar);
+/* Offsets of each history (nhist+1 entries): per-item arrays, delta_tot block, scalefact. */
+int kn_batch_layout(const kn_batch *b, long long *off_k, long long *off_y, long long *off_s);
+/* delta_nu of every history (synchronous; returns 0 or an error code). */
+int kn_batch_solve(kn_batch *b, void *stream);
+/* Adjoint of the last solve for the gbar seeds (synchronous). */
+int kn_batch_adjoint(kn_batch *b, void *stream);
+/* Append one step per history: d_loga[nhist], d_rows[items] (device). */
+int kn_batch_append(kn_batch *b, const double *d_loga, const double *d_rows, void *stream);
+/* Capture [append +] solve [+ adjoint] into one graph; launch replays it (async). */
+int kn_batch_graph_capture(kn_batch *b, int with_adjoint, const double *d_loga, const double *d_rows, void *stream);
+int kn_batch_graph_launch(kn_batch *b, void *stream);
+/* Error state of the last launches (0, 2001 QAG/interpolation failure, 3004 capacity). */
+int kn_batch_status(kn_batch *b);
+/* New neutrino mass (eV) of history h; qc follows it for a hybrid history. */
+int kn_batch_set_mass(kn_batch *b, int h, double mnu);
+/* The history get_delta_nu(d, a, ..., mnu) integrates (delta_tot_table.c:507-555:
+ * tolerance, hybrid qc, IC-only cases), with na = na_max = d->ia. */
+int kn_batch_history_from_table(const _delta_tot_table *d, double a, double mnu, kn_batch_history *out);
+/* The process's hubble_function (or the one registered with
+ * kn_set_hubble_function) in the kn_batch_history callback form. */
+double kn_hubble_ctx(double a, void *ctx);
+/* Pick persistent-grid sizes by timing the forward. */
+int kn_batch_autotune(kn_batch *b, void *stream);
+
+/* ---- adjoint of get_delta_nu (delta_tot_table.c:507-611) ----
+ * gbar[nk] = dL/d delta_nu.  Outputs (host arrays, any may be NULL): the
+ * forward delta_nu[nk], dL/d delta_tot[k][0..ia) (row stride ia), dL/d
+ * delta_nu_init[nk] and dL/d mnu (the function's mnu argument; the hybrid
+ * state of the table held fixed).  The QAG subdivision is that of the
+ * forward evaluation, so the result is the exact derivative of the computed
+ * quadrature.  Built on the batch kernels: returns 3005 while those are
+ * switched off. */
+int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, const double *wavenum, double mnu,
+                            const double *gbar, double *delta_nu, double *dtot_bar, double *init_bar,
+                            double *mnu_bar);
+
 #ifdef __cplusplus
 }
 #endif

@@ -256,27 +256,34 @@ __device__ __forceinline__ void qag_sort(QagWs &w, int n, int limit, int &nrmax,
 
 // QAG (QUADPACK dqage semantics, GSL's qag.c control flow) by one wavefront.
 // Returns 0 on success or the GSL errno (11 iteration limit, 18 roundoff,
-// 21 bad integrand); *result is the sum over the final subintervals.
+// 21 bad integrand); *result is the sum over the final subintervals.  Every
+// lane keeps the same scalar state; lane 0 alone writes the interval list and
+// keeps it ordered, and the shuffles that broadcast its choice (imax, nrmax)
+// order those writes before the next reads by the other lanes.
 template <class F>
 __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int limit, QagWs &w, double *result,
                           int *nint)
 {
+    const bool lead = (threadIdx.x & 63) == 0;
     GK g0, gx;
     gk61_pair(f, a, b, a, b, g0, gx);
     *result = g0.res;
     *nint = 1;
-    w.lo[0] = a;
-    w.hi[0] = b;
-    w.r[0] = g0.res;
-    w.e[0] = g0.err;
-    w.ord[0] = 0;
-    w.lev[0] = 0;
+    if (lead) {
+        w.lo[0] = a;
+        w.hi[0] = b;
+        w.r[0] = g0.res;
+        w.e[0] = g0.err;
+        w.ord[0] = 0;
+        w.lev[0] = 0;
+    }
+    int imax = __shfl(0, 0, 64), nrmax = 0;
     double tol = epsrel * fabs(g0.res);
     if (g0.err <= 50 * DBL_EPSILON * g0.resabs && g0.err > tol) return 18;
     if ((g0.err <= tol && g0.err != g0.resasc) || g0.err == 0.0) return 0;
     if (limit == 1) return 11;
     double area = g0.res, errsum = g0.err;
-    int n = 1, nrmax = 0, imax = 0, it = 1, rt1 = 0, rt2 = 0, etype = 0;
+    int n = 1, it = 1, rt1 = 0, rt2 = 0, etype = 0;
     do {
         const int im = imax;
         const double ai = w.lo[im], bi = w.hi[im], ri = w.r[im], ei = w.e[im];
@@ -296,28 +303,32 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
             if (rt1 >= 6 || rt2 >= 20) etype = 2;
             if (fabs(a1) <= t && fabs(b2) <= t) etype = 3;
         }
-        const int lv = w.lev[im] + 1;
-        if (g2.err > g1.err) {
-            w.lo[im] = a2;
-            w.r[im] = g2.res;
-            w.e[im] = g2.err;
-            w.lo[n] = a1;
-            w.hi[n] = b1;
-            w.r[n] = g1.res;
-            w.e[n] = g1.err;
-        } else {
-            w.hi[im] = b1;
-            w.r[im] = g1.res;
-            w.e[im] = g1.err;
-            w.lo[n] = a2;
-            w.hi[n] = b2;
-            w.r[n] = g2.res;
-            w.e[n] = g2.err;
+        if (lead) {
+            const int lv = w.lev[im] + 1;
+            if (g2.err > g1.err) {
+                w.lo[im] = a2;
+                w.r[im] = g2.res;
+                w.e[im] = g2.err;
+                w.lo[n] = a1;
+                w.hi[n] = b1;
+                w.r[n] = g1.res;
+                w.e[n] = g1.err;
+            } else {
+                w.hi[im] = b1;
+                w.r[im] = g1.res;
+                w.e[im] = g1.err;
+                w.lo[n] = a2;
+                w.hi[n] = b2;
+                w.r[n] = g2.res;
+                w.e[n] = g2.err;
+            }
+            w.lev[im] = lv;
+            w.lev[n] = lv;
+            qag_sort(w, n + 1, limit, nrmax, imax);
         }
-        w.lev[im] = lv;
-        w.lev[n] = lv;
         n++;
-        qag_sort(w, n, limit, nrmax, imax);
+        imax = __shfl(imax, 0, 64);
+        nrmax = __shfl(nrmax, 0, 64);
         it++;
     } while (it < limit && !etype && errsum > tol);
     double s = 0.0;
@@ -379,6 +390,116 @@ __device__ inline void spline_window(const double *x, const double *y, int n, in
     if (lo == 0) c[0] = 0.0;
     if (hi >= n) c[n - 1] = 0.0;
     if (lo <= n - 1 && n - 1 < hi) c[n - 1] = 0.0;
+}
+
+// ---- derivatives for the adjoint ----------------------------------------
+
+// j0'(z) = (z cos z - sin z) / z^2, Taylor series below |z| = 0.5
+__device__ __forceinline__ double sph_j0_prime(double z)
+{
+    if (fabs(z) < 0.5) {
+        const double y = z * z;
+        // d/dz of 1 - z^2/6 + z^4/120 - z^6/5040 + z^8/362880 - z^10/39916800 + z^12/6227020800
+        return z * (-2.0 / 6.0 + y * (4.0 / 120.0 + y * (-6.0 / 5040.0 + y * (8.0 / 362880.0 +
+                    y * (-10.0 / 39916800.0 + y * (12.0 / 6227020800.0))))));
+    }
+    double s, c;
+    sincos(z, &s, &c);
+    return (z * c - s) / (z * z);
+}
+
+// specialJ(x, qc) with its partial derivatives d/dx and d/dqc.
+__device__ inline double specialJ_d(double x, const JPar &P, double &dJdx, double &dJdqc)
+{
+    if (P.qc > 0) {
+        const double qc = P.qc;
+        const double z = qc * x;
+        double s, c;
+        sincos(z, &s, &c);
+        const double j0 = fabs(z) < 0.5 ? sph_j0(z) : s / z;
+        const double j0p = sph_j0_prime(z);
+        double J = 0.0, Jx = 0.0, Jq = 0.0;
+        for (int n = 1; n < 20; n++) {
+            const double nn = (double)(n * n), d = nn + x * x;
+            const double A = (n * n + n * n * n * qc + n * qc * x * x - x * x);
+            const double B = (2 * n + n * n * qc + qc * x * x);
+            const double ii = A * qc * j0 + B * c;
+            const double Ax = 2.0 * n * qc * x - 2.0 * x, Bx = 2.0 * qc * x;
+            const double Aq = (double)(n * n * n) + n * x * x, Bq = nn + x * x;
+            const double iix = Ax * qc * j0 + A * qc * qc * j0p + Bx * c - B * qc * s;
+            const double iiq = Aq * qc * j0 + A * (j0 + z * j0p) + Bq * c - B * x * s;
+            const double sgn = (n & 1) ? 1.0 : -1.0;
+            const double w = sgn * P.en[n] / d / d;
+            J += w * ii;
+            Jx += w * (iix - 4.0 * x / d * ii);
+            Jq += w * (iiq - n * ii);
+        }
+        const double Z = 1.5 * 1.202056903159594 * (1 - P.nufrac_low);
+        dJdx = Jx / Z;
+        dJdqc = Jq / Z;
+        return J / Z;
+    }
+    dJdqc = 0.0;
+    if (x <= 0.) {
+        dJdx = 0.0;
+        return 1.;
+    }
+    const double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x7 = x4 * x3, x8 = x4 * x4;
+    const double p = exp(4.1811 * log(x));
+    const double N = 1. + 0.0168 * x2 + 0.0407 * x4;
+    const double D = 1. + 2.1734 * x2 + 1.6787 * p + 0.1467 * x8;
+    const double Nx = 0.0336 * x + 0.1628 * x3;
+    const double Dx = 4.3468 * x + 1.6787 * 4.1811 * p / x + 1.1736 * x7;
+    dJdx = (Nx * D - N * Dx) / (D * D);
+    return N / D;
+}
+
+// Weights of S(x) = y_i + dx (b + dx (c_i + dx d)) on y_i, y_{i+1}, c_i, c_{i+1}.
+__device__ __forceinline__ void cspline_grad(const double *xa, int i, double x, double &wy0, double &wy1, double &wc0,
+                                             double &wc1)
+{
+    const double h = xa[i + 1] - xa[i], dx = x - xa[i];
+    const double t = dx / h;
+    wy0 = 1.0 - t;
+    wy1 = t;
+    wc0 = -dx * h * (2.0 / 3.0) + dx * dx - dx * dx * dx / (3.0 * h);
+    wc1 = -dx * h / 3.0 + dx * dx * dx / (3.0 * h);
+}
+
+// Windowed solve of the natural-spline system M z = rhs (interior rows
+// 1..n-2, z[0] = z[n-1] = 0); same window scheme as spline_window.
+__device__ inline void tridiag_window(const double *x, const double *rhs, int n, int lo, int hi, double *z_out)
+{
+    if (n < 3) {
+        for (int i = lo; i < hi; i++) z_out[i] = 0.0;
+        return;
+    }
+    int a = lo - kHalo, b = hi + kHalo;
+    if (a < 1) a = 1;
+    if (b > n - 1) b = n - 1;
+    const int m = b - a;
+    double gam[64 + 2 * kHalo + 2], zz[64 + 2 * kHalo + 2];
+    double gam_prev = 0.0, z_prev = 0.0;
+    for (int r = 0; r < m; r++) {
+        const int i = a + r;
+        const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
+        const double diag = 2.0 * (hr + hl);
+        const double alp = r == 0 ? diag : diag - hl * gam_prev;
+        const double zr = r == 0 ? rhs[i] : rhs[i] - gam_prev * z_prev;
+        gam[r] = hr / alp;
+        zz[r] = zr / alp;
+        gam_prev = gam[r];
+        z_prev = zr;
+    }
+    double next = 0.0;
+    for (int r = m - 1; r >= 0; r--) {
+        const double v = zz[r] - (r == m - 1 ? 0.0 : gam[r] * next);
+        const int i = a + r;
+        if (i >= lo && i < hi) z_out[i] = v;
+        next = v;
+    }
+    if (lo == 0) z_out[0] = 0.0;
+    if (lo <= n - 1 && n - 1 < hi) z_out[n - 1] = 0.0;
 }
 
 }  // namespace kn_dnu

@@ -28,6 +28,7 @@ extern "C" {
 #define KN_ERR_ARG 3002
 #define KN_ERR_NOHUBBLE 3003
 #define KN_ERR_RANGE 3004
+#define KN_ERR_DISABLED 3005
 
 /* ---- host services (host_services.c) ---- */
 double kn_hubble(double a);

@@ -53,6 +53,17 @@ class KspaceParams(C.Structure):
                 ("hybrid_neutrinos_on", C.c_int), ("vcrit", C.c_double), ("nu_crit_time", C.c_double)]
 
 
+class BatchHistory(C.Structure):
+    """kn_batch_history: one get_delta_nu integral of a batch (hubble: C function address)."""
+    _fields_ = [("nk", C.c_int), ("na", C.c_int), ("na_max", C.c_int), ("a", C.c_double), ("a_max", C.c_double),
+                ("time_transfer", C.c_double), ("mnu", C.c_double), ("kBtnu", C.c_double), ("light", C.c_double),
+                ("delta_nu_prefac", C.c_double), ("qc", C.c_double), ("relerr", C.c_double),
+                ("nufrac_low", C.c_double), ("vcrit", C.c_double), ("ic_only", C.c_int), ("hubble", C.c_void_p),
+                ("hubble_ctx", C.c_void_p)]
+
+
+vpp = C.POINTER(C.c_void_p)
+
 # name -> (restype, argtypes); the reference entry points first, then kn_ additions
 SIGNATURES = {
     "total_powerspectrum": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, dp, llp, dp, C.c_int]),
@@ -102,6 +113,23 @@ SIGNATURES = {
     "kn_rescale_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                     C.c_int, C.c_double, C.c_void_p]),
     "kn_last_error": (C.c_char_p, []),
+    "kn_batch_create": (C.c_int, [vpp, C.c_int, C.POINTER(BatchHistory)]),
+    "kn_batch_destroy": (None, [C.c_void_p]),
+    "kn_batch_views": (C.c_int, [C.c_void_p] + [vpp] * 9),
+    "kn_batch_layout": (C.c_int, [C.c_void_p, llp, llp, llp]),
+    "kn_batch_solve": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kn_batch_adjoint": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kn_batch_append": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kn_batch_graph_capture": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kn_batch_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kn_batch_status": (C.c_int, [C.c_void_p]),
+    "kn_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kn_batch_set_mass": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "kn_batch_history_from_table": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, C.c_double,
+                                              C.POINTER(BatchHistory)]),
+    "kn_hubble_ctx": (C.c_double, [C.c_double, C.c_void_p]),
+    "kn_get_delta_nu_adjoint": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, dp, C.c_double, dp, dp, dp, dp,
+                                          dp]),
 }
 
 _lib = None
@@ -126,3 +154,215 @@ def lib():
 def exported_symbols():
     """Names this binding expects the library to export."""
     return sorted(SIGNATURES)
+
+
+# ---------------------------------------------------------------- batches
+
+H2D, D2H, D2D = 1, 2, 3
+_hip = None
+
+
+def _memcpy(dst, src, nbytes, kind):
+    """hipMemcpy through the HIP runtime the library itself links."""
+    global _hip
+    if _hip is None:
+        lib()
+        _hip = C.CDLL("libamdhip64.so.7")
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    if nbytes:
+        rc = _hip.hipMemcpy(dst, src, nbytes, kind)
+        if rc:
+            raise RuntimeError("hipMemcpy failed: %d" % rc)
+
+
+def _check(rc, what, L=None):
+    if rc:
+        raise RuntimeError("%s failed with code %d: %s" % (what, rc, (L or lib()).kn_last_error().decode()))
+
+
+def bind(L):
+    """Set this module's signatures on another build of the same ABI (returns L)."""
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name, None)
+        if f is not None:
+            f.restype = res
+            f.argtypes = args
+    return L
+
+
+class Batch:
+    """Owner of a kn_batch: loads histories into its device arrays and reads results back.
+
+    hists: list of BatchHistory (their hubble_ctx objects must outlive the batch).
+    L, memcpy: the library build and its device copy (defaults: this library, hipMemcpy).
+    """
+
+    VIEWS = ("scalefact", "delta_tot", "wavenum", "delta_nu_init", "delta_nu", "gbar", "dtot_bar", "init_bar",
+             "mnu_bar")
+
+    def __init__(self, hists, keep=(), L=None, memcpy=None):
+        L = L or lib()
+        self.L = L
+        self._mc = memcpy or _memcpy
+        self.h = None
+        self.n = len(hists)
+        self.hists = list(hists)
+        self._keep = keep
+        arr = (BatchHistory * self.n)(*hists)
+        h = C.c_void_p()
+        _check(L.kn_batch_create(C.byref(h), self.n, arr), "kn_batch_create", L)
+        self.h = h
+        ptrs = [C.c_void_p() for _ in self.VIEWS]
+        _check(L.kn_batch_views(h, *[C.byref(p) for p in ptrs]), "kn_batch_views", L)
+        self.ptr = {name: p.value for name, p in zip(self.VIEWS, ptrs)}
+        self.off_k = (C.c_longlong * (self.n + 1))()
+        self.off_y = (C.c_longlong * (self.n + 1))()
+        self.off_s = (C.c_longlong * (self.n + 1))()
+        _check(L.kn_batch_layout(h, self.off_k, self.off_y, self.off_s), "kn_batch_layout", L)
+
+    def _put(self, name, off, arr):
+        import numpy as np
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self._mc(self.ptr[name] + 8 * off, a.ctypes.data, a.nbytes, H2D)
+
+    def _get(self, name, off, n):
+        import numpy as np
+        a = np.empty(n, dtype=np.float64)
+        self._mc(a.ctypes.data, self.ptr[name] + 8 * off, a.nbytes, D2H)
+        return a
+
+    def load(self, i, logs, rows, wavenum, init):
+        """History i: stored log a [na], delta_tot rows [nk][na], wavenumbers [nk], delta_nu_init [nk]."""
+        import numpy as np
+        hs = self.hists[i]
+        rows = np.asarray(rows, dtype=np.float64)
+        blk = np.zeros((hs.nk, hs.na_max))
+        blk[:, :rows.shape[1]] = rows
+        self._put("scalefact", self.off_s[i], logs)
+        self._put("delta_tot", self.off_y[i], blk)
+        self._put("wavenum", self.off_k[i], wavenum)
+        self._put("delta_nu_init", self.off_k[i], init)
+
+    def solve(self):
+        _check(self.L.kn_batch_solve(self.h, None), "kn_batch_solve", self.L)
+
+    def adjoint(self, gbar_all=None):
+        if gbar_all is not None:
+            self._put("gbar", 0, gbar_all)
+        _check(self.L.kn_batch_adjoint(self.h, None), "kn_batch_adjoint", self.L)
+
+    def append(self, loga, rows):
+        """One more stored step per history (host arrays: log a per history, rows per item)."""
+        import numpy as np
+        la = np.ascontiguousarray(loga, dtype=np.float64)
+        rw = np.ascontiguousarray(rows, dtype=np.float64)
+        # staged in the batch's gbar (rows) and mnu_bar (log a) arrays, which this overwrites
+        self._put("gbar", 0, rw)
+        self._put("mnu_bar", 0, la)
+        _check(self.L.kn_batch_append(self.h, C.c_void_p(self.ptr["mnu_bar"]), C.c_void_p(self.ptr["gbar"]), None),
+               "kn_batch_append", self.L)
+
+    def status(self):
+        return self.L.kn_batch_status(self.h)
+
+    def delta_nu(self, i=None):
+        if i is None:
+            return self._get("delta_nu", 0, self.off_k[self.n])
+        return self._get("delta_nu", self.off_k[i], self.hists[i].nk)
+
+    def dtot_bar(self, i, na):
+        hs = self.hists[i]
+        return self._get("dtot_bar", self.off_y[i], hs.nk * hs.na_max).reshape(hs.nk, hs.na_max)[:, :na]
+
+    def init_bar(self, i):
+        return self._get("init_bar", self.off_k[i], self.hists[i].nk)
+
+    def mnu_bar(self):
+        return self._get("mnu_bar", 0, self.n)
+
+    def set_mass(self, i, mnu):
+        _check(self.L.kn_batch_set_mass(self.h, i, mnu), "kn_batch_set_mass", self.L)
+
+    def close(self):
+        if self.h:
+            self.L.kn_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def delta_nu_function():
+    """torch.autograd.Function over get_delta_nu of one table:
+    apply(delta_tot[nk, na], delta_nu_init[nk], mnu[()], state) -> delta_nu[nk],
+    where state = DeltaNuState(table, a, wavenum).  Gradients come from the
+    C-ABI adjoint (kn_batch_adjoint) with the forward's QAG subdivision."""
+    import torch
+
+    class DeltaNuFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, delta_tot, delta_nu_init, mnu, state):
+            out = state.run(delta_tot, delta_nu_init, float(mnu))
+            ctx.state = state
+            ctx.save_for_backward(delta_tot, delta_nu_init, mnu)
+            return out
+
+        @staticmethod
+        def backward(ctx, g):
+            delta_tot, delta_nu_init, mnu = ctx.saved_tensors
+            db, ib, mb = ctx.state.grad(delta_tot, delta_nu_init, float(mnu), g)
+            return db, ib, torch.tensor(mb, dtype=mnu.dtype, device=mnu.device), None
+
+    return DeltaNuFunction
+
+
+class DeltaNuState:
+    """A one-history batch for get_delta_nu(table, a, wavenum, .) whose inputs
+    come from torch tensors (device memory, float64)."""
+
+    def __init__(self, table, a, wavenum, mnu, L=None, memcpy=None):
+        import numpy as np
+        L = L or lib()
+        self._mc = memcpy or _memcpy
+        hh = BatchHistory()
+        _check(L.kn_batch_history_from_table(C.byref(table), a, mnu, C.byref(hh)), "history_from_table", L)
+        hh.hubble = C.cast(L.kn_hubble_ctx, C.c_void_p).value
+        self.nk, self.na = hh.nk, hh.na
+        self.batch = Batch([hh], L=L, memcpy=memcpy)
+        logs = np.ctypeslib.as_array(table.scalefact, shape=(self.na,)).copy()
+        self.batch._put("scalefact", 0, logs)
+        self.batch._put("wavenum", 0, np.asarray(wavenum, dtype=np.float64))
+
+    def _load(self, delta_tot, delta_nu_init, mnu):
+        assert delta_tot.is_contiguous() and tuple(delta_tot.shape) == (self.nk, self.na)
+        assert delta_nu_init.is_contiguous() and tuple(delta_nu_init.shape) == (self.nk,)
+        b = self.batch
+        b.set_mass(0, mnu)
+        self._mc(b.ptr["delta_tot"], delta_tot.data_ptr(), 8 * self.nk * self.na, D2D)
+        self._mc(b.ptr["delta_nu_init"], delta_nu_init.data_ptr(), 8 * self.nk, D2D)
+        b.solve()
+
+    def run(self, delta_tot, delta_nu_init, mnu):
+        import torch
+        self._load(delta_tot, delta_nu_init, mnu)
+        out = torch.empty(self.nk, dtype=torch.float64, device=delta_tot.device)
+        self._mc(out.data_ptr(), self.batch.ptr["delta_nu"], 8 * self.nk, D2D)
+        return out
+
+    def grad(self, delta_tot, delta_nu_init, mnu, g):
+        import torch
+        self._load(delta_tot, delta_nu_init, mnu)
+        gc = g.detach().to(torch.float64).contiguous()
+        b = self.batch
+        self._mc(b.ptr["gbar"], gc.data_ptr(), 8 * self.nk, D2D)
+        b.adjoint()
+        db = torch.empty((self.nk, self.na), dtype=torch.float64, device=delta_tot.device)
+        ib = torch.empty(self.nk, dtype=torch.float64, device=delta_tot.device)
+        self._mc(db.data_ptr(), b.ptr["dtot_bar"], 8 * self.nk * self.na, D2D)
+        self._mc(ib.data_ptr(), b.ptr["init_bar"], 8 * self.nk, D2D)
+        mb = b.mnu_bar()[0]
+        return db, ib, mb

@@ -172,8 +172,33 @@ static int subinterval_too_small(double a1, double a2, double b2)
 
 /* Return codes follow GSL: 0 success, 14 EMAXITER-like failures are reported as
  * the GSL errno value (GSL_EMAXITER=11, GSL_EROUND=18, GSL_ESING=21). */
+static int qag61_impl(orc_fn f, void *p, double a, double b, double epsabs, double epsrel,
+                      int limit, double *result, double *abserr, int *nintervals, double *ivl);
+
 int orc_qag61(orc_fn f, void *p, double a, double b, double epsabs, double epsrel,
               int limit, double *result, double *abserr, int *nintervals)
+{
+    return qag61_impl(f, p, a, b, epsabs, epsrel, limit, result, abserr, nintervals, NULL);
+}
+
+int orc_qag61_record(orc_fn f, void *p, double a, double b, double epsabs, double epsrel,
+                     int limit, double *result, double *abserr, int *nintervals, double *ivl)
+{
+    return qag61_impl(f, p, a, b, epsabs, epsrel, limit, result, abserr, nintervals, ivl);
+}
+
+static void record_ivl(const orc_ws *w, double *ivl)
+{
+    int k;
+    if (!ivl) return;
+    for (k = 0; k < w->size; k++) {
+        ivl[2 * k] = w->alist[k];
+        ivl[2 * k + 1] = w->blist[k];
+    }
+}
+
+static int qag61_impl(orc_fn f, void *p, double a, double b, double epsabs, double epsrel,
+                      int limit, double *result, double *abserr, int *nintervals, double *ivl)
 {
     orc_ws w;
     double area, errsum, result0, abserr0, resabs0, resasc0, tolerance, round_off;
@@ -184,6 +209,7 @@ int orc_qag61(orc_fn f, void *p, double a, double b, double epsabs, double epsre
     if (nintervals) *nintervals = 1;
     orc_qk61(f, p, a, b, &result0, &abserr0, &resabs0, &resasc0);
     w.size = 1; w.rlist[0] = result0; w.elist[0] = abserr0;
+    record_ivl(&w, ivl);
     tolerance = fmax(epsabs, epsrel * fabs(result0));
     round_off = 50 * DBL_EPSILON * resabs0;
     if (abserr0 <= round_off && abserr0 > tolerance) {
@@ -231,6 +257,7 @@ int orc_qag61(orc_fn f, void *p, double a, double b, double epsabs, double epsre
     }
     *abserr = errsum;
     if (nintervals) *nintervals = w.size;
+    record_ivl(&w, ivl);
     if (errsum <= tolerance) return 0;
     if (error_type == 2) return 18;
     if (error_type == 3) return 21;
@@ -1075,4 +1102,271 @@ int orc_rescale(int dims, double *field, int startslab, int nslab, double BoxSiz
                 field[2 * ip + 1] *= smth;
             }
     return 0;
+}
+
+/* ======================= adjoint of get_delta_nu (test infrastructure) =======================
+ *
+ * The reference has no adjoint; these define the derivative the GPU adjoint
+ * must reproduce: the derivative of the computed quadrature, i.e. of
+ *   IC(k) + prefac * sum over the final QAG subintervals of GK61(I_k),
+ * with the subdivision of the forward evaluation held fixed.  They are built
+ * from independent means so that they pin the GPU's hand-derived formulas:
+ *  - d/d delta_tot[k][j]: evaluation of the natural spline of the unit vector
+ *    e_j (one spline per j, the oracle's own spline routine) at every node;
+ *  - d/d mnu: complex-step differentiation (Im f(m + i h) / h) of a complex
+ *    restatement of specialJ inside the integrand.
+ * Their own pins (tests/test_oracle_adjoint.py): central differences of
+ * orc_dnu_frozen, which is linear in delta_tot and smooth in mnu. */
+#include <complex.h>
+
+static double complex cj0(double complex z)
+{
+    if (cabs(z) < 0.5) {
+        const double complex y = z * z;
+        return 1.0 + y * (-1.0 / 6.0 + y * (1.0 / 120.0 + y * (-1.0 / 5040.0 + y * (1.0 / 362880.0 +
+                     y * (-1.0 / 39916800.0 + y * (1.0 / 6227020800.0))))));
+    }
+    return csin(z) / z;
+}
+
+/* specialJ (delta_tot_table.c:417-463) on complex x and qc; hybrid when creal(qc) > 0. */
+static double complex specialJ_c(double complex x, double complex qc, double nufrac_low)
+{
+    if (creal(qc) > 0) {
+        double complex integ = 0;
+        int n;
+        for (n = 1; n < 20; n++) {
+            const double complex ii = (n * n + n * n * n * qc + n * qc * x * x - x * x) * qc * cj0(qc * x) +
+                                      (2 * n + n * n * qc + qc * x * x) * ccos(qc * x);
+            integ += -1 * pow(-1, n) * cexp(-n * qc) / (n * n + x * x) / (n * n + x * x) * ii;
+        }
+        return integ / (1.5 * 1.202056903159594 * (1 - nufrac_low));
+    }
+    if (creal(x) <= 0.) return 1.;
+    {
+        const double complex x2 = x * x, x4 = x2 * x2, x8 = x4 * x4;
+        return (1. + 0.0168 * x2 + 0.0407 * x4) / (1. + 2.1734 * x2 + 1.6787 * cexp(4.1811 * clog(x)) + 0.1467 * x8);
+    }
+}
+
+/* d specialJ(k fsl / mu, vcrit mu [hybrid]) / d mu by complex step. */
+static double dJ_dmu(double kfsl, double mu, double vcrit, int hybrid, double nufrac_low)
+{
+    const double h = 1e-30 * mu;
+    const double complex m = mu + I * h;
+    const double complex J = specialJ_c(kfsl / m, hybrid ? vcrit * m : 0.0, nufrac_low);
+    return cimag(J) / h;
+}
+
+double orc_specialJ_dmu(double kfsl, double mu, double vcrit, int hybrid, double nufrac_low)
+{
+    return dJ_dmu(kfsl, mu, vcrit, hybrid, nufrac_low);
+}
+
+typedef struct {
+    double fsl_A0a, deriv_prefac, qc, relerr, mnubykT;
+    int integrate, hybrid, Na, Nfs;
+    double *fsscales, *fslengths;
+    orc_spline fs;
+} dnu_setup;
+
+/* The steps of delta_tot_table.c:507-575 that precede the per-k integrals. */
+static void setup_dnu(orc_dtt *d, double a, double mnu, dnu_setup *S)
+{
+    double partnu;
+    int i;
+    memset(S, 0, sizeof *S);
+    S->Na = d->ia;
+    S->mnubykT = mnu / d->omnu->kBtnu;
+    S->relerr = 1e-6;
+    S->fsl_A0a = orc_fslength(log(d->TimeTransfer), log(a), d->light);
+    S->deriv_prefac = d->TimeTransfer * (orc_hubble(d->TimeTransfer) / d->light) * d->TimeTransfer;
+    partnu = orc_particle_nu_fraction(&d->omnu->hybnu, a, 0);
+    S->integrate = 1;
+    if (partnu > 0) {
+        if (1 - partnu < 1e-3) S->integrate = 0;
+        S->qc = d->omnu->hybnu.vcrit * S->mnubykT;
+        S->hybrid = 1;
+        S->relerr /= (1. + 1e-5 - orc_particle_nu_fraction(&d->omnu->hybnu, a, 0));
+    }
+    if (!(S->Na > 1 && S->mnubykT > 0)) S->integrate = 0;
+    if (!S->integrate) return;
+    S->Nfs = S->Na * 16;
+    S->fslengths = malloc(sizeof(double) * (size_t)S->Nfs);
+    S->fsscales = malloc(sizeof(double) * (size_t)S->Nfs);
+    for (i = 0; i < S->Nfs; i++) {
+        S->fsscales[i] = log(d->TimeTransfer) + i * (log(a) - log(d->TimeTransfer)) / (S->Nfs - 1.);
+        S->fslengths[i] = orc_fslength(S->fsscales[i], log(a), d->light);
+    }
+    orc_spline_init(&S->fs, S->fsscales, S->fslengths, S->Nfs, 0);
+}
+
+static void free_setup(dnu_setup *S)
+{
+    if (!S->integrate) return;
+    orc_spline_free(&S->fs);
+    free(S->fsscales);
+    free(S->fslengths);
+}
+
+static double ic_value(const orc_dtt *d, const dnu_setup *S, double k, double init)
+{
+    const double specJ = orc_specialJ(k * S->fsl_A0a / (S->mnubykT > 0 ? S->mnubykT : 1), 0,
+                                      d->omnu->hybnu.nufrac_low[0]);
+    return specJ * init * (1. + S->deriv_prefac * S->fsl_A0a);
+}
+
+/* Forward of orc_get_delta_nu (mode 0) that also records each k's final
+ * subdivision: ivl[(k * ORC_GSL_VAL + t) * 2 + {0,1}], nint[k]. */
+int orc_get_delta_nu_record(orc_dtt *d, double a, const double wavenum[], double out[], double mnu, double *ivl,
+                            int *nint)
+{
+    dnu_setup S;
+    int ik;
+    g_qag_error = 0;
+    setup_dnu(d, a, mnu, &S);
+    for (ik = 0; ik < d->nk; ik++) {
+        out[ik] = ic_value(d, &S, wavenum[ik], d->delta_nu_init[ik]);
+        nint[ik] = 0;
+    }
+    if (S.integrate) {
+        double *row = malloc(sizeof(double) * (size_t)S.Na);
+        dnu_params P;
+        memset(&P, 0, sizeof P);
+        P.mnubykT = S.mnubykT;
+        P.qc = S.qc;
+        P.nufrac_low = d->omnu->hybnu.nufrac_low[0];
+        P.light = d->light;
+        P.fs = S.fs;
+        for (ik = 0; ik < d->nk; ik++) {
+            double res = 0, abserr;
+            int j;
+            for (j = 0; j < S.Na; j++) row[j] = DT(d, ik, j);
+            P.k = wavenum[ik];
+            orc_spline_init(&P.dt, d->scalefact, row, S.Na, S.Na <= 2);
+            if (orc_qag61_record(dnu_integrand, &P, log(d->TimeTransfer), log(a), 0, S.relerr, ORC_GSL_VAL, &res,
+                                 &abserr, &nint[ik], ivl + (size_t)ik * ORC_GSL_VAL * 2))
+                g_qag_error = 2001;
+            out[ik] += d->delta_nu_prefac * res;
+            orc_spline_free(&P.dt);
+        }
+        free(row);
+        if (P.err) g_qag_error = P.err;
+    }
+    free_setup(&S);
+    return g_qag_error;
+}
+
+/* get_delta_nu with the recorded subdivision held fixed (the same sum of the
+ * same GK61 applications, so it equals the recorded forward bit for bit). */
+int orc_dnu_frozen(orc_dtt *d, double a, const double wavenum[], double out[], double mnu, const double *ivl,
+                   const int *nint)
+{
+    dnu_setup S;
+    int ik;
+    g_qag_error = 0;
+    setup_dnu(d, a, mnu, &S);
+    for (ik = 0; ik < d->nk; ik++) out[ik] = ic_value(d, &S, wavenum[ik], d->delta_nu_init[ik]);
+    if (S.integrate) {
+        double *row = malloc(sizeof(double) * (size_t)S.Na);
+        dnu_params P;
+        memset(&P, 0, sizeof P);
+        P.mnubykT = S.mnubykT;
+        P.qc = S.qc;
+        P.nufrac_low = d->omnu->hybnu.nufrac_low[0];
+        P.light = d->light;
+        P.fs = S.fs;
+        for (ik = 0; ik < d->nk; ik++) {
+            double res = 0;
+            int j, t;
+            for (j = 0; j < S.Na; j++) row[j] = DT(d, ik, j);
+            P.k = wavenum[ik];
+            orc_spline_init(&P.dt, d->scalefact, row, S.Na, S.Na <= 2);
+            for (t = 0; t < nint[ik]; t++) {
+                const double *iv = ivl + ((size_t)ik * ORC_GSL_VAL + t) * 2;
+                double r, e, ra, rs;
+                orc_qk61(dnu_integrand, &P, iv[0], iv[1], &r, &e, &ra, &rs);
+                res += r;
+            }
+            out[ik] += d->delta_nu_prefac * res;
+            orc_spline_free(&P.dt);
+        }
+        free(row);
+        if (P.err) g_qag_error = P.err;
+    }
+    free_setup(&S);
+    return g_qag_error;
+}
+
+/* Adjoint for seeds gbar[nk]: out[nk] the forward, dtot_bar[k * Na + j],
+ * init_bar[nk], *mnu_bar (d/d mnu with the table's hybrid state fixed). */
+int orc_get_delta_nu_adjoint(orc_dtt *d, double a, const double wavenum[], double mnu, const double gbar[],
+                             double out[], double dtot_bar[], double init_bar[], double *mnu_bar)
+{
+    const int nk = d->nk, Na = d->ia;
+    double *ivl = malloc(sizeof(double) * (size_t)nk * ORC_GSL_VAL * 2);
+    int *nint = malloc(sizeof(int) * (size_t)nk);
+    const double nfl = d->omnu->hybnu.nufrac_low[0];
+    double mbar = 0;
+    dnu_setup S;
+    int ik, rc;
+    rc = orc_get_delta_nu_record(d, a, wavenum, out, mnu, ivl, nint);
+    setup_dnu(d, a, mnu, &S);
+    memset(dtot_bar, 0, sizeof(double) * (size_t)nk * (size_t)Na);
+    for (ik = 0; ik < nk; ik++) {
+        const double m = S.mnubykT > 0 ? S.mnubykT : 1;
+        const double lin = 1. + S.deriv_prefac * S.fsl_A0a;
+        init_bar[ik] = gbar[ik] * orc_specialJ(wavenum[ik] * S.fsl_A0a / m, 0, nfl) * lin;
+        if (S.mnubykT > 0)
+            mbar += gbar[ik] * dJ_dmu(wavenum[ik] * S.fsl_A0a, S.mnubykT, 0, 0, nfl) * d->delta_nu_init[ik] * lin;
+    }
+    if (S.integrate) {
+        orc_spline *basis = malloc(sizeof(orc_spline) * (size_t)Na);
+        double *unit = calloc((size_t)Na * (size_t)Na, sizeof(double));
+        double *row = malloc(sizeof(double) * (size_t)Na);
+        double *phi = malloc(sizeof(double) * (size_t)Na);
+        int j, err = 0;
+        for (j = 0; j < Na; j++) {
+            unit[(size_t)j * Na + j] = 1.0;
+            orc_spline_init(&basis[j], d->scalefact, unit + (size_t)j * Na, Na, Na <= 2);
+        }
+        for (ik = 0; ik < nk; ik++) {
+            orc_spline dt;
+            int t;
+            for (j = 0; j < Na; j++) row[j] = DT(d, ik, j);
+            orc_spline_init(&dt, d->scalefact, row, Na, Na <= 2);
+            for (t = 0; t < nint[ik]; t++) {
+                const double *iv = ivl + ((size_t)ik * ORC_GSL_VAL + t) * 2;
+                const double c = 0.5 * (iv[0] + iv[1]), hl = 0.5 * (iv[1] - iv[0]);
+                int q, side;
+                for (q = 0; q < 31; q++) {
+                    for (side = 0; side < (q < 30 ? 2 : 1); side++) {
+                        const double x = q < 30 ? (side ? c + hl * orc_xgk61[q] : c - hl * orc_xgk61[q]) : c;
+                        const double w = orc_wgk61[q] * hl * d->delta_nu_prefac * gbar[ik];
+                        const double fsl = orc_spline_eval(&S.fs, x, &err);
+                        const double ai = exp(x);
+                        const double base = fsl / (ai * orc_hubble(ai));
+                        const double J = orc_specialJ(wavenum[ik] * fsl / S.mnubykT, S.qc, nfl);
+                        const double Sv = orc_spline_eval(&dt, x, &err);
+                        for (j = 0; j < Na; j++) phi[j] = orc_spline_eval(&basis[j], x, &err);
+                        for (j = 0; j < Na; j++) dtot_bar[(size_t)ik * Na + j] += w * base * J * phi[j];
+                        mbar += w * base * Sv *
+                                dJ_dmu(wavenum[ik] * fsl, S.mnubykT, d->omnu->hybnu.vcrit, S.hybrid, nfl);
+                    }
+                }
+            }
+            orc_spline_free(&dt);
+        }
+        for (j = 0; j < Na; j++) orc_spline_free(&basis[j]);
+        free(basis);
+        free(unit);
+        free(row);
+        free(phi);
+        if (err && !rc) rc = err;
+    }
+    *mnu_bar = mbar / d->omnu->kBtnu;
+    free_setup(&S);
+    free(ivl);
+    free(nint);
+    return rc;
 }

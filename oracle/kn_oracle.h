@@ -120,6 +120,17 @@ int  orc_get_delta_nu_combined(orc_dtt *d, double a, const double wavenum[], dou
 int  orc_get_delta_nu_update(orc_dtt *d, double a, int nk_in, const double keff[], const double dcdm[],
                              double out[], const orc_transfer *t);
 
+/* ---- adjoint of get_delta_nu (no reference counterpart; checker of the GPU adjoint) ---- */
+int orc_qag61_record(orc_fn f, void *p, double a, double b, double epsabs, double epsrel,
+                     int limit, double *result, double *abserr, int *nintervals, double *ivl);
+int orc_get_delta_nu_record(orc_dtt *d, double a, const double wavenum[], double out[], double mnu, double *ivl,
+                            int *nint);
+int orc_dnu_frozen(orc_dtt *d, double a, const double wavenum[], double out[], double mnu, const double *ivl,
+                   const int *nint);
+int orc_get_delta_nu_adjoint(orc_dtt *d, double a, const double wavenum[], double mnu, const double gbar[],
+                             double out[], double dtot_bar[], double init_bar[], double *mnu_bar);
+double orc_specialJ_dmu(double kfsl, double mu, double vcrit, int hybrid, double nufrac_low);
+
 /* ---- k-shell binning and rescale (powerspectrum.c:7-117, interface_gadget.c:158-194) ---- */
 int orc_total_powerspectrum(int dims, const double *field /* re,im interleaved */, int nrbins,
                             int startslab, int nslab, double *power, long long *count, double *keffs);

@@ -1,0 +1,112 @@
+"""Shared helpers of the batched-solver tests (GPU and CPU-emulation builds):
+matched product/oracle cosmologies with a per-history H(a) context, mixed
+history sets, and comparison of adjoints against the oracle."""
+import ctypes as C
+import math
+
+import numpy as np
+
+import knhip
+import oracle_py as O
+from conftest import UNIT_LENGTH, UNIT_TIME, load_testhost
+from dnu_common import Pair, synthetic_history
+
+RTOL = 1e-9
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a / b - 1)))
+
+
+class HCtx(C.Structure):
+    _fields_ = [("omnu", C.c_void_p), ("Omega_nonu", C.c_double), ("OmegaLambda", C.c_double),
+                ("Hubble", C.c_double)]
+
+
+class Cosmo:
+    """One cosmology on both sides: product omega_nu + per-history H context, oracle omega_nu."""
+
+    def __init__(self, masses, hybrid=None, omega0=0.3):
+        self.P = Pair(masses, omega0, 4, 2.0, hybrid=hybrid)
+        self.masses = masses
+        self.omega0 = omega0
+        host = load_testhost()
+        self.ctx = HCtx()
+        host.kn_test_hctx_init.argtypes = [C.POINTER(HCtx), C.c_void_p, C.c_double, C.c_double]
+        host.kn_test_hctx_init(C.byref(self.ctx), C.addressof(self.P.om), omega0, UNIT_TIME)
+        self.hfn = C.cast(host.kn_test_hubble_ctx, C.c_void_p).value
+
+    def history(self, nk, na, na_max, a, a_max, mnu):
+        """BatchHistory as get_delta_nu derives it (kn_batch_history_from_table on the product table)."""
+        d = self.P.d
+        d.ia = na
+        hh = knhip.BatchHistory()
+        assert knhip.lib().kn_batch_history_from_table(C.byref(d), a, mnu, C.byref(hh)) == 0
+        hh.nk, hh.na, hh.na_max, hh.a_max = nk, na, na_max, a_max
+        hh.hubble = self.hfn
+        hh.hubble_ctx = C.addressof(self.ctx)
+        return hh
+
+    def oracle(self, nk, logs, rows, init):
+        O.lib().orc_init_hubble(C.byref(self.P.oom), self.omega0, UNIT_TIME)
+        T = O.Table(self.P.oom, nk, 0.01, 2.0, self.omega0, UNIT_TIME, UNIT_LENGTH)
+        na = len(logs)
+        T.scalefact[:na] = logs
+        T.delta_tot[:, :na] = rows
+        T.delta_nu_init[:] = init
+        T.d.ia = na
+        return T
+
+    def free(self):
+        self.P.free()
+
+
+def adjoint_case(P, a, k, m, seed=3, L=None):
+    nk, na = P.nk, P.d.ia
+    g = np.random.default_rng(seed).uniform(-1, 1, nk)
+    dnu, db, ib = np.zeros(nk), np.zeros(nk * na), np.zeros(nk)
+    mb = C.c_double()
+    kk = np.ascontiguousarray(k, dtype=np.float64)
+    dp = C.POINTER(C.c_double)
+    rc = (L or knhip.lib()).kn_get_delta_nu_adjoint(C.byref(P.d), a, kk.ctypes.data_as(dp), m, g.ctypes.data_as(dp),
+                                             dnu.ctypes.data_as(dp), db.ctypes.data_as(dp), ib.ctypes.data_as(dp),
+                                             C.byref(mb))
+    assert rc == 0
+    orc = P.T.get_delta_nu_adjoint(a, k, m, g)
+    assert orc[0] == 0
+    return (dnu, db.reshape(nk, na), ib, mb.value), orc
+
+
+def compare_adjoint(name, gpu, orc):
+    dnu, db, ib, mb = gpu
+    _, odnu, odb, oib, omb = orc
+    e_f = rel(dnu, odnu)
+    scale = np.maximum(np.max(np.abs(odb), axis=1, keepdims=True), 1e-300)
+    e_y = float(np.max(np.abs(db - odb) / scale))
+    e_i = float(np.max(np.abs(ib - oib) / np.max(np.abs(oib))))
+    e_m = abs(mb / omb - 1) if omb != 0 else abs(mb)
+    print("  %s: delta_nu %.2e  d/d delta_tot %.2e  d/d init %.2e  d/d mnu %.2e" % (name, e_f, e_y, e_i, e_m))
+    assert e_f < RTOL and e_y < RTOL and e_i < RTOL and e_m < RTOL
+
+
+def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None):
+    """Histories over three cosmologies with mixed lengths and k counts."""
+    rng = np.random.default_rng(seed)
+    cosmos = [Cosmo([0.15, 0, 0]), Cosmo([0.1, 0.1, 0.1]), Cosmo([0.3, 0.06, 0.06], hybrid=(500.0, 0.3333))]
+    specs = []
+    for i in range(ncosmo_hists * len(cosmos)):
+        c = i % len(cosmos)
+        na = int(rng.choice([2, 3, 7, 25, 60, 110, 150]))
+        nk = int(rng.choice([4, 8, 16, 24]))
+        amax = float(np.exp(math.log(0.01) + (na / 150.0) * (math.log(0.95) - math.log(0.01)))) if na > 2 else 0.03
+        logs, rows, k, init = synthetic_history(na, nk, amax, seed=i, noise=0.05)
+        m = cosmos[c].masses[0] if c != 1 else 0.1
+        specs.append(dict(c=c, na=na, nk=nk, amax=amax, logs=logs, rows=rows, k=k, init=init, m=m))
+    hists = [cosmos[s["c"]].history(s["nk"], s["na"], s["na"] + extra, s["amax"], 1.0 if extra else s["amax"],
+                                    s["m"]) for s in specs]
+    B = knhip.Batch(hists, keep=cosmos, L=L, memcpy=memcpy)
+    for i, s in enumerate(specs):
+        B.load(i, s["logs"], s["rows"], s["k"], s["init"])
+    return cosmos, specs, B
+
+

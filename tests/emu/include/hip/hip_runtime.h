@@ -1,0 +1,216 @@
+// SIMT emulation on the CPU of the HIP subset the batch kernels use.
+//
+// TEST INFRASTRUCTURE ONLY (tests/emu): compiles the library's HIP kernel
+// source with g++ and runs every kernel on the host.  Each lane is a fiber
+// (its own stack, switched by emu_ctx_switch on one host thread); a wavefront
+// runs its 64 lanes round-robin up to the next wavefront operation
+// (__shfl, __shfl_xor, readfirstlane, __syncthreads), where every lane must
+// arrive, as converged hardware code does; otherwise the emulation stops
+// with a message.  Waves and blocks run one after another, so __syncthreads
+// is supported in one-wave blocks only.  "Device" memory is host memory
+// filled with garbage when allocated; graphs are not emulated.
+#pragma once
+#include <atomic>
+#include <cstdint>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+#include <math.h>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __constant__
+#define __shared__ static
+#define __launch_bounds__(...)
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct emu_idx { unsigned x, y, z; };
+extern unsigned emu_tid, emu_bid, emu_bdim, emu_gdim;
+inline emu_idx emu_threadIdx() { return {emu_tid, 0, 0}; }
+inline emu_idx emu_blockIdx() { return {emu_bid, 0, 0}; }
+inline emu_idx emu_blockDim() { return {emu_bdim, 1, 1}; }
+inline emu_idx emu_gridDim() { return {emu_gdim, 1, 1}; }
+#define threadIdx emu_threadIdx()
+#define blockIdx emu_blockIdx()
+#define blockDim emu_blockDim()
+#define gridDim emu_gridDim()
+
+// x86-64 System V stack switch: saves the callee-saved registers on the
+// current stack, stores its pointer in *save_sp and resumes load_sp.
+extern "C" void emu_ctx_switch(void **save_sp, void *load_sp);
+
+struct EmuWaveSched {
+    void *main_sp;
+    void *sp[64];
+    bool done[64];
+    unsigned base;  // threadIdx.x of lane 0
+    int cur;
+    unsigned long long x[64];
+    void (*body)(void *);
+    void *arg;
+    std::vector<char> stacks;
+};
+extern EmuWaveSched *emu_wave;
+
+inline void emu_yield()
+{
+    EmuWaveSched &S = *emu_wave;
+    const int lane = S.cur;
+    emu_ctx_switch(&S.sp[lane], S.main_sp);
+}
+inline unsigned long long emu_exchange(unsigned long long v, int src)
+{
+    EmuWaveSched &S = *emu_wave;
+    const int lane = S.cur;
+    S.x[lane] = v;
+    emu_yield();  // every lane has written
+    const unsigned long long r = S.x[src & 63];
+    emu_yield();  // every lane has read
+    return r;
+}
+inline double __shfl(double v, int src, int = 64)
+{
+    unsigned long long b;
+    memcpy(&b, &v, 8);
+    b = emu_exchange(b, src);
+    double r;
+    memcpy(&r, &b, 8);
+    return r;
+}
+inline int __shfl(int v, int src, int = 64) { return (int)emu_exchange((unsigned long long)(unsigned)v, src); }
+inline double __shfl_xor(double v, int m, int = 64) { return __shfl(v, (int)(emu_tid & 63) ^ m); }
+inline int __shfl_xor(int v, int m, int = 64) { return __shfl(v, (int)(emu_tid & 63) ^ m); }
+inline int __builtin_amdgcn_readfirstlane(int v) { return __shfl(v, 0); }
+inline void __builtin_amdgcn_wave_barrier() {}
+inline void __syncthreads()
+{
+    if (emu_bdim > 64) {
+        fprintf(stderr, "emulation: __syncthreads in a block of more than one wave\n");
+        abort();
+    }
+    emu_yield();
+}
+
+inline int atomicAdd(int *p, int v) { const int o = *p; *p = o + v; return o; }
+inline int atomicOr(int *p, int v) { const int o = *p; *p = o | v; return o; }
+inline double atomicAdd(double *p, double v) { const double o = *p; *p = o + v; return o; }
+
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorNotSupported = 801 };
+typedef void *hipStream_t;
+typedef void *hipEvent_t;
+typedef void *hipGraph_t;
+typedef void *hipGraphExec_t;
+enum hipMemcpyKind { hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
+enum { hipDeviceAttributeMultiprocessorCount = 1 };
+enum { hipStreamCaptureModeThreadLocal = 1 };
+enum hipMemoryType { hipMemoryTypeHost, hipMemoryTypeDevice, hipMemoryTypeManaged };
+struct hipPointerAttribute_t { hipMemoryType type; };
+#define HIP_SYMBOL(x) (&(x))
+extern int emu_cus, emu_per_cu;
+template <class T>
+inline hipError_t hipMalloc(T **p, size_t n)
+{
+    *p = (T *)malloc(n ? n : 1);
+    memset((void *)*p, 0xA5, n);  // not zero: kernels must not rely on fresh memory
+    return hipSuccess;
+}
+inline hipError_t hipFree(void *p) { free(p); return hipSuccess; }
+inline hipError_t hipMemcpy(void *d, const void *s, size_t n, int) { memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipMemset(void *d, int v, size_t n) { memset(d, v, n); return hipSuccess; }
+inline hipError_t hipMemcpyToSymbol(const void *sym, const void *s, size_t n) { memcpy((void *)sym, s, n); return hipSuccess; }
+inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+inline hipError_t hipDeviceGetAttribute(int *v, int, int) { *v = emu_cus; return hipSuccess; }
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, const void *, int, size_t)
+{
+    *n = emu_per_cu;
+    return hipSuccess;
+}
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline const char *hipGetErrorString(hipError_t) { return "emulated"; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamBeginCapture(hipStream_t, int) { return hipErrorNotSupported; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *) { return hipErrorNotSupported; }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, void *, void *, size_t) { return hipErrorNotSupported; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorNotSupported; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t *) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 1; return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipPointerGetAttributes(hipPointerAttribute_t *a, const void *) { a->type = hipMemoryTypeDevice; return hipSuccess; }
+
+constexpr size_t kEmuStack = 256 * 1024;
+
+extern "C" void emu_fiber_entry();
+
+// Runs body(arg) on the 64 lanes of one wave, lane l as threadIdx.x = base + l.
+inline void emu_run_wave(unsigned base, void (*body)(void *), void *arg)
+{
+    static EmuWaveSched S;
+    if (S.stacks.empty()) S.stacks.resize(64 * kEmuStack);
+    S.base = base;
+    S.body = body;
+    S.arg = arg;
+    for (int l = 0; l < 64; l++) {
+        S.done[l] = false;
+        uintptr_t top = ((uintptr_t)(S.stacks.data() + (l + 1) * kEmuStack)) & ~(uintptr_t)15;
+        void **sp = (void **)top;
+        *--sp = nullptr;                      // return address of the entry (never used)
+        *--sp = (void *)&emu_fiber_entry;     // where the first switch returns to
+        for (int r = 0; r < 6; r++) *--sp = nullptr;  // rbp rbx r12 r13 r14 r15
+        S.sp[l] = sp;
+    }
+    emu_wave = &S;
+    for (;;) {
+        int yielded = 0, finished = 0;
+        for (int l = 0; l < 64; l++) {
+            if (S.done[l]) continue;
+            S.cur = l;
+            emu_tid = base + l;
+            emu_ctx_switch(&S.main_sp, S.sp[l]);
+            if (S.done[l]) finished++;
+            else yielded++;
+        }
+        if (yielded == 0) break;
+        if (finished) {
+            fprintf(stderr, "emulation: lanes of wave %u diverged at a wavefront operation\n", base / 64);
+            abort();
+        }
+        for (int l = 0; l < 64; l++)
+            if (S.done[l]) {
+                fprintf(stderr, "emulation: lanes of wave %u diverged at a wavefront operation\n", base / 64);
+                abort();
+            }
+    }
+}
+
+template <typename F>
+void emu_call(void *p) { (*(F *)p)(); }
+
+template <typename... KA, typename... A>
+void emu_launch(void (*k)(KA...), dim3 g, dim3 b, size_t, hipStream_t, A... args)
+{
+    if (b.x % 64) {
+        fprintf(stderr, "emulation: block size %u is not a multiple of 64\n", b.x);
+        abort();
+    }
+    emu_bdim = b.x;
+    emu_gdim = g.x;
+    auto body = [&]() { k(args...); };
+    for (unsigned bx = 0; bx < g.x; bx++) {
+        emu_bid = bx;
+        for (unsigned w = 0; w < b.x / 64; w++) emu_run_wave(w * 64, &emu_call<decltype(body)>, &body);
+    }
+}
+#define hipLaunchKernelGGL(k, g, b, sh, st, ...) emu_launch(k, dim3(g), dim3(b), sh, st, ##__VA_ARGS__)

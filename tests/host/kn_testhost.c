@@ -35,6 +35,29 @@ double hubble_function(double a)
     return h_Hubble * sqrt(omega_tot);
 }
 
+/* Per-history form of the same H(a) for batched solves: ctx is a kn_test_hctx. */
+typedef struct {
+    const _omega_nu *omnu;
+    double Omega_nonu, OmegaLambda, Hubble;
+} kn_test_hctx;
+
+void kn_test_hctx_init(kn_test_hctx *c, const _omega_nu *omnu, double Omega0, double UnitTime_in_s)
+{
+    c->omnu = omnu;
+    c->Omega_nonu = Omega0 - get_omega_nu(omnu, 1);
+    c->OmegaLambda = 1 - Omega0;
+    c->Hubble = 3.24077929e-18 * UnitTime_in_s;
+}
+
+double kn_test_hubble_ctx(double a, void *ctx)
+{
+    const kn_test_hctx *c = (const kn_test_hctx *)ctx;
+    double omega_tot = c->Omega_nonu / pow(a, 3) + c->OmegaLambda;
+    omega_tot += get_omega_nu(c->omnu, a);
+    omega_tot += get_omegag(c->omnu, a);
+    return c->Hubble * sqrt(omega_tot);
+}
+
 long long kn_test_hubble_calls(void) { return n_hubble_calls; }
 long long kn_test_messages(void) { return n_messages; }
 

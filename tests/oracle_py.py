@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libkn_oracle.so")
 dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)
 llp = C.POINTER(C.c_longlong)
+QAG_LIMIT = 200  # GSL_VAL: QAG subinterval limit (kspace_neutrino_const.h)
 
 
 class OrcRhoNu(C.Structure):
@@ -88,6 +89,11 @@ def lib():
             "orc_get_dnudcdm": (C.c_double, [C.POINTER(OrcDeltaPow), C.c_double]),
             "orc_rescale": (C.c_int, [C.c_int, dp, C.c_int, C.c_int, C.c_double, C.POINTER(OrcDeltaPow)]),
             "orc_kspace_prefac": (C.c_double, [C.POINTER(OrcDtt), C.c_double]),
+            "orc_get_delta_nu_record": (C.c_int, [C.POINTER(OrcDtt), C.c_double, dp, dp, C.c_double, dp, ip]),
+            "orc_dnu_frozen": (C.c_int, [C.POINTER(OrcDtt), C.c_double, dp, dp, C.c_double, dp, ip]),
+            "orc_get_delta_nu_adjoint": (C.c_int, [C.POINTER(OrcDtt), C.c_double, dp, C.c_double, dp, dp, dp, dp,
+                                                   dp]),
+            "orc_specialJ_dmu": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -175,6 +181,35 @@ class Table:
         w = np.ascontiguousarray(wavenum, dtype=np.float64)
         rc = lib().orc_get_delta_nu_combined(C.byref(self.d), a, ptr(w), ptr(out))
         return rc, out
+
+    def get_delta_nu_record(self, a, wavenum, mnu):
+        """Forward plus each k's final QAG subdivision (ivl[k, t] = (lo, hi), nint[k])."""
+        nk = self.d.nk
+        out = np.zeros(nk)
+        ivl = np.zeros((nk, QAG_LIMIT, 2))
+        nint = np.zeros(nk, dtype=np.int32)
+        w = np.ascontiguousarray(wavenum, dtype=np.float64)
+        rc = lib().orc_get_delta_nu_record(C.byref(self.d), a, ptr(w), ptr(out), mnu, ptr(ivl),
+                                           nint.ctypes.data_as(ip))
+        return rc, out, ivl, nint
+
+    def dnu_frozen(self, a, wavenum, mnu, ivl, nint):
+        out = np.zeros(self.d.nk)
+        w = np.ascontiguousarray(wavenum, dtype=np.float64)
+        rc = lib().orc_dnu_frozen(C.byref(self.d), a, ptr(w), ptr(out), mnu, ptr(ivl), nint.ctypes.data_as(ip))
+        return rc, out
+
+    def get_delta_nu_adjoint(self, a, wavenum, mnu, gbar):
+        """(rc, delta_nu, dtot_bar[nk, Na], init_bar[nk], mnu_bar)."""
+        nk, na = self.d.nk, self.d.ia
+        out, ib = np.zeros(nk), np.zeros(nk)
+        db = np.zeros((nk, na))
+        mb = C.c_double()
+        w = np.ascontiguousarray(wavenum, dtype=np.float64)
+        g = np.ascontiguousarray(gbar, dtype=np.float64)
+        rc = lib().orc_get_delta_nu_adjoint(C.byref(self.d), a, ptr(w), mnu, ptr(g), ptr(out), ptr(db), ptr(ib),
+                                            C.byref(mb))
+        return rc, out, db, ib, mb.value
 
     def delta_tot_init(self, wavenum, delta_cdm, transfer, time):
         w = np.ascontiguousarray(wavenum, dtype=np.float64)
