@@ -25,6 +25,12 @@ sys.path[:0] = [os.path.join(ROOT, "kspace-neutrinos_amd"), os.path.join(ROOT, "
 
 HBM_PEAK_GBS = 8000.0
 DP = C.POINTER(C.c_double)
+# HBM bytes per k_rescale dispatch at 1024^3, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch) scaled by what the same counters read for a 2 GiB
+# device copy (FETCH_SIZE x2.0000, WRITE_SIZE x1.0000); commands and tables in
+# profiles/bin_rescale_1024_hbm_counters.txt.
+RESCALE_1024_FETCH_KIB, RESCALE_1024_WRITE_KIB = 4202627.3125, 8404992.125
+FETCH_SCALE, WRITE_SCALE = 2.0000, 1.0000
 
 
 def bbks_amplitude(N, box, rms=1e-2, ns=0.96, gamma=0.2 / 1000.0):
@@ -63,6 +69,45 @@ def cpu_baseline(N, nrbins, box, d_grid, torch, nslab):
                       % (nslab, N, N, modes, t1 - t0, t2 - t1)}
 
 
+def dnu_bench(kn, nk, na=2000, calls=3, cpu_k=16, no_cpu=False):
+    """get_delta_nu_combined (the per-step free-streaming integral) on a hybrid
+    history of the 1024^3 step: nk bins, na stored steps over a in [0.01, 1]
+    with delta_tot = a sqrt(P(k)),
+    masses 0.06/0.06/0.3 eV with particles from a = 0.3333 (vcrit 500 km/s), so
+    two distinct masses: 2 nk QAG integrals per call.  Plain path: one C-ABI call,
+    inputs on the host as the reference's caller holds them."""
+    from dnu_common import Pair, synthetic_history
+    P = Pair([0.06, 0.06, 0.3], 0.3, nk, 0.01 + na / 100.0 + 0.02, hybrid=(500.0, 0.3333))
+    logs, rows, k, init = synthetic_history(na, nk, 1.0, noise=0.0)  # delta_tot = a sqrt(P(k))
+    P.set_history(logs, rows, init)
+    out = np.zeros(nk)
+    kk = np.ascontiguousarray(k)
+    kn.get_delta_nu_combined(C.byref(P.d), 1.0, kk.ctypes.data_as(DP), out.ctypes.data_as(DP))  # warm-up
+    t = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        kn.get_delta_nu_combined(C.byref(P.d), 1.0, kk.ctypes.data_as(DP), out.ctypes.data_as(DP))
+        t.append(time.perf_counter() - t0)
+    tg = float(np.median(t))
+    ksteps = 2.0 * nk * na
+    res = {"workload": "get_delta_nu_combined, hybrid 0.06/0.06/0.3 eV, %d k x %d steps, a=1" % (nk, na),
+           "ms_per_call": round(tg * 1e3, 3), "kstep_per_s": round(ksteps / tg, 1),
+           "note": "k.step = one (k bin, stored step) pair per distinct mass; plain C-ABI call, host in/out"}
+    if not no_cpu:
+        sel = np.linspace(0, nk - 1, cpu_k).astype(int)
+        Q = Pair([0.06, 0.06, 0.3], 0.3, cpu_k, 0.01 + na / 100.0 + 0.02, hybrid=(500.0, 0.3333))
+        Q.set_history(logs, rows[sel], init[sel])
+        t0 = time.perf_counter()
+        rc, o = Q.T.get_delta_nu_combined(1.0, k[sel])
+        tc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"kstep_per_s": round(2.0 * cpu_k * na / tc, 1), "cores": 1, "kind": "port",
+                               "sample": "%d of the %d k bins, oracle get_delta_nu_combined %.2fs" % (cpu_k, nk, tc)}
+        res["max_rel_err_vs_oracle"] = float(np.max(np.abs(out[sel] / o - 1)))
+        Q.free()
+    P.free()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,12 +117,24 @@ def main():
     ap.add_argument("--nrbins", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slabs", type=int, default=192)
+    ap.add_argument("--no-dnu", action="store_true", help="skip the free-streaming integral timing")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 2 GiB (reference for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the hot path here is the single-GPU kernel set; "
-              "multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
+              "this build has no multi-GPU run (the slab decomposition is a design note).", file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.copy_calibration:
+        nbytes = 2 << 30
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}))
+        return
     import conftest
     conftest.load_testhost()
     import knhip
@@ -141,8 +198,10 @@ def main():
         "config": {"workload": "bin+rescale of a %d^3 r2c FP64 Gaussian field (BBKS, 512 Mpc/h), %d log k-bins"
                                % (N, nrbins), "grid_modes": modes, "nrbins": nrbins},
         "roofline": {"bound": "hbm", "achieved": round(res_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(res_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_rescale (32 B/mode algorithmic)"},
+                     "frac": round(res_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": (round(1024 * (RESCALE_1024_FETCH_KIB * FETCH_SCALE + RESCALE_1024_WRITE_KIB * WRITE_SCALE))
+                                 if N == 1024 else None),
+                     "kernel": "k_rescale_rows_pf (32 B/mode algorithmic, %d modes per launch)" % modes},
         "bin_sweep": {"ms": round(t_bin.mean() * 1e3, 4), "GB/s": round(bin_gbs, 1),
                       "frac": round(bin_gbs / HBM_PEAK_GBS, 4)},
         "rescale_sweep": {"ms": round(t_res.mean() * 1e3, 4), "GB/s": round(res_gbs, 1)},
@@ -150,6 +209,10 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, nrbins, box, grid, torch, min(args.cpu_slabs, N))
+    if not args.no_dnu:
+        del grid
+        torch.cuda.empty_cache()
+        out["delta_nu"] = dnu_bench(kn, n, no_cpu=args.no_cpu)
     print(json.dumps(out))
 
 

@@ -252,6 +252,16 @@ int kn_rescale_device(fftw_complex *d_field, int dims, int startslab, int nslab,
 /* Last GPU error text (empty if none). */
 const char *kn_last_error(void);
 
+/* ---- adjoint of (a): d/dF of total_powerspectrum's output (powerspectrum.c:33-117) ----
+ * gbar[nout] = dL/dpower[n] for the nout compacted bins the forward returns on
+ * the same field (host array).  d_field_bar (device, the field's layout)
+ * receives dL/dRe F + i dL/dIm F:
+ *   2 c_m W_m^-4 gbar_b F_m / (|F_0|^2 N_b) for m != 0 (c_m the multiplicity),
+ *   -2 F_0 / |F_0|^2 sum_n gbar_n power_n   for the zero mode.
+ * Needs the zero mode (startslab 0).  Synchronous on stream. */
+int kn_bin_power_adjoint(const fftw_complex *d_field, int dims, int nrbins, int startslab, int nslab,
+                         const double *gbar, int nout, fftw_complex *d_field_bar, void *stream);
+
 /* ---- (b) batched: many get_delta_nu integrals in one launch set ----
  * One history is what one get_delta_nu call integrates
  * (delta_tot_table.c:507-611): its stored log a values and delta_tot rows,

@@ -53,6 +53,8 @@ struct BinGeom {
     double *d_partial;      // max(grid, ggrid) * nrbins
     unsigned *d_cpartial;   // ggrid * nrbins (geometry pass)
     double *d_sum;          // nrbins + 1 (last: |F0|^2)
+    double *d_alpha;        // nrbins, adjoint coefficients (allocated on first use)
+    int agrid;              // adjoint-sweep grid
 };
 
 std::vector<BinGeom *> g_geoms;
@@ -263,6 +265,74 @@ void launch_power(BinGeom *g, const double2 *F, hipStream_t st)
     KN_CHECK(hipGetLastError());
 }
 
+// Adjoint sweep of the binning: Fbar_m = s_m F_m with s_m = 2 c_m W_m^-4
+// alpha[bin(m)] (c_m the multiplicity) and s_0 = beta for the zero mode.
+// Reads 16 B and writes 16 B per mode; software-pipelined like the rescale sweep.
+constexpr int kAdjWaves = 8;
+constexpr int kAdjUnroll = 2;
+
+__global__ __launch_bounds__(64 * kAdjWaves) void k_bin_adjoint(const double2 *__restrict__ F,
+                                                                double2 *__restrict__ Fb, int dims, int startslab,
+                                                                long long rows, int nrbins,
+                                                                const int *__restrict__ thr_g,
+                                                                const double *__restrict__ iw_g,
+                                                                const double *__restrict__ alpha_g, float half_bpu,
+                                                                double beta)
+{
+    extern __shared__ double smem[];
+    double *alpha = smem;
+    double *iw = alpha + nrbins;
+    int *thr = (int *)(iw + dims);
+    constexpr int B = 64 * kAdjWaves;
+    for (int t = threadIdx.x; t < nrbins; t += B) alpha[t] = alpha_g[t];
+    for (int t = threadIdx.x; t < dims; t += B) iw[t] = iw_g[t];
+    for (int t = threadIdx.x; t <= nrbins; t += B) thr[t] = thr_g[t];
+    __syncthreads();
+    const int rowlen = dims / 2 + 1, half = dims / 2;
+    const long long r0 = rows * blockIdx.x / gridDim.x, r1 = rows * (blockIdx.x + 1) / gridDim.x;
+    const long long m1 = r1 * rowlen;
+    constexpr long long S = (long long)B * kAdjUnroll;
+    Pos p = kn_bin::pos_of(threadIdx.x, r0, rowlen, dims);
+    long long m = r0 * rowlen + threadIdx.x;
+    double2 cur[kAdjUnroll];
+#pragma unroll
+    for (int u = 0; u < kAdjUnroll; u++) {
+        const long long mu = m + (long long)B * u;
+        cur[u] = mu < m1 ? F[mu] : make_double2(0.0, 0.0);
+    }
+    for (; m < m1; m += S) {
+        double2 nxt[kAdjUnroll];
+#pragma unroll
+        for (int u = 0; u < kAdjUnroll; u++) {
+            const long long mu = m + S + (long long)B * u;
+            nxt[u] = mu < m1 ? F[mu] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < kAdjUnroll; u++) {
+            const long long mu = m + (long long)B * u;
+            const Pos q = p;
+            advance(p, B, rowlen, dims);
+            if (mu >= m1) continue;
+            const int kx = kval(startslab + q.i, dims), ky = kval(q.j, dims);
+            const int k2 = kx * kx + ky * ky + q.z * q.z;
+            double sc;
+            if (k2 == 0) {
+                sc = beta;
+            } else {
+                const int b = find_bin(k2, half_bpu, thr, nrbins);
+                const double w = iw[startslab + q.i] * iw[q.j] * iw[q.z];
+                const double w2 = w * w;
+                sc = ((q.z != 0 && q.z != half) ? 4.0 : 2.0) * (w2 * w2) * alpha[b];
+            }
+            Fb[mu] = make_double2(sc * cur[u].x, sc * cur[u].y);
+        }
+#pragma unroll
+        for (int u = 0; u < kAdjUnroll; u++) cur[u] = nxt[u];
+    }
+}
+
+size_t adjoint_smem(int dims, int nrbins) { return sizeof(double) * (size_t)(nrbins + dims) + sizeof(int) * (nrbins + 1); }
+
 }  // namespace
 
 extern "C" int kn_bin_power_device(const fftw_complex *d_field, int dims, int nrbins, int startslab, int nslab,
@@ -346,4 +416,54 @@ extern "C" int total_powerspectrum(const int dims, fftw_complex *outfield, const
     memcpy(count, g->count.data(), sizeof(long long) * nrbins);
     memcpy(keffs, g->keff_sum.data(), sizeof(double) * nrbins);
     return kn_finish_powerspectrum(nrbins, sums[nrbins], power, count, keffs);
+}
+
+// Adjoint of total_powerspectrum with respect to the field (device arrays):
+// gbar[nout] = dL/dpower[n] for the nout compacted bins of the forward on the
+// same field (host array); d_field_bar receives dL/dF in the field's layout
+// (Re and Im parts: dL/dRe F + i dL/dIm F).  Needs the zero mode (startslab 0).
+extern "C" int kn_bin_power_adjoint(const fftw_complex *d_field, int dims, int nrbins, int startslab, int nslab,
+                                    const double *gbar, int nout, fftw_complex *d_field_bar, void *stream)
+{
+    if (startslab != 0 || !gbar || !d_field || !d_field_bar) {
+        kn_set_last_error("kn_bin_power_adjoint: needs startslab 0 (the zero mode) and non-null arrays");
+        return KN_ERR_ARG;
+    }
+    BinGeom *g = get_geom(dims, nrbins, startslab, nslab);
+    hipStream_t st = (hipStream_t)stream;
+    launch_power(g, (const double2 *)d_field, st);
+    std::vector<double> sums(nrbins + 1), alpha(nrbins, 0.0);
+    KN_CHECK(hipMemcpyAsync(sums.data(), g->d_sum, sizeof(double) * (nrbins + 1), hipMemcpyDeviceToHost, st));
+    KN_CHECK(hipStreamSynchronize(st));
+    const double M = sums[nrbins];
+    double gp = 0.0;
+    int n = 0;
+    for (int b = 0; b < nrbins; b++) {
+        if (!g->count[b]) continue;
+        if (n >= nout) break;
+        const double P = sums[b] / M / (double)g->count[b];
+        alpha[b] = gbar[n] / (M * (double)g->count[b]);
+        gp += gbar[n] * P;
+        n++;
+    }
+    int nonzero = 0;
+    for (int b = 0; b < nrbins; b++) nonzero += g->count[b] ? 1 : 0;
+    if (nonzero != nout) {
+        kn_set_last_error("kn_bin_power_adjoint: nout=%d but the forward has %d non-empty bins", nout, nonzero);
+        return KN_ERR_ARG;
+    }
+    if (!g->d_alpha) {
+        KN_CHECK(hipMalloc(&g->d_alpha, sizeof(double) * nrbins));
+        const long long rows = (long long)nslab * dims;
+        const long long ag = (long long)resident_blocks(k_bin_adjoint, 64 * kAdjWaves, adjoint_smem(dims, nrbins)) *
+                             num_cus();
+        g->agrid = (int)(rows < ag ? rows : ag);
+    }
+    KN_CHECK(hipMemcpyAsync(g->d_alpha, alpha.data(), sizeof(double) * nrbins, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_bin_adjoint, dim3(g->agrid), dim3(64 * kAdjWaves), adjoint_smem(dims, nrbins), st,
+                       (const double2 *)d_field, (double2 *)d_field_bar, dims, startslab, (long long)nslab * dims,
+                       nrbins, g->d_thr, g->d_iw, g->d_alpha, g->half_bpu, -2.0 * gp / M);
+    KN_CHECK(hipGetLastError());
+    KN_CHECK(hipStreamSynchronize(st));
+    return 0;
 }

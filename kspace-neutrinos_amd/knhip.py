@@ -113,6 +113,8 @@ SIGNATURES = {
     "kn_rescale_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                     C.c_int, C.c_double, C.c_void_p]),
     "kn_last_error": (C.c_char_p, []),
+    "kn_bin_power_adjoint": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_void_p,
+                                       C.c_void_p]),
     "kn_batch_create": (C.c_int, [vpp, C.c_int, C.POINTER(BatchHistory)]),
     "kn_batch_destroy": (None, [C.c_void_p]),
     "kn_batch_views": (C.c_int, [C.c_void_p] + [vpp] * 9),
@@ -294,6 +296,40 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def power_spectrum_function():
+    """torch.autograd.Function over total_powerspectrum on a device grid:
+    apply(field[dims, dims, dims/2+1, 2] float64 cuda, nrbins) -> power[n]
+    (the compacted, normalised bins); the gradient comes from
+    kn_bin_power_adjoint."""
+    import numpy as np
+    import torch
+
+    class PowerSpectrumFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, field, nrbins):
+            assert field.is_cuda and field.dtype == torch.float64 and field.is_contiguous()
+            dims = field.shape[0]
+            power, keffs = np.zeros(nrbins), np.zeros(nrbins)
+            count = np.zeros(nrbins, dtype=np.int64)
+            n = lib().total_powerspectrum(dims, field.data_ptr(), nrbins, 0, dims, power.ctypes.data_as(dp),
+                                          count.ctypes.data_as(llp), keffs.ctypes.data_as(dp), 0)
+            ctx.save_for_backward(field)
+            ctx.nrbins, ctx.n = nrbins, n
+            return torch.from_numpy(power[:n].copy()).to(field.device)
+
+        @staticmethod
+        def backward(ctx, g):
+            (field,) = ctx.saved_tensors
+            gb = np.ascontiguousarray(g.detach().cpu().numpy(), dtype=np.float64)
+            fb = torch.empty_like(field)
+            _check(lib().kn_bin_power_adjoint(field.data_ptr(), field.shape[0], ctx.nrbins, 0, field.shape[0],
+                                              gb.ctypes.data_as(dp), ctx.n, fb.data_ptr(), None),
+                   "kn_bin_power_adjoint")
+            return fb, None
+
+    return PowerSpectrumFunction
 
 
 def delta_nu_function():

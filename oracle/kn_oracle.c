@@ -1370,3 +1370,68 @@ int orc_get_delta_nu_adjoint(orc_dtt *d, double a, const double wavenum[], doubl
     free(nint);
     return rc;
 }
+
+/* ======================= adjoint of total_powerspectrum (test infrastructure) =======================
+ *
+ * L = sum_n gbar[n] power[n] over the nout compacted bins of
+ * orc_total_powerspectrum on the same field.  With S_b = sum over the bin's
+ * modes of c_m w_m |F_m|^2 (c_m the multiplicity, w_m = invwindow^2),
+ * M = |F_0|^2 and N_b the bin count, power_b = S_b / (M N_b), so
+ *   dL/dF_m = 2 c_m w_m gbar_b F_m / (M N_b)      (m != 0; Re and Im parts),
+ *   dL/dF_0 = -2 F_0 / M * sum_n gbar_n power_n    (startslab == 0).
+ * field_bar uses the field's layout (re, im interleaved).  Returns 0, or -1
+ * when nout is not the forward's number of non-empty bins.  Pinned by central
+ * differences of the forward (quadratic in F) in tests/test_oracle_pins.py. */
+#define KVALD(n, d) ((n) <= (d) / 2 ? (n) : ((n) - (d)))
+int orc_bin_power_adjoint(int dims, const double *field, int nrbins, int startslab, int nslab, const double *gbar,
+                          int nout, double *field_bar)
+{
+    double *power = malloc(sizeof(double) * (size_t)nrbins), *keffs = malloc(sizeof(double) * (size_t)nrbins);
+    double *alpha = calloc((size_t)nrbins, sizeof(double));
+    long long *count = malloc(sizeof(long long) * (size_t)nrbins), *rawcount = calloc((size_t)nrbins, sizeof(long long));
+    const double binsperunit = (nrbins - 1) / log(sqrt(3) * dims / 2.0);
+    const size_t rowlen = (size_t)(dims / 2 + 1);
+    double M = 0, gp = 0;
+    int n, b, i;
+    if (startslab == 0) M = field[0] * field[0] + field[1] * field[1];
+    n = orc_total_powerspectrum(dims, field, nrbins, startslab, nslab, power, count, keffs);
+    if (n != nout) {
+        free(power); free(keffs); free(alpha); free(count); free(rawcount);
+        return -1;
+    }
+    /* raw bin counts, to map compacted outputs back to bins */
+    for (i = startslab; i < startslab + nslab; i++) {
+        int j, k;
+        for (j = 0; j < dims; j++)
+            for (k = 0; k <= dims / 2; k++) {
+                const double kk = sqrt(pow(KVALD(i, dims), 2) + pow(KVALD(j, dims), 2) + pow(KVALD(k, dims), 2));
+                if (kk > 0) rawcount[(int)floor(binsperunit * log(kk))] += (k == 0 || k == dims / 2) ? 1 : 2;
+            }
+    }
+    for (b = 0, n = 0; b < nrbins; b++)
+        if (rawcount[b]) {
+            alpha[b] = gbar[n] / (M * (double)rawcount[b]);
+            gp += gbar[n] * power[n];
+            n++;
+        }
+    for (i = startslab; i < startslab + nslab; i++) {
+        int j, k;
+        for (j = 0; j < dims; j++)
+            for (k = 0; k <= dims / 2; k++) {
+                const size_t idx = ((size_t)(i - startslab) * (size_t)dims + (size_t)j) * rowlen + (size_t)k;
+                const double kk = sqrt(pow(KVALD(i, dims), 2) + pow(KVALD(j, dims), 2) + pow(KVALD(k, dims), 2));
+                double s;
+                if (kk > 0) {
+                    const int c = (k == 0 || k == dims / 2) ? 1 : 2;
+                    const double w = invwindow(KVALD(i, dims), KVALD(j, dims), KVALD(k, dims), dims);
+                    s = 2.0 * c * w * w * alpha[(int)floor(binsperunit * log(kk))];
+                } else {
+                    s = -2.0 * gp / M;
+                }
+                field_bar[2 * idx] = s * field[2 * idx];
+                field_bar[2 * idx + 1] = s * field[2 * idx + 1];
+            }
+    }
+    free(power); free(keffs); free(alpha); free(count); free(rawcount);
+    return 0;
+}

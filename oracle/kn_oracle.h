@@ -143,6 +143,10 @@ double orc_get_dnudcdm(const orc_delta_pow *p, double logk);
 /* In-place rescale of the slab by 1 + get_dnudcdm(log(|k| 2 pi / L)); returns 5 on NaN. */
 int orc_rescale(int dims, double *field, int startslab, int nslab, double BoxSize, const orc_delta_pow *p);
 
+/* Adjoint of orc_total_powerspectrum: field_bar = dL/dF for L = sum_n gbar[n] power[n]. */
+int orc_bin_power_adjoint(int dims, const double *field, int nrbins, int startslab, int nslab, const double *gbar,
+                          int nout, double *field_bar);
+
 /* kspace_prefac of interface_common.c:139-145 */
 double orc_kspace_prefac(const orc_dtt *d, double Time);
 

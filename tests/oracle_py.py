@@ -94,6 +94,7 @@ def lib():
             "orc_get_delta_nu_adjoint": (C.c_int, [C.POINTER(OrcDtt), C.c_double, dp, C.c_double, dp, dp, dp, dp,
                                                    dp]),
             "orc_specialJ_dmu": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double]),
+            "orc_bin_power_adjoint": (C.c_int, [C.c_int, dp, C.c_int, C.c_int, C.c_int, dp, C.c_int, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -128,6 +129,16 @@ def total_powerspectrum(field_c, nrbins, startslab=0, nslab=None):
     n = lib().orc_total_powerspectrum(dims, ptr(f), nrbins, startslab, nslab, ptr(power),
                                       count.ctypes.data_as(llp), ptr(keffs))
     return n, power, count, keffs
+
+
+def bin_power_adjoint(field_c, nrbins, gbar, startslab=0):
+    """dL/dF (complex array like field_c) for L = sum_n gbar[n] power[n]."""
+    dims = field_c.shape[1]
+    f = np.ascontiguousarray(field_c).view(np.float64).ravel()
+    g = np.ascontiguousarray(gbar, dtype=np.float64)
+    out = np.zeros_like(f)
+    rc = lib().orc_bin_power_adjoint(dims, ptr(f), nrbins, startslab, field_c.shape[0], ptr(g), len(g), ptr(out))
+    return rc, out.view(np.complex128).reshape(field_c.shape)
 
 
 def rescale(field_c, boxsize, logkk, ratio, norm, startslab=0):

@@ -100,7 +100,7 @@ def test_synth_grid_binning_256(kn, torch_cuda):
 
 @pytest.mark.parametrize("N,nrbins", [(512, 300), (1024, 1024)])
 def test_binning_full_size_invariants(kn, torch_cuda, N, nrbins):
-    """Full BASELINE sizes: counts sum to N^3 - 1, and sum_b P_b N_b |F0|^2 equals
+    """Full sizes (512^3, 1024^3): counts sum to N^3 - 1, and sum_b P_b N_b |F0|^2 equals
     sum over modes of multiplicity W^-4 |F|^2 computed by an independent torch reduction."""
     torch = torch_cuda
     buf = torch.empty((N, N, N // 2 + 1), dtype=torch.complex128, device="cuda")

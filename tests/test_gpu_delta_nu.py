@@ -1,7 +1,7 @@
 """(b) get_delta_nu and its update/init flows on the GPU, through the C ABI,
 against the oracle's restatement of the reference QAG procedure.
 
-Bar: 1e-9 relative (the north-star bound).  The GPU replays the reference's
+Bar: 1e-9 relative.  The GPU replays the reference's
 own interpolants and QAG decisions, so agreement is typically ~1e-13; the
 max error against the oracle is printed per case."""
 import ctypes as C

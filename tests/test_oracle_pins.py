@@ -310,6 +310,39 @@ def test_powerspectrum_single_mode_and_slabs():
     assert ca[:n].sum() + cb.sum() == c[:n].sum()
 
 
+def test_powerspectrum_adjoint_central_differences():
+    """dL/dF of L = sum_n g_n P_n: P is quadratic in each F_m over |F_0|^2, so
+    central differences in Re/Im of single entries are exact up to rounding.
+    Entries: the zero mode, the kz = 0 and kz = N/2 planes, interior modes."""
+    N, nrbins = 8, 12
+    rng = np.random.default_rng(3)
+    F = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+    F[0, 0, 0] = N ** 3 * (1 + 0.3j)
+    n, p, c, k = O.total_powerspectrum(F, nrbins)
+    g = rng.uniform(-1, 1, n)
+    rc, Fb = O.bin_power_adjoint(F, nrbins, g)
+    assert rc == 0
+    L = lambda G: float(np.dot(g, O.total_powerspectrum(G, nrbins)[1][:n]))
+    for idx in [(0, 0, 0), (0, 1, 0), (3, 5, 0), (2, 7, N // 2), (1, 2, 3), (7, 7, 1), (4, 0, 2)]:
+        for part, unit in [("re", 1.0), ("im", 1j)]:
+            h = 1e-3 * max(1.0, abs(F[idx]))
+
+            def D(h):
+                Fp, Fm = F.copy(), F.copy()
+                Fp[idx] += h * unit
+                Fm[idx] -= h * unit
+                return (L(Fp) - L(Fm)) / (2 * h)
+            # exact for the quadratic modes; the zero mode enters through 1/|F_0|^2: Richardson
+            fd = D(h) if idx != (0, 0, 0) else (4 * D(h / 2) - D(h)) / 3
+            an = Fb[idx].real if part == "re" else Fb[idx].imag
+            assert fd == pytest.approx(an, rel=1e-7, abs=1e-12 * abs(L(F)) / h), (idx, part)
+    # P is homogeneous of degree 0 in F: sum_m Re(conj(F_m) dL/dF_m) = 0, the
+    # zero mode balancing the others (-2 sum g P against +2 sum g P)
+    dots = (F.conj() * Fb).real
+    assert dots[0, 0, 0] == pytest.approx(-2 * np.dot(g, p[:n]), rel=1e-12)
+    assert abs(dots.sum()) < 1e-12 * abs(dots[0, 0, 0])
+
+
 # ----------------------------------------------------------------- delta_pow
 
 def _read_powerspec_fixtures():
