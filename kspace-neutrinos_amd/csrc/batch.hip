@@ -1010,9 +1010,10 @@ extern "C" int kn_batch_history_from_table(const _delta_tot_table *d, double a, 
     return 0;
 }
 
-extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, const double *wavenum, double mnu,
-                                       const double *gbar, double *delta_nu, double *dtot_bar, double *init_bar,
-                                       double *mnu_bar)
+// kn_get_delta_nu_adjoint computed through a one-history batch (same arguments).
+extern "C" int kn_batch_get_delta_nu_adjoint(const _delta_tot_table *d, double a, const double *wavenum, double mnu,
+                                             const double *gbar, double *delta_nu, double *dtot_bar, double *init_bar,
+                                             double *mnu_bar)
 {
     kn_batch_history hh;
     int rc = kn_batch_history_from_table(d, a, mnu, &hh);

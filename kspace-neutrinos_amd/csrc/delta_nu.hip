@@ -146,7 +146,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_dnu_qag(
     const double *__restrict__ sx, const double *__restrict__ Y, const double *__restrict__ C, const int *__restrict__ sb,
     int ns, int nsb, double sinv_bw, const double *__restrict__ wavenum, int nk, const SpeciesDev *__restrict__ sp,
     const int *__restrict__ item_species, int nitems, double x0, double xa, double *__restrict__ out,
-    int *__restrict__ nint_out, int *__restrict__ status)
+    int *__restrict__ nint_out, int *__restrict__ status, double *__restrict__ ivl)
 {
     __shared__ QagWs ws[kWavesPerBlock];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -177,10 +177,112 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_dnu_qag(
     double r;
     int ni;
     const int rc = wave_qag61(f, x0, xa, sp[s].relerr, kLimit, ws[wave], &r, &ni);
+    if (ivl)  // the final subdivision, for the adjoint
+        for (int t = lane; t < ni; t += 64) {
+            ivl[((size_t)item * kLimit + t) * 2] = ws[wave].lo[t];
+            ivl[((size_t)item * kLimit + t) * 2 + 1] = ws[wave].hi[t];
+        }
     if (lane == 0) {
         out[item] = r;
         nint_out[item] = ni;
         if (rc) atomicOr(status, 2);
+    }
+}
+
+// Adjoint of one item's integral with its recorded subdivision held fixed:
+// node weights onto delta_tot[k][.] (y) and the spline coefficients (c) are
+// accumulated in LDS (ds_add_f64: the row belongs to this wave alone), plus
+// the derivative in m/kT.  One wavefront per block, one item per block.
+__global__ __launch_bounds__(64) void k_dnu_adj(
+    HTab H, const double *__restrict__ fx, const double *__restrict__ fy, const double *__restrict__ fc, int nfs,
+    const double *__restrict__ sx, const double *__restrict__ Y, const double *__restrict__ C,
+    const int *__restrict__ sb, int ns, int nsb, double sinv_bw, const double *__restrict__ wavenum, int nk,
+    const SpeciesDev *__restrict__ sp, double vcrit, const double *__restrict__ ivl, const int *__restrict__ nint,
+    const double *__restrict__ gbar, double prefac, double *__restrict__ ybar, double *__restrict__ cbar,
+    double *__restrict__ mu_bar)
+{
+    HIP_DYNAMIC_SHARED(double, lds)  // ybar row [ns], cbar row [ns]
+    const int lane = threadIdx.x, item = blockIdx.x, k = item % nk;
+    const int hf = lane >> 5, j = lane & 31;
+    for (int t = lane; t < 2 * ns; t += 64) lds[t] = 0.0;
+    __syncthreads();
+    const double *syk = Y + (size_t)k * ns, *sck = C + (size_t)k * ns;
+    const bool linear = ns <= 2;
+    const double kk = wavenum[k], mnubykT = sp[0].mnubykT, gb = gbar[item];
+    const JPar &J = sp[0].J;
+    const int ni = nint[item];
+    const double fx0 = fx[0], finv_h = (nfs - 1) / (fx[nfs - 1] - fx[0]), sx0 = sx[0];
+    double mu_acc = 0.0;
+    for (int t0 = 0; t0 < ni; t0 += 2) {
+        const int t = t0 + hf;
+        if (t < ni && j <= 30) {
+            const double a = ivl[((size_t)item * kLimit + t) * 2], b = ivl[((size_t)item * kLimit + t) * 2 + 1];
+            const double c = 0.5 * (a + b), hl = 0.5 * (b - a);
+            const int npt = j < 30 ? 2 : 1;
+            for (int side = 0; side < npt; side++) {
+                const double x = j < 30 ? (side ? c + hl * c_xgk[j] : c - hl * c_xgk[j]) : c;
+                const double w = c_wgk[j] * hl * prefac * gb;
+                // the pieces of delta_tot_table.c:492-500
+                const int fi = settle(fx, nfs, (int)((x - fx0) * finv_h), x);
+                const double fsl = cspline_at(fx, fy, fc, fi, x);
+                int q = (int)((x - sx0) * sinv_bw);
+                q = q < 0 ? 0 : (q > nsb ? nsb : q);
+                const int si = settle(sx, ns, sb[q], x);
+                const double kx = kk * fsl / mnubykT;
+                double Jx, Jq;
+                const double Jv = specialJ_d(kx, J, Jx, Jq);
+                const double base = fsl / (exp(x) * exp(ln_hubble(H, x)));
+                const double S = linear ? linear_at(sx, syk, si, x) : cspline_at(sx, syk, sck, si, x);
+                mu_acc += w * base * S * (Jx * (-kx / mnubykT) + Jq * vcrit);  // d/d(m/kT), qc = vcrit m/kT
+                const double wj = w * base * Jv;
+                if (linear) {
+                    const double t1 = (x - sx[si]) / (sx[si + 1] - sx[si]);
+                    atomicAdd(&lds[si], wj * (1.0 - t1));
+                    atomicAdd(&lds[si + 1], wj * t1);
+                } else {
+                    double wy0, wy1, wc0, wc1;
+                    cspline_grad(sx, si, x, wy0, wy1, wc0, wc1);
+                    atomicAdd(&lds[si], wj * wy0);
+                    atomicAdd(&lds[si + 1], wj * wy1);
+                    atomicAdd(&lds[ns + si], wj * wc0);
+                    atomicAdd(&lds[ns + si + 1], wj * wc1);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) mu_acc += __shfl_xor(mu_acc, m, 64);
+    __syncthreads();
+    for (int t = lane; t < ns; t += 64) {
+        ybar[(size_t)k * ns + t] = lds[t];
+        cbar[(size_t)k * ns + t] = lds[ns + t];
+    }
+    if (lane == 0) mu_bar[item] = mu_acc;
+}
+
+// Per row of the adjoint: phase 0 solves M z = cbar (the natural-spline matrix
+// is symmetric, so its transpose is itself); phase 1 adds z through the
+// right-hand side 3((y_{i+1}-y_i)/h_i - (y_i-y_{i-1})/h_{i-1}) onto ybar.
+__global__ void k_dnu_adj_rows(const double *__restrict__ x, int n, int nrows, const double *__restrict__ cbar,
+                               double *__restrict__ z, double *__restrict__ ybar, int phase)
+{
+    const int nchunk = (n + kChunk - 1) / kChunk;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nchunk * nrows) return;
+    const int r = (int)(t / nchunk), ch = (int)(t % nchunk);
+    const int lo = ch * kChunk, hi = lo + kChunk < n ? lo + kChunk : n;
+    const size_t row = (size_t)r * n;
+    if (phase == 0) {
+        tridiag_window(x, cbar + row, n, lo, hi, z + row);
+        return;
+    }
+    const double *zr = z + row;
+    for (int i = lo; i < hi; i++) {
+        double acc = 0.0;
+        if (i - 1 >= 1 && i - 1 <= n - 2) acc += zr[i - 1] * 3.0 / (x[i] - x[i - 1]);
+        if (i >= 1 && i <= n - 2) acc -= zr[i] * (3.0 / (x[i + 1] - x[i]) + 3.0 / (x[i] - x[i - 1]));
+        if (i + 1 >= 1 && i + 1 <= n - 2) acc += zr[i + 1] * 3.0 / (x[i + 1] - x[i]);
+        ybar[row + i] += acc;
     }
 }
 
@@ -195,6 +297,9 @@ struct DnuDev {
     int *nint = nullptr, *item_species = nullptr;
     SpeciesDev *sp = nullptr;
     int *status = nullptr;
+    size_t cap_ivl = 0, cap_adj = 0, cap_adj_k = 0;
+    double *ivl = nullptr;                                            // recorded subdivisions
+    double *gbar = nullptr, *ybar = nullptr, *cbar = nullptr, *zbuf = nullptr, *mubar = nullptr;  // adjoint
 };
 std::unordered_map<const void *, DnuDev *> g_dev;
 int g_last_max_intervals = 0;
@@ -298,16 +403,26 @@ extern "C" void kn_dnu_release(const _delta_tot_table *d)
     auto it = g_dev.find(d);
     if (it == g_dev.end()) return;
     DnuDev *v = it->second;
-    void *ptrs[] = {v->fx, v->fy, v->fc, v->sx, v->Y, v->C, v->wavenum, v->bucket, v->out, v->nint, v->item_species,
-                    v->sp, v->status};
+    void *ptrs[] = {v->fx,  v->fy,   v->fc,   v->sx,   v->Y,    v->C,    v->wavenum, v->bucket, v->out,
+                    v->nint, v->item_species, v->sp, v->status, v->ivl, v->gbar, v->ybar, v->cbar, v->zbuf,
+                    v->mubar};
     for (void *p : ptrs)
         if (p) KN_CHECK(hipFree(p));
     delete v;
     g_dev.erase(it);
 }
 
+static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum, int nspecies,
+                      const kn_species *sp, double *out, bool record);
+
 extern "C" void kn_dnu_solve(const _delta_tot_table *d, double a, const double *wavenum, int nspecies,
                              const kn_species *sp, double *out)
+{
+    dnu_solve(d, a, wavenum, nspecies, sp, out, false);
+}
+
+static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum, int nspecies,
+                      const kn_species *sp, double *out, bool record)
 {
     const int Na = d->ia, nk = d->nk;
     const double x0 = log(d->TimeTransfer), xa = log(a);
@@ -384,6 +499,11 @@ extern "C" void kn_dnu_solve(const _delta_tot_table *d, double a, const double *
     KN_CHECK(hipMemcpy(dev->sp, sd, sizeof(SpeciesDev) * need.size(), hipMemcpyHostToDevice));
     KN_CHECK(hipMemcpy(dev->item_species, ispec, sizeof(int) * need.size(), hipMemcpyHostToDevice));
     KN_CHECK(hipMemset(dev->status, 0, sizeof(int) * 2));
+    if (record && dev->cap_ivl < (size_t)nitems) {
+        if (dev->ivl) KN_CHECK(hipFree(dev->ivl));
+        KN_CHECK(hipMalloc(&dev->ivl, sizeof(double) * (size_t)nitems * kLimit * 2));
+        dev->cap_ivl = nitems;
+    }
 
     const HTab H = device_htab();
     hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, d->light,
@@ -406,7 +526,8 @@ extern "C" void kn_dnu_solve(const _delta_tot_table *d, double a, const double *
     KN_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nitems)), dim3(64 * kWavesPerBlock), 0, 0, H, dev->fx, dev->fy,
                        dev->fc, nfs, dev->sx, dev->Y, dev->C, dev->bucket, Na, nsb, sinv_bw, dev->wavenum, nk, dev->sp,
-                       dev->item_species, nitems, x0, xa, dev->out, dev->nint, dev->status);
+                       dev->item_species, nitems, x0, xa, dev->out, dev->nint, dev->status,
+                       record ? dev->ivl : nullptr);
     KN_CHECK(hipGetLastError());
     std::vector<double> res(nitems);
     std::vector<int> nint(nitems);
@@ -421,4 +542,93 @@ extern "C" void kn_dnu_solve(const _delta_tot_table *d, double a, const double *
     for (size_t i = 0; i < need.size(); i++)
         for (int k = 0; k < nk; k++)
             out[(size_t)need[i] * nk + k] += d->delta_nu_prefac * res[i * nk + k];
+}
+
+// d specialJ / dx of the rational fit (delta_tot_table.c:417-428), the branch
+// the initial-condition piece always takes (qc = 0).
+static double fit_dJdx(double x)
+{
+    if (x <= 0.) return 0.0;
+    const double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x7 = x4 * x3, x8 = x4 * x4;
+    const double p = exp(4.1811 * log(x));
+    const double N = 1. + 0.0168 * x2 + 0.0407 * x4;
+    const double D = 1. + 2.1734 * x2 + 1.6787 * p + 0.1467 * x8;
+    const double Nx = 0.0336 * x + 0.1628 * x3;
+    const double Dx = 4.3468 * x + 1.6787 * 4.1811 * p / x + 1.1736 * x7;
+    return (Nx * D - N * Dx) / (D * D);
+}
+
+// Adjoint of get_delta_nu for one mass (delta_tot_table.c:507-611): the
+// forward above with its final QAG subdivisions recorded, then k_dnu_adj and
+// the two row passes on the device; the initial-condition piece on the host.
+extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, const double *wavenum, double mnu,
+                                       const double *gbar, double *delta_nu, double *dtot_bar, double *init_bar,
+                                       double *mnu_bar)
+{
+    if (!d || !wavenum || !gbar || d->ia < 1 || d->nk < 1) return KN_ERR_ARG;
+    kn_species s;
+    kn_species_for(d, a, mnu, &s);
+    const int nk = d->nk, Na = d->ia;
+    std::vector<double> out(nk);
+    dnu_solve(d, a, wavenum, 1, &s, out.data(), true);
+    if (delta_nu) memcpy(delta_nu, out.data(), sizeof(double) * nk);
+    // initial-condition piece (delta_tot_table.c:522-535)
+    const double fsl_A0a = fslength(log(d->TimeTransfer), log(a), d->light);
+    const double lin = 1. + kn_deriv_prefac(d) * fsl_A0a;
+    const double m = s.mnubykT > 0 ? s.mnubykT : 1;
+    const double nfl0 = d->omnu->hybnu.nufrac_low[0];
+    double mb = 0.0;
+    for (int k = 0; k < nk; k++) {
+        const double kk = wavenum[k] * fsl_A0a / m;
+        if (init_bar) init_bar[k] = gbar[k] * specialJ(kk, 0, nfl0) * lin;
+        if (s.mnubykT > 0) mb += gbar[k] * fit_dJdx(kk) * (-kk / m) * d->delta_nu_init[k] * lin;
+    }
+    if (dtot_bar) memset(dtot_bar, 0, sizeof(double) * (size_t)nk * Na);
+    if (!s.ic_only && Na >= 2) {
+        DnuDev *dev = g_dev[d];
+        const size_t ny = (size_t)nk * Na;
+        if (dev->cap_adj < ny) {
+            for (double **p : {&dev->ybar, &dev->cbar, &dev->zbuf}) {
+                if (*p) KN_CHECK(hipFree(*p));
+                KN_CHECK(hipMalloc(p, sizeof(double) * ny));
+            }
+            dev->cap_adj = ny;
+        }
+        if (dev->cap_adj_k < (size_t)nk) {
+            for (double **p : {&dev->gbar, &dev->mubar}) {
+                if (*p) KN_CHECK(hipFree(*p));
+                KN_CHECK(hipMalloc(p, sizeof(double) * nk));
+            }
+            dev->cap_adj_k = nk;
+        }
+        KN_CHECK(hipMemcpy(dev->gbar, gbar, sizeof(double) * nk, hipMemcpyHostToDevice));
+        const size_t lds = sizeof(double) * 2 * (size_t)Na;
+        if (lds > 160 * 1024) {
+            kn_set_last_error("kn_get_delta_nu_adjoint: %d stored steps exceed the LDS row budget", Na);
+            return KN_ERR_RANGE;
+        }
+        if (lds > 64 * 1024)
+            KN_CHECK(hipFuncSetAttribute((const void *)k_dnu_adj, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const int nfs = 16 * Na, nsb = 4 * Na;
+        const double sinv_bw = nsb / (d->scalefact[Na - 1] - d->scalefact[0]);
+        const double vcrit = s.qc > 0 ? d->omnu->hybnu.vcrit : 0.0;
+        hipLaunchKernelGGL(k_dnu_adj, dim3(nk), dim3(64), lds, 0, device_htab(), dev->fx, dev->fy, dev->fc, nfs,
+                           dev->sx, dev->Y, dev->C, dev->bucket, Na, nsb, sinv_bw, dev->wavenum, nk, dev->sp, vcrit,
+                           dev->ivl, dev->nint, dev->gbar, d->delta_nu_prefac, dev->ybar, dev->cbar, dev->mubar);
+        KN_CHECK(hipGetLastError());
+        if (Na > 2) {
+            const long long t = (long long)((Na + kChunk - 1) / kChunk) * nk;
+            for (int phase = 0; phase < 2; phase++) {
+                hipLaunchKernelGGL(k_dnu_adj_rows, dim3((int)((t + 127) / 128)), dim3(128), 0, 0, dev->sx, Na, nk,
+                                   dev->cbar, dev->zbuf, dev->ybar, phase);
+                KN_CHECK(hipGetLastError());
+            }
+        }
+        std::vector<double> mub(nk);
+        KN_CHECK(hipMemcpy(mub.data(), dev->mubar, sizeof(double) * nk, hipMemcpyDeviceToHost));
+        for (int k = 0; k < nk; k++) mb += mub[k];
+        if (dtot_bar) KN_CHECK(hipMemcpy(dtot_bar, dev->ybar, sizeof(double) * ny, hipMemcpyDeviceToHost));
+    }
+    if (mnu_bar) *mnu_bar = mb / d->omnu->kBtnu;
+    return 0;
 }

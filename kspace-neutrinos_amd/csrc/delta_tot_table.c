@@ -118,7 +118,7 @@ double kn_deriv_prefac(const _delta_tot_table *d)
 
 /* ---- species parameters as get_delta_nu derives them (delta_tot_table.c:513-551) ---- */
 
-static void species_for(const _delta_tot_table *d, double a, double mnu, kn_species *s)
+void kn_species_for(const _delta_tot_table *d, double a, double mnu, kn_species *s)
 {
     const double partnu = particle_nu_fraction(&d->omnu->hybnu, a, 0);
     s->mnubykT = mnu / d->omnu->kBtnu;
@@ -137,7 +137,7 @@ void get_delta_nu(const _delta_tot_table *const d, const double a, const double 
                   const double mnu)
 {
     kn_species s;
-    species_for(d, a, mnu, &s);
+    kn_species_for(d, a, mnu, &s);
     if (d->debug)
         kn_message(0, "Start get_delta_nu: a=%g Na =%d wavenum[0]=%g delta_tot[0]=%g m_nu=%g\n", a, d->ia,
                    wavenum[0], d->delta_tot[0][d->ia - 1], mnu);
@@ -155,7 +155,7 @@ void get_delta_nu_combined(const _delta_tot_table *const d, const double a, cons
     for (i = 0; i < NUSPECIES; i++) {
         if (d->omnu->nu_degeneracies[i] > 0) {
             om[ns] = d->omnu->nu_degeneracies[i] * omega_nu_single(d->omnu, a, i);
-            species_for(d, a, d->omnu->RhoNuTab[i]->mnu, &sp[ns]);
+            kn_species_for(d, a, d->omnu->RhoNuTab[i]->mnu, &sp[ns]);
             idx[ns++] = i;
         }
     }

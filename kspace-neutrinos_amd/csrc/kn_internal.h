@@ -67,6 +67,9 @@ typedef struct {
     int ic_only;        /* 1: only the initial-condition piece (hybrid, all in particles) */
 } kn_species;
 
+/* The species parameters get_delta_nu derives from (d, a, mnu) (delta_tot_table.c:507-555). */
+void kn_species_for(const _delta_tot_table *d, double a, double mnu, kn_species *s);
+
 /* delta_nu_out[s*nk + k] receives get_delta_nu(..., mnu_s) for each species. */
 void kn_dnu_solve(const _delta_tot_table *d, double a, const double *wavenum, int nspecies, const kn_species *sp,
                   double *delta_nu_out);

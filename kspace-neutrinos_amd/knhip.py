@@ -132,6 +132,8 @@ SIGNATURES = {
     "kn_hubble_ctx": (C.c_double, [C.c_double, C.c_void_p]),
     "kn_get_delta_nu_adjoint": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, dp, C.c_double, dp, dp, dp, dp,
                                           dp]),
+    "kn_batch_get_delta_nu_adjoint": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, dp, C.c_double, dp, dp, dp,
+                                                dp, dp]),
 }
 
 _lib = None
@@ -354,6 +356,57 @@ def delta_nu_function():
             return db, ib, torch.tensor(mb, dtype=mnu.dtype, device=mnu.device), None
 
     return DeltaNuFunction
+
+
+def table_views(table):
+    """numpy views of a _delta_tot_table's history block: (scalefact[namax], delta_tot[nk_allocated][namax],
+    delta_nu_init[nk])."""
+    import numpy as np
+    namax, nka = table.namax, table.nk_allocated
+    blk = np.ctypeslib.as_array(table.scalefact, shape=(namax * (nka + 1),))
+    return blk[:namax], blk[namax:].reshape(nka, namax), np.ctypeslib.as_array(table.delta_nu_init,
+                                                                               shape=(table.nk,))
+
+
+def delta_nu_plain_function():
+    """torch.autograd.Function over get_delta_nu on a table (per-call path):
+    apply(delta_tot[nk, ia], delta_nu_init[nk], mnu[()], table, a, wavenum) ->
+    delta_nu[nk], CPU float64 tensors as the reference's caller holds them.
+    Forward: get_delta_nu; backward: kn_get_delta_nu_adjoint with the same
+    inputs written back into the table."""
+    import numpy as np
+    import torch
+
+    def load(table, dt, di):
+        _, rows, init = table_views(table)
+        rows[:table.nk, :table.ia] = dt.detach().numpy()
+        init[:] = di.detach().numpy()
+
+    class DeltaNuPlainFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, delta_tot, delta_nu_init, mnu, table, a, wavenum):
+            load(table, delta_tot, delta_nu_init)
+            kk = np.ascontiguousarray(wavenum, dtype=np.float64)
+            out = np.zeros(table.nk)
+            lib().get_delta_nu(C.byref(table), a, kk.ctypes.data_as(dp), out.ctypes.data_as(dp), float(mnu))
+            ctx.save_for_backward(delta_tot, delta_nu_init, mnu)
+            ctx.table, ctx.a, ctx.k = table, a, kk
+            return torch.from_numpy(out)
+
+        @staticmethod
+        def backward(ctx, g):
+            delta_tot, delta_nu_init, mnu = ctx.saved_tensors
+            t = ctx.table
+            load(t, delta_tot, delta_nu_init)
+            gb = np.ascontiguousarray(g.detach().numpy(), dtype=np.float64)
+            db, ib, mb = np.zeros((t.nk, t.ia)), np.zeros(t.nk), C.c_double()
+            _check(lib().kn_get_delta_nu_adjoint(C.byref(t), ctx.a, ctx.k.ctypes.data_as(dp), float(mnu),
+                                                 gb.ctypes.data_as(dp), None, db.ctypes.data_as(dp),
+                                                 ib.ctypes.data_as(dp), C.byref(mb)), "kn_get_delta_nu_adjoint")
+            return (torch.from_numpy(db), torch.from_numpy(ib), torch.tensor(mb.value, dtype=mnu.dtype),
+                    None, None, None)
+
+    return DeltaNuPlainFunction
 
 
 class DeltaNuState:

@@ -61,14 +61,14 @@ class Cosmo:
         self.P.free()
 
 
-def adjoint_case(P, a, k, m, seed=3, L=None):
+def adjoint_case(P, a, k, m, seed=3, L=None, fn="kn_get_delta_nu_adjoint"):
     nk, na = P.nk, P.d.ia
     g = np.random.default_rng(seed).uniform(-1, 1, nk)
     dnu, db, ib = np.zeros(nk), np.zeros(nk * na), np.zeros(nk)
     mb = C.c_double()
     kk = np.ascontiguousarray(k, dtype=np.float64)
     dp = C.POINTER(C.c_double)
-    rc = (L or knhip.lib()).kn_get_delta_nu_adjoint(C.byref(P.d), a, kk.ctypes.data_as(dp), m, g.ctypes.data_as(dp),
+    rc = getattr(L or knhip.lib(), fn)(C.byref(P.d), a, kk.ctypes.data_as(dp), m, g.ctypes.data_as(dp),
                                              dnu.ctypes.data_as(dp), db.ctypes.data_as(dp), ib.ctypes.data_as(dp),
                                              C.byref(mb))
     assert rc == 0

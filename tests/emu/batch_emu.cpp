@@ -4,6 +4,7 @@
 unsigned emu_tid, emu_bid, emu_bdim, emu_gdim;
 EmuWaveSched *emu_wave;
 int emu_cus = 2, emu_per_cu = 1;
+double emu_dyn_lds[160 * 1024 / sizeof(double)];
 
 // Stack switch (x86-64 System V): push callee-saved registers, swap stack
 // pointers, pop the other context's registers and return into it.

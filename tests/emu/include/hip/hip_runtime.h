@@ -27,6 +27,8 @@
 #define __constant__
 #define __shared__ static
 #define __launch_bounds__(...)
+extern double emu_dyn_lds[];  // dynamic LDS of the running block (160 KiB)
+#define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)emu_dyn_lds;
 
 struct dim3 {
     unsigned x, y, z;
@@ -111,6 +113,8 @@ typedef void *hipGraph_t;
 typedef void *hipGraphExec_t;
 enum hipMemcpyKind { hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
 enum { hipDeviceAttributeMultiprocessorCount = 1 };
+enum { hipFuncAttributeMaxDynamicSharedMemorySize = 8 };
+inline hipError_t hipFuncSetAttribute(const void *, int, int) { return 0; }
 enum { hipStreamCaptureModeThreadLocal = 1 };
 enum hipMemoryType { hipMemoryTypeHost, hipMemoryTypeDevice, hipMemoryTypeManaged };
 struct hipPointerAttribute_t { hipMemoryType type; };

@@ -21,6 +21,7 @@ from conftest import ROOT
 from dnu_common import Pair, synthetic_history
 
 EMU_LIB = os.path.join(ROOT, "tests", "emu", "_build", "libkn_batch_emu.so")
+DNU_EMU_LIB = os.path.join(ROOT, "tests", "emu", "_build", "libkn_dnu_emu.so")
 
 
 def host_memcpy(dst, src, n, kind):
@@ -32,6 +33,35 @@ def emu(kn):
     if not os.path.exists(EMU_LIB):
         raise RuntimeError("emulation build missing: make -C tests/emu")
     return knhip.bind(C.CDLL(EMU_LIB))
+
+
+@pytest.fixture(scope="module")
+def dnu_emu(kn):
+    if not os.path.exists(DNU_EMU_LIB):
+        raise RuntimeError("emulation build missing: make -C tests/emu")
+    return knhip.bind(C.CDLL(DNU_EMU_LIB))
+
+
+@pytest.mark.parametrize("case", [
+    dict(name="cspline", masses=[0.15, 0, 0], Na=40, nk=10, amax=0.4, hybrid=None),
+    dict(name="linear Na=2", masses=[0.2, 0, 0], Na=2, nk=8, amax=0.03, hybrid=None),
+    dict(name="hybrid", masses=[0.3, 0.06, 0.06], Na=80, nk=8, amax=0.9, hybrid=(500.0, 0.3333)),
+    dict(name="noisy", masses=[0.3, 0, 0], Na=150, nk=6, amax=1.0, hybrid=None, noise=0.1),
+    dict(name="massless", masses=[0.15, 0, 0], Na=20, nk=8, amax=0.2, hybrid=None, m=0.0),
+], ids=lambda c: c["name"])
+def test_emu_plain_adjoint_matches_oracle(dnu_emu, case):
+    """The per-call path (csrc/delta_nu.hip: k_dnu_qag with recorded
+    subdivisions, k_dnu_adj, k_dnu_adj_rows) in the emulation."""
+    P = Pair(case["masses"], 0.3, case["nk"], 0.01 + case["Na"] / 100.0 + 0.02, hybrid=case["hybrid"])
+    logs, rows, k, init = synthetic_history(case["Na"], case["nk"], case["amax"], noise=case.get("noise", 0.0))
+    P.set_history(logs, rows, init)
+    m = case.get("m", case["masses"][0])
+    gpu, orc = adjoint_case(P, case["amax"], k, m, L=dnu_emu)
+    if m > 0:
+        compare_adjoint(case["name"], gpu, orc)
+    else:
+        assert np.all(gpu[1] == 0) and gpu[3] == 0.0 and rel(gpu[2], orc[3]) < 1e-14
+    P.free()
 
 
 def test_product_build_keeps_the_batch_switched_off(kn):
@@ -52,7 +82,7 @@ def test_emu_get_delta_nu_adjoint_matches_oracle(emu, case):
     P = Pair(case["masses"], 0.3, case["nk"], 0.01 + case["Na"] / 100.0 + 0.02, hybrid=case["hybrid"])
     logs, rows, k, init = synthetic_history(case["Na"], case["nk"], case["amax"], noise=case.get("noise", 0.0))
     P.set_history(logs, rows, init)
-    gpu, orc = adjoint_case(P, case["amax"], k, case["masses"][0], L=emu)
+    gpu, orc = adjoint_case(P, case["amax"], k, case["masses"][0], L=emu, fn="kn_batch_get_delta_nu_adjoint")
     compare_adjoint(case["name"], gpu, orc)
     P.free()
 
@@ -61,7 +91,7 @@ def test_emu_adjoint_massless_is_ic_only(emu):
     P = Pair([0.15, 0, 0], 0.3, 8, 0.5)
     logs, rows, k, init = synthetic_history(20, 8, 0.2)
     P.set_history(logs, rows, init)
-    (dnu, db, ib, mb), orc = adjoint_case(P, 0.2, k, 0.0, L=emu)
+    (dnu, db, ib, mb), orc = adjoint_case(P, 0.2, k, 0.0, L=emu, fn="kn_batch_get_delta_nu_adjoint")
     assert np.all(db == 0) and mb == 0.0
     assert rel(dnu, orc[1]) < 1e-14 and rel(ib, orc[3]) < 1e-14
     P.free()

@@ -14,6 +14,7 @@ import pytest
 import knhip
 import oracle_py as O
 from conftest import REFDATA, UNIT_LENGTH
+from batch_common import adjoint_case, compare_adjoint
 from dnu_common import DP, Pair, synthetic_history
 from test_oracle_pins import _read_powerspec_fixtures, load_camb, read_all_nu_state
 
@@ -187,4 +188,64 @@ def test_reproduce_linear_theory_on_gpu(kn, torch_cuda):
         worst = max(worst, rel(out, o))
     print("  99-step linear theory: worst rel err vs oracle %.3e" % worst)
     assert worst < RTOL
+    P.free()
+
+
+# ------------------------------------------------------------- adjoint (per-call path)
+
+@pytest.mark.parametrize("case", [
+    dict(name="cspline", masses=[0.15, 0, 0], Na=40, nk=10, amax=0.4, hybrid=None),
+    dict(name="linear Na=2", masses=[0.2, 0, 0], Na=2, nk=8, amax=0.03, hybrid=None),
+    dict(name="hybrid", masses=[0.3, 0.06, 0.06], Na=80, nk=8, amax=0.9, hybrid=(500.0, 0.3333)),
+    dict(name="noisy", masses=[0.3, 0, 0], Na=150, nk=6, amax=1.0, hybrid=None, noise=0.1),
+    dict(name="C1-like", masses=[0.15, 0, 0], Na=100, nk=64, amax=0.5, hybrid=None),
+], ids=lambda c: c["name"])
+def test_get_delta_nu_adjoint_per_call_matches_oracle(kn, torch_cuda, case):
+    """kn_get_delta_nu_adjoint (csrc/delta_nu.hip) against the oracle's adjoint:
+    d/d delta_tot, d/d delta_nu_init and d/d mnu to 1e-9, and the forward it
+    returns against the oracle's get_delta_nu."""
+    P = Pair(case["masses"], 0.3, case["nk"], 0.01 + case["Na"] / 100.0 + 0.02, hybrid=case["hybrid"])
+    logs, rows, k, init = synthetic_history(case["Na"], case["nk"], case["amax"], noise=case.get("noise", 0.0))
+    P.set_history(logs, rows, init)
+    print("adjoint (per call)", case["name"], flush=True)
+    gpu, orc = adjoint_case(P, case["amax"], k, case["masses"][0])
+    compare_adjoint(case["name"], gpu, orc)
+    P.free()
+
+
+def test_get_delta_nu_adjoint_per_call_massless(kn, torch_cuda):
+    P = Pair([0.15, 0, 0], 0.3, 8, 0.5)
+    logs, rows, k, init = synthetic_history(20, 8, 0.2)
+    P.set_history(logs, rows, init)
+    (dnu, db, ib, mb), orc = adjoint_case(P, 0.2, k, 0.0)
+    assert np.all(db == 0) and mb == 0.0
+    assert rel(dnu, orc[1]) < 1e-14 and rel(ib, orc[3]) < 1e-14
+    P.free()
+
+
+def test_get_delta_nu_adjoint_per_call_gradcheck(kn, torch_cuda):
+    """torch.autograd.gradcheck of the per-call binding (central differences,
+    relative step 1e-6 on O(1) inputs), then against the oracle's gradient."""
+    import torch
+    P = Pair([0.15, 0, 0], 0.3, 4, 0.5)
+    logs, rows, k, init = synthetic_history(10, 4, 0.3, noise=0.05)
+    scale = np.max(np.abs(rows))
+    rows, init = rows / scale, init / scale
+    P.set_history(logs, rows, init)
+    F = knhip.delta_nu_plain_function()
+    dt = torch.tensor(rows, dtype=torch.float64, requires_grad=True)
+    di = torch.tensor(init, dtype=torch.float64, requires_grad=True)
+    m = torch.tensor(0.15, dtype=torch.float64, requires_grad=True)
+    print("per-call gradcheck", flush=True)
+    # linear in delta_tot and delta_nu_init: differences exact up to rounding (~1e-9)
+    assert torch.autograd.gradcheck(lambda a, b, c: F.apply(a, b, c, P.d, 0.3, k), (dt, di, m), eps=1e-6,
+                                    atol=1e-8, rtol=1e-6, nondet_tol=1e-12)
+    g = torch.linspace(-1, 1, 4, dtype=torch.float64)
+    (F.apply(dt, di, m, P.d, 0.3, k) * g).sum().backward()
+    P.set_history(logs, rows, init)
+    orc = P.T.get_delta_nu_adjoint(0.3, k, 0.15, g.numpy())
+    e = max(float(np.max(np.abs(dt.grad.numpy() - orc[2]) / np.max(np.abs(orc[2])))),
+            rel(di.grad.numpy(), orc[3]), abs(float(m.grad) / orc[4] - 1))
+    print("  autograd vs oracle gradient: %.3e" % e)
+    assert e < RTOL
     P.free()
