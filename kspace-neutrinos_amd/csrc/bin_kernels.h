@@ -213,8 +213,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_bin_power_coalesced(BinArgs a)
     store_partial<WAVES>(a, hist);
 }
 
-// Software-pipelined runs variant: the next strip's loads are issued before
-// the current strip is binned, doubling the bytes in flight per wave.
+// Software-pipelined runs: each lane bins RUN consecutive modes of the
+// flattened grid while its next RUN modes are in flight.  Along a row |k|^2
+// grows with z, so after the first mode of a run the bin is found
+// incrementally: the next threshold is kept in a register and at most one
+// bin step is taken before falling back to the full search (near the origin a
+// unit step in z crosses many log bins).  Equal-bin runs are summed in
+// registers and flushed to the wave's LDS histogram.
 template <int WAVES, int RUN>
 __global__ __launch_bounds__(64 * WAVES) void k_bin_power_runs_pf(BinArgs a)
 {
@@ -237,7 +242,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_bin_power_runs_pf(BinArgs a)
 #pragma unroll
         for (int r = 0; r < RUN; r++) nxt[r] = (m + S + r < m1) ? a.F[m + S + r] : make_double2(0.0, 0.0);
         Pos q = p;
-        int curb = -1;
+        int curb = -1, b = -1, bnext = 0;
         double acc = 0.0;
 #pragma unroll
         for (int r = 0; r < RUN; r++) {
@@ -246,7 +251,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_bin_power_runs_pf(BinArgs a)
             const int kx = kval(a.startslab + q.i, a.dims), ky = kval(q.j, a.dims);
             const int k2 = kx * kx + ky * ky + q.z * q.z;
             if (k2 == 0) continue;
-            const int b = find_bin(k2, a.half_bpu, thr, a.nrbins);
+            if (b < 0 || q.z == 0) {  // run start or a new row: full search
+                b = find_bin(k2, a.half_bpu, thr, a.nrbins);
+                bnext = thr[b + 1];
+            } else if (k2 >= bnext) {
+                const int t2 = thr[b + 2];  // here b + 1 <= nrbins - 1, so b + 2 <= nrbins
+                if (k2 < t2) {
+                    b++;
+                    bnext = t2;
+                } else {
+                    b = find_bin(k2, a.half_bpu, thr, a.nrbins);
+                    bnext = thr[b + 1];
+                }
+            }
             const double v = mode_weight(cur[r], __dmul_rn(iw[a.startslab + q.i], iw[q.j]), iw[q.z], q.z, half);
             if (b != curb) {
                 if (curb >= 0) atomicAdd(&my[curb], acc);

@@ -39,8 +39,8 @@ using kn_bin::kval;
 
 constexpr int kBlock = 256;  // geometry pass
 constexpr int kWaves = kBlock / 64;
-constexpr int kPowWaves = 8;  // power pass: 8 waves, 2 modes per lane per step, next step prefetched
-constexpr int kPowRun = 2;
+constexpr int kPowWaves = 8;  // power pass: 8 waves, 4 consecutive modes per lane per step, next step prefetched
+constexpr int kPowRun = 4;
 
 struct BinGeom {
     int dims, nrbins, startslab, nslab;
@@ -243,8 +243,12 @@ BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
     return g;
 }
 
-void launch_power(BinGeom *g, const double2 *F, hipStream_t st)
+// Power pass; the bin sums land in sum_out[nrbins] and |F0|^2 in mass_out
+// (default: the geometry's own buffer, sums then |F0|^2).
+void launch_power(BinGeom *g, const double2 *F, hipStream_t st, double *sum_out = nullptr, double *mass_out = nullptr)
 {
+    if (!sum_out) sum_out = g->d_sum;
+    if (!mass_out) mass_out = g->d_sum + g->nrbins;
     kn_bin::BinArgs a;
     a.F = F;
     a.dims = g->dims;
@@ -259,9 +263,9 @@ void launch_power(BinGeom *g, const double2 *F, hipStream_t st)
                        power_smem(g->dims, g->nrbins), st, a);
     KN_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_reduce_partials, dim3((g->nrbins + 63) / 64), dim3(64 * kRedWaves), 0, st, g->d_partial,
-                       g->grid, g->nrbins, g->d_sum);
+                       g->grid, g->nrbins, sum_out);
     KN_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_total_mass2, dim3(1), dim3(1), 0, st, F, g->startslab == 0 ? 1 : 0, g->d_sum + g->nrbins);
+    hipLaunchKernelGGL(k_total_mass2, dim3(1), dim3(1), 0, st, F, g->startslab == 0 ? 1 : 0, mass_out);
     KN_CHECK(hipGetLastError());
 }
 
@@ -339,12 +343,7 @@ extern "C" int kn_bin_power_device(const fftw_complex *d_field, int dims, int nr
                                    double *d_power, double *d_total_mass2, void *stream)
 {
     BinGeom *g = get_geom(dims, nrbins, startslab, nslab);
-    hipStream_t st = (hipStream_t)stream;
-    launch_power(g, (const double2 *)d_field, st);
-    if (d_power)
-        KN_CHECK(hipMemcpyAsync(d_power, g->d_sum, sizeof(double) * nrbins, hipMemcpyDeviceToDevice, st));
-    if (d_total_mass2)
-        KN_CHECK(hipMemcpyAsync(d_total_mass2, g->d_sum + nrbins, sizeof(double), hipMemcpyDeviceToDevice, st));
+    launch_power(g, (const double2 *)d_field, (hipStream_t)stream, d_power, d_total_mass2);
     return 0;
 }
 
