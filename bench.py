@@ -108,6 +108,100 @@ def dnu_bench(kn, nk, na=2000, calls=3, cpu_k=16, no_cpu=False):
     return res
 
 
+def c4_plain(kn, nhist=256, seed=1000):
+    """256 independent histories through the plain C-ABI path, one get_delta_nu
+    call per history (the baseline of batching): Na ~ U[100, 3000] stored steps
+    over a in [0.01, 1], nk ~ U[64, 512] log-spaced k in [2 pi/512, sqrt(3) pi 1024/512]
+    h/Mpc, one species m ~ U[0.06, 0.6] eV, Omega_m ~ U[0.25, 0.35], each with
+    its own H(a).  delta_tot = a sqrt(P(k)) (1 + 0.1 N(0,1)) with the noise drawn
+    per k: noise drawn per (k, a) entry makes the reference's own QAG fail
+    (GSL error, terminate 2001) on long histories."""
+    import conftest
+    host = conftest.load_testhost()
+    import knhip
+    rng = np.random.default_rng(seed)
+    tables, total_ksteps, t_total = [], 0.0, 0.0
+    for h in range(nhist):
+        na, nk = int(rng.integers(100, 3001)), int(rng.integers(64, 513))
+        m, om0 = float(rng.uniform(0.06, 0.6)), float(rng.uniform(0.25, 0.35))
+        om = knhip.OmegaNu()
+        kn.init_omega_nu(C.byref(om), (C.c_double * 3)(m, 0.0, 0.0), 0.01, 0.7, 2.7255)
+        d = knhip.DeltaTotTable()
+        kn.allocate_delta_tot_table(C.byref(d), nk, 0.01, 0.01 + na / 100.0 + 0.02, om0, C.byref(om),
+                                    conftest.UNIT_TIME, conftest.UNIT_LENGTH, 0)
+        sf, rows, init = knhip.table_views(d)
+        logs = np.linspace(math.log(0.01), 0.0, na)
+        k = np.exp(np.linspace(math.log(2 * math.pi / 512), math.log(math.sqrt(3) * math.pi * 1024 / 512), nk)) / 1e3
+        q = k * 1000 / 0.2
+        t = np.log(1 + 2.34 * q) / (2.34 * q) * (1 + 3.89 * q + (16.1 * q) ** 2 + (5.46 * q) ** 3 + (6.71 * q) ** 4) ** -0.25
+        amp = np.sqrt(1e9 * (k * 1000) ** 0.96 * t * t) * (1 + 0.1 * rng.standard_normal(nk))
+        sf[:na] = logs
+        rows[:nk, :na] = np.outer(amp, np.exp(logs))
+        init[:] = 0.4 * rows[:nk, 0]
+        d.ia = na
+        out = np.zeros(nk)
+        kk = np.ascontiguousarray(k)
+        host.kn_test_init_hubble(C.byref(om), om0, conftest.UNIT_TIME)
+        t0 = time.perf_counter()
+        kn.get_delta_nu(C.byref(d), 1.0, kk.ctypes.data_as(DP), out.ctypes.data_as(DP), m)
+        t_total += time.perf_counter() - t0
+        total_ksteps += nk * na
+        kn.free_delta_tot_table(C.byref(d))
+    return {"workload": "%d histories, Na~U[100,3000], nk~U[64,512], m~U[0.06,0.6] eV, Omega_m~U[0.25,0.35]" % nhist,
+            "plain_s": round(t_total, 4), "plain_kstep_per_s": round(total_ksteps / t_total, 1),
+            "ksteps": int(total_ksteps), "batched": None,
+            "note": "batched path switched off in this build (kn_batch_create returns 3005); "
+                    "plain = one C-ABI get_delta_nu call per history, host H(a) per history"}
+
+
+def c5_adjoints(kn, torch):
+    """Adjoint timings at the differentiation workload's shapes: get_delta_nu's
+    adjoint for one 1024 k x 2000 step history (m = 0.3 eV), and the binning
+    adjoint on a 256^3 grid (one forward pass of 16 B/mode + one 32 B/mode sweep)."""
+    from dnu_common import Pair, synthetic_history
+    import knhip
+    nk, na = 1024, 2000
+    P = Pair([0.3, 0.0, 0.0], 0.3, nk, 0.01 + na / 100.0 + 0.02)
+    logs, rows, k, init = synthetic_history(na, nk, 1.0, noise=0.0)
+    P.set_history(logs, rows, init)
+    g = np.random.default_rng(3).uniform(-1, 1, nk)
+    db, ib, dnu, mb = np.zeros(nk * na), np.zeros(nk), np.zeros(nk), C.c_double()
+    kk = np.ascontiguousarray(k)
+    args = (C.byref(P.d), 1.0, kk.ctypes.data_as(DP), 0.3, g.ctypes.data_as(DP), dnu.ctypes.data_as(DP),
+            db.ctypes.data_as(DP), ib.ctypes.data_as(DP), C.byref(mb))
+    assert kn.kn_get_delta_nu_adjoint(*args) == 0
+    t = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        kn.kn_get_delta_nu_adjoint(*args)
+        t.append(time.perf_counter() - t0)
+    ta = float(np.median(t))
+    P.free()
+    N, nrbins = 256, 256
+    d = torch.empty((N, N, N // 2 + 1), dtype=torch.complex128, device="cuda")
+    assert kn.kn_synth_grid(d.data_ptr(), N, 0, N, 512000.0, 1e3, 0.96, 0.2 / 1000.0, 5) == 0
+    power, keffs = np.zeros(nrbins), np.zeros(nrbins)
+    count = np.zeros(nrbins, dtype=np.int64)
+    n = kn.total_powerspectrum(N, d.data_ptr(), nrbins, 0, N, power.ctypes.data_as(DP),
+                               count.ctypes.data_as(C.POINTER(C.c_longlong)), keffs.ctypes.data_as(DP), 0)
+    gb = np.ascontiguousarray(np.random.default_rng(4).uniform(-1, 1, n))
+    fb = torch.empty_like(d)
+    kn.kn_bin_power_adjoint(d.data_ptr(), N, nrbins, 0, N, gb.ctypes.data_as(DP), n, fb.data_ptr(), None)
+    t = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        kn.kn_bin_power_adjoint(d.data_ptr(), N, nrbins, 0, N, gb.ctypes.data_as(DP), n, fb.data_ptr(), None)
+        t.append(time.perf_counter() - t0)
+    tb = float(np.median(t))
+    modes = N * N * (N // 2 + 1)
+    return {"delta_nu_adjoint": {"workload": "kn_get_delta_nu_adjoint, 1 species 0.3 eV, %d k x %d steps" % (nk, na),
+                                 "ms_per_call": round(ta * 1e3, 3), "kstep_per_s": round(nk * na / ta, 1),
+                                 "note": "includes the forward with recorded subdivisions; host in/out"},
+            "bin_adjoint": {"workload": "kn_bin_power_adjoint, 256^3, 256 bins",
+                            "ms_per_call": round(tb * 1e3, 3), "GB_per_s": round(48.0 * modes / tb / 1e9, 1),
+                            "note": "forward bin pass (16 B/mode) + adjoint sweep (32 B/mode), host sync included"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,7 +211,7 @@ def main():
     ap.add_argument("--nrbins", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slabs", type=int, default=192)
-    ap.add_argument("--no-dnu", action="store_true", help="skip the free-streaming integral timing")
+    ap.add_argument("--no-dnu", action="store_true", help="skip the free-streaming integral timings")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 2 GiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -183,7 +277,7 @@ def main():
     res_gbs = 32.0 * modes / t_res.mean() / 1e9
     bin_gbs = 16.0 * modes / t_bin.mean() / 1e9
     out = {
-        "metric": "P(k) bin+rescale GB/s at 1024^3",
+        "metric": "P(k) bin+rescale GB/s at 1024^3; delta_nu k*step/s batched vs plain @1e-9",
         "value": round(value, 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -213,6 +307,8 @@ def main():
         del grid
         torch.cuda.empty_cache()
         out["delta_nu"] = dnu_bench(kn, n, no_cpu=args.no_cpu)
+        out["delta_nu_batch"] = c4_plain(kn)
+        out["adjoints"] = c5_adjoints(kn, torch)
     print(json.dumps(out))
 
 
