@@ -212,6 +212,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slabs", type=int, default=192)
     ap.add_argument("--no-dnu", action="store_true", help="skip the free-streaming integral timings")
+    ap.add_argument("--only-dnu", action="store_true",
+                    help="only the get_delta_nu_combined timing (757 k x 2000 steps), for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 2 GiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -233,6 +235,9 @@ def main():
     conftest.load_testhost()
     import knhip
     kn = knhip.lib()
+    if args.only_dnu:
+        print(json.dumps({"delta_nu": dnu_bench(kn, 757, no_cpu=True)}))
+        return
 
     N, nrbins, box = args.n, args.nrbins, 512000.0
     modes = N * N * (N // 2 + 1)
