@@ -249,3 +249,34 @@ def test_get_delta_nu_adjoint_per_call_gradcheck(kn, torch_cuda):
     print("  autograd vs oracle gradient: %.3e" % e)
     assert e < RTOL
     P.free()
+
+
+@pytest.mark.parametrize("cfg", ["medium", "full"])
+def test_get_delta_nu_at_configuration_shapes(kn, torch_cuda, cfg):
+    """The free-streaming integral at the medium and full configurations'
+    shapes, delta_tot = a sqrt(P(k)) over a in [0.01, 1]: 300 k x 1000 steps
+    with three degenerate 0.1 eV species (all k against the oracle), and
+    757 k x 2000 steps hybrid 0.06/0.06/0.3 eV (vcrit 500 km/s, particles
+    from a = 0.3333; 24 k against the oracle)."""
+    if cfg == "medium":
+        masses, hybrid, nk, na, sel = [0.1, 0.1, 0.1], None, 300, 1000, None
+    else:
+        masses, hybrid, nk, na = [0.06, 0.06, 0.3], (500.0, 0.3333), 757, 2000
+        sel = np.linspace(0, nk - 1, 24).astype(int)
+    P = Pair(masses, 0.3, nk, 0.01 + na / 100.0 + 0.02, hybrid=hybrid)
+    logs, rows, k, init = synthetic_history(na, nk, 1.0, noise=0.0)
+    P.set_history(logs, rows, init)
+    print("get_delta_nu_combined", cfg, nk, "x", na, flush=True)
+    g = P.gpu_get_delta_nu_combined(1.0, k)
+    if sel is None:
+        rc, o = P.T.get_delta_nu_combined(1.0, k)
+        e = rel(g, o)
+    else:
+        Q = Pair(masses, 0.3, len(sel), 0.01 + na / 100.0 + 0.02, hybrid=hybrid)
+        Q.set_history(logs, rows[sel], init[sel])
+        rc, o = Q.T.get_delta_nu_combined(1.0, k[sel])
+        e = rel(g[sel], o)
+        Q.free()
+    print("  %s: max rel err vs oracle %.3e" % (cfg, e))
+    assert rc == 0 and e < RTOL
+    P.free()
