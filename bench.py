@@ -108,19 +108,33 @@ def dnu_bench(kn, nk, na=2000, calls=3, cpu_k=16, no_cpu=False):
     return res
 
 
-def c4_plain(kn, nhist=256, seed=1000):
-    """256 independent histories through the plain C-ABI path, one get_delta_nu
-    call per history (the baseline of batching): Na ~ U[100, 3000] stored steps
-    over a in [0.01, 1], nk ~ U[64, 512] log-spaced k in [2 pi/512, sqrt(3) pi 1024/512]
+class _HCtx(C.Structure):
+    _fields_ = [("omnu", C.c_void_p), ("Omega_nonu", C.c_double), ("OmegaLambda", C.c_double),
+                ("Hubble", C.c_double)]
+
+
+def c4_batching(kn, torch, nhist=256, seed=1000, reps=3):
+    """256 independent histories: Na ~ U[100, 3000] stored steps over a in
+    [0.01, 1], nk ~ U[64, 512] log-spaced k in [2 pi/512, sqrt(3) pi 1024/512]
     h/Mpc, one species m ~ U[0.06, 0.6] eV, Omega_m ~ U[0.25, 0.35], each with
-    its own H(a).  delta_tot = a sqrt(P(k)) (1 + 0.1 N(0,1)) with the noise drawn
-    per k: noise drawn per (k, a) entry makes the reference's own QAG fail
-    (GSL error, terminate 2001) on long histories."""
+    its own H(a); delta_tot = a sqrt(P(k)) (1 + 0.1 N(0,1)) with the noise drawn
+    per k (noise drawn per (k, a) entry makes the reference's own QAG fail, GSL
+    error -> terminate 2001, on long histories).
+
+    plain: one C-ABI get_delta_nu call per history (and one
+    kn_get_delta_nu_adjoint call for the adjoint), host tables in and out; the
+    process has one hubble_function, so its H table is rebuilt per history.
+    group: the same histories resident in HBM in one kn_group (built once: one
+    H table per history); a step is kn_group_solve + kn_group_adjoint, or the
+    graph of both replayed."""
     import conftest
     host = conftest.load_testhost()
     import knhip
+    host.kn_test_hctx_init.argtypes = [C.POINTER(_HCtx), C.c_void_p, C.c_double, C.c_double]
+    hfn = C.cast(host.kn_test_hubble_ctx, C.c_void_p).value
     rng = np.random.default_rng(seed)
-    tables, total_ksteps, t_total = [], 0.0, 0.0
+    keep, hists, data = [], [], []
+    ksteps = 0
     for h in range(nhist):
         na, nk = int(rng.integers(100, 3001)), int(rng.integers(64, 513))
         m, om0 = float(rng.uniform(0.06, 0.6)), float(rng.uniform(0.25, 0.35))
@@ -139,19 +153,89 @@ def c4_plain(kn, nhist=256, seed=1000):
         rows[:nk, :na] = np.outer(amp, np.exp(logs))
         init[:] = 0.4 * rows[:nk, 0]
         d.ia = na
-        out = np.zeros(nk)
-        kk = np.ascontiguousarray(k)
-        host.kn_test_init_hubble(C.byref(om), om0, conftest.UNIT_TIME)
+        ctx = _HCtx()
+        host.kn_test_hctx_init(C.byref(ctx), C.addressof(om), om0, conftest.UNIT_TIME)
+        hh = knhip.BatchHistory()
+        assert kn.kn_batch_history_from_table(C.byref(d), 1.0, m, C.byref(hh)) == 0
+        hh.hubble, hh.hubble_ctx = hfn, C.addressof(ctx)
+        keep.append((om, d, ctx))
+        hists.append(hh)
+        data.append(dict(na=na, nk=nk, m=m, om0=om0, logs=logs, rows=np.array(rows[:nk, :na]), k=k,
+                         init=np.array(init)))
+        ksteps += nk * na
+    g_all = rng.uniform(-1, 1, sum(x["nk"] for x in data))
+    # plain path, one history at a time
+    plain_out, t_fwd, t_adj = [], 0.0, 0.0
+    off = 0
+    for (om, d, ctx), x in zip(keep, data):
+        host.kn_test_init_hubble(C.byref(om), x["om0"], conftest.UNIT_TIME)
+        out = np.zeros(x["nk"])
+        kk = np.ascontiguousarray(x["k"])
         t0 = time.perf_counter()
-        kn.get_delta_nu(C.byref(d), 1.0, kk.ctypes.data_as(DP), out.ctypes.data_as(DP), m)
-        t_total += time.perf_counter() - t0
-        total_ksteps += nk * na
+        kn.get_delta_nu(C.byref(d), 1.0, kk.ctypes.data_as(DP), out.ctypes.data_as(DP), x["m"])
+        t_fwd += time.perf_counter() - t0
+        gb = np.ascontiguousarray(g_all[off:off + x["nk"]])
+        db, ib, mb = np.zeros(x["nk"] * x["na"]), np.zeros(x["nk"]), C.c_double()
+        t0 = time.perf_counter()
+        rc = kn.kn_get_delta_nu_adjoint(C.byref(d), 1.0, kk.ctypes.data_as(DP), x["m"], gb.ctypes.data_as(DP),
+                                        None, db.ctypes.data_as(DP), ib.ctypes.data_as(DP), C.byref(mb))
+        t_adj += time.perf_counter() - t0
+        assert rc == 0
+        plain_out.append(out)
+        off += x["nk"]
+    # group: built once, then steps of solve + adjoint with everything resident
+    t0 = time.perf_counter()
+    B = knhip.Batch(hists, keep=keep, api="kn_group")
+    t_create = time.perf_counter() - t0
+    for i, x in enumerate(data):
+        B.load(i, x["logs"], x["rows"], x["k"], x["init"])
+    B._put("gbar", 0, g_all)
+    assert kn.kn_group_autotune(B.h, None) == 0
+    nstream = kn.kn_group_streams(B.h)
+    B.solve()
+    B.adjoint()
+    torch.cuda.synchronize()
+    ts_f, ts_fa = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        assert kn.kn_group_solve(B.h, None) == 0
+        t1 = time.perf_counter()
+        assert kn.kn_group_adjoint(B.h, None) == 0
+        t2 = time.perf_counter()
+        ts_f.append(t1 - t0)
+        ts_fa.append(t2 - t0)
+    s = torch.cuda.Stream()
+    assert kn.kn_group_graph_capture(B.h, 1, None, None, C.c_void_p(s.cuda_stream)) == 0
+    assert kn.kn_group_graph_launch(B.h, C.c_void_p(s.cuda_stream)) == 0
+    s.synchronize()
+    tg = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        kn.kn_group_graph_launch(B.h, C.c_void_p(s.cuda_stream))
+        s.synchronize()
+        tg.append(time.perf_counter() - t0)
+    assert kn.kn_group_status(B.h) == 0
+    dev = B.delta_nu()
+    diff = max(float(np.max(np.abs(dev[B.off_k[i]:B.off_k[i + 1]] / plain_out[i] - 1))) for i in range(nhist))
+    B.close()
+    for om, d, ctx in keep:
         kn.free_delta_tot_table(C.byref(d))
+    tf, tfa, tgr = float(np.median(ts_f)), float(np.median(ts_fa)), float(np.median(tg))
     return {"workload": "%d histories, Na~U[100,3000], nk~U[64,512], m~U[0.06,0.6] eV, Omega_m~U[0.25,0.35]" % nhist,
-            "plain_s": round(t_total, 4), "plain_kstep_per_s": round(total_ksteps / t_total, 1),
-            "ksteps": int(total_ksteps), "batched": None,
-            "note": "batched path switched off in this build (kn_batch_create returns 3005); "
-                    "plain = one C-ABI get_delta_nu call per history, host H(a) per history"}
+            "ksteps": int(ksteps),
+            "plain": {"forward_s": round(t_fwd, 4), "forward_adjoint_s": round(t_fwd + t_adj, 4),
+                      "forward_kstep_per_s": round(ksteps / t_fwd, 1),
+                      "note": "one get_delta_nu + one kn_get_delta_nu_adjoint C-ABI call per history, host in/out, "
+                              "H(a) table rebuilt per history (one hubble_function per process)"},
+            "group": {"forward_s": round(tf, 4), "forward_adjoint_s": round(tfa, 4),
+                      "graph_forward_adjoint_s": round(tgr, 4), "create_s": round(t_create, 3),
+                      "streams": nstream, "forward_kstep_per_s": round(ksteps / tf, 1),
+                      "note": "kn_group: histories resident in HBM, per-call kernels on autotuned HIP streams; "
+                              "create (one H table per history) once, outside the step"},
+            "speedup_forward_adjoint": round((t_fwd + t_adj) / min(tfa, tgr), 2),
+            "speedup_forward": round(t_fwd / tf, 2),
+            "max_rel_diff_group_vs_plain": diff,
+            "persistent_batch": "kn_batch (one persistent launch) switched off in this build: returns 3005"}
 
 
 def c5_adjoints(kn, torch):
@@ -214,6 +298,9 @@ def main():
     ap.add_argument("--no-dnu", action="store_true", help="skip the free-streaming integral timings")
     ap.add_argument("--only-dnu", action="store_true",
                     help="only the get_delta_nu_combined timing (757 k x 2000 steps), for profiling")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="only the 256-history plain vs grouped delta_nu + adjoint timing")
+    ap.add_argument("--batch-hists", type=int, default=256, help="histories for --only-batch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 2 GiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -237,6 +324,9 @@ def main():
     kn = knhip.lib()
     if args.only_dnu:
         print(json.dumps({"delta_nu": dnu_bench(kn, 757, no_cpu=True)}))
+        return
+    if args.only_batch:
+        print(json.dumps({"delta_nu_batch": c4_batching(kn, torch, nhist=args.batch_hists)}))
         return
 
     N, nrbins, box = args.n, args.nrbins, 512000.0
@@ -312,7 +402,7 @@ def main():
         del grid
         torch.cuda.empty_cache()
         out["delta_nu"] = dnu_bench(kn, n, no_cpu=args.no_cpu)
-        out["delta_nu_batch"] = c4_plain(kn)
+        out["delta_nu_batch"] = c4_batching(kn, torch)
         out["adjoints"] = c5_adjoints(kn, torch)
     print(json.dumps(out))
 

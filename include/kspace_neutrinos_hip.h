@@ -323,6 +323,40 @@ double kn_hubble_ctx(double a, void *ctx);
 /* Pick persistent-grid sizes by timing the forward. */
 int kn_batch_autotune(kn_batch *b, void *stream);
 
+/* ---- (b) grouped: many get_delta_nu integrals on the per-call kernels ----
+ * No reference counterpart (the reference evaluates one table per
+ * get_delta_nu call, delta_tot_table.c:507-611); additive.  The same
+ * histories, device views and layout as kn_batch, solved with the kernels of
+ * get_delta_nu: each history's launch sequence runs on one of up to 8 HIP
+ * streams forked from the caller's stream (longest histories first, each to
+ * the least-loaded stream), so the small per-history grids overlap.  Every
+ * solve records its final QAG subdivisions; kn_group_adjoint differentiates
+ * the last solve with them held fixed.  Return codes: 0, 2001 (QAG or
+ * interpolation failure, the history is named in kn_last_error), 3002 bad
+ * arguments, 3004 capacity. */
+typedef struct kn_group kn_group;
+int kn_group_create(kn_group **out, int nhist, const kn_batch_history *hist);
+void kn_group_destroy(kn_group *g);
+int kn_group_views(kn_group *g, double **scalefact, double **delta_tot, double **wavenum, double **delta_nu_init,
+                   double **delta_nu, double **gbar, double **dtot_bar, double **init_bar, double **mnu_bar);
+int kn_group_layout(const kn_group *g, long long *off_k, long long *off_y, long long *off_s);
+int kn_group_solve(kn_group *g, void *stream);   /* synchronous */
+int kn_group_adjoint(kn_group *g, void *stream); /* synchronous */
+/* d_loga[nhist], d_rows[items] in device memory; the host reads the new log a
+ * values (launch sizes follow them).  3004 before any write when a history is
+ * full or the new a is outside its H table (built up to max(a, a_max)). */
+int kn_group_append(kn_group *g, const double *d_loga, const double *d_rows, void *stream);
+/* Capture solve [+ adjoint] on a non-null stream; replayable until the next
+ * append or autotune (new masses and new values in the device arrays are read
+ * at replay).  d_loga and d_rows must be NULL: append is not capturable. */
+int kn_group_graph_capture(kn_group *g, int with_adjoint, const double *d_loga, const double *d_rows, void *stream);
+int kn_group_graph_launch(kn_group *g, void *stream);
+int kn_group_status(kn_group *g);
+int kn_group_set_mass(kn_group *g, int h, double mnu);
+/* Pick the stream count (1, 2, 4 or 8) by timing the forward. */
+int kn_group_autotune(kn_group *g, void *stream);
+int kn_group_streams(const kn_group *g);
+
 /* ---- adjoint of get_delta_nu (delta_tot_table.c:507-611) ----
  * gbar[nk] = dL/d delta_nu.  Outputs (host arrays, any may be NULL): the
  * forward delta_nu[nk], dL/d delta_tot[k][0..ia) (row stride ia), dL/d

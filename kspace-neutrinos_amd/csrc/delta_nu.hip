@@ -18,6 +18,7 @@
 // host's hubble_function on fixed panels (built once, extended on demand).
 #include <math.h>
 #include <string.h>
+#include <algorithm>
 #include <unordered_map>
 #include <vector>
 
@@ -55,19 +56,27 @@ void h_reset(double origin)
     g_h.dirty = true;
 }
 
-void h_build_panel(int p)
+// Chebyshev coefficients of ln H(e^x) on the panel [lo, lo + width].
+void cheb_panel(double lo, double width, double (*hub)(double, void *), void *ctx, double *coef)
 {
-    const double lo = g_h.x0 + p * g_h.width, half = 0.5 * g_h.width, mid = lo + half;
+    const double half = 0.5 * width, mid = lo + half;
     double g[kHnc];
     for (int j = 0; j < kHnc; j++) {
         const double t = cos(M_PI * (j + 0.5) / kHnc);
-        g[j] = log(kn_hubble(exp(mid + half * t)));
+        g[j] = log(hub(exp(mid + half * t), ctx));
     }
     for (int k = 0; k < kHnc; k++) {
         double s = 0.0;
         for (int j = 0; j < kHnc; j++) s += g[j] * cos(M_PI * k * (j + 0.5) / kHnc);
-        g_h.coef[(size_t)p * kHnc + k] = (k == 0 ? 1.0 : 2.0) * s / kHnc;
+        coef[k] = (k == 0 ? 1.0 : 2.0) * s / kHnc;
     }
+}
+
+double hub_process(double a, void *) { return kn_hubble(a); }
+
+void h_build_panel(int p)
+{
+    cheb_panel(g_h.x0 + p * g_h.width, g_h.width, hub_process, nullptr, &g_h.coef[(size_t)p * kHnc]);
     g_h.built[p] = 1;
 }
 
@@ -632,3 +641,632 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
     if (mnu_bar) *mnu_bar = mb / d->omnu->kBtnu;
     return 0;
 }
+
+// ================================================================ groups
+//
+// Many get_delta_nu integrals (one kn_batch_history each: its own H(a), mass,
+// stored steps and k bins) solved with the per-call kernels above.  Each
+// history's launch sequence goes to one of several HIP streams forked from
+// the caller's stream (longest histories first, each to the least-loaded
+// stream), so the small per-history grids run side by side on the device.
+// Histories live in device memory in the caller's layout (delta_tot rows of
+// stride na_max); while na < na_max the kernels read a packed copy (row
+// stride na) made by a 2-D device copy.  Every solve records the final QAG
+// subdivisions, which the adjoint then holds fixed.
+
+namespace {
+
+constexpr int kGroupStreams = 8;
+constexpr double kGroupHWidth = 1.0 / 256.0;  // ln H panel width (the per-call table's)
+
+// delta_nu[k] = IC(k) + prefac * raw[k]: the initial-condition piece of
+// delta_tot_table.c:522-535 (specialJ with qc = 0, m/kT or 1 when massless)
+// plus the integral; raw == nullptr for IC-only histories.
+__global__ void k_group_ic(const double *__restrict__ wavenum, const double *__restrict__ init,
+                           const double *__restrict__ raw, const SpeciesDev *__restrict__ sp, int nk, double fsl0,
+                           double lin, double prefac, double *__restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    const double m = sp->mnubykT > 0 ? sp->mnubykT : 1;
+    JPar J0;
+    J0.qc = 0.0;
+    J0.nufrac_low = sp->J.nufrac_low;
+    double v = specialJ(wavenum[k] * fsl0 / m, J0) * init[k] * lin;
+    if (raw) v += prefac * raw[k];
+    out[k] = v;
+}
+
+// Adjoint of the initial-condition piece: dL/d delta_nu_init[k] and the
+// piece's d/d(m/kT) term per k (zero when massless).
+__global__ void k_group_ic_adj(const double *__restrict__ wavenum, const double *__restrict__ init,
+                               const double *__restrict__ gbar, const SpeciesDev *__restrict__ sp, int nk, double fsl0,
+                               double lin, double *__restrict__ init_bar, double *__restrict__ mbar_ic)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    const double m = sp->mnubykT > 0 ? sp->mnubykT : 1;
+    JPar J0;
+    J0.qc = 0.0;
+    J0.nufrac_low = sp->J.nufrac_low;
+    const double kk = wavenum[k] * fsl0 / m;
+    double Jx, Jq;
+    const double J = specialJ_d(kk, J0, Jx, Jq);
+    init_bar[k] = gbar[k] * J * lin;
+    mbar_ic[k] = sp->mnubykT > 0 ? gbar[k] * Jx * (-kk / m) * init[k] * lin : 0.0;
+}
+
+struct GroupMeta {
+    long long off_k;
+    int nk, integ;
+    double kBtnu;
+};
+
+// dL/d mnu per history: the IC terms, then the integral's terms, in k order.
+__global__ void k_group_mass_bar(const GroupMeta *__restrict__ meta, int nhist, const double *__restrict__ mbar_ic,
+                                 const double *__restrict__ mubar, double *__restrict__ out)
+{
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nhist) return;
+    const GroupMeta g = meta[h];
+    double s = 0.0;
+    for (int k = 0; k < g.nk; k++) s += mbar_ic[g.off_k + k];
+    if (g.integ)
+        for (int k = 0; k < g.nk; k++) s += mubar[g.off_k + k];
+    out[h] = s / g.kBtnu;
+}
+
+__global__ void k_group_zero(double *__restrict__ p, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+struct HubCtx {
+    double (*fn)(double, void *);
+    void *ctx;
+};
+
+double fsl_integrand_ctx(double x, void *p)
+{
+    const HubCtx *h = (const HubCtx *)p;
+    const double a = exp(x);
+    return 1. / a / (a * h->fn(a, h->ctx));
+}
+
+struct GroupHist {
+    kn_batch_history spec;
+    int na, integ, htab, stream;
+    double x0, xa, xmax, fsl0, lin, vcrit;
+    long long off_k, off_y, off_s, off_fs, off_b;
+};
+
+}  // namespace
+
+struct kn_group {
+    int nhist = 0, nstream = 4, cap_err = 0;
+    long long nk_tot = 0, y_tot = 0, s_tot = 0, fs_tot = 0, b_tot = 0;
+    std::vector<GroupHist> h;
+    std::vector<HTab> htabs;
+    std::vector<double *> hcoef;
+    std::vector<int> order;
+    double *sx = nullptr, *Y = nullptr, *Yp = nullptr, *C = nullptr, *dbar = nullptr, *ybar = nullptr,
+           *cbar = nullptr, *zbuf = nullptr, *fx = nullptr, *fy = nullptr, *fc = nullptr, *wavenum = nullptr,
+           *init = nullptr, *out = nullptr, *raw = nullptr, *gbar = nullptr, *ibar = nullptr, *mbic = nullptr,
+           *mubar = nullptr, *mbar = nullptr, *ivl = nullptr;
+    int *bucket = nullptr, *nint = nullptr, *status = nullptr, *zeros = nullptr, *spec0 = nullptr;
+    SpeciesDev *sp = nullptr;
+    GroupMeta *meta = nullptr;
+    hipStream_t st[kGroupStreams] = {};
+    hipEvent_t fork = nullptr, join[kGroupStreams] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int have_graph = 0;
+    long long gen = 0, graph_gen = -1;
+};
+
+namespace {
+
+SpeciesDev species_dev(const kn_batch_history &s)
+{
+    SpeciesDev d;
+    d.mnubykT = s.mnu / s.kBtnu;
+    d.relerr = s.relerr;
+    d.J.qc = s.qc;
+    d.J.nufrac_low = s.nufrac_low;
+    d.J.en[0] = 1.0;
+    for (int n = 1; n < 20; n++) d.J.en[n] = exp(-n * s.qc);
+    return d;
+}
+
+// fslength(log T0, log a) with the history's own H (delta_tot_table.c:394-407),
+// and the initial-condition factor 1 + deriv_prefac fsl.
+int group_refresh_ic(GroupHist &q)
+{
+    HubCtx hc = {q.spec.hubble, q.spec.hubble_ctx};
+    double v = 0.0, e = 0.0;
+    if (q.x0 < q.xa && kn_qag61(fsl_integrand_ctx, &hc, q.x0, q.xa, 0, 1e-6, KN_GSL_VAL, &v, &e)) {
+        kn_set_last_error("kn_group: QAG failed in fslength(%g, %g)", q.x0, q.xa);
+        return 2001;
+    }
+    q.fsl0 = q.spec.light * v;
+    const double T0 = q.spec.time_transfer;
+    const double deriv_prefac = T0 * (q.spec.hubble(T0, q.spec.hubble_ctx) / q.spec.light) * T0;
+    q.lin = 1. + deriv_prefac * q.fsl0;
+    return 0;
+}
+
+void group_set_integ(GroupHist &q) { q.integ = !q.spec.ic_only && q.spec.mnu > 0 && q.na >= 2; }
+
+// Longest-first issue order, each history to the least-loaded stream.
+void group_schedule(kn_group *g)
+{
+    std::vector<double> cost(g->nhist);
+    for (int i = 0; i < g->nhist; i++) {
+        const GroupHist &q = g->h[i];
+        cost[i] = q.integ ? (double)(q.spec.nk + 16) * q.na : 1.0;
+    }
+    g->order.resize(g->nhist);
+    for (int i = 0; i < g->nhist; i++) g->order[i] = i;
+    std::stable_sort(g->order.begin(), g->order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<double> load(g->nstream, 0.0);
+    for (int i : g->order) {
+        int best = 0;
+        for (int s = 1; s < g->nstream; s++)
+            if (load[s] < load[best]) best = s;
+        g->h[i].stream = best;
+        load[best] += cost[i];
+    }
+}
+
+void group_meta_upload(kn_group *g)
+{
+    std::vector<GroupMeta> m(g->nhist);
+    for (int i = 0; i < g->nhist; i++) {
+        m[i].off_k = g->h[i].off_k;
+        m[i].nk = g->h[i].spec.nk;
+        m[i].integ = g->h[i].integ;
+        m[i].kBtnu = g->h[i].spec.kBtnu;
+    }
+    KN_CHECK(hipMemcpy(g->meta, m.data(), sizeof(GroupMeta) * g->nhist, hipMemcpyHostToDevice));
+}
+
+inline double group_sinv_bw(const GroupHist &q)
+{
+    // any positive bucket width gives the same segments (the lookup walks from
+    // its bucket); this one spans [log T0, log a]
+    return 4.0 * q.na / (q.xa - q.x0);
+}
+
+// Forward of one history on stream s, recording its subdivisions.
+void group_forward_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
+{
+    const int nk = q.spec.nk, na = q.na, na_max = q.spec.na_max;
+    const HTab H = g->htabs[q.htab];
+    const double *raw = nullptr;
+    if (q.integ) {
+        const int nfs = 16 * na, nsb = 4 * na;
+        const double sinv_bw = group_sinv_bw(q);
+        const double *Yk = g->Y + q.off_y;
+        if (na < na_max) {
+            KN_CHECK(hipMemcpy2DAsync(g->Yp + q.off_y, sizeof(double) * na, g->Y + q.off_y, sizeof(double) * na_max,
+                                      sizeof(double) * na, nk, hipMemcpyDeviceToDevice, s));
+            Yk = g->Yp + q.off_y;
+        }
+        double *fx = g->fx + q.off_fs, *fy = g->fy + q.off_fs, *fc = g->fc + q.off_fs;
+        const double *sx = g->sx + q.off_s;
+        hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, s, H, nfs, q.x0, q.xa,
+                           q.spec.light, fx, fy, g->status + 2 * i);
+        {
+            const long long t = (long long)((nfs + kChunk - 1) / kChunk);
+            hipLaunchKernelGGL(k_spline, dim3((int)((t + 127) / 128)), dim3(128), 0, s, fx, fy, fc, nfs, 1, 0LL, 0LL);
+        }
+        if (na > 2) {
+            const long long t = (long long)((na + kChunk - 1) / kChunk) * nk;
+            hipLaunchKernelGGL(k_spline, dim3((int)((t + 127) / 128)), dim3(128), 0, s, sx, Yk, g->C + q.off_y, na,
+                               nk, (long long)na, (long long)na);
+        }
+        hipLaunchKernelGGL(k_bucket, dim3((nsb + 256) / 256), dim3(256), 0, s, sx, na, nsb, sinv_bw,
+                           g->bucket + q.off_b);
+        hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nk)), dim3(64 * kWavesPerBlock), 0, s, H, fx, fy, fc, nfs, sx, Yk,
+                           g->C + q.off_y, g->bucket + q.off_b, na, nsb, sinv_bw, g->wavenum + q.off_k, nk, g->sp + i,
+                           g->spec0, nk, q.x0, q.xa, g->raw + q.off_k, g->nint + q.off_k, g->status + 2 * i,
+                           g->ivl + (size_t)q.off_k * kLimit * 2);
+        raw = g->raw + q.off_k;
+    }
+    hipLaunchKernelGGL(k_group_ic, dim3((nk + 63) / 64), dim3(64), 0, s, g->wavenum + q.off_k, g->init + q.off_k, raw,
+                       g->sp + i, nk, q.fsl0, q.lin, q.spec.delta_nu_prefac, g->out + q.off_k);
+}
+
+// Adjoint of one history's last forward on stream s.
+void group_adjoint_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
+{
+    const int nk = q.spec.nk, na = q.na, na_max = q.spec.na_max;
+    const long long nblk = (long long)nk * na_max;
+    double *dbar = g->dbar + q.off_y;
+    if (q.integ) {
+        const int nfs = 16 * na, nsb = 4 * na;
+        const double *Yk = na < na_max ? g->Yp + q.off_y : g->Y + q.off_y;
+        double *yb = na < na_max ? g->ybar + q.off_y : dbar;
+        const size_t lds = sizeof(double) * 2 * (size_t)na;
+        hipLaunchKernelGGL(k_dnu_adj, dim3(nk), dim3(64), lds, s, g->htabs[q.htab], g->fx + q.off_fs,
+                           g->fy + q.off_fs, g->fc + q.off_fs, nfs, g->sx + q.off_s, Yk, g->C + q.off_y,
+                           g->bucket + q.off_b, na, nsb, group_sinv_bw(q), g->wavenum + q.off_k, nk, g->sp + i,
+                           q.vcrit, g->ivl + (size_t)q.off_k * kLimit * 2, g->nint + q.off_k, g->gbar + q.off_k,
+                           q.spec.delta_nu_prefac, yb, g->cbar + q.off_y, g->mubar + q.off_k);
+        if (na > 2) {
+            const long long t = (long long)((na + kChunk - 1) / kChunk) * nk;
+            for (int phase = 0; phase < 2; phase++)
+                hipLaunchKernelGGL(k_dnu_adj_rows, dim3((int)((t + 127) / 128)), dim3(128), 0, s, g->sx + q.off_s, na,
+                                   nk, g->cbar + q.off_y, g->zbuf + q.off_y, yb, phase);
+        }
+        if (na < na_max) {
+            hipLaunchKernelGGL(k_group_zero, dim3((int)((nblk + 255) / 256)), dim3(256), 0, s, dbar, nblk);
+            KN_CHECK(hipMemcpy2DAsync(dbar, sizeof(double) * na_max, yb, sizeof(double) * na, sizeof(double) * na, nk,
+                                      hipMemcpyDeviceToDevice, s));
+        }
+    } else {
+        hipLaunchKernelGGL(k_group_zero, dim3((int)((nblk + 255) / 256)), dim3(256), 0, s, dbar, nblk);
+    }
+    hipLaunchKernelGGL(k_group_ic_adj, dim3((nk + 63) / 64), dim3(64), 0, s, g->wavenum + q.off_k, g->init + q.off_k,
+                       g->gbar + q.off_k, g->sp + i, nk, q.fsl0, q.lin, g->ibar + q.off_k, g->mbic + q.off_k);
+}
+
+// Fork the caller's stream into the group's streams, issue the histories,
+// join back.  Everything is asynchronous on origin (and capturable).
+void group_launch(kn_group *g, hipStream_t origin, bool forward, bool adjoint)
+{
+    if (forward)
+        KN_CHECK(hipMemcpyAsync(g->status, g->zeros, sizeof(int) * 2 * g->nhist, hipMemcpyDeviceToDevice, origin));
+    KN_CHECK(hipEventRecord(g->fork, origin));
+    for (int s = 0; s < g->nstream; s++) KN_CHECK(hipStreamWaitEvent(g->st[s], g->fork, 0));
+    for (int i : g->order) {
+        const GroupHist &q = g->h[i];
+        if (forward) group_forward_one(g, q, i, g->st[q.stream]);
+        if (adjoint) group_adjoint_one(g, q, i, g->st[q.stream]);
+    }
+    KN_CHECK(hipGetLastError());
+    for (int s = 0; s < g->nstream; s++) {
+        KN_CHECK(hipEventRecord(g->join[s], g->st[s]));
+        KN_CHECK(hipStreamWaitEvent(origin, g->join[s], 0));
+    }
+    if (adjoint) {
+        hipLaunchKernelGGL(k_group_mass_bar, dim3((g->nhist + 127) / 128), dim3(128), 0, origin, g->meta, g->nhist,
+                           g->mbic, g->mubar, g->mbar);
+        KN_CHECK(hipGetLastError());
+    }
+}
+
+int group_status(kn_group *g)
+{
+    if (g->cap_err) {
+        kn_set_last_error("kn_group: capacity (na_max or the H table range) exceeded");
+        return KN_ERR_RANGE;
+    }
+    std::vector<int> st(2 * (size_t)g->nhist);
+    KN_CHECK(hipMemcpy(st.data(), g->status, sizeof(int) * st.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < g->nhist; i++) {
+        if (st[2 * i + 1]) {
+            kn_set_last_error("kn_group: history %d: delta_nu integrand evaluated outside a stored table (GSL_EDOM)", i);
+            return 2001;
+        }
+        if (st[2 * i]) {  // GSL error -> terminate(2001) in the reference
+            kn_set_last_error("kn_group: history %d: QAG failed (%s)", i, (st[2 * i] & 1) ? "fslength" : "delta_nu integral");
+            return 2001;
+        }
+    }
+    return 0;
+}
+
+template <typename T>
+T *group_alloc(size_t n)
+{
+    T *p = nullptr;
+    KN_CHECK(hipMalloc(&p, sizeof(T) * (n > 0 ? n : 1)));
+    return p;
+}
+
+}  // namespace
+
+extern "C" void kn_group_destroy(kn_group *g)
+{
+    if (!g) return;
+    if (g->have_graph) {
+        KN_CHECK(hipGraphExecDestroy(g->exec));
+        KN_CHECK(hipGraphDestroy(g->graph));
+    }
+    void *ptrs[] = {g->sx,    g->Y,    g->Yp,     g->C,   g->dbar,  g->ybar,  g->cbar,   g->zbuf, g->fx,
+                    g->fy,    g->fc,   g->wavenum, g->init, g->out,  g->raw,   g->gbar,   g->ibar, g->mbic,
+                    g->mubar, g->mbar, g->ivl,    g->bucket, g->nint, g->status, g->zeros, g->spec0, g->sp,
+                    g->meta};
+    for (void *p : ptrs)
+        if (p) KN_CHECK(hipFree(p));
+    for (double *p : g->hcoef) KN_CHECK(hipFree(p));
+    for (int s = 0; s < kGroupStreams; s++) {
+        if (g->st[s]) KN_CHECK(hipStreamDestroy(g->st[s]));
+        if (g->join[s]) KN_CHECK(hipEventDestroy(g->join[s]));
+    }
+    if (g->fork) KN_CHECK(hipEventDestroy(g->fork));
+    delete g;
+}
+
+extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history *hist)
+{
+    if (!out || nhist < 1 || !hist) return KN_ERR_ARG;
+    for (int i = 0; i < nhist; i++) {
+        const kn_batch_history &s = hist[i];
+        if (s.nk < 1 || s.na < 1 || s.na_max < s.na || !s.hubble || !(s.a > s.time_transfer) || !(s.kBtnu > 0)) {
+            kn_set_last_error("kn_group_create: history %d: bad sizes, range or missing hubble", i);
+            return KN_ERR_ARG;
+        }
+        if (sizeof(double) * 2 * (size_t)s.na_max > 160 * 1024) {
+            kn_set_last_error("kn_group_create: history %d: na_max %d exceeds the adjoint's LDS row budget", i,
+                              s.na_max);
+            return KN_ERR_RANGE;
+        }
+    }
+    upload_constants();
+    kn_group *g = new kn_group();
+    g->nhist = nhist;
+    g->h.resize(nhist);
+    int na_big = 0;
+    for (int i = 0; i < nhist; i++) {
+        GroupHist &q = g->h[i];
+        q.spec = hist[i];
+        q.na = q.spec.na;
+        q.x0 = log(q.spec.time_transfer);
+        q.xa = log(q.spec.a);
+        q.xmax = log(q.spec.a_max > q.spec.a ? q.spec.a_max : q.spec.a);
+        q.vcrit = q.spec.qc > 0 ? q.spec.vcrit : 0.0;
+        group_set_integ(q);
+        q.off_k = g->nk_tot;
+        q.off_y = g->y_tot;
+        q.off_s = g->s_tot;
+        q.off_fs = g->fs_tot;
+        q.off_b = g->b_tot;
+        g->nk_tot += q.spec.nk;
+        g->y_tot += (long long)q.spec.nk * q.spec.na_max;
+        g->s_tot += q.spec.na_max;
+        g->fs_tot += 16LL * q.spec.na_max;
+        g->b_tot += 4LL * q.spec.na_max + 1;
+        na_big = q.spec.na_max > na_big ? q.spec.na_max : na_big;
+        const int rc = group_refresh_ic(q);
+        if (rc) {
+            kn_group_destroy(g);
+            return rc;
+        }
+        // one ln H table per (H, T0): panel edges on log T0, up to log max(a, a_max)
+        int found = -1;
+        for (int j = 0; j < i && found < 0; j++) {
+            const GroupHist &p = g->h[j];
+            if (p.spec.hubble == q.spec.hubble && p.spec.hubble_ctx == q.spec.hubble_ctx && p.x0 == q.x0 &&
+                p.xmax >= q.xmax)
+                found = p.htab;
+        }
+        if (found < 0) {
+            const int npanel = (int)ceil((q.xmax - q.x0) / kGroupHWidth) + 1;
+            std::vector<double> coef((size_t)npanel * kHnc);
+            for (int p = 0; p < npanel; p++)
+                cheb_panel(q.x0 + p * kGroupHWidth, kGroupHWidth, q.spec.hubble, q.spec.hubble_ctx,
+                           &coef[(size_t)p * kHnc]);
+            double *d = group_alloc<double>(coef.size());
+            KN_CHECK(hipMemcpy(d, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice));
+            HTab H;
+            H.coef = d;
+            H.x0 = q.x0;
+            H.inv_w = 1.0 / kGroupHWidth;
+            H.npanel = npanel;
+            g->hcoef.push_back(d);
+            g->htabs.push_back(H);
+            found = (int)g->htabs.size() - 1;
+        }
+        q.htab = found;
+    }
+    const size_t ny = (size_t)g->y_tot, nk = (size_t)g->nk_tot;
+    g->sx = group_alloc<double>(g->s_tot);
+    for (double **p : {&g->Y, &g->Yp, &g->C, &g->dbar, &g->ybar, &g->cbar, &g->zbuf}) *p = group_alloc<double>(ny);
+    for (double **p : {&g->fx, &g->fy, &g->fc}) *p = group_alloc<double>(g->fs_tot);
+    for (double **p : {&g->wavenum, &g->init, &g->out, &g->raw, &g->gbar, &g->ibar, &g->mbic, &g->mubar})
+        *p = group_alloc<double>(nk);
+    g->mbar = group_alloc<double>(nhist);
+    g->ivl = group_alloc<double>(nk * kLimit * 2);
+    g->bucket = group_alloc<int>(g->b_tot);
+    g->nint = group_alloc<int>(nk);
+    g->status = group_alloc<int>(2 * (size_t)nhist);
+    g->zeros = group_alloc<int>(2 * (size_t)nhist);
+    g->spec0 = group_alloc<int>(1);
+    KN_CHECK(hipMemset(g->zeros, 0, sizeof(int) * 2 * nhist));
+    KN_CHECK(hipMemset(g->status, 0, sizeof(int) * 2 * nhist));
+    KN_CHECK(hipMemset(g->spec0, 0, sizeof(int)));
+    g->sp = group_alloc<SpeciesDev>(nhist);
+    {
+        std::vector<SpeciesDev> sd(nhist);
+        for (int i = 0; i < nhist; i++) sd[i] = species_dev(g->h[i].spec);
+        KN_CHECK(hipMemcpy(g->sp, sd.data(), sizeof(SpeciesDev) * nhist, hipMemcpyHostToDevice));
+    }
+    g->meta = group_alloc<GroupMeta>(nhist);
+    group_meta_upload(g);
+    if (sizeof(double) * 2 * (size_t)na_big > 64 * 1024)
+        KN_CHECK(hipFuncSetAttribute((const void *)k_dnu_adj, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(sizeof(double) * 2 * (size_t)na_big)));
+    for (int s = 0; s < kGroupStreams; s++) {
+        KN_CHECK(hipStreamCreateWithFlags(&g->st[s], hipStreamNonBlocking));
+        KN_CHECK(hipEventCreateWithFlags(&g->join[s], hipEventDisableTiming));
+    }
+    KN_CHECK(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+    group_schedule(g);
+    *out = g;
+    return 0;
+}
+
+// Same views and layout as kn_batch_views / kn_batch_layout.
+extern "C" int kn_group_views(kn_group *g, double **scalefact, double **delta_tot, double **wavenum,
+                              double **delta_nu_init, double **delta_nu, double **gbar, double **dtot_bar,
+                              double **init_bar, double **mnu_bar)
+{
+    if (scalefact) *scalefact = g->sx;
+    if (delta_tot) *delta_tot = g->Y;
+    if (wavenum) *wavenum = g->wavenum;
+    if (delta_nu_init) *delta_nu_init = g->init;
+    if (delta_nu) *delta_nu = g->out;
+    if (gbar) *gbar = g->gbar;
+    if (dtot_bar) *dtot_bar = g->dbar;
+    if (init_bar) *init_bar = g->ibar;
+    if (mnu_bar) *mnu_bar = g->mbar;
+    return 0;
+}
+
+extern "C" int kn_group_layout(const kn_group *g, long long *off_k, long long *off_y, long long *off_s)
+{
+    for (int i = 0; i < g->nhist; i++) {
+        if (off_k) off_k[i] = g->h[i].off_k;
+        if (off_y) off_y[i] = g->h[i].off_y;
+        if (off_s) off_s[i] = g->h[i].off_s;
+    }
+    if (off_k) off_k[g->nhist] = g->nk_tot;
+    if (off_y) off_y[g->nhist] = g->y_tot;
+    if (off_s) off_s[g->nhist] = g->s_tot;
+    return 0;
+}
+
+extern "C" int kn_group_solve(kn_group *g, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    group_launch(g, st, true, false);
+    KN_CHECK(hipStreamSynchronize(st));
+    return group_status(g);
+}
+
+extern "C" int kn_group_adjoint(kn_group *g, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    group_launch(g, st, false, true);
+    KN_CHECK(hipStreamSynchronize(st));
+    return group_status(g);
+}
+
+// One more stored step per history: d_loga[nhist] and d_rows[items] in device
+// memory.  The host reads the new log a values (they set each history's upper
+// limit and its launch sizes), checks capacity and the H table range for all
+// histories before writing any, then copies scalefact[na] and column na of
+// delta_tot on the device.
+extern "C" int kn_group_append(kn_group *g, const double *d_loga, const double *d_rows, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<double> la(g->nhist);
+    KN_CHECK(hipMemcpyAsync(la.data(), d_loga, sizeof(double) * g->nhist, hipMemcpyDeviceToHost, st));
+    KN_CHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < g->nhist; i++) {
+        const GroupHist &q = g->h[i];
+        if (q.na >= q.spec.na_max || !(la[i] <= q.xmax) || !(la[i] > q.x0)) {
+            g->cap_err = 1;
+            kn_set_last_error("kn_group_append: history %d: capacity (na_max %d) or H table range exceeded", i,
+                              q.spec.na_max);
+            return KN_ERR_RANGE;
+        }
+    }
+    for (int i = 0; i < g->nhist; i++) {
+        GroupHist &q = g->h[i];
+        KN_CHECK(hipMemcpyAsync(g->sx + q.off_s + q.na, d_loga + i, sizeof(double), hipMemcpyDeviceToDevice, st));
+        KN_CHECK(hipMemcpy2DAsync(g->Y + q.off_y + q.na, sizeof(double) * q.spec.na_max, d_rows + q.off_k,
+                                  sizeof(double), sizeof(double), q.spec.nk, hipMemcpyDeviceToDevice, st));
+        q.na++;
+        q.xa = la[i];
+        q.spec.a = exp(la[i]);
+        group_set_integ(q);
+        const int rc = group_refresh_ic(q);
+        if (rc) return rc;
+    }
+    group_meta_upload(g);
+    group_schedule(g);
+    g->gen++;
+    KN_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Capture solve [+ adjoint] on stream (not the null stream) into one graph.
+// Launch sizes follow the stored steps, so a graph is replayable until the
+// next append (new masses and new delta_tot/wavenum/init values in the
+// device arrays are read at replay).  Append is host-driven and cannot be
+// captured: d_loga and d_rows must be NULL.
+extern "C" int kn_group_graph_capture(kn_group *g, int with_adjoint, const double *d_loga, const double *d_rows,
+                                      void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (!st || d_loga || d_rows) {
+        kn_set_last_error("kn_group_graph_capture: needs a non-null stream; append is not capturable");
+        return KN_ERR_ARG;
+    }
+    if (g->have_graph) {
+        KN_CHECK(hipGraphExecDestroy(g->exec));
+        KN_CHECK(hipGraphDestroy(g->graph));
+        g->have_graph = 0;
+    }
+    KN_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    group_launch(g, st, true, with_adjoint != 0);
+    KN_CHECK(hipStreamEndCapture(st, &g->graph));
+    KN_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    g->have_graph = 1;
+    g->graph_gen = g->gen;
+    return 0;
+}
+
+extern "C" int kn_group_graph_launch(kn_group *g, void *stream)
+{
+    if (!g->have_graph || g->graph_gen != g->gen) {
+        kn_set_last_error("kn_group_graph_launch: no graph for the current stored steps (capture after append)");
+        return KN_ERR_ARG;
+    }
+    KN_CHECK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int kn_group_status(kn_group *g) { return group_status(g); }
+
+// New neutrino mass (eV) of history h; qc follows it for a hybrid history.
+// Only the device species record changes, so a captured graph stays valid.
+extern "C" int kn_group_set_mass(kn_group *g, int h, double mnu)
+{
+    if (h < 0 || h >= g->nhist || !(mnu > 0)) return KN_ERR_ARG;
+    GroupHist &q = g->h[h];
+    q.spec.mnu = mnu;
+    if (q.spec.qc > 0) q.spec.qc = q.spec.vcrit * (mnu / q.spec.kBtnu);
+    const SpeciesDev sd = species_dev(q.spec);
+    KN_CHECK(hipMemcpy(g->sp + h, &sd, sizeof sd, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Pick the stream count by timing the forward (1, 2, 4, 8 streams).
+extern "C" int kn_group_autotune(kn_group *g, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const int cand[] = {1, 2, 4, 8};
+    float best = 1e30f;
+    int bs = g->nstream;
+    hipEvent_t e0, e1;
+    KN_CHECK(hipEventCreate(&e0));
+    KN_CHECK(hipEventCreate(&e1));
+    for (int c : cand) {
+        g->nstream = c;
+        group_schedule(g);
+        group_launch(g, st, true, false);
+        KN_CHECK(hipEventRecord(e0, st));
+        group_launch(g, st, true, false);
+        KN_CHECK(hipEventRecord(e1, st));
+        KN_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        KN_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            bs = c;
+        }
+    }
+    KN_CHECK(hipEventDestroy(e0));
+    KN_CHECK(hipEventDestroy(e1));
+    g->nstream = bs;
+    group_schedule(g);
+    g->gen++;  // the stream assignment changed: a captured graph is stale
+    return group_status(g);
+}
+
+extern "C" int kn_group_streams(const kn_group *g) { return g->nstream; }

@@ -130,6 +130,19 @@ SIGNATURES = {
     "kn_batch_history_from_table": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, C.c_double,
                                               C.POINTER(BatchHistory)]),
     "kn_hubble_ctx": (C.c_double, [C.c_double, C.c_void_p]),
+    "kn_group_create": (C.c_int, [vpp, C.c_int, C.POINTER(BatchHistory)]),
+    "kn_group_destroy": (None, [C.c_void_p]),
+    "kn_group_views": (C.c_int, [C.c_void_p] + [vpp] * 9),
+    "kn_group_layout": (C.c_int, [C.c_void_p, llp, llp, llp]),
+    "kn_group_solve": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kn_group_adjoint": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kn_group_append": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kn_group_graph_capture": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kn_group_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kn_group_status": (C.c_int, [C.c_void_p]),
+    "kn_group_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kn_group_set_mass": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "kn_group_streams": (C.c_int, [C.c_void_p]),
     "kn_get_delta_nu_adjoint": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, dp, C.c_double, dp, dp, dp, dp,
                                           dp]),
     "kn_batch_get_delta_nu_adjoint": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, dp, C.c_double, dp, dp, dp,
@@ -196,7 +209,8 @@ def bind(L):
 
 
 class Batch:
-    """Owner of a kn_batch: loads histories into its device arrays and reads results back.
+    """Owner of a kn_batch (api "kn_batch") or kn_group (api "kn_group"): loads
+    histories into its device arrays and reads results back.
 
     hists: list of BatchHistory (their hubble_ctx objects must outlive the batch).
     L, memcpy: the library build and its device copy (defaults: this library, hipMemcpy).
@@ -205,9 +219,10 @@ class Batch:
     VIEWS = ("scalefact", "delta_tot", "wavenum", "delta_nu_init", "delta_nu", "gbar", "dtot_bar", "init_bar",
              "mnu_bar")
 
-    def __init__(self, hists, keep=(), L=None, memcpy=None):
+    def __init__(self, hists, keep=(), L=None, memcpy=None, api="kn_batch"):
         L = L or lib()
         self.L = L
+        self.api = api
         self._mc = memcpy or _memcpy
         self.h = None
         self.n = len(hists)
@@ -215,15 +230,18 @@ class Batch:
         self._keep = keep
         arr = (BatchHistory * self.n)(*hists)
         h = C.c_void_p()
-        _check(L.kn_batch_create(C.byref(h), self.n, arr), "kn_batch_create", L)
+        _check(self._f("create")(C.byref(h), self.n, arr), self.api + "_create", L)
         self.h = h
         ptrs = [C.c_void_p() for _ in self.VIEWS]
-        _check(L.kn_batch_views(h, *[C.byref(p) for p in ptrs]), "kn_batch_views", L)
+        _check(self._f("views")(h, *[C.byref(p) for p in ptrs]), self.api + "_views", L)
         self.ptr = {name: p.value for name, p in zip(self.VIEWS, ptrs)}
         self.off_k = (C.c_longlong * (self.n + 1))()
         self.off_y = (C.c_longlong * (self.n + 1))()
         self.off_s = (C.c_longlong * (self.n + 1))()
-        _check(L.kn_batch_layout(h, self.off_k, self.off_y, self.off_s), "kn_batch_layout", L)
+        _check(self._f("layout")(h, self.off_k, self.off_y, self.off_s), self.api + "_layout", L)
+
+    def _f(self, name):
+        return getattr(self.L, self.api + "_" + name)
 
     def _put(self, name, off, arr):
         import numpy as np
@@ -249,12 +267,12 @@ class Batch:
         self._put("delta_nu_init", self.off_k[i], init)
 
     def solve(self):
-        _check(self.L.kn_batch_solve(self.h, None), "kn_batch_solve", self.L)
+        _check(self._f("solve")(self.h, None), self.api + "_solve", self.L)
 
     def adjoint(self, gbar_all=None):
         if gbar_all is not None:
             self._put("gbar", 0, gbar_all)
-        _check(self.L.kn_batch_adjoint(self.h, None), "kn_batch_adjoint", self.L)
+        _check(self._f("adjoint")(self.h, None), self.api + "_adjoint", self.L)
 
     def append(self, loga, rows):
         """One more stored step per history (host arrays: log a per history, rows per item)."""
@@ -264,11 +282,11 @@ class Batch:
         # staged in the batch's gbar (rows) and mnu_bar (log a) arrays, which this overwrites
         self._put("gbar", 0, rw)
         self._put("mnu_bar", 0, la)
-        _check(self.L.kn_batch_append(self.h, C.c_void_p(self.ptr["mnu_bar"]), C.c_void_p(self.ptr["gbar"]), None),
-               "kn_batch_append", self.L)
+        _check(self._f("append")(self.h, C.c_void_p(self.ptr["mnu_bar"]), C.c_void_p(self.ptr["gbar"]), None),
+               self.api + "_append", self.L)
 
     def status(self):
-        return self.L.kn_batch_status(self.h)
+        return self._f("status")(self.h)
 
     def delta_nu(self, i=None):
         if i is None:
@@ -286,11 +304,11 @@ class Batch:
         return self._get("mnu_bar", 0, self.n)
 
     def set_mass(self, i, mnu):
-        _check(self.L.kn_batch_set_mass(self.h, i, mnu), "kn_batch_set_mass", self.L)
+        _check(self._f("set_mass")(self.h, i, mnu), self.api + "_set_mass", self.L)
 
     def close(self):
         if self.h:
-            self.L.kn_batch_destroy(self.h)
+            self._f("destroy")(self.h)
             self.h = None
 
     def __del__(self):

@@ -49,8 +49,8 @@ class Cosmo:
 
     def oracle(self, nk, logs, rows, init):
         O.lib().orc_init_hubble(C.byref(self.P.oom), self.omega0, UNIT_TIME)
-        T = O.Table(self.P.oom, nk, 0.01, 2.0, self.omega0, UNIT_TIME, UNIT_LENGTH)
         na = len(logs)
+        T = O.Table(self.P.oom, nk, 0.01, max(2.0, 0.01 + na / 100.0 + 0.02), self.omega0, UNIT_TIME, UNIT_LENGTH)
         T.scalefact[:na] = logs
         T.delta_tot[:, :na] = rows
         T.delta_nu_init[:] = init
@@ -89,8 +89,8 @@ def compare_adjoint(name, gpu, orc):
     assert e_f < RTOL and e_y < RTOL and e_i < RTOL and e_m < RTOL
 
 
-def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None):
-    """Histories over three cosmologies with mixed lengths and k counts."""
+def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None, api="kn_batch"):
+    """Histories over three cosmologies with mixed lengths and k counts (a kn_batch or a kn_group)."""
     rng = np.random.default_rng(seed)
     cosmos = [Cosmo([0.15, 0, 0]), Cosmo([0.1, 0.1, 0.1]), Cosmo([0.3, 0.06, 0.06], hybrid=(500.0, 0.3333))]
     specs = []
@@ -104,7 +104,7 @@ def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None):
         specs.append(dict(c=c, na=na, nk=nk, amax=amax, logs=logs, rows=rows, k=k, init=init, m=m))
     hists = [cosmos[s["c"]].history(s["nk"], s["na"], s["na"] + extra, s["amax"], 1.0 if extra else s["amax"],
                                     s["m"]) for s in specs]
-    B = knhip.Batch(hists, keep=cosmos, L=L, memcpy=memcpy)
+    B = knhip.Batch(hists, keep=cosmos, L=L, memcpy=memcpy, api=api)
     for i, s in enumerate(specs):
         B.load(i, s["logs"], s["rows"], s["k"], s["init"])
     return cosmos, specs, B

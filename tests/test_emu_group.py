@@ -1,0 +1,150 @@
+"""(b) grouped get_delta_nu (kn_group_*) in the CPU SIMT emulation build of
+csrc/delta_nu.hip (tests/emu), against the oracle.
+
+A group solves many histories with the per-call kernels, one HIP stream per
+history slot; in the emulation every kernel and copy runs when issued, so
+these tests check the per-history launch sequences, offsets, packing of
+histories with spare capacity, the initial-condition kernels, the adjoint
+bookkeeping and the host-side checks.  Bars as on the device: 1e-9 relative.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+import knhip
+from batch_common import RTOL, Cosmo, compare_adjoint, mixed_batch, rel
+from conftest import ROOT
+from dnu_common import synthetic_history
+
+DNU_EMU_LIB = os.path.join(ROOT, "tests", "emu", "_build", "libkn_dnu_emu.so")
+
+
+def host_memcpy(dst, src, n, kind):
+    C.memmove(dst, src, n)
+
+
+@pytest.fixture(scope="module")
+def emu(kn):
+    if not os.path.exists(DNU_EMU_LIB):
+        raise RuntimeError("emulation build missing: make -C tests/emu")
+    return knhip.bind(C.CDLL(DNU_EMU_LIB))
+
+
+def _forward_errors(B, cosmos, specs):
+    worst = 0.0
+    for i, s in enumerate(specs):
+        T = cosmos[s["c"]].oracle(s["nk"], s["logs"], s["rows"], s["init"])
+        rc, o = T.get_delta_nu(s["amax"], s["k"], s["m"])
+        assert rc == 0
+        worst = max(worst, rel(B.delta_nu(i), o))
+    return worst
+
+
+def test_emu_group_mixed_forward_and_adjoint(emu):
+    """Twelve histories over three cosmologies (one hybrid), Na from 2 to 150,
+    some with spare capacity (packed copies), forward and adjoint."""
+    cosmos, specs, B = mixed_batch(4, seed=2, extra=3, L=emu, memcpy=host_memcpy, api="kn_group")
+    B.solve()
+    worst = _forward_errors(B, cosmos, specs)
+    print("  %d histories: worst forward rel err %.3e" % (B.n, worst))
+    assert worst < RTOL
+    g = np.random.default_rng(7).uniform(-1, 1, B.off_k[B.n])
+    B.adjoint(g)
+    mb = B.mnu_bar()
+    for i, s in enumerate(specs):
+        T = cosmos[s["c"]].oracle(s["nk"], s["logs"], s["rows"], s["init"])
+        gi = g[B.off_k[i]:B.off_k[i] + s["nk"]]
+        orc = T.get_delta_nu_adjoint(s["amax"], s["k"], s["m"], gi)
+        compare_adjoint("hist %d (Na=%d nk=%d)" % (i, s["na"], s["nk"]),
+                        (B.delta_nu(i), B.dtot_bar(i, s["na"]), B.init_bar(i), mb[i]), orc)
+        # entries beyond the stored steps are zero
+        hs = B.hists[i]
+        full = B._get("dtot_bar", B.off_y[i], hs.nk * hs.na_max).reshape(hs.nk, hs.na_max)
+        assert np.all(full[:, s["na"]:] == 0)
+    B.close()
+    for c in cosmos:
+        c.free()
+
+
+def test_emu_group_append_steps_and_capacity(emu):
+    """Two appended steps per history, then a third that overflows na_max:
+    3004 before anything is written, and the group's status stays 3004."""
+    cosmos, specs, B = mixed_batch(2, seed=4, extra=2, L=emu, memcpy=host_memcpy, api="kn_group")
+    rng = np.random.default_rng(5)
+    for step in range(2):
+        loga, rows = np.zeros(B.n), np.zeros(B.off_k[B.n])
+        for i, s in enumerate(specs):
+            new_a = math.exp(s["logs"][-1]) * 1.02
+            s["logs"] = np.append(s["logs"], math.log(new_a))
+            r = s["rows"][:, -1] * 1.02 * (1 + 0.01 * rng.standard_normal(s["nk"]))
+            s["rows"] = np.concatenate([s["rows"], r[:, None]], axis=1)
+            s["amax"] = new_a
+            loga[i] = math.log(new_a)
+            rows[B.off_k[i]:B.off_k[i] + s["nk"]] = r
+        B.append(loga, rows)
+        B.solve()
+        worst = _forward_errors(B, cosmos, specs)
+        print("  append step %d: worst rel err %.3e" % (step, worst))
+        assert worst < RTOL
+    sf_before = B._get("scalefact", 0, B.off_s[B.n])
+    with pytest.raises(RuntimeError, match="code 3004"):
+        B.append(np.zeros(B.n) + math.log(0.99), np.zeros(B.off_k[B.n]))
+    assert np.array_equal(B._get("scalefact", 0, B.off_s[B.n]), sf_before)
+    assert B.status() == 3004
+    B.close()
+    for c in cosmos:
+        c.free()
+
+
+def test_emu_group_set_mass(emu):
+    """A new mass for a plain and a hybrid history (qc follows the mass)."""
+    cosmos, specs, B = mixed_batch(1, seed=6, L=emu, memcpy=host_memcpy, api="kn_group")
+    B.solve()
+    for i, s in enumerate(specs):
+        if s["na"] > 2:
+            s["m"] = s["m"] * 1.3
+            B.set_mass(i, s["m"])
+    B.solve()
+    worst = _forward_errors(B, cosmos, specs)
+    print("  after set_mass: worst rel err %.3e" % worst)
+    assert worst < RTOL
+    B.close()
+    for c in cosmos:
+        c.free()
+
+
+def test_emu_group_out_of_range_a_reports_2001(emu):
+    """An upper limit beyond the stored steps makes the integrand leave the
+    delta_tot table: GSL_EDOM -> 2001, with the history named."""
+    c = Cosmo([0.15, 0, 0])
+    logs, rows, k, init = synthetic_history(10, 4, 0.2)
+    hh = c.history(4, 10, 10, 0.25, 0.25, 0.15)
+    B = knhip.Batch([hh], keep=(c,), L=emu, memcpy=host_memcpy, api="kn_group")
+    B.load(0, logs, rows, k, init)
+    with pytest.raises(RuntimeError, match="code 2001.*history 0"):
+        B.solve()
+    B.close()
+    c.free()
+
+
+def test_emu_group_argument_checks(emu):
+    c = Cosmo([0.15, 0, 0])
+    hh = c.history(4, 10, 10, 0.25, 0.25, 0.15)
+    g = C.c_void_p()
+    bad = knhip.BatchHistory.from_buffer_copy(hh)
+    bad.na_max = 20000  # adjoint row of 2 x 20000 doubles does not fit in LDS
+    assert emu.kn_group_create(C.byref(g), 1, C.byref(bad)) == 3004
+    bad = knhip.BatchHistory.from_buffer_copy(hh)
+    bad.a = bad.time_transfer
+    assert emu.kn_group_create(C.byref(g), 1, C.byref(bad)) == 3002
+    assert emu.kn_group_create(C.byref(g), 1, C.byref(hh)) == 0
+    # append is not capturable; a null stream cannot be captured; no graph yet
+    assert emu.kn_group_graph_capture(g, 0, C.c_void_p(8), None, C.c_void_p(1)) == 3002
+    assert emu.kn_group_graph_capture(g, 0, None, None, None) == 3002
+    assert emu.kn_group_graph_launch(g, None) == 3002
+    assert emu.kn_group_set_mass(g, 1, 0.2) == 3002 and emu.kn_group_set_mass(g, 0, 0.0) == 3002
+    emu.kn_group_destroy(g)
+    c.free()

@@ -1,0 +1,209 @@
+"""(b) grouped get_delta_nu on the GPU (kn_group_*): many histories, each with
+its own H(a), mass, stored steps and k bins, solved with the per-call kernels
+on several HIP streams; forward and adjoint against the oracle, appended
+steps, new masses, graph replays and the stream-count autotuner.
+
+Bars: delta_nu and its gradients within 1e-9 relative of the oracle (the
+d/d delta_tot rows relative to their largest entry); graph replays and
+re-solves bit-identical to the eager solve they repeat.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import knhip
+from batch_common import RTOL, Cosmo, compare_adjoint, mixed_batch, rel
+from dnu_common import synthetic_history
+
+pytestmark = pytest.mark.gpu
+
+
+def _forward_errors(B, cosmos, specs):
+    worst = 0.0
+    for i, s in enumerate(specs):
+        T = cosmos[s["c"]].oracle(s["nk"], s["logs"], s["rows"], s["init"])
+        rc, o = T.get_delta_nu(s["amax"], s["k"], s["m"])
+        assert rc == 0
+        worst = max(worst, rel(B.delta_nu(i), o))
+    return worst
+
+
+def _free(cosmos):
+    for c in cosmos:
+        c.free()
+
+
+@pytest.mark.parametrize("extra", [0, 3])
+def test_group_mixed_forward_and_adjoint(kn, torch_cuda, extra):
+    """48 histories over three cosmologies (one hybrid), Na 2..150, nk 4..24;
+    extra > 0 leaves spare capacity, so the kernels read packed copies."""
+    print("group mixed, extra capacity %d" % extra, flush=True)
+    cosmos, specs, B = mixed_batch(16, seed=2, extra=extra, api="kn_group")
+    B.solve()
+    worst = _forward_errors(B, cosmos, specs)
+    print("  %d histories: worst forward rel err %.3e" % (B.n, worst))
+    assert worst < RTOL
+    g = np.random.default_rng(7).uniform(-1, 1, B.off_k[B.n])
+    B.adjoint(g)
+    mb = B.mnu_bar()
+    for i, s in enumerate(specs):
+        T = cosmos[s["c"]].oracle(s["nk"], s["logs"], s["rows"], s["init"])
+        gi = g[B.off_k[i]:B.off_k[i] + s["nk"]]
+        orc = T.get_delta_nu_adjoint(s["amax"], s["k"], s["m"], gi)
+        compare_adjoint("hist %d (Na=%d nk=%d)" % (i, s["na"], s["nk"]),
+                        (B.delta_nu(i), B.dtot_bar(i, s["na"]), B.init_bar(i), mb[i]), orc)
+    B.close()
+    _free(cosmos)
+
+
+def test_group_append_steps_and_capacity(kn, torch_cuda):
+    print("group append", flush=True)
+    cosmos, specs, B = mixed_batch(4, seed=4, extra=2, api="kn_group")
+    rng = np.random.default_rng(5)
+    for step in range(2):
+        loga, rows = np.zeros(B.n), np.zeros(B.off_k[B.n])
+        for i, s in enumerate(specs):
+            new_a = math.exp(s["logs"][-1]) * 1.02
+            s["logs"] = np.append(s["logs"], math.log(new_a))
+            r = s["rows"][:, -1] * 1.02 * (1 + 0.01 * rng.standard_normal(s["nk"]))
+            s["rows"] = np.concatenate([s["rows"], r[:, None]], axis=1)
+            s["amax"] = new_a
+            loga[i] = math.log(new_a)
+            rows[B.off_k[i]:B.off_k[i] + s["nk"]] = r
+        B.append(loga, rows)
+        B.solve()
+        worst = _forward_errors(B, cosmos, specs)
+        print("  append step %d: worst rel err %.3e" % (step, worst))
+        assert worst < RTOL
+    with pytest.raises(RuntimeError, match="code 3004"):
+        B.append(np.zeros(B.n) + math.log(0.99), np.zeros(B.off_k[B.n]))
+    assert B.status() == 3004
+    B.close()
+    _free(cosmos)
+
+
+def test_group_set_mass_and_out_of_range(kn, torch_cuda):
+    print("group set_mass", flush=True)
+    cosmos, specs, B = mixed_batch(2, seed=6, api="kn_group")
+    B.solve()
+    for i, s in enumerate(specs):
+        if s["na"] > 2:
+            s["m"] = s["m"] * 1.3
+            B.set_mass(i, s["m"])
+    B.solve()
+    worst = _forward_errors(B, cosmos, specs)
+    print("  after set_mass: worst rel err %.3e" % worst)
+    assert worst < RTOL
+    B.close()
+    _free(cosmos)
+    c = Cosmo([0.15, 0, 0])
+    logs, rows, k, init = synthetic_history(10, 4, 0.2)
+    hh = c.history(4, 10, 10, 0.25, 0.25, 0.15)
+    B = knhip.Batch([hh], keep=(c,), api="kn_group")
+    B.load(0, logs, rows, k, init)
+    print("group out-of-range a", flush=True)
+    with pytest.raises(RuntimeError, match="code 2001.*history 0"):
+        B.solve()
+    B.close()
+    c.free()
+
+
+def test_group_graph_replays_and_autotune(kn, torch_cuda):
+    """solve + adjoint captured once on a torch stream and replayed three
+    times with eager solves in between (each replay bit-identical to the eager
+    result), then a new mass read at replay; the autotuner's choice gives the
+    same results bit for bit."""
+    torch = torch_cuda
+    cosmos, specs, B = mixed_batch(4, seed=8, api="kn_group")
+    g = np.random.default_rng(9).uniform(-1, 1, B.off_k[B.n])
+    B.solve()
+    B.adjoint(g)
+    ref = (B.delta_nu(), B._get("dtot_bar", 0, B.off_y[B.n]), B._get("init_bar", 0, B.off_k[B.n]), B.mnu_bar())
+    s = torch.cuda.Stream()
+    print("group graph capture", flush=True)
+    assert kn.kn_group_graph_capture(B.h, 1, None, None, C.c_void_p(s.cuda_stream)) == 0
+    for rep in range(3):
+        B._put("delta_nu", 0, np.zeros(B.off_k[B.n]))
+        B._put("mnu_bar", 0, np.zeros(B.n))
+        print("  replay %d" % rep, flush=True)
+        assert kn.kn_group_graph_launch(B.h, C.c_void_p(s.cuda_stream)) == 0
+        s.synchronize()
+        assert kn.kn_group_status(B.h) == 0
+        got = (B.delta_nu(), B._get("dtot_bar", 0, B.off_y[B.n]), B._get("init_bar", 0, B.off_k[B.n]), B.mnu_bar())
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+        B.solve()  # eager work between replays
+    i = next(j for j, sp in enumerate(specs) if sp["na"] > 2)
+    specs[i]["m"] *= 1.2
+    B.set_mass(i, specs[i]["m"])
+    assert kn.kn_group_graph_launch(B.h, C.c_void_p(s.cuda_stream)) == 0
+    s.synchronize()
+    replay = B.delta_nu()
+    B.solve()
+    assert np.array_equal(replay, B.delta_nu())
+    worst = _forward_errors(B, cosmos, specs)
+    print("  new mass through the graph: worst rel err %.3e" % worst)
+    assert worst < RTOL
+    print("group autotune", flush=True)
+    assert kn.kn_group_autotune(B.h, None) == 0
+    ns = kn.kn_group_streams(B.h)
+    assert ns in (1, 2, 4, 8)
+    before = B.delta_nu()
+    B.solve()
+    assert np.array_equal(before, B.delta_nu())
+    print("  autotuned streams: %d" % ns)
+    # the graph is stale after autotune (new stream assignment)
+    assert kn.kn_group_graph_launch(B.h, C.c_void_p(s.cuda_stream)) == 3002
+    B.close()
+    _free(cosmos)
+
+
+def test_group_at_batching_workload_shape(kn, torch_cuda):
+    """24 histories drawn like the 256-history batching workload (Na ~
+    U[100, 3000], nk ~ U[64, 512], m ~ U[0.06, 0.6] eV, Omega_m ~ U[0.25, 0.35],
+    per-k noise), forward and adjoint on the GPU; 3 sampled k per history
+    against the oracle."""
+    rng = np.random.default_rng(1000)
+    cosmos, specs, hists = [], [], []
+    for h in range(24):
+        na, nk = int(rng.integers(100, 3001)), int(rng.integers(64, 513))
+        m, om0 = float(rng.uniform(0.06, 0.6)), float(rng.uniform(0.25, 0.35))
+        c = Cosmo([m, 0.0, 0.0], omega0=om0)
+        logs = np.linspace(math.log(0.01), 0.0, na)
+        k = np.exp(np.linspace(math.log(2 * math.pi / 512), math.log(math.sqrt(3) * math.pi * 1024 / 512), nk)) / 1e3
+        q = k * 1000 / 0.2
+        t = np.log(1 + 2.34 * q) / (2.34 * q) * (1 + 3.89 * q + (16.1 * q) ** 2 + (5.46 * q) ** 3 + (6.71 * q) ** 4) ** -0.25
+        amp = np.sqrt(1e9 * (k * 1000) ** 0.96 * t * t) * (1 + 0.1 * rng.standard_normal(nk))
+        rows = np.outer(amp, np.exp(logs))
+        cosmos.append(c)
+        specs.append(dict(na=na, nk=nk, m=m, logs=logs, rows=rows, k=k, init=0.4 * rows[:, 0]))
+        hists.append(c.history(nk, na, na, 1.0, 1.0, m))
+    B = knhip.Batch(hists, keep=cosmos, api="kn_group")
+    for i, s in enumerate(specs):
+        B.load(i, s["logs"], s["rows"], s["k"], s["init"])
+    print("group at the batching workload shape: %d histories" % B.n, flush=True)
+    B.solve()
+    g = rng.uniform(-1, 1, B.off_k[B.n])
+    B.adjoint(g)
+    mb = B.mnu_bar()
+    worst_f, worst_b = 0.0, 0.0
+    for i, s in enumerate(specs):
+        idx = np.array(sorted(rng.choice(s["nk"], 3, replace=False)))
+        T = cosmos[i].oracle(len(idx), s["logs"], s["rows"][idx], s["init"][idx])
+        gi = np.zeros(len(idx))
+        gi[:] = g[B.off_k[i] + idx]
+        orc = T.get_delta_nu_adjoint(1.0, s["k"][idx], s["m"], gi)
+        assert orc[0] == 0
+        dnu = B.delta_nu(i)[idx]
+        worst_f = max(worst_f, rel(dnu, orc[1]))
+        db = B.dtot_bar(i, s["na"])[idx]
+        scale = np.maximum(np.max(np.abs(orc[2]), axis=1, keepdims=True), 1e-300)
+        worst_b = max(worst_b, float(np.max(np.abs(db - orc[2]) / scale)),
+                      float(np.max(np.abs(B.init_bar(i)[idx] - orc[3]) / np.max(np.abs(orc[3])))))
+    print("  sampled k: forward %.3e, d/d delta_tot and d/d init %.3e" % (worst_f, worst_b))
+    assert worst_f < RTOL and worst_b < RTOL
+    assert np.all(np.isfinite(mb))
+    B.close()
+    _free(cosmos)
