@@ -9,7 +9,8 @@
 //                reference's uniform log-a grid, each a QAG-GK61 integral of
 //                1/(a^2 H) (relerr 1e-6), one wavefront per node;
 //   k_spline     natural cubic spline coefficients of that table and of every
-//                delta_tot[k][0..Na-1] row (windowed parallel tridiagonal solve);
+//                delta_tot[k][0..Na-1] row (one wavefront per row window: scans of
+//                the substitution recurrences);
 //   k_bucket     a uniform bucket index over the stored log a values;
 //   k_dnu_qag    one wavefront per (species, k): QAG-GK61 of the integrand
 //                fsl/(a H) * specialJ * S_k over [log T0, log a] at the
@@ -30,7 +31,6 @@ namespace {
 using namespace kn_dnu;
 
 constexpr int kWavesPerBlock = 4;
-constexpr int kChunk = 32;  // spline rows per thread (window chunk)
 
 // ---------------------------------------------------------------- H table
 
@@ -118,16 +118,28 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_fs_nodes(HTab H, int nf
     }
 }
 
-// rows of natural-spline coefficients: row r uses x[0..n), y + r*ystride, c + r*cstride
-__global__ void k_spline(const double *__restrict__ x, const double *__restrict__ y, double *__restrict__ c, int n,
-                         int nrows, long long ystride, long long cstride)
+// Rows of natural-spline coefficients (row r: x[0..n), y + r*ystride,
+// c + r*cstride): one wavefront per (row, window of kKeep rows), four per block.
+__global__ __launch_bounds__(256) void k_spline(const double *__restrict__ x, const double *__restrict__ y,
+                                                double *__restrict__ c, int n, int nrows, long long ystride,
+                                                long long cstride)
 {
-    const int nchunk = (n + kChunk - 1) / kChunk;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)nchunk * nrows) return;
-    const int r = (int)(t / nchunk), ch = (int)(t % nchunk);
-    const int lo = ch * kChunk, hi = lo + kChunk < n ? lo + kChunk : n;
-    spline_window(x, y + r * ystride, n, lo, hi, c + r * cstride);
+    const int nwin = (n + kKeep - 1) / kKeep;
+    const long long item = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= (long long)nwin * nrows) return;  // wave-uniform
+    const int r = (int)(item / nwin), w = (int)(item % nwin);
+    const int lo = w * kKeep, hi = lo + kKeep < n ? lo + kKeep : n;
+    SplineRhs rhs;
+    rhs.x = x;
+    rhs.y = y + r * ystride;
+    tridiag_window_wave(x, n, rhs, lo, hi, c + r * cstride);
+}
+
+// Blocks of k_spline / k_adj_solve for nrows rows of n knots.
+inline int spline_blocks(int n, int nrows)
+{
+    const long long items = (long long)((n + kKeep - 1) / kKeep) * nrows;
+    return (int)((items + 3) / 4);
 }
 
 // bucket[q] = largest i <= n-2 with x[i] <= x[0] + q / inv_bw
@@ -269,30 +281,35 @@ __global__ __launch_bounds__(64) void k_dnu_adj(
     if (lane == 0) mu_bar[item] = mu_acc;
 }
 
-// Per row of the adjoint: phase 0 solves M z = cbar (the natural-spline matrix
-// is symmetric, so its transpose is itself); phase 1 adds z through the
-// right-hand side 3((y_{i+1}-y_i)/h_i - (y_i-y_{i-1})/h_{i-1}) onto ybar.
-__global__ void k_dnu_adj_rows(const double *__restrict__ x, int n, int nrows, const double *__restrict__ cbar,
-                               double *__restrict__ z, double *__restrict__ ybar, int phase)
+// Per row of the adjoint, first M z = cbar (the natural-spline matrix is
+// symmetric, so its transpose is itself), by wavefronts as k_spline ...
+__global__ __launch_bounds__(256) void k_adj_solve(const double *__restrict__ x, int n, int nrows,
+                                                   const double *__restrict__ cbar, double *__restrict__ z)
 {
-    const int nchunk = (n + kChunk - 1) / kChunk;
+    const int nwin = (n + kKeep - 1) / kKeep;
+    const long long item = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= (long long)nwin * nrows) return;  // wave-uniform
+    const int r = (int)(item / nwin), w = (int)(item % nwin);
+    const int lo = w * kKeep, hi = lo + kKeep < n ? lo + kKeep : n;
+    ArrayRhs rhs;
+    rhs.r = cbar + (size_t)r * n;
+    tridiag_window_wave(x, n, rhs, lo, hi, z + (size_t)r * n);
+}
+
+// ... then z added through the right-hand side 3((y_{i+1}-y_i)/h_i - (y_i-y_{i-1})/h_{i-1})
+// onto ybar (one thread per entry).
+__global__ void k_dnu_adj_rows(const double *__restrict__ x, int n, int nrows, const double *__restrict__ z,
+                               double *__restrict__ ybar)
+{
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)nchunk * nrows) return;
-    const int r = (int)(t / nchunk), ch = (int)(t % nchunk);
-    const int lo = ch * kChunk, hi = lo + kChunk < n ? lo + kChunk : n;
-    const size_t row = (size_t)r * n;
-    if (phase == 0) {
-        tridiag_window(x, cbar + row, n, lo, hi, z + row);
-        return;
-    }
-    const double *zr = z + row;
-    for (int i = lo; i < hi; i++) {
-        double acc = 0.0;
-        if (i - 1 >= 1 && i - 1 <= n - 2) acc += zr[i - 1] * 3.0 / (x[i] - x[i - 1]);
-        if (i >= 1 && i <= n - 2) acc -= zr[i] * (3.0 / (x[i + 1] - x[i]) + 3.0 / (x[i] - x[i - 1]));
-        if (i + 1 >= 1 && i + 1 <= n - 2) acc += zr[i + 1] * 3.0 / (x[i + 1] - x[i]);
-        ybar[row + i] += acc;
-    }
+    if (t >= (long long)n * nrows) return;
+    const int i = (int)(t % n);
+    const double *zr = z + (t - i);
+    double acc = 0.0;
+    if (i - 1 >= 1 && i - 1 <= n - 2) acc += zr[i - 1] * 3.0 / (x[i] - x[i - 1]);
+    if (i >= 1 && i <= n - 2) acc -= zr[i] * (3.0 / (x[i + 1] - x[i]) + 3.0 / (x[i] - x[i - 1]));
+    if (i + 1 >= 1 && i + 1 <= n - 2) acc += zr[i + 1] * 3.0 / (x[i + 1] - x[i]);
+    ybar[t] += acc;
 }
 
 // ---------------------------------------------------------------- state
@@ -518,15 +535,11 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, d->light,
                        dev->fx, dev->fy, dev->status);
     KN_CHECK(hipGetLastError());
-    {
-        const long long t = (long long)((nfs + kChunk - 1) / kChunk);
-        hipLaunchKernelGGL(k_spline, dim3((int)((t + 127) / 128)), dim3(128), 0, 0, dev->fx, dev->fy, dev->fc, nfs, 1,
-                           0LL, 0LL);
-        KN_CHECK(hipGetLastError());
-    }
+    hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, 0, dev->fx, dev->fy, dev->fc, nfs, 1, 0LL,
+                       0LL);
+    KN_CHECK(hipGetLastError());
     if (Na > 2) {
-        const long long t = (long long)((Na + kChunk - 1) / kChunk) * nk;
-        hipLaunchKernelGGL(k_spline, dim3((int)((t + 127) / 128)), dim3(128), 0, 0, dev->sx, dev->Y, dev->C, Na, nk,
+        hipLaunchKernelGGL(k_spline, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, dev->Y, dev->C, Na, nk,
                            (long long)Na, (long long)Na);
         KN_CHECK(hipGetLastError());
     }
@@ -626,12 +639,12 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
                            dev->ivl, dev->nint, dev->gbar, d->delta_nu_prefac, dev->ybar, dev->cbar, dev->mubar);
         KN_CHECK(hipGetLastError());
         if (Na > 2) {
-            const long long t = (long long)((Na + kChunk - 1) / kChunk) * nk;
-            for (int phase = 0; phase < 2; phase++) {
-                hipLaunchKernelGGL(k_dnu_adj_rows, dim3((int)((t + 127) / 128)), dim3(128), 0, 0, dev->sx, Na, nk,
-                                   dev->cbar, dev->zbuf, dev->ybar, phase);
-                KN_CHECK(hipGetLastError());
-            }
+            const long long t = (long long)Na * nk;
+            hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, Na, nk, dev->cbar,
+                               dev->zbuf);
+            hipLaunchKernelGGL(k_dnu_adj_rows, dim3((int)((t + 255) / 256)), dim3(256), 0, 0, dev->sx, Na, nk,
+                               dev->zbuf, dev->ybar);
+            KN_CHECK(hipGetLastError());
         }
         std::vector<double> mub(nk);
         KN_CHECK(hipMemcpy(mub.data(), dev->mubar, sizeof(double) * nk, hipMemcpyDeviceToHost));
@@ -857,15 +870,10 @@ void group_forward_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
         const double *sx = g->sx + q.off_s;
         hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, s, H, nfs, q.x0, q.xa,
                            q.spec.light, fx, fy, g->status + 2 * i);
-        {
-            const long long t = (long long)((nfs + kChunk - 1) / kChunk);
-            hipLaunchKernelGGL(k_spline, dim3((int)((t + 127) / 128)), dim3(128), 0, s, fx, fy, fc, nfs, 1, 0LL, 0LL);
-        }
-        if (na > 2) {
-            const long long t = (long long)((na + kChunk - 1) / kChunk) * nk;
-            hipLaunchKernelGGL(k_spline, dim3((int)((t + 127) / 128)), dim3(128), 0, s, sx, Yk, g->C + q.off_y, na,
-                               nk, (long long)na, (long long)na);
-        }
+        hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, s, fx, fy, fc, nfs, 1, 0LL, 0LL);
+        if (na > 2)
+            hipLaunchKernelGGL(k_spline, dim3(spline_blocks(na, nk)), dim3(256), 0, s, sx, Yk, g->C + q.off_y, na, nk,
+                               (long long)na, (long long)na);
         hipLaunchKernelGGL(k_bucket, dim3((nsb + 256) / 256), dim3(256), 0, s, sx, na, nsb, sinv_bw,
                            g->bucket + q.off_b);
         hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nk)), dim3(64 * kWavesPerBlock), 0, s, H, fx, fy, fc, nfs, sx, Yk,
@@ -895,10 +903,11 @@ void group_adjoint_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
                            q.vcrit, g->ivl + (size_t)q.off_k * kLimit * 2, g->nint + q.off_k, g->gbar + q.off_k,
                            q.spec.delta_nu_prefac, yb, g->cbar + q.off_y, g->mubar + q.off_k);
         if (na > 2) {
-            const long long t = (long long)((na + kChunk - 1) / kChunk) * nk;
-            for (int phase = 0; phase < 2; phase++)
-                hipLaunchKernelGGL(k_dnu_adj_rows, dim3((int)((t + 127) / 128)), dim3(128), 0, s, g->sx + q.off_s, na,
-                                   nk, g->cbar + q.off_y, g->zbuf + q.off_y, yb, phase);
+            const long long t = (long long)na * nk;
+            hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(na, nk)), dim3(256), 0, s, g->sx + q.off_s, na, nk,
+                               g->cbar + q.off_y, g->zbuf + q.off_y);
+            hipLaunchKernelGGL(k_dnu_adj_rows, dim3((int)((t + 255) / 256)), dim3(256), 0, s, g->sx + q.off_s, na, nk,
+                               g->zbuf + q.off_y, yb);
         }
         if (na < na_max) {
             hipLaunchKernelGGL(k_group_zero, dim3((int)((nblk + 255) / 256)), dim3(256), 0, s, dbar, nblk);

@@ -350,6 +350,12 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
 // reaching the kept rows is below 0.268^48 ~ 1e-27 relative.
 constexpr int kHalo = 48;
 
+// Rows per batch of loads: a thread issues the loads of kSB rows at once
+// and then runs their (serial) elimination steps, so the window's memory
+// round trips are m / kSB instead of m.  Same arithmetic, step by step, as
+// the one-row-at-a-time loops.
+constexpr int kSB = 16;
+
 __device__ inline void spline_window(const double *x, const double *y, int n, int lo, int hi, double *c)
 {
     // unknowns c[1..n-2]; c[0] = c[n-1] = 0
@@ -361,35 +367,203 @@ __device__ inline void spline_window(const double *x, const double *y, int n, in
     if (a < 1) a = 1;
     if (b > n - 1) b = n - 1;
     const int m = b - a;
-    // forward elimination (L D L^T of the window), then back substitution into c
-    double gam_prev = 0.0, z_prev = 0.0, alp = 0.0;
-    // store gamma and z per row in c[] temporarily is not possible (c is shared); keep a small ring:
-    // the window is at most hi-lo+2*kHalo rows, solved twice: forward for z and gamma, backward for c.
+    // forward elimination (L D L^T of the window), then back substitution into c;
+    // the window is at most hi-lo+2*kHalo rows
+    double gam_prev = 0.0, z_prev = 0.0;
     double gam[64 + 2 * kHalo + 2], zz[64 + 2 * kHalo + 2];
-    for (int r = 0; r < m; r++) {
-        const int i = a + r;  // unknown c[i], row couples c[i-1], c[i], c[i+1]
-        const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
-        const double gl = hl != 0.0 ? 1.0 / hl : 0.0, gr = hr != 0.0 ? 1.0 / hr : 0.0;
-        const double diag = 2.0 * (hr + hl);
-        const double rhs = 3.0 * ((y[i + 1] - y[i]) * gr - (y[i] - y[i - 1]) * gl);
-        alp = r == 0 ? diag : diag - hl * gam_prev;
-        const double g = hr / alp;
-        const double z = r == 0 ? rhs : rhs - gam_prev * z_prev;
-        gam[r] = g;
-        zz[r] = z / alp;
-        gam_prev = g;
-        z_prev = z;
+    for (int r0 = 0; r0 < m; r0 += kSB) {
+        double xs[kSB + 2], ys[kSB + 2];  // x, y at rows a+r0-1 .. a+r0+kSB (only up to b is read)
+#pragma unroll
+        for (int j = 0; j < kSB + 2; j++) {
+            const int i = a + r0 - 1 + j;
+            xs[j] = i <= b ? x[i] : 0.0;
+            ys[j] = i <= b ? y[i] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < kSB; j++) {
+            const int r = r0 + j;  // unknown c[a+r], row couples c[i-1], c[i], c[i+1]
+            if (r < m) {
+                const double hl = xs[j + 1] - xs[j], hr = xs[j + 2] - xs[j + 1];
+                const double gl = hl != 0.0 ? 1.0 / hl : 0.0, gr = hr != 0.0 ? 1.0 / hr : 0.0;
+                const double diag = 2.0 * (hr + hl);
+                const double rhs = 3.0 * ((ys[j + 2] - ys[j + 1]) * gr - (ys[j + 1] - ys[j]) * gl);
+                const double alp = r == 0 ? diag : diag - hl * gam_prev;
+                const double g = hr / alp;
+                const double z = r == 0 ? rhs : rhs - gam_prev * z_prev;
+                gam[r] = g;
+                zz[r] = z / alp;
+                gam_prev = g;
+                z_prev = z;
+            }
+        }
     }
     double cnext = 0.0;
-    for (int r = m - 1; r >= 0; r--) {
-        const double ci = zz[r] - (r == m - 1 ? 0.0 : gam[r] * cnext);
-        const int i = a + r;
-        if (i >= lo && i < hi) c[i] = ci;
-        cnext = ci;
+    for (int r1 = m; r1 > 0; r1 -= kSB) {
+        double gs[kSB], zs[kSB];
+#pragma unroll
+        for (int j = 0; j < kSB; j++) {
+            const int r = r1 - 1 - j;
+            gs[j] = r >= 0 ? gam[r] : 0.0;
+            zs[j] = r >= 0 ? zz[r] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < kSB; j++) {
+            const int r = r1 - 1 - j;
+            if (r >= 0) {
+                const double ci = zs[j] - (r == m - 1 ? 0.0 : gs[j] * cnext);
+                const int i = a + r;
+                if (i >= lo && i < hi) c[i] = ci;
+                cnext = ci;
+            }
+        }
     }
     if (lo == 0) c[0] = 0.0;
     if (hi >= n) c[n - 1] = 0.0;
     if (lo <= n - 1 && n - 1 < hi) c[n - 1] = 0.0;
+}
+
+// ---- natural-spline rows by wavefront scans -------------------------------
+// One wavefront solves a window of up to 64 kRun interior rows of
+// M c = rhs (rows i = 1..n-2, c_0 = c_{n-1} = 0), lane l taking kRun
+// consecutive rows in registers, and keeps the rows [lo, hi) that lie kHalo
+// rows or more inside the window (zero values are assumed just outside it,
+// as in spline_window).  M's L D L^T factors
+//   alp_i = 2(hl_i + hr_i) - hl_i gam_{i-1},  gam_i = hr_i / alp_i
+// are those of the elimination from row 1; the gam recurrence contracts by
+// at most (hl/alp)(hr/alp) <= 1/4 per row, so a lane that starts it kWarm
+// rows before its run (alp = diagonal there) reaches them to below
+// 0.25^32 ~ 5e-20 relative.  The forward substitution
+// z_i = rhs_i - gam_{i-1} z_{i-1} and the back substitution
+// c_i = z_i / alp_i - gam_i c_{i+1} are first-order linear recurrences: each
+// lane composes its run into one affine map, a 64-lane scan of the maps
+// gives every lane its incoming value, and the lane replays its run.
+constexpr int kWarm = 32;
+constexpr int kRun = 10;                        // rows per lane
+constexpr int kKeep = 64 * kRun - 2 * kHalo;    // rows kept per window (544)
+
+// Right-hand side of the natural-spline system (GSL cspline): 3 (dy/h right - dy/h left).
+struct SplineRhs {
+    const double *x, *y;
+    __device__ __forceinline__ double operator()(int i) const
+    {
+        const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
+        const double gl = hl != 0.0 ? 1.0 / hl : 0.0, gr = hr != 0.0 ? 1.0 / hr : 0.0;
+        return 3.0 * ((y[i + 1] - y[i]) * gr - (y[i] - y[i - 1]) * gl);
+    }
+};
+
+struct ArrayRhs {
+    const double *r;
+    __device__ __forceinline__ double operator()(int i) const { return r[i]; }
+};
+
+// Called by all 64 lanes of a wavefront together, with hi - lo <= kKeep.
+template <class RHS>
+__device__ inline void tridiag_window_wave(const double *x, int n, const RHS &rhs, int lo, int hi, double *out)
+{
+    const int lane = threadIdx.x & 63;
+    if (n < 3) {
+        for (int i = lo + lane; i < hi; i += 64) out[i] = 0.0;
+        return;
+    }
+    int wa = lo - kHalo, wb = hi + kHalo;  // window of interior rows [wa, wb)
+    if (wa < 1) wa = 1;
+    if (wb > n - 1) wb = n - 1;
+    const int s = wa + lane * kRun;
+    const int len = s < wb ? (wb - s < kRun ? wb - s : kRun) : 0;
+    double g[kRun], al[kRun], zz[kRun];
+    double g_before = 0.0;  // gam_{s-1}
+    {
+        int a = s - kWarm;
+        if (a < 1) a = 1;
+        double gp = 0.0;
+        for (int r = a; r < s && r < wb; r++) {
+            const double hl = x[r] - x[r - 1], hr = x[r + 1] - x[r];
+            const double diag = 2.0 * (hr + hl);
+            const double alr = r == a ? diag : diag - hl * gp;
+            gp = hr / alr;
+        }
+        g_before = gp;
+#pragma unroll
+        for (int j = 0; j < kRun; j++) {
+            if (j < len) {
+                const int i = s + j;
+                const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
+                const double diag = 2.0 * (hr + hl);
+                al[j] = i == a ? diag : diag - hl * gp;
+                g[j] = hr / al[j];
+                gp = g[j];
+            } else {
+                al[j] = 1.0;
+                g[j] = 0.0;
+            }
+        }
+    }
+    double rv[kRun];
+#pragma unroll
+    for (int j = 0; j < kRun; j++) rv[j] = j < len ? rhs(s + j) : 0.0;
+    // forward: z_i = rhs_i - gam_{i-1} z_{i-1}, z_{wa-1} = 0
+    double A = 0.0, B = 1.0;
+#pragma unroll
+    for (int j = 0; j < kRun; j++) {
+        if (j < len) {
+            const double gm = s + j == wa ? 0.0 : (j == 0 ? g_before : g[j - 1]);
+            A = rv[j] - gm * A;
+            B = -gm * B;
+        }
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double Ap = __shfl_up(A, d, 64), Bp = __shfl_up(B, d, 64);
+        if (lane >= d) {
+            A = A + B * Ap;
+            B = B * Bp;
+        }
+    }
+    double z = __shfl_up(A, 1, 64);
+    if (lane == 0) z = 0.0;
+#pragma unroll
+    for (int j = 0; j < kRun; j++) {
+        if (j < len) {
+            const double gm = s + j == wa ? 0.0 : (j == 0 ? g_before : g[j - 1]);
+            z = rv[j] - gm * z;
+            zz[j] = z / al[j];
+        } else {
+            zz[j] = 0.0;
+        }
+    }
+    // back: c_i = zz_i - gam_i c_{i+1}, c_{wb} = 0
+    A = 0.0;
+    B = 1.0;
+#pragma unroll
+    for (int j = kRun - 1; j >= 0; j--) {
+        if (j < len) {
+            A = zz[j] - g[j] * A;
+            B = -g[j] * B;
+        }
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double An = __shfl_down(A, d, 64), Bn = __shfl_down(B, d, 64);
+        if (lane + d < 64) {
+            A = A + B * An;
+            B = B * Bn;
+        }
+    }
+    double c = __shfl_down(A, 1, 64);
+    if (lane == 63) c = 0.0;
+#pragma unroll
+    for (int j = kRun - 1; j >= 0; j--) {
+        if (j < len) {
+            const int i = s + j;
+            c = zz[j] - g[j] * c;
+            if (i >= lo && i < hi) out[i] = c;
+        }
+    }
+    if (lane == 0) {
+        if (lo == 0) out[0] = 0.0;
+        if (lo <= n - 1 && n - 1 < hi) out[n - 1] = 0.0;
+    }
 }
 
 // ---- derivatives for the adjoint ----------------------------------------
@@ -480,23 +654,49 @@ __device__ inline void tridiag_window(const double *x, const double *rhs, int n,
     const int m = b - a;
     double gam[64 + 2 * kHalo + 2], zz[64 + 2 * kHalo + 2];
     double gam_prev = 0.0, z_prev = 0.0;
-    for (int r = 0; r < m; r++) {
-        const int i = a + r;
-        const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
-        const double diag = 2.0 * (hr + hl);
-        const double alp = r == 0 ? diag : diag - hl * gam_prev;
-        const double zr = r == 0 ? rhs[i] : rhs[i] - gam_prev * z_prev;
-        gam[r] = hr / alp;
-        zz[r] = zr / alp;
-        gam_prev = gam[r];
-        z_prev = zr;
+    for (int r0 = 0; r0 < m; r0 += kSB) {
+        double xs[kSB + 2], rs[kSB];  // x at rows a+r0-1 .. a+r0+kSB (up to b), rhs at a+r0 .. a+r0+kSB-1
+#pragma unroll
+        for (int j = 0; j < kSB + 2; j++) {
+            const int i = a + r0 - 1 + j;
+            xs[j] = i <= b ? x[i] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < kSB; j++) rs[j] = r0 + j < m ? rhs[a + r0 + j] : 0.0;
+#pragma unroll
+        for (int j = 0; j < kSB; j++) {
+            const int r = r0 + j;
+            if (r < m) {
+                const double hl = xs[j + 1] - xs[j], hr = xs[j + 2] - xs[j + 1];
+                const double diag = 2.0 * (hr + hl);
+                const double alp = r == 0 ? diag : diag - hl * gam_prev;
+                const double zr = r == 0 ? rs[j] : rs[j] - gam_prev * z_prev;
+                gam[r] = hr / alp;
+                zz[r] = zr / alp;
+                gam_prev = gam[r];
+                z_prev = zr;
+            }
+        }
     }
     double next = 0.0;
-    for (int r = m - 1; r >= 0; r--) {
-        const double v = zz[r] - (r == m - 1 ? 0.0 : gam[r] * next);
-        const int i = a + r;
-        if (i >= lo && i < hi) z_out[i] = v;
-        next = v;
+    for (int r1 = m; r1 > 0; r1 -= kSB) {
+        double gs[kSB], zs[kSB];
+#pragma unroll
+        for (int j = 0; j < kSB; j++) {
+            const int r = r1 - 1 - j;
+            gs[j] = r >= 0 ? gam[r] : 0.0;
+            zs[j] = r >= 0 ? zz[r] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < kSB; j++) {
+            const int r = r1 - 1 - j;
+            if (r >= 0) {
+                const double v = zs[j] - (r == m - 1 ? 0.0 : gs[j] * next);
+                const int i = a + r;
+                if (i >= lo && i < hi) z_out[i] = v;
+                next = v;
+            }
+        }
     }
     if (lo == 0) z_out[0] = 0.0;
     if (lo <= n - 1 && n - 1 < hi) z_out[n - 1] = 0.0;

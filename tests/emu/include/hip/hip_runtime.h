@@ -91,6 +91,16 @@ inline int __shfl(int v, int src, int = 64) { return (int)emu_exchange((unsigned
 inline double __shfl_xor(double v, int m, int = 64) { return __shfl(v, (int)(emu_tid & 63) ^ m); }
 inline int __shfl_xor(int v, int m, int = 64) { return __shfl(v, (int)(emu_tid & 63) ^ m); }
 inline int __builtin_amdgcn_readfirstlane(int v) { return __shfl(v, 0); }
+inline double __shfl_up(double v, unsigned d, int = 64)
+{
+    const int l = (int)(emu_tid & 63);
+    return __shfl(v, l >= (int)d ? l - (int)d : l);
+}
+inline double __shfl_down(double v, unsigned d, int = 64)
+{
+    const int l = (int)(emu_tid & 63);
+    return __shfl(v, l + (int)d < 64 ? l + (int)d : l);
+}
 inline void __builtin_amdgcn_wave_barrier() {}
 inline void __syncthreads()
 {
