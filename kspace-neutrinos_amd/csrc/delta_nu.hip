@@ -92,14 +92,11 @@ HTab device_htab()
 
 // ---------------------------------------------------------------- kernels
 
-__global__ __launch_bounds__(64 * kWavesPerBlock) void k_fs_nodes(HTab H, int nfs, double x0, double xa,
-                                                                   double light, double *__restrict__ fx,
-                                                                   double *__restrict__ fy, int *__restrict__ status)
+// fslength node `node` of nfs (one wavefront).
+__device__ inline void fs_node_wave(const HTab &H, int nfs, double x0, double xa, double light, double *__restrict__ fx,
+                                    double *__restrict__ fy, int *__restrict__ status, int node, QagWs &ws)
 {
-    __shared__ QagWs ws[kWavesPerBlock];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int node = blockIdx.x * kWavesPerBlock + wave;
-    if (node >= nfs) return;
+    const int lane = threadIdx.x & 63;
     // delta_tot_table.c:582: log T0 + i (log a - log T0) / (Nfs - 1.)
     const double xi = x0 + node * (xa - x0) / (nfs - 1.);
     double v = 0.0;
@@ -108,7 +105,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_fs_nodes(HTab H, int nf
         f.H = H;
         double r;
         int ni;
-        const int rc = wave_qag61(f, xi, xa, 1e-6, kLimit, ws[wave], &r, &ni);
+        const int rc = wave_qag61(f, xi, xa, 1e-6, kLimit, ws, &r, &ni);
         if (rc && lane == 0) atomicOr(status, 1);
         v = light * r;
     }
@@ -118,21 +115,39 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_fs_nodes(HTab H, int nf
     }
 }
 
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_fs_nodes(HTab H, int nfs, double x0, double xa,
+                                                                   double light, double *__restrict__ fx,
+                                                                   double *__restrict__ fy, int *__restrict__ status)
+{
+    __shared__ QagWs ws[kWavesPerBlock];
+    const int wave = threadIdx.x >> 6;
+    const int node = blockIdx.x * kWavesPerBlock + wave;
+    if (node >= nfs) return;
+    fs_node_wave(H, nfs, x0, xa, light, fx, fy, status, node, ws[wave]);
+}
+
 // Rows of natural-spline coefficients (row r: x[0..n), y + r*ystride,
-// c + r*cstride): one wavefront per (row, window of kKeep rows), four per block.
-__global__ __launch_bounds__(256) void k_spline(const double *__restrict__ x, const double *__restrict__ y,
-                                                double *__restrict__ c, int n, int nrows, long long ystride,
-                                                long long cstride)
+// c + r*cstride): one wavefront per (row, window of kKeep rows).
+__device__ inline void spline_item_wave(const double *__restrict__ x, const double *__restrict__ y,
+                                        double *__restrict__ c, int n, long long ystride, long long cstride,
+                                        long long item)
 {
     const int nwin = (n + kKeep - 1) / kKeep;
-    const long long item = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (item >= (long long)nwin * nrows) return;  // wave-uniform
     const int r = (int)(item / nwin), w = (int)(item % nwin);
     const int lo = w * kKeep, hi = lo + kKeep < n ? lo + kKeep : n;
     SplineRhs rhs;
     rhs.x = x;
     rhs.y = y + r * ystride;
     tridiag_window_wave(x, n, rhs, lo, hi, c + r * cstride);
+}
+
+__global__ __launch_bounds__(256) void k_spline(const double *__restrict__ x, const double *__restrict__ y,
+                                                double *__restrict__ c, int n, int nrows, long long ystride,
+                                                long long cstride)
+{
+    const long long item = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= (long long)((n + kKeep - 1) / kKeep) * nrows) return;  // wave-uniform
+    spline_item_wave(x, y, c, n, ystride, cstride, item);
 }
 
 // Blocks of k_spline / k_adj_solve for nrows rows of n knots.
@@ -143,10 +158,8 @@ inline int spline_blocks(int n, int nrows)
 }
 
 // bucket[q] = largest i <= n-2 with x[i] <= x[0] + q / inv_bw
-__global__ void k_bucket(const double *__restrict__ x, int n, int nb, double inv_bw, int *__restrict__ bucket)
+__device__ inline void bucket_at(const double *__restrict__ x, int n, double inv_bw, int *__restrict__ bucket, int q)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q > nb) return;
     const double edge = x[0] + q / inv_bw;
     int lo = 0, hi = n - 1;
     while (hi > lo + 1) {
@@ -157,17 +170,41 @@ __global__ void k_bucket(const double *__restrict__ x, int n, int nb, double inv
     bucket[q] = lo;
 }
 
+__global__ void k_bucket(const double *__restrict__ x, int n, int nb, double inv_bw, int *__restrict__ bucket)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > nb) return;
+    bucket_at(x, n, inv_bw, bucket, q);
+}
+
 struct SpeciesDev {
     double mnubykT, relerr;
     JPar J;
 };
+
+// Initial-condition piece (delta_tot_table.c:522-535) added in the QAG
+// kernel when init != nullptr: specialJ(k fsl0 / m, qc = 0) init[k] lin,
+// m = m/kT (1 when massless); then delta_nu = IC + prefac * integral.
+struct IcArgs {
+    const double *init;
+    double fsl0, lin, prefac;
+};
+
+__device__ __forceinline__ double ic_piece(double kk, double init, const SpeciesDev &sp, const IcArgs &ic)
+{
+    const double m = sp.mnubykT > 0 ? sp.mnubykT : 1;
+    JPar J0;
+    J0.qc = 0.0;
+    J0.nufrac_low = sp.J.nufrac_low;
+    return specialJ(kk * ic.fsl0 / m, J0) * init * ic.lin;
+}
 
 __global__ __launch_bounds__(64 * kWavesPerBlock) void k_dnu_qag(
     HTab H, const double *__restrict__ fx, const double *__restrict__ fy, const double *__restrict__ fc, int nfs,
     const double *__restrict__ sx, const double *__restrict__ Y, const double *__restrict__ C, const int *__restrict__ sb,
     int ns, int nsb, double sinv_bw, const double *__restrict__ wavenum, int nk, const SpeciesDev *__restrict__ sp,
     const int *__restrict__ item_species, int nitems, double x0, double xa, double *__restrict__ out,
-    int *__restrict__ nint_out, int *__restrict__ status, double *__restrict__ ivl)
+    int *__restrict__ nint_out, int *__restrict__ status, double *__restrict__ ivl, IcArgs ic)
 {
     __shared__ QagWs ws[kWavesPerBlock];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -193,7 +230,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_dnu_qag(
     f.sinv_bw = sinv_bw;
     f.k = wavenum[k];
     f.mnubykT = sp[s].mnubykT;
-    f.J = sp[s].J;
+    f.J = &sp[s].J;
     f.edom = status + 1;
     double r;
     int ni;
@@ -204,10 +241,33 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_dnu_qag(
             ivl[((size_t)item * kLimit + t) * 2 + 1] = ws[wave].hi[t];
         }
     if (lane == 0) {
-        out[item] = r;
+        out[item] = ic.init ? ic_piece(wavenum[k], ic.init[k], sp[s], ic) + ic.prefac * r : r;
         nint_out[item] = ni;
         if (rc) atomicOr(status, 2);
     }
+}
+
+// Adjoint of the initial-condition piece at k: dL/d delta_nu_init[k] and the
+// piece's d/d(m/kT) term (zero when massless).
+struct IcAdjArgs {
+    const double *init;
+    double fsl0, lin;
+    double *init_bar, *mbar_ic;
+};
+
+__device__ __forceinline__ void ic_adjoint_at(int k, const double *__restrict__ wavenum,
+                                              const double *__restrict__ gbar, const SpeciesDev &sp,
+                                              const IcAdjArgs &a)
+{
+    const double m = sp.mnubykT > 0 ? sp.mnubykT : 1;
+    JPar J0;
+    J0.qc = 0.0;
+    J0.nufrac_low = sp.J.nufrac_low;
+    const double kk = wavenum[k] * a.fsl0 / m;
+    double Jx, Jq;
+    const double J = specialJ_d(kk, J0, Jx, Jq);
+    a.init_bar[k] = gbar[k] * J * a.lin;
+    a.mbar_ic[k] = sp.mnubykT > 0 ? gbar[k] * Jx * (-kk / m) * a.init[k] * a.lin : 0.0;
 }
 
 // Adjoint of one item's integral with its recorded subdivision held fixed:
@@ -220,9 +280,14 @@ __global__ __launch_bounds__(64) void k_dnu_adj(
     const int *__restrict__ sb, int ns, int nsb, double sinv_bw, const double *__restrict__ wavenum, int nk,
     const SpeciesDev *__restrict__ sp, double vcrit, const double *__restrict__ ivl, const int *__restrict__ nint,
     const double *__restrict__ gbar, double prefac, double *__restrict__ ybar, double *__restrict__ cbar,
-    double *__restrict__ mu_bar)
+    double *__restrict__ mu_bar, IcAdjArgs ica)
 {
     HIP_DYNAMIC_SHARED(double, lds)  // ybar row [ns], cbar row [ns]
+    if ((int)blockIdx.x >= nk) {  // blocks past the items: the initial-condition piece, 64 k per block
+        const int kk = ((int)blockIdx.x - nk) * 64 + (int)threadIdx.x;
+        if (kk < nk) ic_adjoint_at(kk, wavenum, gbar, *sp, ica);
+        return;
+    }
     const int lane = threadIdx.x, item = blockIdx.x, k = item % nk;
     const int hf = lane >> 5, j = lane & 31;
     for (int t = lane; t < 2 * ns; t += 64) lds[t] = 0.0;
@@ -532,24 +597,24 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     }
 
     const HTab H = device_htab();
+    const double sinv_bw = nsb / (d->scalefact[Na - 1] - d->scalefact[0]);
     hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, d->light,
                        dev->fx, dev->fy, dev->status);
+    if (Na > 2)
+        hipLaunchKernelGGL(k_spline, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, dev->Y, dev->C, Na, nk,
+                           (long long)Na, (long long)Na);
+    hipLaunchKernelGGL(k_bucket, dim3((nsb + 256) / 256), dim3(256), 0, 0, dev->sx, Na, nsb, sinv_bw, dev->bucket);
     KN_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, 0, dev->fx, dev->fy, dev->fc, nfs, 1, 0LL,
                        0LL);
     KN_CHECK(hipGetLastError());
-    if (Na > 2) {
-        hipLaunchKernelGGL(k_spline, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, dev->Y, dev->C, Na, nk,
-                           (long long)Na, (long long)Na);
-        KN_CHECK(hipGetLastError());
-    }
-    const double sinv_bw = nsb / (d->scalefact[Na - 1] - d->scalefact[0]);
-    hipLaunchKernelGGL(k_bucket, dim3((nsb + 256) / 256), dim3(256), 0, 0, dev->sx, Na, nsb, sinv_bw, dev->bucket);
-    KN_CHECK(hipGetLastError());
+    IcArgs no_ic;
+    no_ic.init = nullptr;
+    no_ic.fsl0 = no_ic.lin = no_ic.prefac = 0.0;
     hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nitems)), dim3(64 * kWavesPerBlock), 0, 0, H, dev->fx, dev->fy,
                        dev->fc, nfs, dev->sx, dev->Y, dev->C, dev->bucket, Na, nsb, sinv_bw, dev->wavenum, nk, dev->sp,
                        dev->item_species, nitems, x0, xa, dev->out, dev->nint, dev->status,
-                       record ? dev->ivl : nullptr);
+                       record ? dev->ivl : nullptr, no_ic);
     KN_CHECK(hipGetLastError());
     std::vector<double> res(nitems);
     std::vector<int> nint(nitems);
@@ -636,7 +701,8 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
         const double vcrit = s.qc > 0 ? d->omnu->hybnu.vcrit : 0.0;
         hipLaunchKernelGGL(k_dnu_adj, dim3(nk), dim3(64), lds, 0, device_htab(), dev->fx, dev->fy, dev->fc, nfs,
                            dev->sx, dev->Y, dev->C, dev->bucket, Na, nsb, sinv_bw, dev->wavenum, nk, dev->sp, vcrit,
-                           dev->ivl, dev->nint, dev->gbar, d->delta_nu_prefac, dev->ybar, dev->cbar, dev->mubar);
+                           dev->ivl, dev->nint, dev->gbar, d->delta_nu_prefac, dev->ybar, dev->cbar, dev->mubar,
+                           IcAdjArgs{nullptr, 0.0, 0.0, nullptr, nullptr});
         KN_CHECK(hipGetLastError());
         if (Na > 2) {
             const long long t = (long long)Na * nk;
@@ -672,41 +738,24 @@ namespace {
 constexpr int kGroupStreams = 8;
 constexpr double kGroupHWidth = 1.0 / 256.0;  // ln H panel width (the per-call table's)
 
-// delta_nu[k] = IC(k) + prefac * raw[k]: the initial-condition piece of
-// delta_tot_table.c:522-535 (specialJ with qc = 0, m/kT or 1 when massless)
-// plus the integral; raw == nullptr for IC-only histories.
+// delta_nu[k] = IC(k) of an IC-only history (delta_tot_table.c:522-535,
+// :543 and :551: hybrid with all neutrinos in particles, massless, or one
+// stored step).
 __global__ void k_group_ic(const double *__restrict__ wavenum, const double *__restrict__ init,
-                           const double *__restrict__ raw, const SpeciesDev *__restrict__ sp, int nk, double fsl0,
-                           double lin, double prefac, double *__restrict__ out)
+                           const SpeciesDev *__restrict__ sp, int nk, IcArgs ic, double *__restrict__ out)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nk) return;
-    const double m = sp->mnubykT > 0 ? sp->mnubykT : 1;
-    JPar J0;
-    J0.qc = 0.0;
-    J0.nufrac_low = sp->J.nufrac_low;
-    double v = specialJ(wavenum[k] * fsl0 / m, J0) * init[k] * lin;
-    if (raw) v += prefac * raw[k];
-    out[k] = v;
+    out[k] = ic_piece(wavenum[k], init[k], *sp, ic);
 }
 
-// Adjoint of the initial-condition piece: dL/d delta_nu_init[k] and the
-// piece's d/d(m/kT) term per k (zero when massless).
-__global__ void k_group_ic_adj(const double *__restrict__ wavenum, const double *__restrict__ init,
-                               const double *__restrict__ gbar, const SpeciesDev *__restrict__ sp, int nk, double fsl0,
-                               double lin, double *__restrict__ init_bar, double *__restrict__ mbar_ic)
+// Adjoint of the initial-condition piece alone (IC-only histories).
+__global__ void k_group_ic_adj(const double *__restrict__ wavenum, const double *__restrict__ gbar,
+                               const SpeciesDev *__restrict__ sp, int nk, IcAdjArgs ica)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nk) return;
-    const double m = sp->mnubykT > 0 ? sp->mnubykT : 1;
-    JPar J0;
-    J0.qc = 0.0;
-    J0.nufrac_low = sp->J.nufrac_low;
-    const double kk = wavenum[k] * fsl0 / m;
-    double Jx, Jq;
-    const double J = specialJ_d(kk, J0, Jx, Jq);
-    init_bar[k] = gbar[k] * J * lin;
-    mbar_ic[k] = sp->mnubykT > 0 ? gbar[k] * Jx * (-kk / m) * init[k] * lin : 0.0;
+    ic_adjoint_at(k, wavenum, gbar, *sp, ica);
 }
 
 struct GroupMeta {
@@ -765,7 +814,7 @@ struct kn_group {
     std::vector<int> order;
     double *sx = nullptr, *Y = nullptr, *Yp = nullptr, *C = nullptr, *dbar = nullptr, *ybar = nullptr,
            *cbar = nullptr, *zbuf = nullptr, *fx = nullptr, *fy = nullptr, *fc = nullptr, *wavenum = nullptr,
-           *init = nullptr, *out = nullptr, *raw = nullptr, *gbar = nullptr, *ibar = nullptr, *mbic = nullptr,
+           *init = nullptr, *out = nullptr, *gbar = nullptr, *ibar = nullptr, *mbic = nullptr,
            *mubar = nullptr, *mbar = nullptr, *ivl = nullptr;
     int *bucket = nullptr, *nint = nullptr, *status = nullptr, *zeros = nullptr, *spec0 = nullptr;
     SpeciesDev *sp = nullptr;
@@ -851,39 +900,49 @@ inline double group_sinv_bw(const GroupHist &q)
     return 4.0 * q.na / (q.xa - q.x0);
 }
 
-// Forward of one history on stream s, recording its subdivisions.
+// Forward of one history on stream s, recording its subdivisions: the
+// preparations in one launch, the fslength spline, then QAG with the
+// initial-condition piece added per k (k_group_ic alone when IC-only).
 void group_forward_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
 {
     const int nk = q.spec.nk, na = q.na, na_max = q.spec.na_max;
-    const HTab H = g->htabs[q.htab];
-    const double *raw = nullptr;
-    if (q.integ) {
-        const int nfs = 16 * na, nsb = 4 * na;
-        const double sinv_bw = group_sinv_bw(q);
-        const double *Yk = g->Y + q.off_y;
-        if (na < na_max) {
-            KN_CHECK(hipMemcpy2DAsync(g->Yp + q.off_y, sizeof(double) * na, g->Y + q.off_y, sizeof(double) * na_max,
-                                      sizeof(double) * na, nk, hipMemcpyDeviceToDevice, s));
-            Yk = g->Yp + q.off_y;
-        }
-        double *fx = g->fx + q.off_fs, *fy = g->fy + q.off_fs, *fc = g->fc + q.off_fs;
-        const double *sx = g->sx + q.off_s;
-        hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, s, H, nfs, q.x0, q.xa,
-                           q.spec.light, fx, fy, g->status + 2 * i);
-        hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, s, fx, fy, fc, nfs, 1, 0LL, 0LL);
-        if (na > 2)
-            hipLaunchKernelGGL(k_spline, dim3(spline_blocks(na, nk)), dim3(256), 0, s, sx, Yk, g->C + q.off_y, na, nk,
-                               (long long)na, (long long)na);
-        hipLaunchKernelGGL(k_bucket, dim3((nsb + 256) / 256), dim3(256), 0, s, sx, na, nsb, sinv_bw,
-                           g->bucket + q.off_b);
-        hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nk)), dim3(64 * kWavesPerBlock), 0, s, H, fx, fy, fc, nfs, sx, Yk,
-                           g->C + q.off_y, g->bucket + q.off_b, na, nsb, sinv_bw, g->wavenum + q.off_k, nk, g->sp + i,
-                           g->spec0, nk, q.x0, q.xa, g->raw + q.off_k, g->nint + q.off_k, g->status + 2 * i,
-                           g->ivl + (size_t)q.off_k * kLimit * 2);
-        raw = g->raw + q.off_k;
+    if (!q.integ) {
+        IcArgs ic;
+        ic.init = g->init + q.off_k;
+        ic.fsl0 = q.fsl0;
+        ic.lin = q.lin;
+        ic.prefac = q.spec.delta_nu_prefac;
+        hipLaunchKernelGGL(k_group_ic, dim3((nk + 63) / 64), dim3(64), 0, s, g->wavenum + q.off_k, g->init + q.off_k,
+                           g->sp + i, nk, ic, g->out + q.off_k);
+        return;
     }
-    hipLaunchKernelGGL(k_group_ic, dim3((nk + 63) / 64), dim3(64), 0, s, g->wavenum + q.off_k, g->init + q.off_k, raw,
-                       g->sp + i, nk, q.fsl0, q.lin, q.spec.delta_nu_prefac, g->out + q.off_k);
+    const HTab H = g->htabs[q.htab];
+    const int nfs = 16 * na, nsb = 4 * na;
+    const double sinv_bw = group_sinv_bw(q);
+    const double *Yk = g->Y + q.off_y;
+    if (na < na_max) {
+        KN_CHECK(hipMemcpy2DAsync(g->Yp + q.off_y, sizeof(double) * na, g->Y + q.off_y, sizeof(double) * na_max,
+                                  sizeof(double) * na, nk, hipMemcpyDeviceToDevice, s));
+        Yk = g->Yp + q.off_y;
+    }
+    double *fx = g->fx + q.off_fs, *fy = g->fy + q.off_fs, *fc = g->fc + q.off_fs;
+    const double *sx = g->sx + q.off_s;
+    hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, s, H, nfs, q.x0, q.xa,
+                       q.spec.light, fx, fy, g->status + 2 * i);
+    if (na > 2)
+        hipLaunchKernelGGL(k_spline, dim3(spline_blocks(na, nk)), dim3(256), 0, s, sx, Yk, g->C + q.off_y, na, nk,
+                           (long long)na, (long long)na);
+    hipLaunchKernelGGL(k_bucket, dim3((nsb + 256) / 256), dim3(256), 0, s, sx, na, nsb, sinv_bw, g->bucket + q.off_b);
+    hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, s, fx, fy, fc, nfs, 1, 0LL, 0LL);
+    IcArgs ic;
+    ic.init = g->init + q.off_k;
+    ic.fsl0 = q.fsl0;
+    ic.lin = q.lin;
+    ic.prefac = q.spec.delta_nu_prefac;
+    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nk)), dim3(64 * kWavesPerBlock), 0, s, H, fx, fy, fc, nfs, sx, Yk,
+                       g->C + q.off_y, g->bucket + q.off_b, na, nsb, sinv_bw, g->wavenum + q.off_k, nk, g->sp + i,
+                       g->spec0, nk, q.x0, q.xa, g->out + q.off_k, g->nint + q.off_k, g->status + 2 * i,
+                       g->ivl + (size_t)q.off_k * kLimit * 2, ic);
 }
 
 // Adjoint of one history's last forward on stream s.
@@ -892,33 +951,40 @@ void group_adjoint_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
     const int nk = q.spec.nk, na = q.na, na_max = q.spec.na_max;
     const long long nblk = (long long)nk * na_max;
     double *dbar = g->dbar + q.off_y;
-    if (q.integ) {
-        const int nfs = 16 * na, nsb = 4 * na;
-        const double *Yk = na < na_max ? g->Yp + q.off_y : g->Y + q.off_y;
-        double *yb = na < na_max ? g->ybar + q.off_y : dbar;
-        const size_t lds = sizeof(double) * 2 * (size_t)na;
-        hipLaunchKernelGGL(k_dnu_adj, dim3(nk), dim3(64), lds, s, g->htabs[q.htab], g->fx + q.off_fs,
-                           g->fy + q.off_fs, g->fc + q.off_fs, nfs, g->sx + q.off_s, Yk, g->C + q.off_y,
-                           g->bucket + q.off_b, na, nsb, group_sinv_bw(q), g->wavenum + q.off_k, nk, g->sp + i,
-                           q.vcrit, g->ivl + (size_t)q.off_k * kLimit * 2, g->nint + q.off_k, g->gbar + q.off_k,
-                           q.spec.delta_nu_prefac, yb, g->cbar + q.off_y, g->mubar + q.off_k);
-        if (na > 2) {
-            const long long t = (long long)na * nk;
-            hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(na, nk)), dim3(256), 0, s, g->sx + q.off_s, na, nk,
-                               g->cbar + q.off_y, g->zbuf + q.off_y);
-            hipLaunchKernelGGL(k_dnu_adj_rows, dim3((int)((t + 255) / 256)), dim3(256), 0, s, g->sx + q.off_s, na, nk,
-                               g->zbuf + q.off_y, yb);
-        }
-        if (na < na_max) {
-            hipLaunchKernelGGL(k_group_zero, dim3((int)((nblk + 255) / 256)), dim3(256), 0, s, dbar, nblk);
-            KN_CHECK(hipMemcpy2DAsync(dbar, sizeof(double) * na_max, yb, sizeof(double) * na, sizeof(double) * na, nk,
-                                      hipMemcpyDeviceToDevice, s));
-        }
-    } else {
+    IcAdjArgs ica;
+    ica.init = g->init + q.off_k;
+    ica.fsl0 = q.fsl0;
+    ica.lin = q.lin;
+    ica.init_bar = g->ibar + q.off_k;
+    ica.mbar_ic = g->mbic + q.off_k;
+    if (!q.integ) {
         hipLaunchKernelGGL(k_group_zero, dim3((int)((nblk + 255) / 256)), dim3(256), 0, s, dbar, nblk);
+        hipLaunchKernelGGL(k_group_ic_adj, dim3((nk + 63) / 64), dim3(64), 0, s, g->wavenum + q.off_k,
+                           g->gbar + q.off_k, g->sp + i, nk, ica);
+        return;
     }
-    hipLaunchKernelGGL(k_group_ic_adj, dim3((nk + 63) / 64), dim3(64), 0, s, g->wavenum + q.off_k, g->init + q.off_k,
-                       g->gbar + q.off_k, g->sp + i, nk, q.fsl0, q.lin, g->ibar + q.off_k, g->mbic + q.off_k);
+    const int nfs = 16 * na, nsb = 4 * na;
+    const double *Yk = na < na_max ? g->Yp + q.off_y : g->Y + q.off_y;
+    double *yb = na < na_max ? g->ybar + q.off_y : dbar;
+    const size_t lds = sizeof(double) * 2 * (size_t)na;
+    // nk item blocks, then the initial-condition piece's blocks
+    hipLaunchKernelGGL(k_dnu_adj, dim3(nk + (nk + 63) / 64), dim3(64), lds, s, g->htabs[q.htab], g->fx + q.off_fs,
+                       g->fy + q.off_fs, g->fc + q.off_fs, nfs, g->sx + q.off_s, Yk, g->C + q.off_y,
+                       g->bucket + q.off_b, na, nsb, group_sinv_bw(q), g->wavenum + q.off_k, nk, g->sp + i, q.vcrit,
+                       g->ivl + (size_t)q.off_k * kLimit * 2, g->nint + q.off_k, g->gbar + q.off_k,
+                       q.spec.delta_nu_prefac, yb, g->cbar + q.off_y, g->mubar + q.off_k, ica);
+    if (na > 2) {
+        const long long t = (long long)na * nk;
+        hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(na, nk)), dim3(256), 0, s, g->sx + q.off_s, na, nk,
+                           g->cbar + q.off_y, g->zbuf + q.off_y);
+        hipLaunchKernelGGL(k_dnu_adj_rows, dim3((int)((t + 255) / 256)), dim3(256), 0, s, g->sx + q.off_s, na, nk,
+                           g->zbuf + q.off_y, yb);
+    }
+    if (na < na_max) {
+        hipLaunchKernelGGL(k_group_zero, dim3((int)((nblk + 255) / 256)), dim3(256), 0, s, dbar, nblk);
+        KN_CHECK(hipMemcpy2DAsync(dbar, sizeof(double) * na_max, yb, sizeof(double) * na, sizeof(double) * na, nk,
+                                  hipMemcpyDeviceToDevice, s));
+    }
 }
 
 // Fork the caller's stream into the group's streams, issue the histories,
@@ -985,7 +1051,7 @@ extern "C" void kn_group_destroy(kn_group *g)
         KN_CHECK(hipGraphDestroy(g->graph));
     }
     void *ptrs[] = {g->sx,    g->Y,    g->Yp,     g->C,   g->dbar,  g->ybar,  g->cbar,   g->zbuf, g->fx,
-                    g->fy,    g->fc,   g->wavenum, g->init, g->out,  g->raw,   g->gbar,   g->ibar, g->mbic,
+                    g->fy,    g->fc,   g->wavenum, g->init, g->out,  g->gbar,   g->ibar, g->mbic,
                     g->mubar, g->mbar, g->ivl,    g->bucket, g->nint, g->status, g->zeros, g->spec0, g->sp,
                     g->meta};
     for (void *p : ptrs)
@@ -1075,7 +1141,7 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
     g->sx = group_alloc<double>(g->s_tot);
     for (double **p : {&g->Y, &g->Yp, &g->C, &g->dbar, &g->ybar, &g->cbar, &g->zbuf}) *p = group_alloc<double>(ny);
     for (double **p : {&g->fx, &g->fy, &g->fc}) *p = group_alloc<double>(g->fs_tot);
-    for (double **p : {&g->wavenum, &g->init, &g->out, &g->raw, &g->gbar, &g->ibar, &g->mbic, &g->mubar})
+    for (double **p : {&g->wavenum, &g->init, &g->out, &g->gbar, &g->ibar, &g->mbic, &g->mubar})
         *p = group_alloc<double>(nk);
     g->mbar = group_alloc<double>(nhist);
     g->ivl = group_alloc<double>(nk * kLimit * 2);

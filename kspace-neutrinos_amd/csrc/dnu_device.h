@@ -137,7 +137,7 @@ struct DnuIntegrand {  // delta_tot_table.c:492-500
     int ns, nsb, linear;
     double sx0, sinv_bw;
     double k, mnubykT;
-    JPar J;
+    const JPar *J;  // in device memory: the series weights are read where used
     int *edom;
     __device__ __forceinline__ double operator()(double x) const
     {
@@ -151,7 +151,7 @@ struct DnuIntegrand {  // delta_tot_table.c:492-500
         q = q < 0 ? 0 : (q > nsb ? nsb : q);
         const int si = settle(sx, ns, sbucket[q], x);
         const double dtot = linear ? linear_at(sx, sy, si, x) : cspline_at(sx, sy, sc, si, x);
-        const double specJ = specialJ(k * fsl / mnubykT, J);
+        const double specJ = specialJ(k * fsl / mnubykT, *J);
         const double ai = exp(x);
         return fsl / (ai * exp(ln_hubble(H, x))) * specJ * dtot;
     }
