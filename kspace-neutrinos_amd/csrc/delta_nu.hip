@@ -20,6 +20,11 @@
 #include <math.h>
 #include <string.h>
 #include <algorithm>
+#include <condition_variable>
+#include <cstdlib>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -35,7 +40,8 @@ constexpr int kWavesPerBlock = 4;
 // ---------------------------------------------------------------- H table
 
 struct HostHTab {
-    double x0 = 0.0;             // ln a of panel 0; a panel edge falls on ln TimeTransfer
+    double origin = 0.0;         // ln TimeTransfer the table is anchored on
+    double x0 = 0.0;             // ln a of panel 0 (origin - 20); a panel edge falls on the origin
     double width = 1.0 / 256.0;  // panel width in ln a
     int npanel = 0;
     std::vector<double> coef;
@@ -49,6 +55,7 @@ HostHTab g_h;
 // ln a covered: [origin - 20, origin + 8]
 void h_reset(double origin)
 {
+    g_h.origin = origin;
     g_h.x0 = origin - 20.0;
     g_h.npanel = (int)(28.0 / g_h.width);
     g_h.coef.assign((size_t)g_h.npanel * kHnc, 0.0);
@@ -417,6 +424,110 @@ void upload_constants()
 
 int blocks_for(long long waves) { return (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock); }
 
+// ---------------------------------------------------------------- host staging
+// The per-call inputs go through pinned host memory (grown as needed, kept
+// for the process): host threads pack the delta_tot rows (stride namax on the
+// host) to stride Na there, and one copy stream uploads them while the
+// fslength kernels, which need none of them, already run.
+
+// Threads that split a packing loop with the caller (KN_HOST_THREADS, default
+// min(8, hardware threads), counting the caller).
+class HostPool {
+public:
+    HostPool()
+    {
+        int n = (int)std::thread::hardware_concurrency();
+        n = n < 1 ? 1 : (n > 8 ? 8 : n);
+        if (const char *e = getenv("KN_HOST_THREADS")) n = atoi(e) > 0 ? atoi(e) : 1;
+        for (int i = 1; i < n; i++) th_.emplace_back([this] { work(); });
+    }
+    ~HostPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_work_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    // f(0) .. f(parts - 1), on the pool and the calling thread
+    void run(int parts, const std::function<void(int)> &f)
+    {
+        if (th_.empty() || parts <= 1) {
+            for (int p = 0; p < parts; p++) f(p);
+            return;
+        }
+        std::unique_lock<std::mutex> lk(m_);
+        job_ = &f;
+        nparts_ = parts;
+        next_ = 0;
+        pending_ = parts;
+        gen_++;
+        cv_work_.notify_all();
+        drain(lk);
+        cv_done_.wait(lk, [this] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    void drain(std::unique_lock<std::mutex> &lk)
+    {
+        while (next_ < nparts_) {
+            const int p = next_++;
+            const std::function<void(int)> *f = job_;
+            lk.unlock();
+            (*f)(p);
+            lk.lock();
+            if (--pending_ == 0) cv_done_.notify_all();
+        }
+    }
+    void work()
+    {
+        unsigned long seen = 0;
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            cv_work_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            drain(lk);
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable cv_work_, cv_done_;
+    const std::function<void(int)> *job_ = nullptr;
+    int nparts_ = 0, next_ = 0, pending_ = 0;
+    unsigned long gen_ = 0;
+    bool stop_ = false;
+};
+
+HostPool &host_pool()
+{
+    static HostPool p;
+    return p;
+}
+
+char *g_stage = nullptr;
+size_t g_stage_cap = 0;
+hipStream_t g_copy = nullptr;
+hipEvent_t g_copied = nullptr;
+
+char *pinned_stage(size_t bytes)
+{
+    if (bytes > g_stage_cap) {
+        if (g_stage) KN_CHECK(hipHostFree(g_stage));
+        KN_CHECK(hipHostMalloc((void **)&g_stage, bytes, 0));
+        g_stage_cap = bytes;
+    }
+    if (!g_copy) {
+        KN_CHECK(hipStreamCreateWithFlags(&g_copy, hipStreamNonBlocking));
+        KN_CHECK(hipEventCreateWithFlags(&g_copied, hipEventDisableTiming));
+    }
+    return g_stage;
+}
+
+inline size_t align64(size_t n) { return (n + 63) & ~(size_t)63; }
+
 }  // namespace
 
 // ------------------------------------------------------------ host API
@@ -427,7 +538,7 @@ int blocks_for(long long waves) { return (int)((waves + kWavesPerBlock - 1) / kW
 // H below that limit never enter a panel that the integrals use.
 extern "C" void kn_htable_build_host(double origin, double xlo, double xhi)
 {
-    if (g_h.coef.empty() || fabs(origin - (g_h.x0 + 20.0)) > 0.0) h_reset(origin);
+    if (g_h.coef.empty() || origin != g_h.origin) h_reset(origin);
     const int p0 = (int)floor((xlo - g_h.x0) / g_h.width), p1 = (int)floor((xhi - g_h.x0) / g_h.width);
     if (p0 < 0 || p1 >= g_h.npanel)
         kn_terminate(KN_ERR_RANGE, "scale factor range [%g, %g] outside the Hubble table\n", exp(xlo), exp(xhi));
@@ -449,7 +560,7 @@ static void h_validate(double xlo, double xhi)
         if (p < 0 || p >= g_h.npanel || !g_h.built[p]) continue;
         const double h = kn_hubble(exp(x));
         if (fabs(kn_htable_eval_host(x) / h - 1) > 1e-11) {
-            h_reset(g_h.x0 + 20.0);
+            h_reset(g_h.origin);
             return;
         }
     }
@@ -457,7 +568,7 @@ static void h_validate(double xlo, double xhi)
 
 extern "C" void kn_htable_ensure(double origin, double xlo, double xhi)
 {
-    if (!g_h.coef.empty() && fabs(origin - (g_h.x0 + 20.0)) == 0.0) h_validate(xlo, xhi);
+    if (!g_h.coef.empty() && origin == g_h.origin) h_validate(xlo, xhi);
     kn_htable_build_host(origin, xlo, xhi);
     if (g_h.d_cap < g_h.coef.size()) {
         if (g_h.d_coef) KN_CHECK(hipFree(g_h.d_coef));
@@ -569,14 +680,40 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
         KN_CHECK(hipMalloc(&dev->item_species, sizeof(int) * NUSPECIES));
         KN_CHECK(hipMalloc(&dev->status, sizeof(int) * 2));
     }
-    // history rows packed with stride Na (host rows have stride namax)
-    std::vector<double> Yh((size_t)nk * Na);
-    for (int k = 0; k < nk; k++) memcpy(&Yh[(size_t)k * Na], d->delta_tot[k], sizeof(double) * Na);
-    KN_CHECK(hipMemcpy(dev->Y, Yh.data(), sizeof(double) * Yh.size(), hipMemcpyHostToDevice));
-    KN_CHECK(hipMemcpy(dev->sx, d->scalefact, sizeof(double) * Na, hipMemcpyHostToDevice));
-    KN_CHECK(hipMemcpy(dev->wavenum, wavenum, sizeof(double) * nk, hipMemcpyHostToDevice));
-    SpeciesDev sd[NUSPECIES];
-    int ispec[NUSPECIES];
+    if (record && dev->cap_ivl < (size_t)nitems) {
+        if (dev->ivl) KN_CHECK(hipFree(dev->ivl));
+        KN_CHECK(hipMalloc(&dev->ivl, sizeof(double) * (size_t)nitems * kLimit * 2));
+        dev->cap_ivl = nitems;
+    }
+    // pinned staging: inputs [Y packed | scalefact | wavenum | species | species index], then outputs
+    const size_t nY = (size_t)nk * Na;
+    const size_t b_sp = sizeof(double) * (nY + Na + nk);
+    const size_t b_in = align64(b_sp + sizeof(SpeciesDev) * NUSPECIES + sizeof(int) * NUSPECIES);
+    char *stage = pinned_stage(b_in + align64(sizeof(double) * nitems) + sizeof(int) * ((size_t)nitems + 2));
+    double *stY = (double *)stage, *stX = stY + nY, *stK = stX + Na;
+    SpeciesDev *sd = (SpeciesDev *)(stage + b_sp);
+    int *ispec = (int *)(sd + NUSPECIES);
+    double *res = (double *)(stage + b_in);
+    int *nint = (int *)(stage + b_in + align64(sizeof(double) * nitems)), *status = nint + nitems;
+
+    // the fslength nodes and their spline need only (T0, a): they start now
+    const HTab H = device_htab();
+    KN_CHECK(hipMemsetAsync(dev->status, 0, sizeof(int) * 2, 0));
+    hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, d->light,
+                       dev->fx, dev->fy, dev->status);
+    hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, 0, dev->fx, dev->fy, dev->fc, nfs, 1, 0LL,
+                       0LL);
+    KN_CHECK(hipGetLastError());
+    // meanwhile the history rows are packed to stride Na and uploaded on the copy stream
+    {
+        const int parts = nY < (1u << 16) ? 1 : 32;
+        host_pool().run(parts, [&](int p) {
+            const int k0 = (int)((long long)nk * p / parts), k1 = (int)((long long)nk * (p + 1) / parts);
+            for (int k = k0; k < k1; k++) memcpy(stY + (size_t)k * Na, d->delta_tot[k], sizeof(double) * Na);
+        });
+    }
+    memcpy(stX, d->scalefact, sizeof(double) * Na);
+    memcpy(stK, wavenum, sizeof(double) * nk);
     for (size_t i = 0; i < need.size(); i++) {
         const kn_species &q = sp[need[i]];
         sd[i].mnubykT = q.mnubykT;
@@ -587,27 +724,19 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
         for (int n = 1; n < 20; n++) sd[i].J.en[n] = exp(-n * q.qc);
         ispec[i] = (int)i;
     }
-    KN_CHECK(hipMemcpy(dev->sp, sd, sizeof(SpeciesDev) * need.size(), hipMemcpyHostToDevice));
-    KN_CHECK(hipMemcpy(dev->item_species, ispec, sizeof(int) * need.size(), hipMemcpyHostToDevice));
-    KN_CHECK(hipMemset(dev->status, 0, sizeof(int) * 2));
-    if (record && dev->cap_ivl < (size_t)nitems) {
-        if (dev->ivl) KN_CHECK(hipFree(dev->ivl));
-        KN_CHECK(hipMalloc(&dev->ivl, sizeof(double) * (size_t)nitems * kLimit * 2));
-        dev->cap_ivl = nitems;
-    }
+    KN_CHECK(hipMemcpyAsync(dev->Y, stY, sizeof(double) * nY, hipMemcpyHostToDevice, g_copy));
+    KN_CHECK(hipMemcpyAsync(dev->sx, stX, sizeof(double) * Na, hipMemcpyHostToDevice, g_copy));
+    KN_CHECK(hipMemcpyAsync(dev->wavenum, stK, sizeof(double) * nk, hipMemcpyHostToDevice, g_copy));
+    KN_CHECK(hipMemcpyAsync(dev->sp, sd, sizeof(SpeciesDev) * need.size(), hipMemcpyHostToDevice, g_copy));
+    KN_CHECK(hipMemcpyAsync(dev->item_species, ispec, sizeof(int) * need.size(), hipMemcpyHostToDevice, g_copy));
+    KN_CHECK(hipEventRecord(g_copied, g_copy));
+    KN_CHECK(hipStreamWaitEvent(0, g_copied, 0));
 
-    const HTab H = device_htab();
     const double sinv_bw = nsb / (d->scalefact[Na - 1] - d->scalefact[0]);
-    hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, d->light,
-                       dev->fx, dev->fy, dev->status);
     if (Na > 2)
         hipLaunchKernelGGL(k_spline, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, dev->Y, dev->C, Na, nk,
                            (long long)Na, (long long)Na);
     hipLaunchKernelGGL(k_bucket, dim3((nsb + 256) / 256), dim3(256), 0, 0, dev->sx, Na, nsb, sinv_bw, dev->bucket);
-    KN_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, 0, dev->fx, dev->fy, dev->fc, nfs, 1, 0LL,
-                       0LL);
-    KN_CHECK(hipGetLastError());
     IcArgs no_ic;
     no_ic.init = nullptr;
     no_ic.fsl0 = no_ic.lin = no_ic.prefac = 0.0;
@@ -616,16 +745,14 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
                        dev->item_species, nitems, x0, xa, dev->out, dev->nint, dev->status,
                        record ? dev->ivl : nullptr, no_ic);
     KN_CHECK(hipGetLastError());
-    std::vector<double> res(nitems);
-    std::vector<int> nint(nitems);
-    int status[2];
-    KN_CHECK(hipMemcpy(res.data(), dev->out, sizeof(double) * nitems, hipMemcpyDeviceToHost));
-    KN_CHECK(hipMemcpy(nint.data(), dev->nint, sizeof(int) * nitems, hipMemcpyDeviceToHost));
-    KN_CHECK(hipMemcpy(status, dev->status, sizeof(int) * 2, hipMemcpyDeviceToHost));
+    KN_CHECK(hipMemcpyAsync(res, dev->out, sizeof(double) * nitems, hipMemcpyDeviceToHost, 0));
+    KN_CHECK(hipMemcpyAsync(nint, dev->nint, sizeof(int) * nitems, hipMemcpyDeviceToHost, 0));
+    KN_CHECK(hipMemcpyAsync(status, dev->status, sizeof(int) * 2, hipMemcpyDeviceToHost, 0));
+    KN_CHECK(hipStreamSynchronize(0));
     if (status[1]) kn_terminate(2001, "GSL_ERROR-equivalent: interpolation outside the stored table (GSL_EDOM)\n");
     if (status[0]) kn_terminate(2001, "GSL_ERROR-equivalent: QAG failed in get_delta_nu (status %d)\n", status[0]);
     g_last_max_intervals = 0;
-    for (int v : nint) g_last_max_intervals = v > g_last_max_intervals ? v : g_last_max_intervals;
+    for (int i = 0; i < nitems; i++) g_last_max_intervals = nint[i] > g_last_max_intervals ? nint[i] : g_last_max_intervals;
     for (size_t i = 0; i < need.size(); i++)
         for (int k = 0; k < nk; k++)
             out[(size_t)need[i] * nk + k] += d->delta_nu_prefac * res[i * nk + k];
