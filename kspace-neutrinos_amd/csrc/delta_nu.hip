@@ -797,7 +797,7 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
         if (init_bar) init_bar[k] = gbar[k] * specialJ(kk, 0, nfl0) * lin;
         if (s.mnubykT > 0) mb += gbar[k] * fit_dJdx(kk) * (-kk / m) * d->delta_nu_init[k] * lin;
     }
-    if (dtot_bar) memset(dtot_bar, 0, sizeof(double) * (size_t)nk * Na);
+    if (dtot_bar && (s.ic_only || Na < 2)) memset(dtot_bar, 0, sizeof(double) * (size_t)nk * Na);
     if (!s.ic_only && Na >= 2) {
         DnuDev *dev = g_dev[d];
         const size_t ny = (size_t)nk * Na;
@@ -839,10 +839,19 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
                                dev->zbuf, dev->ybar);
             KN_CHECK(hipGetLastError());
         }
-        std::vector<double> mub(nk);
-        KN_CHECK(hipMemcpy(mub.data(), dev->mubar, sizeof(double) * nk, hipMemcpyDeviceToHost));
+        // results through the pinned stage: [mu_bar (nk) | dL/d delta_tot (nk Na)]
+        double *mub = (double *)pinned_stage(sizeof(double) * (nk + (dtot_bar ? ny : 0))), *yb = mub + nk;
+        KN_CHECK(hipMemcpyAsync(mub, dev->mubar, sizeof(double) * nk, hipMemcpyDeviceToHost, 0));
+        if (dtot_bar) KN_CHECK(hipMemcpyAsync(yb, dev->ybar, sizeof(double) * ny, hipMemcpyDeviceToHost, 0));
+        KN_CHECK(hipStreamSynchronize(0));
         for (int k = 0; k < nk; k++) mb += mub[k];
-        if (dtot_bar) KN_CHECK(hipMemcpy(dtot_bar, dev->ybar, sizeof(double) * ny, hipMemcpyDeviceToHost));
+        if (dtot_bar) {
+            const int parts = ny < (1u << 16) ? 1 : 32;
+            host_pool().run(parts, [&](int p) {
+                const size_t i0 = ny * p / parts, i1 = ny * (p + 1) / parts;
+                memcpy(dtot_bar + i0, yb + i0, sizeof(double) * (i1 - i0));
+            });
+        }
     }
     if (mnu_bar) *mnu_bar = mb / d->omnu->kBtnu;
     return 0;
@@ -891,18 +900,22 @@ struct GroupMeta {
     double kBtnu;
 };
 
-// dL/d mnu per history: the IC terms, then the integral's terms, in k order.
-__global__ void k_group_mass_bar(const GroupMeta *__restrict__ meta, int nhist, const double *__restrict__ mbar_ic,
-                                 const double *__restrict__ mubar, double *__restrict__ out)
+// dL/d mnu per history, one wavefront each: lane l sums k = l, l + 64, ...
+// (the IC term, then the integral's), then a fixed butterfly (deterministic).
+__global__ __launch_bounds__(64) void k_group_mass_bar(const GroupMeta *__restrict__ meta,
+                                                       const double *__restrict__ mbar_ic,
+                                                       const double *__restrict__ mubar, double *__restrict__ out)
 {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= nhist) return;
+    const int h = blockIdx.x;
     const GroupMeta g = meta[h];
     double s = 0.0;
-    for (int k = 0; k < g.nk; k++) s += mbar_ic[g.off_k + k];
-    if (g.integ)
-        for (int k = 0; k < g.nk; k++) s += mubar[g.off_k + k];
-    out[h] = s / g.kBtnu;
+    for (int k = threadIdx.x; k < g.nk; k += 64) {
+        s += mbar_ic[g.off_k + k];
+        if (g.integ) s += mubar[g.off_k + k];
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    if (threadIdx.x == 0) out[h] = s / g.kBtnu;
 }
 
 __global__ void k_group_zero(double *__restrict__ p, long long n)
@@ -1133,8 +1146,8 @@ void group_launch(kn_group *g, hipStream_t origin, bool forward, bool adjoint)
         KN_CHECK(hipStreamWaitEvent(origin, g->join[s], 0));
     }
     if (adjoint) {
-        hipLaunchKernelGGL(k_group_mass_bar, dim3((g->nhist + 127) / 128), dim3(128), 0, origin, g->meta, g->nhist,
-                           g->mbic, g->mubar, g->mbar);
+        hipLaunchKernelGGL(k_group_mass_bar, dim3(g->nhist), dim3(64), 0, origin, g->meta, g->mbic, g->mubar,
+                           g->mbar);
         KN_CHECK(hipGetLastError());
     }
 }
