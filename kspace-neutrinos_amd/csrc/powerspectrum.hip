@@ -206,11 +206,11 @@ BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
     kn_bin::build_invwindow(dims, iw);
     g->half_bpu = (float)(0.5 * (nrbins - 1) / log(sqrt(3) * dims / 2.0));
     const long long rows = (long long)nslab * dims;
-    // two row-range blocks per resident slot: the second round evens out the
-    // blocks' finishing times (1.56 -> 1.52 ms at 1024^3; more rounds lose it
-    // again to partial-histogram stores)
+    // one resident wave of row-range blocks: no tail of late blocks (a second
+    // round of blocks measured slower once the doubled partial histograms are
+    // reduced)
     const long long pgrid = (long long)resident_blocks(kn_bin::k_bin_power_runs_pf<kPowWaves, kPowRun>, 64 * kPowWaves,
-                                                       power_smem(dims, nrbins)) * num_cus() * 2;
+                                                       power_smem(dims, nrbins)) * num_cus();
     g->grid = (int)(rows < pgrid ? rows : pgrid);
     g->ggrid = (int)(rows < 4LL * num_cus() ? rows : 4LL * num_cus());
     const int pmax = g->grid > g->ggrid ? g->grid : g->ggrid;
