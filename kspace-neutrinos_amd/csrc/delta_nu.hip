@@ -8,10 +8,10 @@
 //   k_fs_nodes   the 16 Na free-streaming lengths fslength(x_i, log a) on the
 //                reference's uniform log-a grid, each a QAG-GK61 integral of
 //                1/(a^2 H) (relerr 1e-6), one wavefront per node;
-//   k_spline     natural cubic spline coefficients of that table and of every
-//                delta_tot[k][0..Na-1] row (one wavefront per row window: scans of
-//                the substitution recurrences);
-//   k_bucket     a uniform bucket index over the stored log a values;
+//   k_spline     natural cubic spline coefficients of that table (one
+//                wavefront per row window: scans of the substitution recurrences);
+//   k_rows_prep  the same for every delta_tot[k][0..Na-1] row, and a uniform
+//                bucket index over the stored log a values, in one launch;
 //   k_dnu_qag    one wavefront per (species, k): QAG-GK61 of the integrand
 //                fsl/(a H) * specialJ * S_k over [log T0, log a] at the
 //                species' tolerance, as the reference does per k.
@@ -177,12 +177,24 @@ __device__ inline void bucket_at(const double *__restrict__ x, int n, double inv
     bucket[q] = lo;
 }
 
-__global__ void k_bucket(const double *__restrict__ x, int n, int nb, double inv_bw, int *__restrict__ bucket)
+// The delta_tot rows' splines and the bucket index in one launch: blocks
+// [0, b_sp) four row windows each (none when na <= 2), the rest the bucket.
+__global__ __launch_bounds__(256) void k_rows_prep(const double *__restrict__ sx, const double *__restrict__ Y,
+                                                   double *__restrict__ C, int na, int nk, int b_sp, int nsb,
+                                                   double sinv_bw, int *__restrict__ bucket)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q > nb) return;
-    bucket_at(x, n, inv_bw, bucket, q);
+    const int b = blockIdx.x;
+    if (b < b_sp) {
+        const long long item = (long long)b * 4 + (threadIdx.x >> 6);
+        if (item < (long long)((na + kKeep - 1) / kKeep) * nk)
+            spline_item_wave(sx, Y, C, na, (long long)na, (long long)na, item);
+        return;
+    }
+    const int q = (b - b_sp) * 256 + threadIdx.x;
+    if (q <= nsb) bucket_at(sx, na, sinv_bw, bucket, q);
 }
+
+inline int rows_prep_spline_blocks(int na, int nk) { return na > 2 ? spline_blocks(na, nk) : 0; }
 
 struct SpeciesDev {
     double mnubykT, relerr;
@@ -353,10 +365,12 @@ __global__ __launch_bounds__(64) void k_dnu_adj(
     if (lane == 0) mu_bar[item] = mu_acc;
 }
 
-// Per row of the adjoint, first M z = cbar (the natural-spline matrix is
-// symmetric, so its transpose is itself), by wavefronts as k_spline ...
+// Per row of the adjoint, M z = cbar (the natural-spline matrix is
+// symmetric, so its transpose is itself) by wavefronts as k_spline, and z
+// added through the right-hand side 3((y_{i+1}-y_i)/h_i - (y_i-y_{i-1})/h_{i-1})
+// onto ybar in the same pass.
 __global__ __launch_bounds__(256) void k_adj_solve(const double *__restrict__ x, int n, int nrows,
-                                                   const double *__restrict__ cbar, double *__restrict__ z)
+                                                   const double *__restrict__ cbar, double *__restrict__ ybar)
 {
     const int nwin = (n + kKeep - 1) / kKeep;
     const long long item = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -365,23 +379,7 @@ __global__ __launch_bounds__(256) void k_adj_solve(const double *__restrict__ x,
     const int lo = w * kKeep, hi = lo + kKeep < n ? lo + kKeep : n;
     ArrayRhs rhs;
     rhs.r = cbar + (size_t)r * n;
-    tridiag_window_wave(x, n, rhs, lo, hi, z + (size_t)r * n);
-}
-
-// ... then z added through the right-hand side 3((y_{i+1}-y_i)/h_i - (y_i-y_{i-1})/h_{i-1})
-// onto ybar (one thread per entry).
-__global__ void k_dnu_adj_rows(const double *__restrict__ x, int n, int nrows, const double *__restrict__ z,
-                               double *__restrict__ ybar)
-{
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)n * nrows) return;
-    const int i = (int)(t % n);
-    const double *zr = z + (t - i);
-    double acc = 0.0;
-    if (i - 1 >= 1 && i - 1 <= n - 2) acc += zr[i - 1] * 3.0 / (x[i] - x[i - 1]);
-    if (i >= 1 && i <= n - 2) acc -= zr[i] * (3.0 / (x[i + 1] - x[i]) + 3.0 / (x[i] - x[i - 1]));
-    if (i + 1 >= 1 && i + 1 <= n - 2) acc += zr[i + 1] * 3.0 / (x[i + 1] - x[i]);
-    ybar[t] += acc;
+    tridiag_window_wave<ArrayRhs, true>(x, n, rhs, lo, hi, ybar + (size_t)r * n);
 }
 
 // ---------------------------------------------------------------- state
@@ -397,7 +395,7 @@ struct DnuDev {
     int *status = nullptr;
     size_t cap_ivl = 0, cap_adj = 0, cap_adj_k = 0;
     double *ivl = nullptr;                                            // recorded subdivisions
-    double *gbar = nullptr, *ybar = nullptr, *cbar = nullptr, *zbuf = nullptr, *mubar = nullptr;  // adjoint
+    double *gbar = nullptr, *ybar = nullptr, *cbar = nullptr, *mubar = nullptr;  // adjoint
 };
 std::unordered_map<const void *, DnuDev *> g_dev;
 int g_last_max_intervals = 0;
@@ -606,7 +604,7 @@ extern "C" void kn_dnu_release(const _delta_tot_table *d)
     if (it == g_dev.end()) return;
     DnuDev *v = it->second;
     void *ptrs[] = {v->fx,  v->fy,   v->fc,   v->sx,   v->Y,    v->C,    v->wavenum, v->bucket, v->out,
-                    v->nint, v->item_species, v->sp, v->status, v->ivl, v->gbar, v->ybar, v->cbar, v->zbuf,
+                    v->nint, v->item_species, v->sp, v->status, v->ivl, v->gbar, v->ybar, v->cbar,
                     v->mubar};
     for (void *p : ptrs)
         if (p) KN_CHECK(hipFree(p));
@@ -733,10 +731,11 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     KN_CHECK(hipStreamWaitEvent(0, g_copied, 0));
 
     const double sinv_bw = nsb / (d->scalefact[Na - 1] - d->scalefact[0]);
-    if (Na > 2)
-        hipLaunchKernelGGL(k_spline, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, dev->Y, dev->C, Na, nk,
-                           (long long)Na, (long long)Na);
-    hipLaunchKernelGGL(k_bucket, dim3((nsb + 256) / 256), dim3(256), 0, 0, dev->sx, Na, nsb, sinv_bw, dev->bucket);
+    {
+        const int bsp = rows_prep_spline_blocks(Na, nk);
+        hipLaunchKernelGGL(k_rows_prep, dim3(bsp + (nsb + 256) / 256), dim3(256), 0, 0, dev->sx, dev->Y, dev->C, Na, nk,
+                           bsp, nsb, sinv_bw, dev->bucket);
+    }
     IcArgs no_ic;
     no_ic.init = nullptr;
     no_ic.fsl0 = no_ic.lin = no_ic.prefac = 0.0;
@@ -802,7 +801,7 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
         DnuDev *dev = g_dev[d];
         const size_t ny = (size_t)nk * Na;
         if (dev->cap_adj < ny) {
-            for (double **p : {&dev->ybar, &dev->cbar, &dev->zbuf}) {
+            for (double **p : {&dev->ybar, &dev->cbar}) {
                 if (*p) KN_CHECK(hipFree(*p));
                 KN_CHECK(hipMalloc(p, sizeof(double) * ny));
             }
@@ -832,11 +831,8 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
                            IcAdjArgs{nullptr, 0.0, 0.0, nullptr, nullptr});
         KN_CHECK(hipGetLastError());
         if (Na > 2) {
-            const long long t = (long long)Na * nk;
             hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, Na, nk, dev->cbar,
-                               dev->zbuf);
-            hipLaunchKernelGGL(k_dnu_adj_rows, dim3((int)((t + 255) / 256)), dim3(256), 0, 0, dev->sx, Na, nk,
-                               dev->zbuf, dev->ybar);
+                               dev->ybar);
             KN_CHECK(hipGetLastError());
         }
         // results through the pinned stage: [mu_bar (nk) | dL/d delta_tot (nk Na)]
@@ -953,7 +949,7 @@ struct kn_group {
     std::vector<double *> hcoef;
     std::vector<int> order;
     double *sx = nullptr, *Y = nullptr, *Yp = nullptr, *C = nullptr, *dbar = nullptr, *ybar = nullptr,
-           *cbar = nullptr, *zbuf = nullptr, *fx = nullptr, *fy = nullptr, *fc = nullptr, *wavenum = nullptr,
+           *cbar = nullptr, *fx = nullptr, *fy = nullptr, *fc = nullptr, *wavenum = nullptr,
            *init = nullptr, *out = nullptr, *gbar = nullptr, *ibar = nullptr, *mbic = nullptr,
            *mubar = nullptr, *mbar = nullptr, *ivl = nullptr;
     int *bucket = nullptr, *nint = nullptr, *status = nullptr, *zeros = nullptr, *spec0 = nullptr;
@@ -1069,10 +1065,9 @@ void group_forward_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
     const double *sx = g->sx + q.off_s;
     hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, s, H, nfs, q.x0, q.xa,
                        q.spec.light, fx, fy, g->status + 2 * i);
-    if (na > 2)
-        hipLaunchKernelGGL(k_spline, dim3(spline_blocks(na, nk)), dim3(256), 0, s, sx, Yk, g->C + q.off_y, na, nk,
-                           (long long)na, (long long)na);
-    hipLaunchKernelGGL(k_bucket, dim3((nsb + 256) / 256), dim3(256), 0, s, sx, na, nsb, sinv_bw, g->bucket + q.off_b);
+    const int bsp = rows_prep_spline_blocks(na, nk);
+    hipLaunchKernelGGL(k_rows_prep, dim3(bsp + (nsb + 256) / 256), dim3(256), 0, s, sx, Yk, g->C + q.off_y, na, nk, bsp,
+                       nsb, sinv_bw, g->bucket + q.off_b);
     hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, s, fx, fy, fc, nfs, 1, 0LL, 0LL);
     IcArgs ic;
     ic.init = g->init + q.off_k;
@@ -1113,13 +1108,9 @@ void group_adjoint_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
                        g->bucket + q.off_b, na, nsb, group_sinv_bw(q), g->wavenum + q.off_k, nk, g->sp + i, q.vcrit,
                        g->ivl + (size_t)q.off_k * kLimit * 2, g->nint + q.off_k, g->gbar + q.off_k,
                        q.spec.delta_nu_prefac, yb, g->cbar + q.off_y, g->mubar + q.off_k, ica);
-    if (na > 2) {
-        const long long t = (long long)na * nk;
+    if (na > 2)
         hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(na, nk)), dim3(256), 0, s, g->sx + q.off_s, na, nk,
-                           g->cbar + q.off_y, g->zbuf + q.off_y);
-        hipLaunchKernelGGL(k_dnu_adj_rows, dim3((int)((t + 255) / 256)), dim3(256), 0, s, g->sx + q.off_s, na, nk,
-                           g->zbuf + q.off_y, yb);
-    }
+                           g->cbar + q.off_y, yb);
     if (na < na_max) {
         hipLaunchKernelGGL(k_group_zero, dim3((int)((nblk + 255) / 256)), dim3(256), 0, s, dbar, nblk);
         KN_CHECK(hipMemcpy2DAsync(dbar, sizeof(double) * na_max, yb, sizeof(double) * na, sizeof(double) * na, nk,
@@ -1190,7 +1181,7 @@ extern "C" void kn_group_destroy(kn_group *g)
         KN_CHECK(hipGraphExecDestroy(g->exec));
         KN_CHECK(hipGraphDestroy(g->graph));
     }
-    void *ptrs[] = {g->sx,    g->Y,    g->Yp,     g->C,   g->dbar,  g->ybar,  g->cbar,   g->zbuf, g->fx,
+    void *ptrs[] = {g->sx,    g->Y,    g->Yp,     g->C,   g->dbar,  g->ybar,  g->cbar,   g->fx,
                     g->fy,    g->fc,   g->wavenum, g->init, g->out,  g->gbar,   g->ibar, g->mbic,
                     g->mubar, g->mbar, g->ivl,    g->bucket, g->nint, g->status, g->zeros, g->spec0, g->sp,
                     g->meta};
@@ -1279,7 +1270,7 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
     }
     const size_t ny = (size_t)g->y_tot, nk = (size_t)g->nk_tot;
     g->sx = group_alloc<double>(g->s_tot);
-    for (double **p : {&g->Y, &g->Yp, &g->C, &g->dbar, &g->ybar, &g->cbar, &g->zbuf}) *p = group_alloc<double>(ny);
+    for (double **p : {&g->Y, &g->Yp, &g->C, &g->dbar, &g->ybar, &g->cbar}) *p = group_alloc<double>(ny);
     for (double **p : {&g->fx, &g->fy, &g->fc}) *p = group_alloc<double>(g->fs_tot);
     for (double **p : {&g->wavenum, &g->init, &g->out, &g->gbar, &g->ibar, &g->mbic, &g->mubar})
         *p = group_alloc<double>(nk);

@@ -458,12 +458,16 @@ struct ArrayRhs {
 };
 
 // Called by all 64 lanes of a wavefront together, with hi - lo <= kKeep.
-template <class RHS>
+// ADJ_RHS: instead of storing the solution z, add its image under the
+// transpose of the natural-spline right-hand side's y-derivative onto out
+// (the second half of the spline adjoint) for the kept rows.
+template <class RHS, bool ADJ_RHS = false>
 __device__ inline void tridiag_window_wave(const double *x, int n, const RHS &rhs, int lo, int hi, double *out)
 {
     const int lane = threadIdx.x & 63;
     if (n < 3) {
-        for (int i = lo + lane; i < hi; i += 64) out[i] = 0.0;
+        if (!ADJ_RHS)
+            for (int i = lo + lane; i < hi; i += 64) out[i] = 0.0;
         return;
     }
     int wa = lo - kHalo, wb = hi + kHalo;  // window of interior rows [wa, wb)
@@ -552,17 +556,55 @@ __device__ inline void tridiag_window_wave(const double *x, int n, const RHS &rh
     }
     double c = __shfl_down(A, 1, 64);
     if (lane == 63) c = 0.0;
+    if (!ADJ_RHS) {
+#pragma unroll
+        for (int j = kRun - 1; j >= 0; j--) {
+            if (j < len) {
+                const int i = s + j;
+                c = zz[j] - g[j] * c;
+                if (i >= lo && i < hi) out[i] = c;
+            }
+        }
+        if (lane == 0) {
+            if (lo == 0) out[0] = 0.0;
+            if (lo <= n - 1 && n - 1 < hi) out[n - 1] = 0.0;
+        }
+        return;
+    }
+    // ADJ_RHS: out (+)= d(rhs)/dy^T z for the kept rows, with
+    // rhs_i = 3 ((y_{i+1} - y_i) / h_i - (y_i - y_{i-1}) / h_{i-1}) and z_0 = z_{n-1} = 0
+    double zv[kRun];
 #pragma unroll
     for (int j = kRun - 1; j >= 0; j--) {
         if (j < len) {
-            const int i = s + j;
             c = zz[j] - g[j] * c;
-            if (i >= lo && i < hi) out[i] = c;
+            zv[j] = c;
+        } else {
+            zv[j] = 0.0;
         }
     }
-    if (lane == 0) {
-        if (lo == 0) out[0] = 0.0;
-        if (lo <= n - 1 && n - 1 < hi) out[n - 1] = 0.0;
+    // z just before and after this lane's run (0 at the natural ends; a
+    // window's inner ends are never next to a kept row)
+    double zp = __shfl_up(zv[kRun - 1], 1, 64), zn = __shfl_down(zv[0], 1, 64);
+    if (lane == 0 || s - 1 < 1) zp = 0.0;
+    if (lane == 63 || s + len >= wb) zn = 0.0;
+#pragma unroll
+    for (int j = 0; j < kRun; j++) {
+        const int i = s + j;
+        if (j < len && i >= lo && i < hi) {
+            const double zl = j > 0 ? zv[j - 1] : zp, zr = j + 1 < len ? zv[j + 1] : zn;
+            double acc = 0.0;
+            if (i - 1 >= 1) acc += zl * 3.0 / (x[i] - x[i - 1]);
+            acc -= zv[j] * (3.0 / (x[i + 1] - x[i]) + 3.0 / (x[i] - x[i - 1]));
+            if (i + 1 <= n - 2) acc += zr * 3.0 / (x[i + 1] - x[i]);
+            out[i] += acc;
+        }
+    }
+    if (lane == 0 && lo == 0 && len > 0) out[0] += zv[0] * 3.0 / (x[1] - x[0]);  // row 0: the z_1 term
+    if (lo <= n - 1 && n - 1 < hi) {                                         // row n-1: the z_{n-2} term
+#pragma unroll
+        for (int j = 0; j < kRun; j++)
+            if (j < len && s + j == n - 2) out[n - 1] += zv[j] * 3.0 / (x[n - 1] - x[n - 2]);
     }
 }
 
