@@ -80,7 +80,8 @@ __device__ __forceinline__ double specialJ(double x, const JPar &P)
             const double nn = (double)(n * n);
             const double ii = (n * n + n * n * n * qc + n * qc * x * x - x * x) * qc * j0 + (2 * n + n * n * qc + qc * x * x) * c;
             const double sgn = (n & 1) ? 1.0 : -1.0;   // -1 * pow(-1, n)
-            integ += sgn * P.en[n] / (nn + x * x) / (nn + x * x) * ii;
+            const double d = nn + x * x;
+            integ += sgn * P.en[n] / (d * d) * ii;  // one division per term
         }
         return integ / (1.5 * 1.202056903159594 * (1 - P.nufrac_low));
     }
@@ -645,7 +646,7 @@ __device__ inline double specialJ_d(double x, const JPar &P, double &dJdx, doubl
             const double iix = Ax * qc * j0 + A * qc * qc * j0p + Bx * c - B * qc * s;
             const double iiq = Aq * qc * j0 + A * (j0 + z * j0p) + Bq * c - B * x * s;
             const double sgn = (n & 1) ? 1.0 : -1.0;
-            const double w = sgn * P.en[n] / d / d;
+            const double w = sgn * P.en[n] / (d * d);
             J += w * ii;
             Jx += w * (iix - 4.0 * x / d * ii);
             Jq += w * (iiq - n * ii);
