@@ -113,31 +113,19 @@ class _HCtx(C.Structure):
                 ("Hubble", C.c_double)]
 
 
-def c4_batching(kn, torch, nhist=256, seed=1000, reps=3):
-    """256 independent histories: Na ~ U[100, 3000] stored steps over a in
-    [0.01, 1], nk ~ U[64, 512] log-spaced k in [2 pi/512, sqrt(3) pi 1024/512]
-    h/Mpc, one species m ~ U[0.06, 0.6] eV, Omega_m ~ U[0.25, 0.35], each with
-    its own H(a); delta_tot = a sqrt(P(k)) (1 + 0.1 N(0,1)) with the noise drawn
-    per k (noise drawn per (k, a) entry makes the reference's own QAG fail, GSL
-    error -> terminate 2001, on long histories).
-
-    plain: one C-ABI get_delta_nu call per history (and one
-    kn_get_delta_nu_adjoint call for the adjoint), host tables in and out; the
-    process has one hubble_function, so its H table is rebuilt per history.
-    group: the same histories resident in HBM in one kn_group (built once: one
-    H table per history); a step is kn_group_solve + kn_group_adjoint, or the
-    graph of both replayed."""
+def build_histories(kn, nhist, rng, draw):
+    """nhist tables with their own Omega_nu and H(a): draw(rng) -> (na, nk, m, Omega_m).
+    delta_tot = a sqrt(P(k)) (1 + 0.1 N(0,1)) with the noise drawn per k, log-spaced a
+    in [0.01, 1] and k in [2 pi/512, sqrt(3) pi 1024/512] h/Mpc; returns
+    (keep, hists, data) with the kn_batch_history of each table."""
     import conftest
     host = conftest.load_testhost()
     import knhip
     host.kn_test_hctx_init.argtypes = [C.POINTER(_HCtx), C.c_void_p, C.c_double, C.c_double]
     hfn = C.cast(host.kn_test_hubble_ctx, C.c_void_p).value
-    rng = np.random.default_rng(seed)
     keep, hists, data = [], [], []
-    ksteps = 0
     for h in range(nhist):
-        na, nk = int(rng.integers(100, 3001)), int(rng.integers(64, 513))
-        m, om0 = float(rng.uniform(0.06, 0.6)), float(rng.uniform(0.25, 0.35))
+        na, nk, m, om0 = draw(rng)
         om = knhip.OmegaNu()
         kn.init_omega_nu(C.byref(om), (C.c_double * 3)(m, 0.0, 0.0), 0.01, 0.7, 2.7255)
         d = knhip.DeltaTotTable()
@@ -162,11 +150,19 @@ def c4_batching(kn, torch, nhist=256, seed=1000, reps=3):
         hists.append(hh)
         data.append(dict(na=na, nk=nk, m=m, om0=om0, logs=logs, rows=np.array(rows[:nk, :na]), k=k,
                          init=np.array(init)))
-        ksteps += nk * na
-    g_all = rng.uniform(-1, 1, sum(x["nk"] for x in data))
-    # plain path, one history at a time
-    plain_out, t_fwd, t_adj = [], 0.0, 0.0
-    off = 0
+    return keep, hists, data
+
+
+def plain_vs_group(kn, torch, keep, hists, data, reps=3, seed=5):
+    """The same histories through the per-history path (get_delta_nu, then
+    kn_get_delta_nu_adjoint, per history; host tables) and through one kn_group
+    (solve + adjoint, eager and as a replayed graph; inputs resident in HBM)."""
+    import conftest
+    host = conftest.load_testhost()
+    import knhip
+    nhist = len(hists)
+    g_all = np.random.default_rng(seed).uniform(-1, 1, sum(x["nk"] for x in data))
+    plain_out, t_fwd, t_adj, off = [], 0.0, 0.0, 0
     for (om, d, ctx), x in zip(keep, data):
         host.kn_test_init_hubble(C.byref(om), x["om0"], conftest.UNIT_TIME)
         out = np.zeros(x["nk"])
@@ -183,7 +179,6 @@ def c4_batching(kn, torch, nhist=256, seed=1000, reps=3):
         assert rc == 0
         plain_out.append(out)
         off += x["nk"]
-    # group: built once, then steps of solve + adjoint with everything resident
     t0 = time.perf_counter()
     B = knhip.Batch(hists, keep=keep, api="kn_group")
     t_create = time.perf_counter() - t0
@@ -220,9 +215,9 @@ def c4_batching(kn, torch, nhist=256, seed=1000, reps=3):
     B.close()
     for om, d, ctx in keep:
         kn.free_delta_tot_table(C.byref(d))
+    ksteps = sum(x["nk"] * x["na"] for x in data)
     tf, tfa, tgr = float(np.median(ts_f)), float(np.median(ts_fa)), float(np.median(tg))
-    return {"workload": "%d histories, Na~U[100,3000], nk~U[64,512], m~U[0.06,0.6] eV, Omega_m~U[0.25,0.35]" % nhist,
-            "ksteps": int(ksteps),
+    return {"ksteps": int(ksteps),
             "plain": {"forward_s": round(t_fwd, 4), "forward_adjoint_s": round(t_fwd + t_adj, 4),
                       "forward_kstep_per_s": round(ksteps / t_fwd, 1),
                       "note": "one get_delta_nu + one kn_get_delta_nu_adjoint C-ABI call per history, host in/out, "
@@ -234,8 +229,32 @@ def c4_batching(kn, torch, nhist=256, seed=1000, reps=3):
                               "create (one H table per history) once, outside the step"},
             "speedup_forward_adjoint": round((t_fwd + t_adj) / min(tfa, tgr), 2),
             "speedup_forward": round(t_fwd / tf, 2),
-            "max_rel_diff_group_vs_plain": diff,
-            "persistent_batch": "kn_batch (one persistent launch) switched off in this build: returns 3005"}
+            "max_rel_diff_group_vs_plain": diff}
+
+
+def c4_batching(kn, torch, nhist=256, seed=1000, reps=3):
+    """256 independent histories: Na ~ U[100, 3000], nk ~ U[64, 512], one
+    species m ~ U[0.06, 0.6] eV, Omega_m ~ U[0.25, 0.35], each with its own
+    H(a) (noise drawn per k: drawn per (k, a) entry it makes the reference's own
+    QAG fail, GSL error -> terminate 2001, on long histories)."""
+    draw = lambda r: (int(r.integers(100, 3001)), int(r.integers(64, 513)), float(r.uniform(0.06, 0.6)),
+                      float(r.uniform(0.25, 0.35)))
+    keep, hists, data = build_histories(kn, nhist, np.random.default_rng(seed), draw)
+    res = plain_vs_group(kn, torch, keep, hists, data, reps)
+    res["workload"] = ("%d histories, Na~U[100,3000], nk~U[64,512], m~U[0.06,0.6] eV, Omega_m~U[0.25,0.35]"
+                       % nhist)
+    res["persistent_batch"] = "kn_batch (one persistent launch) switched off in this build: returns 3005"
+    return res
+
+
+def c5_group(kn, torch, nhist=64, seed=2000, reps=3):
+    """The differentiation workload's batch: 64 histories of 1024 k x 2000
+    steps, m ~ U[0.06, 0.6] eV (Omega_m = 0.3), forward + adjoint."""
+    draw = lambda r: (2000, 1024, float(r.uniform(0.06, 0.6)), 0.3)
+    keep, hists, data = build_histories(kn, nhist, np.random.default_rng(seed), draw)
+    res = plain_vs_group(kn, torch, keep, hists, data, reps)
+    res["workload"] = "%d histories of 1024 k x 2000 steps, m~U[0.06,0.6] eV, Omega_m=0.3" % nhist
+    return res
 
 
 def c5_adjoints(kn, torch):
@@ -301,6 +320,8 @@ def main():
     ap.add_argument("--only-batch", action="store_true",
                     help="only the 256-history plain vs grouped delta_nu + adjoint timing")
     ap.add_argument("--batch-hists", type=int, default=256, help="histories for --only-batch")
+    ap.add_argument("--only-adjoint-batch", action="store_true",
+                    help="only the 64-history 1024 k x 2000 step plain vs grouped forward + adjoint timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 2 GiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -327,6 +348,9 @@ def main():
         return
     if args.only_batch:
         print(json.dumps({"delta_nu_batch": c4_batching(kn, torch, nhist=args.batch_hists)}))
+        return
+    if args.only_adjoint_batch:
+        print(json.dumps({"group_64x1024x2000": c5_group(kn, torch)}))
         return
 
     N, nrbins, box = args.n, args.nrbins, 512000.0
@@ -404,6 +428,7 @@ def main():
         out["delta_nu"] = dnu_bench(kn, n, no_cpu=args.no_cpu)
         out["delta_nu_batch"] = c4_batching(kn, torch)
         out["adjoints"] = c5_adjoints(kn, torch)
+        out["adjoints"]["group_64x1024x2000"] = c5_group(kn, torch)
     print(json.dumps(out))
 
 
