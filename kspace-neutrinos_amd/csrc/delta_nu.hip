@@ -5,9 +5,10 @@
 // with the host's own hubble_function (the initial-condition piece,
 // fslength(T0, a), delta_tot_table.c:522-535) and the per-species parameters
 // (:539-548); the device computes, in order:
-//   k_fs_nodes   the 16 Na free-streaming lengths fslength(x_i, log a) on the
-//                reference's uniform log-a grid, each a QAG-GK61 integral of
-//                1/(a^2 H) (relerr 1e-6), one wavefront per node;
+//   k_fs_first,  the 16 Na free-streaming lengths fslength(x_i, log a) on the
+//   k_fs_rest    reference's uniform log-a grid, each a QAG-GK61 integral of
+//                1/(a^2 H) (relerr 1e-6): the first estimate for two nodes per
+//                wavefront, full QAG (one wavefront each) for those it leaves;
 //   k_spline     natural cubic spline coefficients of that table (one
 //                wavefront per row window: scans of the substitution recurrences);
 //   k_rows_prep  the same for every delta_tot[k][0..Na-1] row, and a uniform
@@ -122,16 +123,71 @@ __device__ inline void fs_node_wave(const HTab &H, int nfs, double x0, double xa
     }
 }
 
-__global__ __launch_bounds__(64 * kWavesPerBlock) void k_fs_nodes(HTab H, int nfs, double x0, double xa,
-                                                                   double light, double *__restrict__ fx,
-                                                                   double *__restrict__ fy, int *__restrict__ status)
+// The fslength nodes in two passes with the result of one QAG per node.
+// Pass 1 (k_fs_first, no LDS): two nodes per wavefront, lanes 0-31 and 32-63
+// each forming the first 61-point Gauss-Kronrod estimate over [x_i, log a]
+// exactly as wave_qag61's first step does, and accepting it by the same test
+// (QUADPACK returns that estimate when it already meets the tolerance); a node
+// that needs bisection is appended to a list.  Pass 2 (k_fs_rest) runs the
+// full wavefront QAG for the listed nodes.  Node values are those of one
+// wave_qag61 per node, bit for bit.
+__global__ __launch_bounds__(256) void k_fs_first(HTab H, int nfs, double x0, double xa, double light,
+                                                  double *__restrict__ fx, double *__restrict__ fy,
+                                                  int *__restrict__ status, int *__restrict__ list,
+                                                  int *__restrict__ count)
+{
+    const int lane = threadIdx.x & 63, hf = lane >> 5;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (2 * wave >= nfs) return;  // wave-uniform
+    const int nA = 2 * wave, nB = nA + 1;
+    // delta_tot_table.c:582: log T0 + i (log a - log T0) / (Nfs - 1.)
+    const double xA = x0 + nA * (xa - x0) / (nfs - 1.);
+    const double xB = nB < nfs ? x0 + nB * (xa - x0) / (nfs - 1.) : xa;
+    FslIntegrand f;
+    f.H = H;
+    GK gA, gB;
+    gk61_pair(f, xA < xa ? xA : xa, xa, xB < xa ? xB : xa, xa, gA, gB);
+    if ((lane & 31) == 0) {
+        const int node = hf ? nB : nA;
+        const double xi = hf ? xB : xA;
+        const GK &g = hf ? gB : gA;
+        if (node < nfs) {
+            double v = 0.0;
+            if (xi < xa) {  // fslength returns 0 when logai >= logaf (:402)
+                const double tol = 1e-6 * fabs(g.res);
+                if (g.err <= 50 * DBL_EPSILON * g.resabs && g.err > tol) {
+                    atomicOr(status, 1);  // roundoff on the first estimate (wave_qag61 returns 18)
+                    v = light * g.res;
+                } else if ((g.err <= tol && g.err != g.resasc) || g.err == 0.0) {
+                    v = light * g.res;
+                } else {
+                    list[atomicAdd(count, 1)] = node;
+                    v = 0.0;
+                }
+            }
+            fx[node] = xi;
+            fy[node] = v;
+        }
+    }
+}
+
+// Pass 2: full QAG for the listed nodes (a bounded loop over the list).
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_fs_rest(HTab H, int nfs, double x0, double xa, double light,
+                                                                  double *__restrict__ fx, double *__restrict__ fy,
+                                                                  int *__restrict__ status,
+                                                                  const int *__restrict__ list,
+                                                                  const int *__restrict__ count)
 {
     __shared__ QagWs ws[kWavesPerBlock];
     const int wave = threadIdx.x >> 6;
-    const int node = blockIdx.x * kWavesPerBlock + wave;
-    if (node >= nfs) return;
-    fs_node_wave(H, nfs, x0, xa, light, fx, fy, status, node, ws[wave]);
+    const int n = *count;
+    for (int t = blockIdx.x * kWavesPerBlock + wave; t < n; t += gridDim.x * kWavesPerBlock)
+        fs_node_wave(H, nfs, x0, xa, light, fx, fy, status, list[t], ws[wave]);
 }
+
+constexpr int kFsRestBlocks = 64;
+
+inline int fs_first_blocks(int nfs) { return (nfs + 7) / 8; }  // 4 waves x 2 nodes per block
 
 // Rows of natural-spline coefficients (row r: x[0..n), y + r*ystride,
 // c + r*cstride): one wavefront per (row, window of kKeep rows).
@@ -387,6 +443,7 @@ __global__ __launch_bounds__(256) void k_adj_solve(const double *__restrict__ x,
 struct DnuDev {
     size_t cap_fs = 0, cap_y = 0, cap_x = 0, cap_k = 0, cap_items = 0, cap_b = 0;
     double *fx = nullptr, *fy = nullptr, *fc = nullptr;
+    int *fs_list = nullptr;  // fslength nodes that need bisection, then their count at [cap_fs]
     double *sx = nullptr, *Y = nullptr, *C = nullptr, *wavenum = nullptr;
     int *bucket = nullptr;
     double *out = nullptr;
@@ -603,7 +660,7 @@ extern "C" void kn_dnu_release(const _delta_tot_table *d)
     auto it = g_dev.find(d);
     if (it == g_dev.end()) return;
     DnuDev *v = it->second;
-    void *ptrs[] = {v->fx,  v->fy,   v->fc,   v->sx,   v->Y,    v->C,    v->wavenum, v->bucket, v->out,
+    void *ptrs[] = {v->fs_list, v->fx,  v->fy,   v->fc,   v->sx,   v->Y,    v->C,    v->wavenum, v->bucket, v->out,
                     v->nint, v->item_species, v->sp, v->status, v->ivl, v->gbar, v->ybar, v->cbar,
                     v->mubar};
     for (void *p : ptrs)
@@ -649,11 +706,13 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     const int nfs = 16 * Na;
     const int nsb = 4 * Na;
     const int nitems = (int)need.size() * nk;
-    if (dev->cap_fs < (size_t)nfs) {  // fx, fy, fc
+    if (dev->cap_fs < (size_t)nfs) {  // fx, fy, fc, fs_list
         for (double **p : {&dev->fx, &dev->fy, &dev->fc}) {
             if (*p) KN_CHECK(hipFree(*p));
             KN_CHECK(hipMalloc(p, sizeof(double) * nfs));
         }
+        if (dev->fs_list) KN_CHECK(hipFree(dev->fs_list));
+        KN_CHECK(hipMalloc(&dev->fs_list, sizeof(int) * ((size_t)nfs + 1)));
         dev->cap_fs = nfs;
     }
     grow(dev->sx, dev->cap_x, (size_t)Na);
@@ -697,8 +756,12 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     // the fslength nodes and their spline need only (T0, a): they start now
     const HTab H = device_htab();
     KN_CHECK(hipMemsetAsync(dev->status, 0, sizeof(int) * 2, 0));
-    hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, d->light,
-                       dev->fx, dev->fy, dev->status);
+    int *fs_count = dev->fs_list + dev->cap_fs;
+    KN_CHECK(hipMemsetAsync(fs_count, 0, sizeof(int), 0));
+    hipLaunchKernelGGL(k_fs_first, dim3(fs_first_blocks(nfs)), dim3(256), 0, 0, H, nfs, x0, xa, d->light, dev->fx,
+                       dev->fy, dev->status, dev->fs_list, fs_count);
+    hipLaunchKernelGGL(k_fs_rest, dim3(kFsRestBlocks), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, d->light,
+                       dev->fx, dev->fy, dev->status, dev->fs_list, fs_count);
     hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, 0, dev->fx, dev->fy, dev->fc, nfs, 1, 0LL,
                        0LL);
     KN_CHECK(hipGetLastError());
@@ -952,7 +1015,7 @@ struct kn_group {
            *cbar = nullptr, *fx = nullptr, *fy = nullptr, *fc = nullptr, *wavenum = nullptr,
            *init = nullptr, *out = nullptr, *gbar = nullptr, *ibar = nullptr, *mbic = nullptr,
            *mubar = nullptr, *mbar = nullptr, *ivl = nullptr;
-    int *bucket = nullptr, *nint = nullptr, *status = nullptr, *zeros = nullptr, *spec0 = nullptr;
+    int *bucket = nullptr, *nint = nullptr, *status = nullptr, *zeros = nullptr, *spec0 = nullptr, *fs_list = nullptr;
     SpeciesDev *sp = nullptr;
     GroupMeta *meta = nullptr;
     hipStream_t st[kGroupStreams] = {};
@@ -1063,8 +1126,11 @@ void group_forward_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
     }
     double *fx = g->fx + q.off_fs, *fy = g->fy + q.off_fs, *fc = g->fc + q.off_fs;
     const double *sx = g->sx + q.off_s;
-    hipLaunchKernelGGL(k_fs_nodes, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, s, H, nfs, q.x0, q.xa,
-                       q.spec.light, fx, fy, g->status + 2 * i);
+    int *fs_list = g->fs_list + q.off_fs, *fs_count = g->status + 2 * g->nhist + i;
+    hipLaunchKernelGGL(k_fs_first, dim3(fs_first_blocks(nfs)), dim3(256), 0, s, H, nfs, q.x0, q.xa, q.spec.light, fx,
+                       fy, g->status + 2 * i, fs_list, fs_count);
+    hipLaunchKernelGGL(k_fs_rest, dim3(kFsRestBlocks), dim3(64 * kWavesPerBlock), 0, s, H, nfs, q.x0, q.xa,
+                       q.spec.light, fx, fy, g->status + 2 * i, fs_list, fs_count);
     const int bsp = rows_prep_spline_blocks(na, nk);
     hipLaunchKernelGGL(k_rows_prep, dim3(bsp + (nsb + 256) / 256), dim3(256), 0, s, sx, Yk, g->C + q.off_y, na, nk, bsp,
                        nsb, sinv_bw, g->bucket + q.off_b);
@@ -1123,7 +1189,7 @@ void group_adjoint_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
 void group_launch(kn_group *g, hipStream_t origin, bool forward, bool adjoint)
 {
     if (forward)
-        KN_CHECK(hipMemcpyAsync(g->status, g->zeros, sizeof(int) * 2 * g->nhist, hipMemcpyDeviceToDevice, origin));
+        KN_CHECK(hipMemcpyAsync(g->status, g->zeros, sizeof(int) * 3 * g->nhist, hipMemcpyDeviceToDevice, origin));
     KN_CHECK(hipEventRecord(g->fork, origin));
     for (int s = 0; s < g->nstream; s++) KN_CHECK(hipStreamWaitEvent(g->st[s], g->fork, 0));
     for (int i : g->order) {
@@ -1183,7 +1249,7 @@ extern "C" void kn_group_destroy(kn_group *g)
     }
     void *ptrs[] = {g->sx,    g->Y,    g->Yp,     g->C,   g->dbar,  g->ybar,  g->cbar,   g->fx,
                     g->fy,    g->fc,   g->wavenum, g->init, g->out,  g->gbar,   g->ibar, g->mbic,
-                    g->mubar, g->mbar, g->ivl,    g->bucket, g->nint, g->status, g->zeros, g->spec0, g->sp,
+                    g->mubar, g->mbar, g->ivl,    g->bucket, g->nint, g->status, g->zeros, g->spec0, g->sp, g->fs_list,
                     g->meta};
     for (void *p : ptrs)
         if (p) KN_CHECK(hipFree(p));
@@ -1278,11 +1344,12 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
     g->ivl = group_alloc<double>(nk * kLimit * 2);
     g->bucket = group_alloc<int>(g->b_tot);
     g->nint = group_alloc<int>(nk);
-    g->status = group_alloc<int>(2 * (size_t)nhist);
-    g->zeros = group_alloc<int>(2 * (size_t)nhist);
+    g->status = group_alloc<int>(3 * (size_t)nhist);  // [2 h] QAG, [2 h + 1] domain, [2 nhist + h] fs list count
+    g->zeros = group_alloc<int>(3 * (size_t)nhist);
+    g->fs_list = group_alloc<int>(g->fs_tot);
     g->spec0 = group_alloc<int>(1);
-    KN_CHECK(hipMemset(g->zeros, 0, sizeof(int) * 2 * nhist));
-    KN_CHECK(hipMemset(g->status, 0, sizeof(int) * 2 * nhist));
+    KN_CHECK(hipMemset(g->zeros, 0, sizeof(int) * 3 * nhist));
+    KN_CHECK(hipMemset(g->status, 0, sizeof(int) * 3 * nhist));
     KN_CHECK(hipMemset(g->spec0, 0, sizeof(int)));
     g->sp = group_alloc<SpeciesDev>(nhist);
     {
