@@ -42,3 +42,46 @@ extern "C" void emu_fiber_entry()
 }
 
 #include "delta_nu.hip"
+
+// ---- test hook: the two-pass fslength nodes against one full QAG per node ----
+namespace {
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_fs_full_test(HTab H, int nfs, double x0, double xa,
+                                                                       double light, double *fx, double *fy,
+                                                                       int *status)
+{
+    __shared__ QagWs ws[kWavesPerBlock];
+    const int wave = threadIdx.x >> 6;
+    const int node = blockIdx.x * kWavesPerBlock + wave;
+    if (node >= nfs) return;
+    fs_node_wave(H, nfs, x0, xa, light, fx, fy, status, node, ws[wave]);
+}
+}  // namespace
+
+// fslength at nfs nodes over [x0, xa] with the H(a) of hub(a, ctx), by the
+// two passes the library uses (two_pass) and by one wavefront QAG per node
+// (full); returns the number of nodes the first pass handed to the second.
+extern "C" int emu_fs_two_pass_check(double (*hub)(double, void *), void *ctx, double x0, double xa, int nfs,
+                                     double light, double *two_pass, double *full)
+{
+    upload_constants();
+    const double w = 1.0 / 256.0;
+    const int npanel = (int)ceil((xa - x0) / w) + 1;
+    std::vector<double> coef((size_t)npanel * kHnc);
+    for (int p = 0; p < npanel; p++) cheb_panel(x0 + p * w, w, hub, ctx, &coef[(size_t)p * kHnc]);
+    HTab H;
+    H.coef = coef.data();
+    H.x0 = x0;
+    H.inv_w = 1.0 / w;
+    H.npanel = npanel;
+    std::vector<double> fx(nfs), fy(nfs), gx(nfs);
+    std::vector<int> list(nfs + 1, 0), status(2, 0);
+    hipLaunchKernelGGL(k_fs_first, dim3(fs_first_blocks(nfs)), dim3(256), 0, 0, H, nfs, x0, xa, light, fx.data(),
+                       fy.data(), status.data(), list.data(), list.data() + nfs);
+    const int listed = list[nfs];
+    hipLaunchKernelGGL(k_fs_rest, dim3(kFsRestBlocks), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, light,
+                       fx.data(), fy.data(), status.data(), list.data(), list.data() + nfs);
+    memcpy(two_pass, fy.data(), sizeof(double) * nfs);
+    hipLaunchKernelGGL(k_fs_full_test, dim3(blocks_for(nfs)), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, light,
+                       gx.data(), full, status.data());
+    return listed;
+}

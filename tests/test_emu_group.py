@@ -148,3 +148,27 @@ def test_emu_group_argument_checks(emu):
     assert emu.kn_group_set_mass(g, 1, 0.2) == 3002 and emu.kn_group_set_mass(g, 0, 0.0) == 3002
     emu.kn_group_destroy(g)
     c.free()
+
+
+def test_emu_fslength_two_passes_equal_one_qag_per_node(emu):
+    """The fslength nodes' first pass (two nodes per wavefront, the first
+    Gauss-Kronrod estimate accepted by QAG's own test) plus the second pass
+    (full QAG for the rest) give the values of one full QAG per node, bit for
+    bit: for a smooth H(a) (no node needs the second pass) and for an H(a)
+    with a sharp step near a = 0.1 (the nodes whose range crosses it do)."""
+    lib = C.CDLL(DNU_EMU_LIB)
+    HUB = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+    lib.emu_fs_two_pass_check.restype = C.c_int
+    lib.emu_fs_two_pass_check.argtypes = [HUB, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    smooth = HUB(lambda a, ctx: 0.1 * math.sqrt(0.3 / a ** 3 + 0.7))
+    step = HUB(lambda a, ctx: 0.1 * math.sqrt(0.3 / a ** 3 + 0.7) * (1.3 + 0.3 * math.tanh((math.log(a) - math.log(0.1)) / 0.002)))
+    nfs = 48
+    for name, fn, want_listed in (("smooth", smooth, False), ("step", step, True)):
+        two, full = (C.c_double * nfs)(), (C.c_double * nfs)()
+        listed = lib.emu_fs_two_pass_check(fn, None, math.log(0.01), 0.0, nfs, 1.0, two, full)
+        a, b = np.array(two[:]), np.array(full[:])
+        print("  %s H: %d of %d nodes to the second pass" % (name, listed, nfs))
+        assert np.array_equal(a, b)
+        assert np.all(a[:-1] > 0) and a[-1] == 0.0
+        assert (listed > 0) == want_listed

@@ -207,3 +207,34 @@ def test_group_at_batching_workload_shape(kn, torch_cuda):
     assert np.all(np.isfinite(mb))
     B.close()
     _free(cosmos)
+
+
+def test_group_with_a_stepped_hubble_matches_the_emulation(kn, torch_cuda):
+    """A history whose H(a) has a sharp step near a = 0.1: the fslength nodes
+    whose range crosses it need the second (full QAG) pass.  The device result
+    is compared with the same group run by the CPU emulation build of the same
+    kernels (tests/emu), which the oracle-backed emulation tests pin."""
+    import os
+    from conftest import ROOT
+    emu_path = os.path.join(ROOT, "tests", "emu", "_build", "libkn_dnu_emu.so")
+    emu = knhip.bind(C.CDLL(emu_path))
+    HUB = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+    hub = HUB(lambda a, ctx: 0.1 * math.sqrt(0.3 / a ** 3 + 0.7) *
+              (1.3 + 0.3 * math.tanh((math.log(a) - math.log(0.1)) / 0.002)))
+    logs, rows, k, init = synthetic_history(40, 8, 0.5)
+    hh = knhip.BatchHistory()
+    hh.nk, hh.na, hh.na_max, hh.a, hh.a_max = 8, 40, 40, 0.5, 0.5
+    hh.time_transfer, hh.mnu, hh.kBtnu, hh.light = 0.01, 0.15, 1.68e-4, 1.0e5
+    hh.delta_nu_prefac, hh.qc, hh.relerr, hh.nufrac_low, hh.vcrit, hh.ic_only = 1e-9, 0.0, 1e-6, 0.0, 0.0, 0
+    hh.hubble = C.cast(hub, C.c_void_p).value
+    out = []
+    for L, mc in ((None, None), (emu, lambda d, s_, n, kind: C.memmove(d, s_, n))):
+        B = knhip.Batch([hh], keep=(hub,), L=L, memcpy=mc, api="kn_group")
+        B.load(0, logs, rows, k, init)
+        print("stepped H: %s" % ("device" if L is None else "emulation"), flush=True)
+        B.solve()
+        out.append(B.delta_nu(0))
+        B.close()
+    e = rel(out[0], out[1])
+    print("  device vs emulation: %.2e" % e)
+    assert e < 1e-12
