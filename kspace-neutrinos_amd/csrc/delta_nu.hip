@@ -26,6 +26,7 @@
 #include <functional>
 #include <mutex>
 #include <thread>
+#include <unistd.h>
 #include <unordered_map>
 #include <vector>
 
@@ -486,7 +487,7 @@ int blocks_for(long long waves) { return (int)((waves + kWavesPerBlock - 1) / kW
 // fslength kernels, which need none of them, already run.
 
 // Threads that split a packing loop with the caller (KN_HOST_THREADS, default
-// min(8, hardware threads), counting the caller).
+// min(8, hardware threads), counting the caller); a forked child packs alone.
 class HostPool {
 public:
     HostPool()
@@ -494,21 +495,14 @@ public:
         int n = (int)std::thread::hardware_concurrency();
         n = n < 1 ? 1 : (n > 8 ? 8 : n);
         if (const char *e = getenv("KN_HOST_THREADS")) n = atoi(e) > 0 ? atoi(e) : 1;
+        pid_ = getpid();
         for (int i = 1; i < n; i++) th_.emplace_back([this] { work(); });
-    }
-    ~HostPool()
-    {
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            stop_ = true;
-        }
-        cv_work_.notify_all();
-        for (auto &t : th_) t.join();
     }
     // f(0) .. f(parts - 1), on the pool and the calling thread
     void run(int parts, const std::function<void(int)> &f)
     {
-        if (th_.empty() || parts <= 1) {
+        // a forked child has none of the parent's threads: it packs alone
+        if (th_.empty() || parts <= 1 || getpid() != pid_) {
             for (int p = 0; p < parts; p++) f(p);
             return;
         }
@@ -554,12 +548,15 @@ private:
     int nparts_ = 0, next_ = 0, pending_ = 0;
     unsigned long gen_ = 0;
     bool stop_ = false;
+    pid_t pid_ = 0;
 };
 
+// Never destroyed: its threads wait on a condition variable until the process
+// ends (a forked child, which has none of them, must not tear it down either).
 HostPool &host_pool()
 {
-    static HostPool p;
-    return p;
+    static HostPool *p = new HostPool();
+    return *p;
 }
 
 char *g_stage = nullptr;
