@@ -185,6 +185,16 @@ def plain_vs_group(kn, torch, keep, hists, data, reps=3, seed=5):
     for i, x in enumerate(data):
         B.load(i, x["logs"], x["rows"], x["k"], x["init"])
     B._put("gbar", 0, g_all)
+    modes = {}
+    for ns in (0, 4):  # merged launches, and per-history chains on 4 streams
+        assert kn.kn_group_set_streams(B.h, ns) == 0
+        B.solve()
+        B.adjoint()
+        t0 = time.perf_counter()
+        for _ in range(2):
+            assert kn.kn_group_solve(B.h, None) == 0
+            assert kn.kn_group_adjoint(B.h, None) == 0
+        modes["merged" if ns == 0 else "streams%d" % ns] = round((time.perf_counter() - t0) / 2, 4)
     assert kn.kn_group_autotune(B.h, None) == 0
     nstream = kn.kn_group_streams(B.h)
     B.solve()
@@ -224,9 +234,11 @@ def plain_vs_group(kn, torch, keep, hists, data, reps=3, seed=5):
                               "H(a) table rebuilt per history (one hubble_function per process)"},
             "group": {"forward_s": round(tf, 4), "forward_adjoint_s": round(tfa, 4),
                       "graph_forward_adjoint_s": round(tgr, 4), "create_s": round(t_create, 3),
-                      "streams": nstream, "forward_kstep_per_s": round(ksteps / tf, 1),
-                      "note": "kn_group: histories resident in HBM, per-call kernels on autotuned HIP streams; "
-                              "create (one H table per history) once, outside the step"},
+                      "streams": nstream, "forward_adjoint_s_by_mode": modes,
+                      "forward_kstep_per_s": round(ksteps / tf, 1),
+                      "note": "kn_group: histories resident in HBM; the autotuner's launch mode (streams 0: merged "
+                              "launches, one grid per phase for all histories; n > 0: per-history launch chains on "
+                              "n HIP streams); create (one H table per history) once, outside the step"},
             "speedup_forward_adjoint": round((t_fwd + t_adj) / min(tfa, tgr), 2),
             "speedup_forward": round(t_fwd / tf, 2),
             "max_rel_diff_group_vs_plain": diff}

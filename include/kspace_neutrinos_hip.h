@@ -327,7 +327,9 @@ int kn_batch_autotune(kn_batch *b, void *stream);
  * No reference counterpart (the reference evaluates one table per
  * get_delta_nu call, delta_tot_table.c:507-611); additive.  The same
  * histories, device views and layout as kn_batch, solved with the kernels of
- * get_delta_nu: each history's launch sequence runs on one of up to 8 HIP
+ * get_delta_nu, either in merged launches (the default: one grid per phase
+ * covering every history, so the launch count does not grow with the number
+ * of histories) or with each history's launch sequence on one of up to 8 HIP
  * streams forked from the caller's stream (longest histories first, each to
  * the least-loaded stream), so the small per-history grids overlap.  Every
  * solve records its final QAG subdivisions; kn_group_adjoint differentiates
@@ -353,9 +355,13 @@ int kn_group_graph_capture(kn_group *g, int with_adjoint, const double *d_loga, 
 int kn_group_graph_launch(kn_group *g, void *stream);
 int kn_group_status(kn_group *g);
 int kn_group_set_mass(kn_group *g, int h, double mnu);
-/* Pick the stream count (1, 2, 4 or 8) by timing the forward. */
+/* Pick the launch strategy by timing the forward: merged launches (0) or
+ * per-history sequences on 1, 2, 4 or 8 streams. */
 int kn_group_autotune(kn_group *g, void *stream);
+/* The strategy in use: 0 merged launches, n > 0 per-history sequences on n
+ * streams.  Setting it (0..8; 3002 otherwise) makes a captured graph stale. */
 int kn_group_streams(const kn_group *g);
+int kn_group_set_streams(kn_group *g, int nstream);
 
 /* ---- adjoint of get_delta_nu (delta_tot_table.c:507-611) ----
  * gbar[nk] = dL/d delta_nu.  Outputs (host arrays, any may be NULL): the

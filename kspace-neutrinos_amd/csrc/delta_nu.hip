@@ -131,14 +131,13 @@ __device__ inline void fs_node_wave(const HTab &H, int nfs, double x0, double xa
 // (QUADPACK returns that estimate when it already meets the tolerance); a node
 // that needs bisection is appended to a list.  Pass 2 (k_fs_rest) runs the
 // full wavefront QAG for the listed nodes.  Node values are those of one
-// wave_qag61 per node, bit for bit.
-__global__ __launch_bounds__(256) void k_fs_first(HTab H, int nfs, double x0, double xa, double light,
-                                                  double *__restrict__ fx, double *__restrict__ fy,
-                                                  int *__restrict__ status, int *__restrict__ list,
-                                                  int *__restrict__ count)
+// wave_qag61 per node, bit for bit.  `push(node)` lists a node for pass 2.
+template <class Push>
+__device__ inline void fs_first_wave(const HTab &H, int nfs, double x0, double xa, double light,
+                                     double *__restrict__ fx, double *__restrict__ fy, int *__restrict__ status,
+                                     int wave, const Push &push)
 {
     const int lane = threadIdx.x & 63, hf = lane >> 5;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (2 * wave >= nfs) return;  // wave-uniform
     const int nA = 2 * wave, nB = nA + 1;
     // delta_tot_table.c:582: log T0 + i (log a - log T0) / (Nfs - 1.)
@@ -162,7 +161,7 @@ __global__ __launch_bounds__(256) void k_fs_first(HTab H, int nfs, double x0, do
                 } else if ((g.err <= tol && g.err != g.resasc) || g.err == 0.0) {
                     v = light * g.res;
                 } else {
-                    list[atomicAdd(count, 1)] = node;
+                    push(node);
                     v = 0.0;
                 }
             }
@@ -170,6 +169,20 @@ __global__ __launch_bounds__(256) void k_fs_first(HTab H, int nfs, double x0, do
             fy[node] = v;
         }
     }
+}
+
+struct ListPush {
+    int *list, *count;
+    __device__ void operator()(int node) const { list[atomicAdd(count, 1)] = node; }
+};
+
+__global__ __launch_bounds__(256) void k_fs_first(HTab H, int nfs, double x0, double xa, double light,
+                                                  double *__restrict__ fx, double *__restrict__ fy,
+                                                  int *__restrict__ status, int *__restrict__ list,
+                                                  int *__restrict__ count)
+{
+    const ListPush push = {list, count};
+    fs_first_wave(H, nfs, x0, xa, light, fx, fy, status, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), push);
 }
 
 // Pass 2: full QAG for the listed nodes (a bounded loop over the list).
@@ -236,20 +249,28 @@ __device__ inline void bucket_at(const double *__restrict__ x, int n, double inv
 
 // The delta_tot rows' splines and the bucket index in one launch: blocks
 // [0, b_sp) four row windows each (none when na <= 2), the rest the bucket.
-__global__ __launch_bounds__(256) void k_rows_prep(const double *__restrict__ sx, const double *__restrict__ Y,
-                                                   double *__restrict__ C, int na, int nk, int b_sp, int nsb,
-                                                   double sinv_bw, int *__restrict__ bucket)
+// Rows of Y and C are `ystride` doubles apart.
+__device__ inline void rows_prep_block(const double *__restrict__ sx, const double *__restrict__ Y,
+                                       double *__restrict__ C, int na, int nk, long long ystride, int b_sp, int nsb,
+                                       double sinv_bw, int *__restrict__ bucket, int b)
 {
-    const int b = blockIdx.x;
     if (b < b_sp) {
         const long long item = (long long)b * 4 + (threadIdx.x >> 6);
-        if (item < (long long)((na + kKeep - 1) / kKeep) * nk)
-            spline_item_wave(sx, Y, C, na, (long long)na, (long long)na, item);
+        if (item < (long long)((na + kKeep - 1) / kKeep) * nk) spline_item_wave(sx, Y, C, na, ystride, ystride, item);
         return;
     }
     const int q = (b - b_sp) * 256 + threadIdx.x;
     if (q <= nsb) bucket_at(sx, na, sinv_bw, bucket, q);
 }
+
+__global__ __launch_bounds__(256) void k_rows_prep(const double *__restrict__ sx, const double *__restrict__ Y,
+                                                   double *__restrict__ C, int na, int nk, long long ystride,
+                                                   int b_sp, int nsb, double sinv_bw, int *__restrict__ bucket)
+{
+    rows_prep_block(sx, Y, C, na, nk, ystride, b_sp, nsb, sinv_bw, bucket, blockIdx.x);
+}
+
+inline int rows_prep_blocks(int b_sp, int nsb) { return b_sp + (nsb + 256) / 256; }
 
 inline int rows_prep_spline_blocks(int na, int nk) { return na > 2 ? spline_blocks(na, nk) : 0; }
 
@@ -275,52 +296,74 @@ __device__ __forceinline__ double ic_piece(double kk, double init, const Species
     return specialJ(kk * ic.fsl0 / m, J0) * init * ic.lin;
 }
 
-__global__ __launch_bounds__(64 * kWavesPerBlock) void k_dnu_qag(
-    HTab H, const double *__restrict__ fx, const double *__restrict__ fy, const double *__restrict__ fc, int nfs,
-    const double *__restrict__ sx, const double *__restrict__ Y, const double *__restrict__ C, const int *__restrict__ sb,
-    int ns, int nsb, double sinv_bw, const double *__restrict__ wavenum, int nk, const SpeciesDev *__restrict__ sp,
-    const int *__restrict__ item_species, int nitems, double x0, double xa, double *__restrict__ out,
-    int *__restrict__ nint_out, int *__restrict__ status, double *__restrict__ ivl, IcArgs ic)
+// What one history's QAG items read and write (rows of Y and C `ystride`
+// doubles apart).
+struct QagArgs {
+    HTab H;
+    const double *fx, *fy, *fc;
+    int nfs;
+    const double *sx, *Y, *C;
+    const int *sb;
+    int ns, nsb;
+    long long ystride;
+    double sinv_bw;
+    const double *wavenum;
+    double x0, xa;
+    double *out;
+    int *nint_out, *status;
+    double *ivl;
+    IcArgs ic;
+};
+
+// Item `item` (species sp, wavenumber k) by one wavefront.
+__device__ inline void dnu_qag_item(const QagArgs &a, const SpeciesDev &sp, int item, int k, QagWs &ws)
 {
-    __shared__ QagWs ws[kWavesPerBlock];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int item = blockIdx.x * kWavesPerBlock + wave;
-    if (item >= nitems) return;
-    const int s = item_species[item / nk], k = item % nk;
+    const int lane = threadIdx.x & 63;
     DnuIntegrand f;
-    f.H = H;
-    f.fx = fx;
-    f.fy = fy;
-    f.fc = fc;
-    f.nfs = nfs;
-    f.fx0 = fx[0];
-    f.finv_h = (nfs - 1) / (fx[nfs - 1] - fx[0]);
-    f.sx = sx;
-    f.sy = Y + (size_t)k * ns;
-    f.sc = C + (size_t)k * ns;
-    f.sbucket = sb;
-    f.ns = ns;
-    f.nsb = nsb;
-    f.linear = ns <= 2;
-    f.sx0 = sx[0];
-    f.sinv_bw = sinv_bw;
-    f.k = wavenum[k];
-    f.mnubykT = sp[s].mnubykT;
-    f.J = &sp[s].J;
-    f.edom = status + 1;
+    f.H = a.H;
+    f.fx = a.fx;
+    f.fy = a.fy;
+    f.fc = a.fc;
+    f.nfs = a.nfs;
+    f.fx0 = a.fx[0];
+    f.finv_h = (a.nfs - 1) / (a.fx[a.nfs - 1] - a.fx[0]);
+    f.sx = a.sx;
+    f.sy = a.Y + (size_t)k * a.ystride;
+    f.sc = a.C + (size_t)k * a.ystride;
+    f.sbucket = a.sb;
+    f.ns = a.ns;
+    f.nsb = a.nsb;
+    f.linear = a.ns <= 2;
+    f.sx0 = a.sx[0];
+    f.sinv_bw = a.sinv_bw;
+    f.k = a.wavenum[k];
+    f.mnubykT = sp.mnubykT;
+    f.J = &sp.J;
+    f.edom = a.status + 1;
     double r;
     int ni;
-    const int rc = wave_qag61(f, x0, xa, sp[s].relerr, kLimit, ws[wave], &r, &ni);
-    if (ivl)  // the final subdivision, for the adjoint
+    const int rc = wave_qag61(f, a.x0, a.xa, sp.relerr, kLimit, ws, &r, &ni);
+    if (a.ivl)  // the final subdivision, for the adjoint
         for (int t = lane; t < ni; t += 64) {
-            ivl[((size_t)item * kLimit + t) * 2] = ws[wave].lo[t];
-            ivl[((size_t)item * kLimit + t) * 2 + 1] = ws[wave].hi[t];
+            a.ivl[((size_t)item * kLimit + t) * 2] = ws.lo[t];
+            a.ivl[((size_t)item * kLimit + t) * 2 + 1] = ws.hi[t];
         }
     if (lane == 0) {
-        out[item] = ic.init ? ic_piece(wavenum[k], ic.init[k], sp[s], ic) + ic.prefac * r : r;
-        nint_out[item] = ni;
-        if (rc) atomicOr(status, 2);
+        a.out[item] = a.ic.init ? ic_piece(a.wavenum[k], a.ic.init[k], sp, a.ic) + a.ic.prefac * r : r;
+        a.nint_out[item] = ni;
+        if (rc) atomicOr(a.status, 2);
     }
+}
+
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_dnu_qag(QagArgs a, const SpeciesDev *__restrict__ sp,
+                                                                  const int *__restrict__ item_species, int nk,
+                                                                  int nitems)
+{
+    __shared__ QagWs ws[kWavesPerBlock];
+    const int wave = threadIdx.x >> 6;
+    const int item = blockIdx.x * kWavesPerBlock + wave;
+    if (item >= nitems) return;
+    dnu_qag_item(a, sp[item_species[item / nk]], item, item % nk, ws[wave]);
 }
 
 // Adjoint of the initial-condition piece at k: dL/d delta_nu_init[k] and the
@@ -350,29 +393,53 @@ __device__ __forceinline__ void ic_adjoint_at(int k, const double *__restrict__ 
 // node weights onto delta_tot[k][.] (y) and the spline coefficients (c) are
 // accumulated in LDS (ds_add_f64: the row belongs to this wave alone), plus
 // the derivative in m/kT.  One wavefront per block, one item per block.
-__global__ __launch_bounds__(64) void k_dnu_adj(
-    HTab H, const double *__restrict__ fx, const double *__restrict__ fy, const double *__restrict__ fc, int nfs,
-    const double *__restrict__ sx, const double *__restrict__ Y, const double *__restrict__ C,
-    const int *__restrict__ sb, int ns, int nsb, double sinv_bw, const double *__restrict__ wavenum, int nk,
-    const SpeciesDev *__restrict__ sp, double vcrit, const double *__restrict__ ivl, const int *__restrict__ nint,
-    const double *__restrict__ gbar, double prefac, double *__restrict__ ybar, double *__restrict__ cbar,
-    double *__restrict__ mu_bar, IcAdjArgs ica)
+// Rows of Y, C, ybar and cbar are `ystride` doubles apart; entries
+// [ns, ystride) of the item's ybar row are written as zeros.
+struct AdjArgs {
+    HTab H;
+    const double *fx, *fy, *fc;
+    int nfs;
+    const double *sx, *Y, *C;
+    const int *sb;
+    int ns, nsb, nk;
+    long long ystride;
+    double sinv_bw;
+    const double *wavenum;
+    const SpeciesDev *sp;
+    double vcrit;
+    const double *ivl;
+    const int *nint;
+    const double *gbar;
+    double prefac;
+    double *ybar, *cbar, *mu_bar;
+    IcAdjArgs ica;
+};
+
+// Block `blk` of one history's adjoint: items [0, nk), then the
+// initial-condition piece, 64 k per block.  `lds`: 2 ns doubles.
+__device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
 {
-    HIP_DYNAMIC_SHARED(double, lds)  // ybar row [ns], cbar row [ns]
-    if ((int)blockIdx.x >= nk) {  // blocks past the items: the initial-condition piece, 64 k per block
-        const int kk = ((int)blockIdx.x - nk) * 64 + (int)threadIdx.x;
-        if (kk < nk) ic_adjoint_at(kk, wavenum, gbar, *sp, ica);
+    const int nk = A.nk, ns = A.ns, nfs = A.nfs;
+    const double *__restrict__ fx = A.fx, *__restrict__ fy = A.fy, *__restrict__ fc = A.fc, *__restrict__ sx = A.sx;
+    const int *__restrict__ sb = A.sb;
+    const double prefac = A.prefac, vcrit = A.vcrit, sinv_bw = A.sinv_bw;
+    const int nsb = A.nsb;
+    const HTab &H = A.H;
+    if (blk >= nk) {
+        const int kk = (blk - nk) * 64 + (int)(threadIdx.x & 63);
+        if (kk < nk) ic_adjoint_at(kk, A.wavenum, A.gbar, *A.sp, A.ica);
         return;
     }
-    const int lane = threadIdx.x, item = blockIdx.x, k = item % nk;
+    const int lane = threadIdx.x & 63, item = blk, k = item % nk;
     const int hf = lane >> 5, j = lane & 31;
     for (int t = lane; t < 2 * ns; t += 64) lds[t] = 0.0;
     __syncthreads();
-    const double *syk = Y + (size_t)k * ns, *sck = C + (size_t)k * ns;
+    const double *syk = A.Y + (size_t)k * A.ystride, *sck = A.C + (size_t)k * A.ystride;
     const bool linear = ns <= 2;
-    const double kk = wavenum[k], mnubykT = sp[0].mnubykT, gb = gbar[item];
-    const JPar &J = sp[0].J;
-    const int ni = nint[item];
+    const double kk = A.wavenum[k], mnubykT = A.sp->mnubykT, gb = A.gbar[item];
+    const JPar &J = A.sp->J;
+    const int ni = A.nint[item];
+    const double *__restrict__ ivl = A.ivl;
     const double fx0 = fx[0], finv_h = (nfs - 1) / (fx[nfs - 1] - fx[0]), sx0 = sx[0];
     double mu_acc = 0.0;
     for (int t0 = 0; t0 < ni; t0 += 2) {
@@ -415,28 +482,43 @@ __global__ __launch_bounds__(64) void k_dnu_adj(
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) mu_acc += __shfl_xor(mu_acc, m, 64);
     __syncthreads();
+    double *yrow = A.ybar + (size_t)k * A.ystride, *crow = A.cbar + (size_t)k * A.ystride;
     for (int t = lane; t < ns; t += 64) {
-        ybar[(size_t)k * ns + t] = lds[t];
-        cbar[(size_t)k * ns + t] = lds[ns + t];
+        yrow[t] = lds[t];
+        crow[t] = lds[ns + t];
     }
-    if (lane == 0) mu_bar[item] = mu_acc;
+    for (int t = ns + lane; t < A.ystride; t += 64) yrow[t] = 0.0;
+    if (lane == 0) A.mu_bar[item] = mu_acc;
+}
+
+__global__ __launch_bounds__(64) void k_dnu_adj(AdjArgs A)
+{
+    HIP_DYNAMIC_SHARED(double, lds)  // ybar row [ns], cbar row [ns]
+    dnu_adj_block(A, blockIdx.x, lds);
 }
 
 // Per row of the adjoint, M z = cbar (the natural-spline matrix is
 // symmetric, so its transpose is itself) by wavefronts as k_spline, and z
 // added through the right-hand side 3((y_{i+1}-y_i)/h_i - (y_i-y_{i-1})/h_{i-1})
 // onto ybar in the same pass.
-__global__ __launch_bounds__(256) void k_adj_solve(const double *__restrict__ x, int n, int nrows,
-                                                   const double *__restrict__ cbar, double *__restrict__ ybar)
+// Rows `stride` doubles apart; block `b` of spline_blocks(n, nrows).
+__device__ inline void adj_solve_block(const double *__restrict__ x, int n, int nrows, long long stride,
+                                       const double *__restrict__ cbar, double *__restrict__ ybar, int b)
 {
     const int nwin = (n + kKeep - 1) / kKeep;
-    const long long item = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long long item = (long long)b * 4 + (threadIdx.x >> 6);
     if (item >= (long long)nwin * nrows) return;  // wave-uniform
     const int r = (int)(item / nwin), w = (int)(item % nwin);
     const int lo = w * kKeep, hi = lo + kKeep < n ? lo + kKeep : n;
     ArrayRhs rhs;
-    rhs.r = cbar + (size_t)r * n;
-    tridiag_window_wave<ArrayRhs, true>(x, n, rhs, lo, hi, ybar + (size_t)r * n);
+    rhs.r = cbar + (size_t)r * stride;
+    tridiag_window_wave<ArrayRhs, true>(x, n, rhs, lo, hi, ybar + (size_t)r * stride);
+}
+
+__global__ __launch_bounds__(256) void k_adj_solve(const double *__restrict__ x, int n, int nrows, long long stride,
+                                                   const double *__restrict__ cbar, double *__restrict__ ybar)
+{
+    adj_solve_block(x, n, nrows, stride, cbar, ybar, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- state
@@ -793,16 +875,34 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     const double sinv_bw = nsb / (d->scalefact[Na - 1] - d->scalefact[0]);
     {
         const int bsp = rows_prep_spline_blocks(Na, nk);
-        hipLaunchKernelGGL(k_rows_prep, dim3(bsp + (nsb + 256) / 256), dim3(256), 0, 0, dev->sx, dev->Y, dev->C, Na, nk,
-                           bsp, nsb, sinv_bw, dev->bucket);
+        hipLaunchKernelGGL(k_rows_prep, dim3(rows_prep_blocks(bsp, nsb)), dim3(256), 0, 0, dev->sx, dev->Y, dev->C, Na,
+                           nk, (long long)Na, bsp, nsb, sinv_bw, dev->bucket);
     }
-    IcArgs no_ic;
-    no_ic.init = nullptr;
-    no_ic.fsl0 = no_ic.lin = no_ic.prefac = 0.0;
-    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nitems)), dim3(64 * kWavesPerBlock), 0, 0, H, dev->fx, dev->fy,
-                       dev->fc, nfs, dev->sx, dev->Y, dev->C, dev->bucket, Na, nsb, sinv_bw, dev->wavenum, nk, dev->sp,
-                       dev->item_species, nitems, x0, xa, dev->out, dev->nint, dev->status,
-                       record ? dev->ivl : nullptr, no_ic);
+    QagArgs qa;
+    qa.H = H;
+    qa.fx = dev->fx;
+    qa.fy = dev->fy;
+    qa.fc = dev->fc;
+    qa.nfs = nfs;
+    qa.sx = dev->sx;
+    qa.Y = dev->Y;
+    qa.C = dev->C;
+    qa.sb = dev->bucket;
+    qa.ns = Na;
+    qa.nsb = nsb;
+    qa.ystride = Na;
+    qa.sinv_bw = sinv_bw;
+    qa.wavenum = dev->wavenum;
+    qa.x0 = x0;
+    qa.xa = xa;
+    qa.out = dev->out;
+    qa.nint_out = dev->nint;
+    qa.status = dev->status;
+    qa.ivl = record ? dev->ivl : nullptr;
+    qa.ic.init = nullptr;
+    qa.ic.fsl0 = qa.ic.lin = qa.ic.prefac = 0.0;
+    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nitems)), dim3(64 * kWavesPerBlock), 0, 0, qa, dev->sp,
+                       dev->item_species, nk, nitems);
     KN_CHECK(hipGetLastError());
     KN_CHECK(hipMemcpyAsync(res, dev->out, sizeof(double) * nitems, hipMemcpyDeviceToHost, 0));
     KN_CHECK(hipMemcpyAsync(nint, dev->nint, sizeof(int) * nitems, hipMemcpyDeviceToHost, 0));
@@ -885,14 +985,37 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
         const int nfs = 16 * Na, nsb = 4 * Na;
         const double sinv_bw = nsb / (d->scalefact[Na - 1] - d->scalefact[0]);
         const double vcrit = s.qc > 0 ? d->omnu->hybnu.vcrit : 0.0;
-        hipLaunchKernelGGL(k_dnu_adj, dim3(nk), dim3(64), lds, 0, device_htab(), dev->fx, dev->fy, dev->fc, nfs,
-                           dev->sx, dev->Y, dev->C, dev->bucket, Na, nsb, sinv_bw, dev->wavenum, nk, dev->sp, vcrit,
-                           dev->ivl, dev->nint, dev->gbar, d->delta_nu_prefac, dev->ybar, dev->cbar, dev->mubar,
-                           IcAdjArgs{nullptr, 0.0, 0.0, nullptr, nullptr});
+        AdjArgs A;
+        A.H = device_htab();
+        A.fx = dev->fx;
+        A.fy = dev->fy;
+        A.fc = dev->fc;
+        A.nfs = nfs;
+        A.sx = dev->sx;
+        A.Y = dev->Y;
+        A.C = dev->C;
+        A.sb = dev->bucket;
+        A.ns = Na;
+        A.nsb = nsb;
+        A.nk = nk;
+        A.ystride = Na;
+        A.sinv_bw = sinv_bw;
+        A.wavenum = dev->wavenum;
+        A.sp = dev->sp;
+        A.vcrit = vcrit;
+        A.ivl = dev->ivl;
+        A.nint = dev->nint;
+        A.gbar = dev->gbar;
+        A.prefac = d->delta_nu_prefac;
+        A.ybar = dev->ybar;
+        A.cbar = dev->cbar;
+        A.mu_bar = dev->mubar;
+        A.ica = IcAdjArgs{nullptr, 0.0, 0.0, nullptr, nullptr};
+        hipLaunchKernelGGL(k_dnu_adj, dim3(nk), dim3(64), lds, 0, A);
         KN_CHECK(hipGetLastError());
         if (Na > 2) {
-            hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, Na, nk, dev->cbar,
-                               dev->ybar);
+            hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, Na, nk, (long long)Na,
+                               dev->cbar, dev->ybar);
             KN_CHECK(hipGetLastError());
         }
         // results through the pinned stage: [mu_bar (nk) | dL/d delta_tot (nk Na)]
@@ -916,14 +1039,15 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
 // ================================================================ groups
 //
 // Many get_delta_nu integrals (one kn_batch_history each: its own H(a), mass,
-// stored steps and k bins) solved with the per-call kernels above.  Each
-// history's launch sequence goes to one of several HIP streams forked from
-// the caller's stream (longest histories first, each to the least-loaded
-// stream), so the small per-history grids run side by side on the device.
-// Histories live in device memory in the caller's layout (delta_tot rows of
-// stride na_max); while na < na_max the kernels read a packed copy (row
-// stride na) made by a 2-D device copy.  Every solve records the final QAG
-// subdivisions, which the adjoint then holds fixed.
+// stored steps and k bins) solved with the per-call kernels' device code, in
+// one of two ways: merged launches (default), one grid per phase covering
+// every history (a block finds its history in the phase's block prefix), or
+// each history's per-call launch sequence on one of several HIP streams
+// forked from the caller's stream (longest histories first, each to the
+// least-loaded stream).  Histories live in device memory in the caller's
+// layout (delta_tot rows of stride na_max, which the kernels read and write
+// in place).  Every solve records the final QAG subdivisions, which the
+// adjoint then holds fixed.
 
 namespace {
 
@@ -999,22 +1123,236 @@ struct GroupHist {
     long long off_k, off_y, off_s, off_fs, off_b;
 };
 
+// ---- merged launches: one grid per phase for every history of the group.
+// Block b of a phase belongs to history h with pre[h] <= b < pre[h + 1]
+// (pre: the phase's exclusive prefix of per-history block counts) and is
+// that history's block b - pre[h] of the per-call kernel's grid, running the
+// same device code on the history's arrays (descriptor below).
+
+// Per-history arrays and scalars (device copy; rows of Y, C, dbar and cbar
+// are na_max doubles apart).
+struct GroupDesc {
+    HTab H;
+    const double *sx, *Y, *wavenum, *init, *gbar;
+    double *C, *fx, *fy, *fc, *out, *ivl, *dbar, *cbar, *mubar, *ibar, *mbic;
+    int *bucket, *nint, *status;
+    const SpeciesDev *sp;
+    int nk, na, na_max, nfs, nsb, bsp;
+    double x0, xa, light, sinv_bw, fsl0, lin, prefac, vcrit;
+};
+
+// phases (rows of the prefix table); the adjoint's item blocks are split in
+// classes of na (LDS rows of 2 na doubles): class c holds na <= 256 << c
+enum { kPhFs, kPhRows, kPhFsSpline, kPhQag, kPhIc, kPhSolve, kPhAdj0 };
+constexpr int kAdjClasses = 7;
+constexpr int kPhases = kPhAdj0 + kAdjClasses;
+constexpr int kMergedRestBlocks = 1280;
+
+__device__ __forceinline__ int hist_of(const int *__restrict__ pre, int nhist, int blk)
+{
+    int lo = 0, hi = nhist;  // pre[lo] <= blk < pre[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= blk) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// pass-2 list entries: (history << 32) | node
+struct GroupPush {
+    long long *list;
+    int *count;
+    int h;
+    __device__ void operator()(int node) const { list[atomicAdd(count, 1)] = ((long long)h << 32) | node; }
+};
+
+__global__ __launch_bounds__(256) void k_m_fs_first(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
+                                                    int nhist, long long *__restrict__ list, int *__restrict__ count)
+{
+    const int h = hist_of(pre, nhist, blockIdx.x);
+    const GroupDesc &d = D[h];
+    const GroupPush push = {list, count, h};
+    fs_first_wave(d.H, d.nfs, d.x0, d.xa, d.light, d.fx, d.fy, d.status,
+                  (int)(((blockIdx.x - pre[h]) * blockDim.x + threadIdx.x) >> 6), push);
+}
+
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_m_fs_rest(const GroupDesc *__restrict__ D,
+                                                                    const long long *__restrict__ list,
+                                                                    const int *__restrict__ count)
+{
+    __shared__ QagWs ws[kWavesPerBlock];
+    const int wave = threadIdx.x >> 6;
+    const int n = *count;
+    for (int t = blockIdx.x * kWavesPerBlock + wave; t < n; t += gridDim.x * kWavesPerBlock) {
+        const long long e = list[t];
+        const GroupDesc &d = D[(int)(e >> 32)];
+        fs_node_wave(d.H, d.nfs, d.x0, d.xa, d.light, d.fx, d.fy, d.status, (int)(e & 0xffffffffLL), ws[wave]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_m_rows_prep(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
+                                                     int nhist)
+{
+    const int h = hist_of(pre, nhist, blockIdx.x);
+    const GroupDesc &d = D[h];
+    rows_prep_block(d.sx, d.Y, d.C, d.na, d.nk, (long long)d.na_max, d.bsp, d.nsb, d.sinv_bw, d.bucket,
+                    blockIdx.x - pre[h]);
+}
+
+__global__ __launch_bounds__(256) void k_m_fs_spline(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
+                                                     int nhist)
+{
+    const int h = hist_of(pre, nhist, blockIdx.x);
+    const GroupDesc &d = D[h];
+    const long long item = (long long)(blockIdx.x - pre[h]) * 4 + (threadIdx.x >> 6);
+    if (item >= (d.nfs + kKeep - 1) / kKeep) return;  // wave-uniform
+    spline_item_wave(d.fx, d.fy, d.fc, d.nfs, 0LL, 0LL, item);
+}
+
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_m_dnu_qag(const GroupDesc *__restrict__ D,
+                                                                    const int *__restrict__ pre, int nhist)
+{
+    __shared__ QagWs ws[kWavesPerBlock];
+    const int h = hist_of(pre, nhist, blockIdx.x);
+    const GroupDesc &d = D[h];
+    const int wave = threadIdx.x >> 6;
+    const int k = (blockIdx.x - pre[h]) * kWavesPerBlock + wave;
+    if (k >= d.nk) return;
+    QagArgs a;
+    a.H = d.H;
+    a.fx = d.fx;
+    a.fy = d.fy;
+    a.fc = d.fc;
+    a.nfs = d.nfs;
+    a.sx = d.sx;
+    a.Y = d.Y;
+    a.C = d.C;
+    a.sb = d.bucket;
+    a.ns = d.na;
+    a.nsb = d.nsb;
+    a.ystride = d.na_max;
+    a.sinv_bw = d.sinv_bw;
+    a.wavenum = d.wavenum;
+    a.x0 = d.x0;
+    a.xa = d.xa;
+    a.out = d.out;
+    a.nint_out = d.nint;
+    a.status = d.status;
+    a.ivl = d.ivl;
+    a.ic.init = d.init;
+    a.ic.fsl0 = d.fsl0;
+    a.ic.lin = d.lin;
+    a.ic.prefac = d.prefac;
+    dnu_qag_item(a, *d.sp, k, k, ws[wave]);
+}
+
+// initial-condition-only histories, 64 k per block
+__global__ __launch_bounds__(64) void k_m_ic(const GroupDesc *__restrict__ D, const int *__restrict__ pre, int nhist)
+{
+    const int h = hist_of(pre, nhist, blockIdx.x);
+    const GroupDesc &d = D[h];
+    const int k = (blockIdx.x - pre[h]) * 64 + threadIdx.x;
+    if (k >= d.nk) return;
+    IcArgs ic;
+    ic.init = d.init;
+    ic.fsl0 = d.fsl0;
+    ic.lin = d.lin;
+    ic.prefac = d.prefac;
+    d.out[k] = ic_piece(d.wavenum[k], d.init[k], *d.sp, ic);
+}
+
+__device__ inline IcAdjArgs group_ica(const GroupDesc &d)
+{
+    IcAdjArgs ica;
+    ica.init = d.init;
+    ica.fsl0 = d.fsl0;
+    ica.lin = d.lin;
+    ica.init_bar = d.ibar;
+    ica.mbar_ic = d.mbic;
+    return ica;
+}
+
+// adjoint of the initial-condition-only histories: the piece's gradients and
+// zero rows of dL/d delta_tot, 64 k (rows) per block
+__global__ __launch_bounds__(64) void k_m_ic_adj(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
+                                                 int nhist)
+{
+    const int h = hist_of(pre, nhist, blockIdx.x);
+    const GroupDesc &d = D[h];
+    const int k0 = (blockIdx.x - pre[h]) * 64;
+    if (k0 + (int)threadIdx.x < d.nk) ic_adjoint_at(k0 + threadIdx.x, d.wavenum, d.gbar, *d.sp, group_ica(d));
+    for (int r = 0; r < 64 && k0 + r < d.nk; r++) {
+        double *row = d.dbar + (size_t)(k0 + r) * d.na_max;
+        for (int t = threadIdx.x; t < d.na_max; t += 64) row[t] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_m_dnu_adj(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
+                                                  int nhist)
+{
+    HIP_DYNAMIC_SHARED(double, lds)  // 2 na doubles of the class's largest na
+    const int h = hist_of(pre, nhist, blockIdx.x);
+    const GroupDesc &d = D[h];
+    AdjArgs A;
+    A.H = d.H;
+    A.fx = d.fx;
+    A.fy = d.fy;
+    A.fc = d.fc;
+    A.nfs = d.nfs;
+    A.sx = d.sx;
+    A.Y = d.Y;
+    A.C = d.C;
+    A.sb = d.bucket;
+    A.ns = d.na;
+    A.nsb = d.nsb;
+    A.nk = d.nk;
+    A.ystride = d.na_max;
+    A.sinv_bw = d.sinv_bw;
+    A.wavenum = d.wavenum;
+    A.sp = d.sp;
+    A.vcrit = d.vcrit;
+    A.ivl = d.ivl;
+    A.nint = d.nint;
+    A.gbar = d.gbar;
+    A.prefac = d.prefac;
+    A.ybar = d.dbar;
+    A.cbar = d.cbar;
+    A.mu_bar = d.mubar;
+    A.ica = group_ica(d);
+    dnu_adj_block(A, blockIdx.x - pre[h], lds);
+}
+
+__global__ __launch_bounds__(256) void k_m_adj_solve(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
+                                                     int nhist)
+{
+    const int h = hist_of(pre, nhist, blockIdx.x);
+    const GroupDesc &d = D[h];
+    adj_solve_block(d.sx, d.na, d.nk, (long long)d.na_max, d.cbar, d.dbar, blockIdx.x - pre[h]);
+}
+
 }  // namespace
 
 struct kn_group {
-    int nhist = 0, nstream = 4, cap_err = 0;
+    int nhist = 0, nstream = 0, cap_err = 0;  // nstream 0: merged launches
     long long nk_tot = 0, y_tot = 0, s_tot = 0, fs_tot = 0, b_tot = 0;
     std::vector<GroupHist> h;
     std::vector<HTab> htabs;
     std::vector<double *> hcoef;
     std::vector<int> order;
-    double *sx = nullptr, *Y = nullptr, *Yp = nullptr, *C = nullptr, *dbar = nullptr, *ybar = nullptr,
-           *cbar = nullptr, *fx = nullptr, *fy = nullptr, *fc = nullptr, *wavenum = nullptr,
-           *init = nullptr, *out = nullptr, *gbar = nullptr, *ibar = nullptr, *mbic = nullptr,
-           *mubar = nullptr, *mbar = nullptr, *ivl = nullptr;
+    double *sx = nullptr, *Y = nullptr, *C = nullptr, *dbar = nullptr, *cbar = nullptr, *fx = nullptr,
+           *fy = nullptr, *fc = nullptr, *wavenum = nullptr, *init = nullptr, *out = nullptr, *gbar = nullptr,
+           *ibar = nullptr, *mbic = nullptr, *mubar = nullptr, *mbar = nullptr, *ivl = nullptr;
     int *bucket = nullptr, *nint = nullptr, *status = nullptr, *zeros = nullptr, *spec0 = nullptr, *fs_list = nullptr;
     SpeciesDev *sp = nullptr;
     GroupMeta *meta = nullptr;
+    // merged launches
+    GroupDesc *desc = nullptr;
+    int *pre = nullptr;      // kPhases x (nhist + 1)
+    long long *mlist = nullptr;  // pass-2 fslength list (fs_tot entries)
+    std::vector<GroupDesc> desc_h;
+    int tot[kPhases] = {};
+    size_t adj_lds[kAdjClasses] = {};
     hipStream_t st[kGroupStreams] = {};
     hipEvent_t fork = nullptr, join[kGroupStreams] = {};
     hipGraph_t graph = nullptr;
@@ -1067,6 +1405,7 @@ void group_schedule(kn_group *g)
     g->order.resize(g->nhist);
     for (int i = 0; i < g->nhist; i++) g->order[i] = i;
     std::stable_sort(g->order.begin(), g->order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    if (g->nstream < 1) return;  // merged launches use no streams of their own
     std::vector<double> load(g->nstream, 0.0);
     for (int i : g->order) {
         int best = 0;
@@ -1096,6 +1435,82 @@ inline double group_sinv_bw(const GroupHist &q)
     return 4.0 * q.na / (q.xa - q.x0);
 }
 
+// Descriptors and per-phase block prefixes for the stored steps (after
+// create and append; launches read them from device memory, so a captured
+// graph needs no host data).
+void group_plan(kn_group *g)
+{
+    const int n = g->nhist;
+    g->desc_h.resize(n);
+    std::vector<int> pre((size_t)kPhases * (n + 1), 0);
+    std::vector<long long> acc(kPhases, 0);
+    size_t lds[kAdjClasses] = {};
+    for (int i = 0; i < n; i++) {
+        const GroupHist &q = g->h[i];
+        GroupDesc &d = g->desc_h[i];
+        d.H = g->htabs[q.htab];
+        d.sx = g->sx + q.off_s;
+        d.Y = g->Y + q.off_y;
+        d.wavenum = g->wavenum + q.off_k;
+        d.init = g->init + q.off_k;
+        d.gbar = g->gbar + q.off_k;
+        d.C = g->C + q.off_y;
+        d.fx = g->fx + q.off_fs;
+        d.fy = g->fy + q.off_fs;
+        d.fc = g->fc + q.off_fs;
+        d.out = g->out + q.off_k;
+        d.ivl = g->ivl + (size_t)q.off_k * kLimit * 2;
+        d.dbar = g->dbar + q.off_y;
+        d.cbar = g->cbar + q.off_y;
+        d.mubar = g->mubar + q.off_k;
+        d.ibar = g->ibar + q.off_k;
+        d.mbic = g->mbic + q.off_k;
+        d.bucket = g->bucket + q.off_b;
+        d.nint = g->nint + q.off_k;
+        d.status = g->status + 2 * i;
+        d.sp = g->sp + i;
+        d.nk = q.spec.nk;
+        d.na = q.na;
+        d.na_max = q.spec.na_max;
+        d.nfs = 16 * q.na;
+        d.nsb = 4 * q.na;
+        d.bsp = rows_prep_spline_blocks(q.na, q.spec.nk);
+        d.x0 = q.x0;
+        d.xa = q.xa;
+        d.light = q.spec.light;
+        d.sinv_bw = group_sinv_bw(q);
+        d.fsl0 = q.fsl0;
+        d.lin = q.lin;
+        d.prefac = q.spec.delta_nu_prefac;
+        d.vcrit = q.vcrit;
+        long long cnt[kPhases] = {};
+        if (q.integ) {
+            cnt[kPhFs] = fs_first_blocks(d.nfs);
+            cnt[kPhRows] = rows_prep_blocks(d.bsp, d.nsb);
+            cnt[kPhFsSpline] = spline_blocks(d.nfs, 1);
+            cnt[kPhQag] = blocks_for(d.nk);
+            cnt[kPhSolve] = q.na > 2 ? spline_blocks(q.na, d.nk) : 0;
+            int c = 0;
+            while (c < kAdjClasses - 1 && q.na > (256 << c)) c++;
+            cnt[kPhAdj0 + c] = d.nk + (d.nk + 63) / 64;
+            const size_t b = sizeof(double) * 2 * (size_t)q.na;
+            lds[c] = b > lds[c] ? b : lds[c];
+        } else {
+            cnt[kPhIc] = (d.nk + 63) / 64;
+        }
+        for (int p = 0; p < kPhases; p++) {
+            acc[p] += cnt[p];
+            if (acc[p] > 0x7fffffffLL) kn_terminate(KN_ERR_RANGE, "kn_group: %lld blocks in one phase\n", acc[p]);
+            pre[(size_t)p * (n + 1) + i + 1] = (int)acc[p];
+        }
+    }
+    for (int p = 0; p < kPhases; p++) g->tot[p] = (int)acc[p];
+    for (int c = 0; c < kAdjClasses; c++) g->adj_lds[c] = lds[c];
+    KN_CHECK(hipMemcpy(g->desc, g->desc_h.data(), sizeof(GroupDesc) * n, hipMemcpyHostToDevice));
+    KN_CHECK(hipMemcpy(g->pre, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice));
+    group_meta_upload(g);
+}
+
 // Forward of one history on stream s, recording its subdivisions: the
 // preparations in one launch, the fslength spline, then QAG with the
 // initial-condition piece added per k (k_group_ic alone when IC-only).
@@ -1112,35 +1527,44 @@ void group_forward_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
                            g->sp + i, nk, ic, g->out + q.off_k);
         return;
     }
-    const HTab H = g->htabs[q.htab];
-    const int nfs = 16 * na, nsb = 4 * na;
-    const double sinv_bw = group_sinv_bw(q);
-    const double *Yk = g->Y + q.off_y;
-    if (na < na_max) {
-        KN_CHECK(hipMemcpy2DAsync(g->Yp + q.off_y, sizeof(double) * na, g->Y + q.off_y, sizeof(double) * na_max,
-                                  sizeof(double) * na, nk, hipMemcpyDeviceToDevice, s));
-        Yk = g->Yp + q.off_y;
-    }
+    const int nfs = 16 * na;
     double *fx = g->fx + q.off_fs, *fy = g->fy + q.off_fs, *fc = g->fc + q.off_fs;
-    const double *sx = g->sx + q.off_s;
     int *fs_list = g->fs_list + q.off_fs, *fs_count = g->status + 2 * g->nhist + i;
-    hipLaunchKernelGGL(k_fs_first, dim3(fs_first_blocks(nfs)), dim3(256), 0, s, H, nfs, q.x0, q.xa, q.spec.light, fx,
-                       fy, g->status + 2 * i, fs_list, fs_count);
-    hipLaunchKernelGGL(k_fs_rest, dim3(kFsRestBlocks), dim3(64 * kWavesPerBlock), 0, s, H, nfs, q.x0, q.xa,
-                       q.spec.light, fx, fy, g->status + 2 * i, fs_list, fs_count);
-    const int bsp = rows_prep_spline_blocks(na, nk);
-    hipLaunchKernelGGL(k_rows_prep, dim3(bsp + (nsb + 256) / 256), dim3(256), 0, s, sx, Yk, g->C + q.off_y, na, nk, bsp,
-                       nsb, sinv_bw, g->bucket + q.off_b);
+    const GroupDesc &d = g->desc_h[i];
+    hipLaunchKernelGGL(k_fs_first, dim3(fs_first_blocks(nfs)), dim3(256), 0, s, d.H, nfs, q.x0, q.xa, q.spec.light, fx,
+                       fy, d.status, fs_list, fs_count);
+    hipLaunchKernelGGL(k_fs_rest, dim3(kFsRestBlocks), dim3(64 * kWavesPerBlock), 0, s, d.H, nfs, q.x0, q.xa,
+                       q.spec.light, fx, fy, d.status, fs_list, fs_count);
+    hipLaunchKernelGGL(k_rows_prep, dim3(rows_prep_blocks(d.bsp, d.nsb)), dim3(256), 0, s, d.sx, d.Y, d.C, na, nk,
+                       (long long)na_max, d.bsp, d.nsb, d.sinv_bw, d.bucket);
     hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, s, fx, fy, fc, nfs, 1, 0LL, 0LL);
-    IcArgs ic;
-    ic.init = g->init + q.off_k;
-    ic.fsl0 = q.fsl0;
-    ic.lin = q.lin;
-    ic.prefac = q.spec.delta_nu_prefac;
-    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nk)), dim3(64 * kWavesPerBlock), 0, s, H, fx, fy, fc, nfs, sx, Yk,
-                       g->C + q.off_y, g->bucket + q.off_b, na, nsb, sinv_bw, g->wavenum + q.off_k, nk, g->sp + i,
-                       g->spec0, nk, q.x0, q.xa, g->out + q.off_k, g->nint + q.off_k, g->status + 2 * i,
-                       g->ivl + (size_t)q.off_k * kLimit * 2, ic);
+    QagArgs a;
+    a.H = d.H;
+    a.fx = fx;
+    a.fy = fy;
+    a.fc = fc;
+    a.nfs = nfs;
+    a.sx = d.sx;
+    a.Y = d.Y;
+    a.C = d.C;
+    a.sb = d.bucket;
+    a.ns = na;
+    a.nsb = d.nsb;
+    a.ystride = na_max;
+    a.sinv_bw = d.sinv_bw;
+    a.wavenum = d.wavenum;
+    a.x0 = q.x0;
+    a.xa = q.xa;
+    a.out = d.out;
+    a.nint_out = d.nint;
+    a.status = d.status;
+    a.ivl = d.ivl;
+    a.ic.init = d.init;
+    a.ic.fsl0 = q.fsl0;
+    a.ic.lin = q.lin;
+    a.ic.prefac = q.spec.delta_nu_prefac;
+    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nk)), dim3(64 * kWavesPerBlock), 0, s, a, g->sp + i, g->spec0, nk,
+                       nk);
 }
 
 // Adjoint of one history's last forward on stream s.
@@ -1161,32 +1585,91 @@ void group_adjoint_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
                            g->gbar + q.off_k, g->sp + i, nk, ica);
         return;
     }
-    const int nfs = 16 * na, nsb = 4 * na;
-    const double *Yk = na < na_max ? g->Yp + q.off_y : g->Y + q.off_y;
-    double *yb = na < na_max ? g->ybar + q.off_y : dbar;
-    const size_t lds = sizeof(double) * 2 * (size_t)na;
+    const GroupDesc &d = g->desc_h[i];
+    AdjArgs A;
+    A.H = d.H;
+    A.fx = d.fx;
+    A.fy = d.fy;
+    A.fc = d.fc;
+    A.nfs = d.nfs;
+    A.sx = d.sx;
+    A.Y = d.Y;
+    A.C = d.C;
+    A.sb = d.bucket;
+    A.ns = na;
+    A.nsb = d.nsb;
+    A.nk = nk;
+    A.ystride = na_max;
+    A.sinv_bw = d.sinv_bw;
+    A.wavenum = d.wavenum;
+    A.sp = d.sp;
+    A.vcrit = q.vcrit;
+    A.ivl = d.ivl;
+    A.nint = d.nint;
+    A.gbar = d.gbar;
+    A.prefac = q.spec.delta_nu_prefac;
+    A.ybar = dbar;  // rows written whole: [0, na) the item's terms, [na, na_max) zeros
+    A.cbar = d.cbar;
+    A.mu_bar = d.mubar;
+    A.ica = ica;
     // nk item blocks, then the initial-condition piece's blocks
-    hipLaunchKernelGGL(k_dnu_adj, dim3(nk + (nk + 63) / 64), dim3(64), lds, s, g->htabs[q.htab], g->fx + q.off_fs,
-                       g->fy + q.off_fs, g->fc + q.off_fs, nfs, g->sx + q.off_s, Yk, g->C + q.off_y,
-                       g->bucket + q.off_b, na, nsb, group_sinv_bw(q), g->wavenum + q.off_k, nk, g->sp + i, q.vcrit,
-                       g->ivl + (size_t)q.off_k * kLimit * 2, g->nint + q.off_k, g->gbar + q.off_k,
-                       q.spec.delta_nu_prefac, yb, g->cbar + q.off_y, g->mubar + q.off_k, ica);
+    hipLaunchKernelGGL(k_dnu_adj, dim3(nk + (nk + 63) / 64), dim3(64), sizeof(double) * 2 * (size_t)na, s, A);
     if (na > 2)
-        hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(na, nk)), dim3(256), 0, s, g->sx + q.off_s, na, nk,
-                           g->cbar + q.off_y, yb);
-    if (na < na_max) {
-        hipLaunchKernelGGL(k_group_zero, dim3((int)((nblk + 255) / 256)), dim3(256), 0, s, dbar, nblk);
-        KN_CHECK(hipMemcpy2DAsync(dbar, sizeof(double) * na_max, yb, sizeof(double) * na, sizeof(double) * na, nk,
-                                  hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(na, nk)), dim3(256), 0, s, d.sx, na, nk, (long long)na_max,
+                           d.cbar, dbar);
+}
+
+// The whole group in one grid per phase, on stream s.
+void group_launch_merged(kn_group *g, hipStream_t s, bool forward, bool adjoint)
+{
+    const int n = g->nhist;
+    auto pre = [&](int ph) { return g->pre + (size_t)ph * (n + 1); };
+    if (forward) {
+        if (g->tot[kPhFs]) {
+            hipLaunchKernelGGL(k_m_fs_first, dim3(g->tot[kPhFs]), dim3(256), 0, s, g->desc, pre(kPhFs), n, g->mlist,
+                               g->status + 3 * n);
+            hipLaunchKernelGGL(k_m_fs_rest, dim3(kMergedRestBlocks), dim3(64 * kWavesPerBlock), 0, s, g->desc,
+                               g->mlist, g->status + 3 * n);
+        }
+        if (g->tot[kPhRows])
+            hipLaunchKernelGGL(k_m_rows_prep, dim3(g->tot[kPhRows]), dim3(256), 0, s, g->desc, pre(kPhRows), n);
+        if (g->tot[kPhFsSpline])
+            hipLaunchKernelGGL(k_m_fs_spline, dim3(g->tot[kPhFsSpline]), dim3(256), 0, s, g->desc, pre(kPhFsSpline), n);
+        if (g->tot[kPhQag])
+            hipLaunchKernelGGL(k_m_dnu_qag, dim3(g->tot[kPhQag]), dim3(64 * kWavesPerBlock), 0, s, g->desc,
+                               pre(kPhQag), n);
+        if (g->tot[kPhIc]) hipLaunchKernelGGL(k_m_ic, dim3(g->tot[kPhIc]), dim3(64), 0, s, g->desc, pre(kPhIc), n);
+    }
+    if (adjoint) {
+        for (int c = 0; c < kAdjClasses; c++)
+            if (g->tot[kPhAdj0 + c])
+                hipLaunchKernelGGL(k_m_dnu_adj, dim3(g->tot[kPhAdj0 + c]), dim3(64), g->adj_lds[c], s, g->desc,
+                                   pre(kPhAdj0 + c), n);
+        if (g->tot[kPhSolve])
+            hipLaunchKernelGGL(k_m_adj_solve, dim3(g->tot[kPhSolve]), dim3(256), 0, s, g->desc, pre(kPhSolve), n);
+        if (g->tot[kPhIc])
+            hipLaunchKernelGGL(k_m_ic_adj, dim3(g->tot[kPhIc]), dim3(64), 0, s, g->desc, pre(kPhIc), n);
     }
 }
 
-// Fork the caller's stream into the group's streams, issue the histories,
-// join back.  Everything is asynchronous on origin (and capturable).
+// Forward and/or adjoint of every history, asynchronous on origin (and
+// capturable): merged launches (nstream 0), or each history's launch sequence
+// on one of nstream streams forked from origin and joined back.
 void group_launch(kn_group *g, hipStream_t origin, bool forward, bool adjoint)
 {
     if (forward)
-        KN_CHECK(hipMemcpyAsync(g->status, g->zeros, sizeof(int) * 3 * g->nhist, hipMemcpyDeviceToDevice, origin));
+        KN_CHECK(hipMemcpyAsync(g->status, g->zeros, sizeof(int) * (3 * (size_t)g->nhist + 1), hipMemcpyDeviceToDevice,
+                                origin));
+    if (g->nstream == 0) {
+        group_launch_merged(g, origin, forward, adjoint);
+        KN_CHECK(hipGetLastError());
+        if (adjoint) {
+            hipLaunchKernelGGL(k_group_mass_bar, dim3(g->nhist), dim3(64), 0, origin, g->meta, g->mbic, g->mubar,
+                               g->mbar);
+            KN_CHECK(hipGetLastError());
+        }
+        return;
+    }
     KN_CHECK(hipEventRecord(g->fork, origin));
     for (int s = 0; s < g->nstream; s++) KN_CHECK(hipStreamWaitEvent(g->st[s], g->fork, 0));
     for (int i : g->order) {
@@ -1244,10 +1727,10 @@ extern "C" void kn_group_destroy(kn_group *g)
         KN_CHECK(hipGraphExecDestroy(g->exec));
         KN_CHECK(hipGraphDestroy(g->graph));
     }
-    void *ptrs[] = {g->sx,    g->Y,    g->Yp,     g->C,   g->dbar,  g->ybar,  g->cbar,   g->fx,
-                    g->fy,    g->fc,   g->wavenum, g->init, g->out,  g->gbar,   g->ibar, g->mbic,
-                    g->mubar, g->mbar, g->ivl,    g->bucket, g->nint, g->status, g->zeros, g->spec0, g->sp, g->fs_list,
-                    g->meta};
+    void *ptrs[] = {g->sx,   g->Y,    g->C,      g->dbar, g->cbar,   g->fx,     g->fy,    g->fc,
+                    g->wavenum, g->init, g->out,  g->gbar,   g->ibar, g->mbic,   g->mubar,  g->mbar,  g->ivl,
+                    g->bucket, g->nint, g->status, g->zeros, g->spec0, g->sp,    g->fs_list, g->meta, g->desc,
+                    g->pre,  g->mlist};
     for (void *p : ptrs)
         if (p) KN_CHECK(hipFree(p));
     for (double *p : g->hcoef) KN_CHECK(hipFree(p));
@@ -1333,7 +1816,7 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
     }
     const size_t ny = (size_t)g->y_tot, nk = (size_t)g->nk_tot;
     g->sx = group_alloc<double>(g->s_tot);
-    for (double **p : {&g->Y, &g->Yp, &g->C, &g->dbar, &g->ybar, &g->cbar}) *p = group_alloc<double>(ny);
+    for (double **p : {&g->Y, &g->C, &g->dbar, &g->cbar}) *p = group_alloc<double>(ny);
     for (double **p : {&g->fx, &g->fy, &g->fc}) *p = group_alloc<double>(g->fs_tot);
     for (double **p : {&g->wavenum, &g->init, &g->out, &g->gbar, &g->ibar, &g->mbic, &g->mubar})
         *p = group_alloc<double>(nk);
@@ -1341,12 +1824,16 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
     g->ivl = group_alloc<double>(nk * kLimit * 2);
     g->bucket = group_alloc<int>(g->b_tot);
     g->nint = group_alloc<int>(nk);
-    g->status = group_alloc<int>(3 * (size_t)nhist);  // [2 h] QAG, [2 h + 1] domain, [2 nhist + h] fs list count
-    g->zeros = group_alloc<int>(3 * (size_t)nhist);
+    // status: [2 h] QAG, [2 h + 1] domain, [2 nhist + h] fs list count, [3 nhist] merged fs list count
+    g->status = group_alloc<int>(3 * (size_t)nhist + 1);
+    g->zeros = group_alloc<int>(3 * (size_t)nhist + 1);
     g->fs_list = group_alloc<int>(g->fs_tot);
+    g->mlist = group_alloc<long long>(g->fs_tot);
     g->spec0 = group_alloc<int>(1);
-    KN_CHECK(hipMemset(g->zeros, 0, sizeof(int) * 3 * nhist));
-    KN_CHECK(hipMemset(g->status, 0, sizeof(int) * 3 * nhist));
+    g->desc = group_alloc<GroupDesc>(nhist);
+    g->pre = group_alloc<int>((size_t)kPhases * (nhist + 1));
+    KN_CHECK(hipMemset(g->zeros, 0, sizeof(int) * (3 * (size_t)nhist + 1)));
+    KN_CHECK(hipMemset(g->status, 0, sizeof(int) * (3 * (size_t)nhist + 1)));
     KN_CHECK(hipMemset(g->spec0, 0, sizeof(int)));
     g->sp = group_alloc<SpeciesDev>(nhist);
     {
@@ -1355,10 +1842,11 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
         KN_CHECK(hipMemcpy(g->sp, sd.data(), sizeof(SpeciesDev) * nhist, hipMemcpyHostToDevice));
     }
     g->meta = group_alloc<GroupMeta>(nhist);
-    group_meta_upload(g);
+    group_plan(g);
     if (sizeof(double) * 2 * (size_t)na_big > 64 * 1024)
-        KN_CHECK(hipFuncSetAttribute((const void *)k_dnu_adj, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)(sizeof(double) * 2 * (size_t)na_big)));
+        for (const void *k : {(const void *)k_dnu_adj, (const void *)k_m_dnu_adj})
+            KN_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)(sizeof(double) * 2 * (size_t)na_big)));
     for (int s = 0; s < kGroupStreams; s++) {
         KN_CHECK(hipStreamCreateWithFlags(&g->st[s], hipStreamNonBlocking));
         KN_CHECK(hipEventCreateWithFlags(&g->join[s], hipEventDisableTiming));
@@ -1447,7 +1935,7 @@ extern "C" int kn_group_append(kn_group *g, const double *d_loga, const double *
         const int rc = group_refresh_ic(q);
         if (rc) return rc;
     }
-    group_meta_upload(g);
+    group_plan(g);
     group_schedule(g);
     g->gen++;
     KN_CHECK(hipStreamSynchronize(st));
@@ -1506,11 +1994,12 @@ extern "C" int kn_group_set_mass(kn_group *g, int h, double mnu)
     return 0;
 }
 
-// Pick the stream count by timing the forward (1, 2, 4, 8 streams).
+// Pick the launch strategy by timing the forward: merged launches (0) or
+// per-history launch sequences on 1, 2, 4 or 8 streams.
 extern "C" int kn_group_autotune(kn_group *g, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
-    const int cand[] = {1, 2, 4, 8};
+    const int cand[] = {0, 1, 2, 4, 8};
     float best = 1e30f;
     int bs = g->nstream;
     hipEvent_t e0, e1;
@@ -1540,3 +2029,12 @@ extern "C" int kn_group_autotune(kn_group *g, void *stream)
 }
 
 extern "C" int kn_group_streams(const kn_group *g) { return g->nstream; }
+
+extern "C" int kn_group_set_streams(kn_group *g, int nstream)
+{
+    if (!g || nstream < 0 || nstream > kGroupStreams) return KN_ERR_ARG;
+    g->nstream = nstream;
+    if (nstream) group_schedule(g);
+    g->gen++;  // a captured graph is stale
+    return 0;
+}

@@ -143,6 +143,7 @@ SIGNATURES = {
     "kn_group_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kn_group_set_mass": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "kn_group_streams": (C.c_int, [C.c_void_p]),
+    "kn_group_set_streams": (C.c_int, [C.c_void_p, C.c_int]),
     "kn_get_delta_nu_adjoint": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, dp, C.c_double, dp, dp, dp, dp,
                                           dp]),
     "kn_batch_get_delta_nu_adjoint": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, dp, C.c_double, dp, dp, dp,
@@ -305,6 +306,13 @@ class Batch:
 
     def set_mass(self, i, mnu):
         _check(self._f("set_mass")(self.h, i, mnu), self.api + "_set_mass", self.L)
+
+    def set_streams(self, n):
+        """kn_group only: 0 merged launches, n > 0 per-history sequences on n streams."""
+        _check(self._f("set_streams")(self.h, n), self.api + "_set_streams", self.L)
+
+    def streams(self):
+        return self._f("streams")(self.h)
 
     def close(self):
         if self.h:

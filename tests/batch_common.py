@@ -89,7 +89,7 @@ def compare_adjoint(name, gpu, orc):
     assert e_f < RTOL and e_y < RTOL and e_i < RTOL and e_m < RTOL
 
 
-def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None, api="kn_batch"):
+def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None, api="kn_batch", streams=None):
     """Histories over three cosmologies with mixed lengths and k counts (a kn_batch or a kn_group)."""
     rng = np.random.default_rng(seed)
     cosmos = [Cosmo([0.15, 0, 0]), Cosmo([0.1, 0.1, 0.1]), Cosmo([0.3, 0.06, 0.06], hybrid=(500.0, 0.3333))]
@@ -105,6 +105,8 @@ def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None, api="kn_b
     hists = [cosmos[s["c"]].history(s["nk"], s["na"], s["na"] + extra, s["amax"], 1.0 if extra else s["amax"],
                                     s["m"]) for s in specs]
     B = knhip.Batch(hists, keep=cosmos, L=L, memcpy=memcpy, api=api)
+    if streams is not None:
+        B.set_streams(streams)
     for i, s in enumerate(specs):
         B.load(i, s["logs"], s["rows"], s["k"], s["init"])
     return cosmos, specs, B

@@ -43,10 +43,13 @@ def _forward_errors(B, cosmos, specs):
     return worst
 
 
-def test_emu_group_mixed_forward_and_adjoint(emu):
+@pytest.mark.parametrize("streams", [0, 4], ids=["merged", "streams4"])
+def test_emu_group_mixed_forward_and_adjoint(emu, streams):
     """Twelve histories over three cosmologies (one hybrid), Na from 2 to 150,
-    some with spare capacity (packed copies), forward and adjoint."""
-    cosmos, specs, B = mixed_batch(4, seed=2, extra=3, L=emu, memcpy=host_memcpy, api="kn_group")
+    some with spare capacity (rows read and written at stride na_max),
+    forward and adjoint, in merged launches and on four streams."""
+    cosmos, specs, B = mixed_batch(4, seed=2, extra=3, L=emu, memcpy=host_memcpy, api="kn_group", streams=streams)
+    assert B.streams() == streams
     B.solve()
     worst = _forward_errors(B, cosmos, specs)
     print("  %d histories: worst forward rel err %.3e" % (B.n, worst))
@@ -172,3 +175,74 @@ def test_emu_fslength_two_passes_equal_one_qag_per_node(emu):
         assert np.array_equal(a, b)
         assert np.all(a[:-1] > 0) and a[-1] == 0.0
         assert (listed > 0) == want_listed
+
+
+def _group_arrays(B):
+    n_y = B.off_y[B.n]
+    return (B.delta_nu(), B._get("dtot_bar", 0, n_y), B._get("init_bar", 0, B.off_k[B.n]), B.mnu_bar())
+
+
+def test_emu_group_merged_equals_streams_bit_for_bit(emu):
+    """The merged launches run each history's blocks with the per-history
+    launches' device code and inputs: identical results, forward and adjoint."""
+    out = []
+    for streams in (0, 2):
+        cosmos, specs, B = mixed_batch(3, seed=9, extra=2, L=emu, memcpy=host_memcpy, api="kn_group", streams=streams)
+        B.solve()
+        B.adjoint(np.random.default_rng(3).uniform(-1, 1, B.off_k[B.n]))
+        out.append(_group_arrays(B))
+        B.close()
+        for c in cosmos:
+            c.free()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_emu_group_merged_with_ic_only_histories(emu):
+    """Massless and one-step histories (initial-condition piece only) between
+    integrated ones, in merged launches: forward against the oracle, adjoint
+    rows of dL/d delta_tot zero for them, init_bar and mnu_bar as the oracle."""
+    c = Cosmo([0.15, 0, 0])
+    cases = [(20, 8, 0.2, 0.15), (20, 8, 0.2, 0.0), (1, 6, 0.011, 0.15), (30, 12, 0.3, 0.15), (12, 70, 0.15, 0.0)]
+    hists, data = [], []
+    for i, (na, nk, amax, m) in enumerate(cases):
+        logs, rows, k, init = synthetic_history(na, nk, amax, seed=20 + i, noise=0.02) if na > 1 else \
+            (np.array([math.log(0.01)]), np.ones((nk, 1)), np.geomspace(1e-3, 1.0, nk), np.ones(nk))
+        hists.append(c.history(nk, na, na + 3, amax, 1.0, m))
+        data.append((na, nk, amax, m, logs, rows, k, init))
+    B = knhip.Batch(hists, keep=(c,), L=emu, memcpy=host_memcpy, api="kn_group")
+    assert B.streams() == 0
+    for i, (na, nk, amax, m, logs, rows, k, init) in enumerate(data):
+        B.load(i, logs, rows, k, init)
+    B.solve()
+    g = np.random.default_rng(11).uniform(-1, 1, B.off_k[B.n])
+    B.adjoint(g)
+    mb = B.mnu_bar()
+    for i, (na, nk, amax, m, logs, rows, k, init) in enumerate(data):
+        T = c.oracle(nk, logs, rows, init)
+        rc, o = T.get_delta_nu(amax, k, m)
+        assert rc == 0
+        assert rel(B.delta_nu(i), o) < RTOL
+        gi = g[B.off_k[i]:B.off_k[i] + nk]
+        hs = B.hists[i]
+        full = B._get("dtot_bar", B.off_y[i], hs.nk * hs.na_max).reshape(hs.nk, hs.na_max)
+        assert np.all(full[:, na:] == 0)
+        if m == 0.0 or na < 2:
+            assert np.all(full == 0)
+        orc = T.get_delta_nu_adjoint(amax, k, m, gi)
+        compare_adjoint("hist %d (Na=%d nk=%d m=%g)" % (i, na, nk, m),
+                        (B.delta_nu(i), B.dtot_bar(i, na), B.init_bar(i), mb[i]), orc)
+    B.close()
+    c.free()
+
+
+def test_emu_group_set_streams_checks(emu):
+    c = Cosmo([0.15, 0, 0])
+    hh = c.history(4, 10, 10, 0.25, 0.25, 0.15)
+    g = C.c_void_p()
+    assert emu.kn_group_create(C.byref(g), 1, C.byref(hh)) == 0
+    assert emu.kn_group_streams(g) == 0
+    assert emu.kn_group_set_streams(g, 9) == 3002 and emu.kn_group_set_streams(g, -1) == 3002
+    assert emu.kn_group_set_streams(g, 8) == 0 and emu.kn_group_streams(g) == 8
+    emu.kn_group_destroy(g)
+    c.free()

@@ -1,7 +1,9 @@
 """(b) grouped get_delta_nu on the GPU (kn_group_*): many histories, each with
-its own H(a), mass, stored steps and k bins, solved with the per-call kernels
-on several HIP streams; forward and adjoint against the oracle, appended
-steps, new masses, graph replays and the stream-count autotuner.
+its own H(a), mass, stored steps and k bins, solved with the per-call kernels'
+device code in merged launches (one grid per phase for all histories) and on
+several HIP streams; forward and adjoint against the oracle, merged against
+streams bit for bit, appended steps, new masses, graph replays and the
+autotuner.
 
 Bars: delta_nu and its gradients within 1e-9 relative of the oracle (the
 d/d delta_tot rows relative to their largest entry); graph replays and
@@ -35,12 +37,17 @@ def _free(cosmos):
         c.free()
 
 
+def _group_arrays(B):
+    return (B.delta_nu(), B._get("dtot_bar", 0, B.off_y[B.n]), B._get("init_bar", 0, B.off_k[B.n]), B.mnu_bar())
+
+
+@pytest.mark.parametrize("streams", [0, 4], ids=["merged", "streams4"])
 @pytest.mark.parametrize("extra", [0, 3])
-def test_group_mixed_forward_and_adjoint(kn, torch_cuda, extra):
+def test_group_mixed_forward_and_adjoint(kn, torch_cuda, extra, streams):
     """48 histories over three cosmologies (one hybrid), Na 2..150, nk 4..24;
-    extra > 0 leaves spare capacity, so the kernels read packed copies."""
-    print("group mixed, extra capacity %d" % extra, flush=True)
-    cosmos, specs, B = mixed_batch(16, seed=2, extra=extra, api="kn_group")
+    extra > 0 leaves spare capacity (rows read and written at stride na_max)."""
+    print("group mixed, extra capacity %d, streams %d" % (extra, streams), flush=True)
+    cosmos, specs, B = mixed_batch(16, seed=2, extra=extra, api="kn_group", streams=streams)
     B.solve()
     worst = _forward_errors(B, cosmos, specs)
     print("  %d histories: worst forward rel err %.3e" % (B.n, worst))
@@ -149,12 +156,12 @@ def test_group_graph_replays_and_autotune(kn, torch_cuda):
     print("group autotune", flush=True)
     assert kn.kn_group_autotune(B.h, None) == 0
     ns = kn.kn_group_streams(B.h)
-    assert ns in (1, 2, 4, 8)
+    assert ns in (0, 1, 2, 4, 8)
     before = B.delta_nu()
     B.solve()
     assert np.array_equal(before, B.delta_nu())
-    print("  autotuned streams: %d" % ns)
-    # the graph is stale after autotune (new stream assignment)
+    print("  autotuned streams: %d (0: merged launches)" % ns)
+    # the graph is stale after autotune (new launch plan)
     assert kn.kn_group_graph_launch(B.h, C.c_void_p(s.cuda_stream)) == 3002
     B.close()
     _free(cosmos)
@@ -205,6 +212,13 @@ def test_group_at_batching_workload_shape(kn, torch_cuda):
     print("  sampled k: forward %.3e, d/d delta_tot and d/d init %.3e" % (worst_f, worst_b))
     assert worst_f < RTOL and worst_b < RTOL
     assert np.all(np.isfinite(mb))
+    merged = _group_arrays(B)
+    B.set_streams(4)
+    print("  the same on 4 streams", flush=True)
+    B.solve()
+    B.adjoint(g)
+    for a, b in zip(merged, _group_arrays(B)):
+        assert np.array_equal(a, b)
     B.close()
     _free(cosmos)
 
@@ -238,3 +252,54 @@ def test_group_with_a_stepped_hubble_matches_the_emulation(kn, torch_cuda):
     e = rel(out[0], out[1])
     print("  device vs emulation: %.2e" % e)
     assert e < 1e-12
+
+
+def test_group_merged_with_ic_only_histories(kn, torch_cuda):
+    """Massless and one-step histories (initial-condition piece only) between
+    integrated ones in merged launches: forward and adjoint against the
+    oracle, their dL/d delta_tot rows zero."""
+    c = Cosmo([0.15, 0, 0])
+    cases = [(20, 8, 0.2, 0.15), (20, 8, 0.2, 0.0), (1, 6, 0.011, 0.15), (30, 12, 0.3, 0.15), (12, 70, 0.15, 0.0)]
+    hists, data = [], []
+    for i, (na, nk, amax, m) in enumerate(cases):
+        logs, rows, k, init = synthetic_history(na, nk, amax, seed=20 + i, noise=0.02) if na > 1 else \
+            (np.array([math.log(0.01)]), np.ones((nk, 1)), np.geomspace(1e-3, 1.0, nk), np.ones(nk))
+        hists.append(c.history(nk, na, na + 3, amax, 1.0, m))
+        data.append((na, nk, amax, m, logs, rows, k, init))
+    B = knhip.Batch(hists, keep=(c,), api="kn_group")
+    assert B.streams() == 0
+    for i, (na, nk, amax, m, logs, rows, k, init) in enumerate(data):
+        B.load(i, logs, rows, k, init)
+    print("group merged with IC-only histories", flush=True)
+    B.solve()
+    g = np.random.default_rng(11).uniform(-1, 1, B.off_k[B.n])
+    B.adjoint(g)
+    mb = B.mnu_bar()
+    for i, (na, nk, amax, m, logs, rows, k, init) in enumerate(data):
+        T = c.oracle(nk, logs, rows, init)
+        hs = B.hists[i]
+        full = B._get("dtot_bar", B.off_y[i], hs.nk * hs.na_max).reshape(hs.nk, hs.na_max)
+        assert np.all(full[:, na:] == 0)
+        if m == 0.0 or na < 2:
+            assert np.all(full == 0)
+        orc = T.get_delta_nu_adjoint(amax, k, m, g[B.off_k[i]:B.off_k[i] + nk])
+        compare_adjoint("hist %d (Na=%d nk=%d m=%g)" % (i, na, nk, m),
+                        (B.delta_nu(i), B.dtot_bar(i, na), B.init_bar(i), mb[i]), orc)
+    B.close()
+    c.free()
+
+
+def test_group_merged_equals_streams_bit_for_bit(kn, torch_cuda):
+    """Merged launches and per-history launches on two streams run the same
+    device code on the same inputs: identical forward and adjoint results."""
+    out = []
+    for streams in (0, 2):
+        print("group mixed, streams %d" % streams, flush=True)
+        cosmos, specs, B = mixed_batch(6, seed=9, extra=2, api="kn_group", streams=streams)
+        B.solve()
+        B.adjoint(np.random.default_rng(3).uniform(-1, 1, B.off_k[B.n]))
+        out.append(_group_arrays(B))
+        B.close()
+        _free(cosmos)
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
