@@ -291,8 +291,7 @@ struct BatchIntegrand {
         const int si = settle(sx, ns, sbucket[q], x);
         const double dtot = linear ? linear_at(sx, sy, si, x) : cspline_at(sx, sy, sc, si, x);
         const double specJ = specialJ(k * fsl / mnubykT, hd->J);
-        const double ai = exp(x);
-        return fsl / (ai * exp(ln_hubble(hd->H, x))) * specJ * dtot;
+        return fsl * exp(-x - ln_hubble(hd->H, x)) * specJ * dtot;  // fsl / (a H) specJ dtot
     }
 };
 
@@ -417,7 +416,7 @@ __global__ __launch_bounds__(64 * kBW, 2) void k_b_adj_nodes(BatchDev B, const H
                         const double kx = f.k * fsl / f.mnubykT;
                         double Jx, Jq;
                         const double J = specialJ_d(kx, d.J, Jx, Jq);
-                        const double base = fsl / (exp(x) * exp(ln_hubble(d.H, x)));
+                        const double base = fsl * exp(-x - ln_hubble(d.H, x));  // fsl / (a H)
                         const double S = f.linear ? linear_at(f.sx, f.sy, si, x) : cspline_at(f.sx, f.sy, f.sc, si, x);
                         // d/dmu of J(k fsl/mu, qc = vcrit mu)
                         mu_acc += w * base * S * (Jx * (-kx / f.mnubykT) + Jq * d.vcrit);

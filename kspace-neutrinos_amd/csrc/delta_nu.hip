@@ -234,6 +234,37 @@ inline int spline_blocks(int n, int nrows)
     return (int)((items + 3) / 4);
 }
 
+// Spline factors of the knots x[0..n) shared by all rows on them (fac,
+// kFacStride doubles per window; none when n < 3) and the reciprocal knot
+// spacings ih[0..n-1): block b's waves factor windows 4b .. 4b+3, its threads
+// the spacings 256b + t.
+__device__ inline void factor_block(const double *__restrict__ x, int n, double *__restrict__ fac,
+                                    double *__restrict__ ih, int b)
+{
+    const int w = b * 4 + (threadIdx.x >> 6);
+    if (n >= 3 && w < fac_windows(n)) tri_factor_store(x, n, w, fac);  // wave-uniform
+    const int i = b * 256 + threadIdx.x;
+    if (i < n - 1) {
+        const double h = x[i + 1] - x[i];
+        ih[i] = h != 0.0 ? 1.0 / h : 0.0;
+    }
+}
+
+inline int factor_blocks(int n)
+{
+    const int bw = (fac_windows(n) + 3) / 4, bi = (n - 1 + 255) / 256;
+    return bw > bi ? bw : bi;
+}
+
+__global__ __launch_bounds__(256) void k_factor(const double *__restrict__ x, int n, double *__restrict__ fac,
+                                                double *__restrict__ ih)
+{
+    factor_block(x, n, fac, ih, blockIdx.x);
+}
+
+// doubles of factor storage for n knots: windows, then the spacings
+inline size_t fac_doubles(int n) { return (size_t)fac_windows(n) * kFacStride + (size_t)n; }
+
 // bucket[q] = largest i <= n-2 with x[i] <= x[0] + q / inv_bw
 __device__ inline void bucket_at(const double *__restrict__ x, int n, double inv_bw, int *__restrict__ bucket, int q)
 {
@@ -247,16 +278,26 @@ __device__ inline void bucket_at(const double *__restrict__ x, int n, double inv
     bucket[q] = lo;
 }
 
-// The delta_tot rows' splines and the bucket index in one launch: blocks
-// [0, b_sp) four row windows each (none when na <= 2), the rest the bucket.
-// Rows of Y and C are `ystride` doubles apart.
+// The delta_tot rows' splines (on the stored factors of the knots sx, see
+// factor_block) and the bucket index in one launch: blocks [0, b_sp) four
+// row windows each (none when na <= 2), the rest the bucket.  Rows of Y and
+// C are `ystride` doubles apart.
 __device__ inline void rows_prep_block(const double *__restrict__ sx, const double *__restrict__ Y,
                                        double *__restrict__ C, int na, int nk, long long ystride, int b_sp, int nsb,
-                                       double sinv_bw, int *__restrict__ bucket, int b)
+                                       double sinv_bw, int *__restrict__ bucket, const double *__restrict__ fac,
+                                       int b)
 {
     if (b < b_sp) {
+        const int nwin = fac_windows(na);
         const long long item = (long long)b * 4 + (threadIdx.x >> 6);
-        if (item < (long long)((na + kKeep - 1) / kKeep) * nk) spline_item_wave(sx, Y, C, na, ystride, ystride, item);
+        if (item < (long long)nwin * nk) {  // wave-uniform
+            const int r = (int)(item / nwin), w = (int)(item % nwin);
+            const int lo = w * kKeep, hi = lo + kKeep < na ? lo + kKeep : na;
+            SplineRhsIh rhs;
+            rhs.ih = fac + (size_t)nwin * kFacStride;
+            rhs.y = Y + r * ystride;
+            tridiag_window_wave_fac(sx, na, rhs, lo, hi, C + r * ystride, fac);
+        }
         return;
     }
     const int q = (b - b_sp) * 256 + threadIdx.x;
@@ -265,9 +306,10 @@ __device__ inline void rows_prep_block(const double *__restrict__ sx, const doub
 
 __global__ __launch_bounds__(256) void k_rows_prep(const double *__restrict__ sx, const double *__restrict__ Y,
                                                    double *__restrict__ C, int na, int nk, long long ystride,
-                                                   int b_sp, int nsb, double sinv_bw, int *__restrict__ bucket)
+                                                   int b_sp, int nsb, double sinv_bw, int *__restrict__ bucket,
+                                                   const double *__restrict__ fac)
 {
-    rows_prep_block(sx, Y, C, na, nk, ystride, b_sp, nsb, sinv_bw, bucket, blockIdx.x);
+    rows_prep_block(sx, Y, C, na, nk, ystride, b_sp, nsb, sinv_bw, bucket, fac, blockIdx.x);
 }
 
 inline int rows_prep_blocks(int b_sp, int nsb) { return b_sp + (nsb + 256) / 256; }
@@ -460,7 +502,7 @@ __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
                 const double kx = kk * fsl / mnubykT;
                 double Jx, Jq;
                 const double Jv = specialJ_d(kx, J, Jx, Jq);
-                const double base = fsl / (exp(x) * exp(ln_hubble(H, x)));
+                const double base = fsl * exp(-x - ln_hubble(H, x));  // fsl / (a H), as the forward
                 const double S = linear ? linear_at(sx, syk, si, x) : cspline_at(sx, syk, sck, si, x);
                 mu_acc += w * base * S * (Jx * (-kx / mnubykT) + Jq * vcrit);  // d/d(m/kT), qc = vcrit m/kT
                 const double wj = w * base * Jv;
@@ -501,9 +543,11 @@ __global__ __launch_bounds__(64) void k_dnu_adj(AdjArgs A)
 // symmetric, so its transpose is itself) by wavefronts as k_spline, and z
 // added through the right-hand side 3((y_{i+1}-y_i)/h_i - (y_i-y_{i-1})/h_{i-1})
 // onto ybar in the same pass.
-// Rows `stride` doubles apart; block `b` of spline_blocks(n, nrows).
+// Rows `stride` doubles apart, on the stored factors of x; block `b` of
+// spline_blocks(n, nrows).
 __device__ inline void adj_solve_block(const double *__restrict__ x, int n, int nrows, long long stride,
-                                       const double *__restrict__ cbar, double *__restrict__ ybar, int b)
+                                       const double *__restrict__ cbar, double *__restrict__ ybar,
+                                       const double *__restrict__ fac, int b)
 {
     const int nwin = (n + kKeep - 1) / kKeep;
     const long long item = (long long)b * 4 + (threadIdx.x >> 6);
@@ -512,13 +556,14 @@ __device__ inline void adj_solve_block(const double *__restrict__ x, int n, int 
     const int lo = w * kKeep, hi = lo + kKeep < n ? lo + kKeep : n;
     ArrayRhs rhs;
     rhs.r = cbar + (size_t)r * stride;
-    tridiag_window_wave<ArrayRhs, true>(x, n, rhs, lo, hi, ybar + (size_t)r * stride);
+    tridiag_window_wave_fac<ArrayRhs, true>(x, n, rhs, lo, hi, ybar + (size_t)r * stride, fac);
 }
 
 __global__ __launch_bounds__(256) void k_adj_solve(const double *__restrict__ x, int n, int nrows, long long stride,
-                                                   const double *__restrict__ cbar, double *__restrict__ ybar)
+                                                   const double *__restrict__ cbar, double *__restrict__ ybar,
+                                                   const double *__restrict__ fac)
 {
-    adj_solve_block(x, n, nrows, stride, cbar, ybar, blockIdx.x);
+    adj_solve_block(x, n, nrows, stride, cbar, ybar, fac, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- state
@@ -535,6 +580,8 @@ struct DnuDev {
     int *status = nullptr;
     size_t cap_ivl = 0, cap_adj = 0, cap_adj_k = 0;
     double *ivl = nullptr;                                            // recorded subdivisions
+    size_t cap_fac = 0;
+    double *fac = nullptr;  // spline factors of scalefact (fac_doubles(Na))
     double *gbar = nullptr, *ybar = nullptr, *cbar = nullptr, *mubar = nullptr;  // adjoint
 };
 std::unordered_map<const void *, DnuDev *> g_dev;
@@ -741,7 +788,7 @@ extern "C" void kn_dnu_release(const _delta_tot_table *d)
     DnuDev *v = it->second;
     void *ptrs[] = {v->fs_list, v->fx,  v->fy,   v->fc,   v->sx,   v->Y,    v->C,    v->wavenum, v->bucket, v->out,
                     v->nint, v->item_species, v->sp, v->status, v->ivl, v->gbar, v->ybar, v->cbar,
-                    v->mubar};
+                    v->mubar, v->fac};
     for (void *p : ptrs)
         if (p) KN_CHECK(hipFree(p));
     delete v;
@@ -804,6 +851,7 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     }
     grow(dev->wavenum, dev->cap_k, (size_t)nk);
     grow(dev->bucket, dev->cap_b, (size_t)nsb + 1);
+    grow(dev->fac, dev->cap_fac, fac_doubles(Na));
     if (dev->cap_items < (size_t)nitems) {
         if (dev->out) KN_CHECK(hipFree(dev->out));
         if (dev->nint) KN_CHECK(hipFree(dev->nint));
@@ -875,8 +923,10 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     const double sinv_bw = nsb / (d->scalefact[Na - 1] - d->scalefact[0]);
     {
         const int bsp = rows_prep_spline_blocks(Na, nk);
+        hipLaunchKernelGGL(k_factor, dim3(factor_blocks(Na)), dim3(256), 0, 0, dev->sx, Na, dev->fac,
+                           dev->fac + (size_t)fac_windows(Na) * kFacStride);
         hipLaunchKernelGGL(k_rows_prep, dim3(rows_prep_blocks(bsp, nsb)), dim3(256), 0, 0, dev->sx, dev->Y, dev->C, Na,
-                           nk, (long long)Na, bsp, nsb, sinv_bw, dev->bucket);
+                           nk, (long long)Na, bsp, nsb, sinv_bw, dev->bucket, dev->fac);
     }
     QagArgs qa;
     qa.H = H;
@@ -1015,7 +1065,7 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
         KN_CHECK(hipGetLastError());
         if (Na > 2) {
             hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, Na, nk, (long long)Na,
-                               dev->cbar, dev->ybar);
+                               dev->cbar, dev->ybar, dev->fac);
             KN_CHECK(hipGetLastError());
         }
         // results through the pinned stage: [mu_bar (nk) | dL/d delta_tot (nk Na)]
@@ -1120,7 +1170,7 @@ struct GroupHist {
     kn_batch_history spec;
     int na, integ, htab, stream;
     double x0, xa, xmax, fsl0, lin, vcrit;
-    long long off_k, off_y, off_s, off_fs, off_b;
+    long long off_k, off_y, off_s, off_fs, off_b, off_fac;
 };
 
 // ---- merged launches: one grid per phase for every history of the group.
@@ -1134,7 +1184,7 @@ struct GroupHist {
 struct GroupDesc {
     HTab H;
     const double *sx, *Y, *wavenum, *init, *gbar;
-    double *C, *fx, *fy, *fc, *out, *ivl, *dbar, *cbar, *mubar, *ibar, *mbic;
+    double *C, *fx, *fy, *fc, *out, *ivl, *dbar, *cbar, *mubar, *ibar, *mbic, *fac;
     int *bucket, *nint, *status;
     const SpeciesDev *sp;
     int nk, na, na_max, nfs, nsb, bsp;
@@ -1143,7 +1193,7 @@ struct GroupDesc {
 
 // phases (rows of the prefix table); the adjoint's item blocks are split in
 // classes of na (LDS rows of 2 na doubles): class c holds na <= 256 << c
-enum { kPhFs, kPhRows, kPhFsSpline, kPhQag, kPhIc, kPhSolve, kPhAdj0 };
+enum { kPhFs, kPhFactor, kPhRows, kPhFsSpline, kPhQag, kPhIc, kPhSolve, kPhAdj0 };
 constexpr int kAdjClasses = 7;
 constexpr int kPhases = kPhAdj0 + kAdjClasses;
 constexpr int kMergedRestBlocks = 1280;
@@ -1191,12 +1241,20 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_m_fs_rest(const GroupDe
     }
 }
 
+__global__ __launch_bounds__(256) void k_m_factor(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
+                                                  int nhist)
+{
+    const int h = hist_of(pre, nhist, blockIdx.x);
+    const GroupDesc &d = D[h];
+    factor_block(d.sx, d.na, d.fac, d.fac + (size_t)fac_windows(d.na) * kFacStride, blockIdx.x - pre[h]);
+}
+
 __global__ __launch_bounds__(256) void k_m_rows_prep(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
                                                      int nhist)
 {
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
-    rows_prep_block(d.sx, d.Y, d.C, d.na, d.nk, (long long)d.na_max, d.bsp, d.nsb, d.sinv_bw, d.bucket,
+    rows_prep_block(d.sx, d.Y, d.C, d.na, d.nk, (long long)d.na_max, d.bsp, d.nsb, d.sinv_bw, d.bucket, d.fac,
                     blockIdx.x - pre[h]);
 }
 
@@ -1328,21 +1386,21 @@ __global__ __launch_bounds__(256) void k_m_adj_solve(const GroupDesc *__restrict
 {
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
-    adj_solve_block(d.sx, d.na, d.nk, (long long)d.na_max, d.cbar, d.dbar, blockIdx.x - pre[h]);
+    adj_solve_block(d.sx, d.na, d.nk, (long long)d.na_max, d.cbar, d.dbar, d.fac, blockIdx.x - pre[h]);
 }
 
 }  // namespace
 
 struct kn_group {
     int nhist = 0, nstream = 0, cap_err = 0;  // nstream 0: merged launches
-    long long nk_tot = 0, y_tot = 0, s_tot = 0, fs_tot = 0, b_tot = 0;
+    long long nk_tot = 0, y_tot = 0, s_tot = 0, fs_tot = 0, b_tot = 0, fac_tot = 0;
     std::vector<GroupHist> h;
     std::vector<HTab> htabs;
     std::vector<double *> hcoef;
     std::vector<int> order;
     double *sx = nullptr, *Y = nullptr, *C = nullptr, *dbar = nullptr, *cbar = nullptr, *fx = nullptr,
            *fy = nullptr, *fc = nullptr, *wavenum = nullptr, *init = nullptr, *out = nullptr, *gbar = nullptr,
-           *ibar = nullptr, *mbic = nullptr, *mubar = nullptr, *mbar = nullptr, *ivl = nullptr;
+           *ibar = nullptr, *mbic = nullptr, *mubar = nullptr, *mbar = nullptr, *ivl = nullptr, *fac = nullptr;
     int *bucket = nullptr, *nint = nullptr, *status = nullptr, *zeros = nullptr, *spec0 = nullptr, *fs_list = nullptr;
     SpeciesDev *sp = nullptr;
     GroupMeta *meta = nullptr;
@@ -1465,6 +1523,7 @@ void group_plan(kn_group *g)
         d.mubar = g->mubar + q.off_k;
         d.ibar = g->ibar + q.off_k;
         d.mbic = g->mbic + q.off_k;
+        d.fac = g->fac + q.off_fac;
         d.bucket = g->bucket + q.off_b;
         d.nint = g->nint + q.off_k;
         d.status = g->status + 2 * i;
@@ -1486,6 +1545,7 @@ void group_plan(kn_group *g)
         long long cnt[kPhases] = {};
         if (q.integ) {
             cnt[kPhFs] = fs_first_blocks(d.nfs);
+            cnt[kPhFactor] = factor_blocks(q.na);
             cnt[kPhRows] = rows_prep_blocks(d.bsp, d.nsb);
             cnt[kPhFsSpline] = spline_blocks(d.nfs, 1);
             cnt[kPhQag] = blocks_for(d.nk);
@@ -1535,8 +1595,10 @@ void group_forward_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
                        fy, d.status, fs_list, fs_count);
     hipLaunchKernelGGL(k_fs_rest, dim3(kFsRestBlocks), dim3(64 * kWavesPerBlock), 0, s, d.H, nfs, q.x0, q.xa,
                        q.spec.light, fx, fy, d.status, fs_list, fs_count);
+    hipLaunchKernelGGL(k_factor, dim3(factor_blocks(na)), dim3(256), 0, s, d.sx, na, d.fac,
+                       d.fac + (size_t)fac_windows(na) * kFacStride);
     hipLaunchKernelGGL(k_rows_prep, dim3(rows_prep_blocks(d.bsp, d.nsb)), dim3(256), 0, s, d.sx, d.Y, d.C, na, nk,
-                       (long long)na_max, d.bsp, d.nsb, d.sinv_bw, d.bucket);
+                       (long long)na_max, d.bsp, d.nsb, d.sinv_bw, d.bucket, d.fac);
     hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, s, fx, fy, fc, nfs, 1, 0LL, 0LL);
     QagArgs a;
     a.H = d.H;
@@ -1616,7 +1678,7 @@ void group_adjoint_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
     hipLaunchKernelGGL(k_dnu_adj, dim3(nk + (nk + 63) / 64), dim3(64), sizeof(double) * 2 * (size_t)na, s, A);
     if (na > 2)
         hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(na, nk)), dim3(256), 0, s, d.sx, na, nk, (long long)na_max,
-                           d.cbar, dbar);
+                           d.cbar, dbar, d.fac);
 }
 
 // The whole group in one grid per phase, on stream s.
@@ -1631,6 +1693,8 @@ void group_launch_merged(kn_group *g, hipStream_t s, bool forward, bool adjoint)
             hipLaunchKernelGGL(k_m_fs_rest, dim3(kMergedRestBlocks), dim3(64 * kWavesPerBlock), 0, s, g->desc,
                                g->mlist, g->status + 3 * n);
         }
+        if (g->tot[kPhFactor])
+            hipLaunchKernelGGL(k_m_factor, dim3(g->tot[kPhFactor]), dim3(256), 0, s, g->desc, pre(kPhFactor), n);
         if (g->tot[kPhRows])
             hipLaunchKernelGGL(k_m_rows_prep, dim3(g->tot[kPhRows]), dim3(256), 0, s, g->desc, pre(kPhRows), n);
         if (g->tot[kPhFsSpline])
@@ -1730,7 +1794,7 @@ extern "C" void kn_group_destroy(kn_group *g)
     void *ptrs[] = {g->sx,   g->Y,    g->C,      g->dbar, g->cbar,   g->fx,     g->fy,    g->fc,
                     g->wavenum, g->init, g->out,  g->gbar,   g->ibar, g->mbic,   g->mubar,  g->mbar,  g->ivl,
                     g->bucket, g->nint, g->status, g->zeros, g->spec0, g->sp,    g->fs_list, g->meta, g->desc,
-                    g->pre,  g->mlist};
+                    g->pre,  g->mlist, g->fac};
     for (void *p : ptrs)
         if (p) KN_CHECK(hipFree(p));
     for (double *p : g->hcoef) KN_CHECK(hipFree(p));
@@ -1776,6 +1840,8 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
         q.off_s = g->s_tot;
         q.off_fs = g->fs_tot;
         q.off_b = g->b_tot;
+        q.off_fac = g->fac_tot;
+        g->fac_tot += (long long)fac_doubles(q.spec.na_max);
         g->nk_tot += q.spec.nk;
         g->y_tot += (long long)q.spec.nk * q.spec.na_max;
         g->s_tot += q.spec.na_max;
@@ -1821,6 +1887,7 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
     for (double **p : {&g->wavenum, &g->init, &g->out, &g->gbar, &g->ibar, &g->mbic, &g->mubar})
         *p = group_alloc<double>(nk);
     g->mbar = group_alloc<double>(nhist);
+    g->fac = group_alloc<double>(g->fac_tot);
     g->ivl = group_alloc<double>(nk * kLimit * 2);
     g->bucket = group_alloc<int>(g->b_tot);
     g->nint = group_alloc<int>(nk);

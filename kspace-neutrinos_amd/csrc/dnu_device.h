@@ -119,13 +119,9 @@ __device__ __forceinline__ int settle(const double *xa, int n, int i, double x)
 }
 
 // ---- integrands ----
-struct FslIntegrand {  // delta_tot_table.c:378-383: 1 / a / (a H(a))
+struct FslIntegrand {  // delta_tot_table.c:378-383: 1 / a / (a H(a)) = exp(-2 ln a - ln H)
     HTab H;
-    __device__ __forceinline__ double operator()(double x) const
-    {
-        const double a = exp(x);
-        return 1. / a / (a * exp(ln_hubble(H, x)));
-    }
+    __device__ __forceinline__ double operator()(double x) const { return exp(-2.0 * x - ln_hubble(H, x)); }
 };
 
 struct DnuIntegrand {  // delta_tot_table.c:492-500
@@ -153,8 +149,7 @@ struct DnuIntegrand {  // delta_tot_table.c:492-500
         const int si = settle(sx, ns, sbucket[q], x);
         const double dtot = linear ? linear_at(sx, sy, si, x) : cspline_at(sx, sy, sc, si, x);
         const double specJ = specialJ(k * fsl / mnubykT, *J);
-        const double ai = exp(x);
-        return fsl / (ai * exp(ln_hubble(H, x))) * specJ * dtot;
+        return fsl * exp(-x - ln_hubble(H, x)) * specJ * dtot;  // fsl / (a H) specJ dtot
     }
 };
 
@@ -170,7 +165,9 @@ __device__ __forceinline__ double gk_rescale(double err, double resabs, double r
 {
     err = fabs(err);
     if (resasc != 0 && err != 0) {
-        const double s = pow(200 * err / resasc, 1.5);
+        // (200 err / resasc)^1.5 as QUADPACK's rescale, formed as t sqrt(t)
+        const double t = 200 * err / resasc;
+        const double s = t * sqrt(t);
         err = s < 1 ? resasc * s : resasc;
     }
     if (resabs > DBL_MIN / (50 * DBL_EPSILON)) {
@@ -458,52 +455,70 @@ struct ArrayRhs {
     __device__ __forceinline__ double operator()(int i) const { return r[i]; }
 };
 
-// Called by all 64 lanes of a wavefront together, with hi - lo <= kKeep.
+// A lane's share of a window: rows [s, s + len) of the window's interior
+// rows [wa, wb).
+struct TriLane {
+    int wa, wb, s, len;
+};
+
+__device__ __forceinline__ TriLane tri_lane(int n, int lo, int hi)
+{
+    const int lane = threadIdx.x & 63;
+    TriLane t;
+    t.wa = lo - kHalo;
+    t.wb = hi + kHalo;  // window of interior rows [wa, wb)
+    if (t.wa < 1) t.wa = 1;
+    if (t.wb > n - 1) t.wb = n - 1;
+    t.s = t.wa + lane * kRun;
+    t.len = t.s < t.wb ? (t.wb - t.s < kRun ? t.wb - t.s : kRun) : 0;
+    return t;
+}
+
+// The lane's factors gam (g) and alp (al) over its run, and gam_{s-1}; past
+// the run al = 1, g = 0.
+__device__ __forceinline__ void tri_factor(const double *x, const TriLane &t, double (&g)[kRun], double (&al)[kRun],
+                                           double &g_before)
+{
+    const int s = t.s, wb = t.wb, len = t.len;
+    int a = s - kWarm;
+    if (a < 1) a = 1;
+    double gp = 0.0;
+    for (int r = a; r < s && r < wb; r++) {
+        const double hl = x[r] - x[r - 1], hr = x[r + 1] - x[r];
+        const double diag = 2.0 * (hr + hl);
+        const double alr = r == a ? diag : diag - hl * gp;
+        gp = hr / alr;
+    }
+    g_before = gp;
+#pragma unroll
+    for (int j = 0; j < kRun; j++) {
+        if (j < len) {
+            const int i = s + j;
+            const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
+            const double diag = 2.0 * (hr + hl);
+            al[j] = i == a ? diag : diag - hl * gp;
+            g[j] = hr / al[j];
+            gp = g[j];
+        } else {
+            al[j] = 1.0;
+            g[j] = 0.0;
+        }
+    }
+}
+
+// Called by all 64 lanes of a wavefront together, with hi - lo <= kKeep and
+// n >= 3, given the lane's factors.
 // ADJ_RHS: instead of storing the solution z, add its image under the
 // transpose of the natural-spline right-hand side's y-derivative onto out
 // (the second half of the spline adjoint) for the kept rows.
-template <class RHS, bool ADJ_RHS = false>
-__device__ inline void tridiag_window_wave(const double *x, int n, const RHS &rhs, int lo, int hi, double *out)
+template <class RHS, bool ADJ_RHS>
+__device__ __forceinline__ void tri_solve(const double *x, int n, const RHS &rhs, int lo, int hi, double *out,
+                                          const TriLane &t, const double (&g)[kRun], const double (&al)[kRun],
+                                          double g_before)
 {
     const int lane = threadIdx.x & 63;
-    if (n < 3) {
-        if (!ADJ_RHS)
-            for (int i = lo + lane; i < hi; i += 64) out[i] = 0.0;
-        return;
-    }
-    int wa = lo - kHalo, wb = hi + kHalo;  // window of interior rows [wa, wb)
-    if (wa < 1) wa = 1;
-    if (wb > n - 1) wb = n - 1;
-    const int s = wa + lane * kRun;
-    const int len = s < wb ? (wb - s < kRun ? wb - s : kRun) : 0;
-    double g[kRun], al[kRun], zz[kRun];
-    double g_before = 0.0;  // gam_{s-1}
-    {
-        int a = s - kWarm;
-        if (a < 1) a = 1;
-        double gp = 0.0;
-        for (int r = a; r < s && r < wb; r++) {
-            const double hl = x[r] - x[r - 1], hr = x[r + 1] - x[r];
-            const double diag = 2.0 * (hr + hl);
-            const double alr = r == a ? diag : diag - hl * gp;
-            gp = hr / alr;
-        }
-        g_before = gp;
-#pragma unroll
-        for (int j = 0; j < kRun; j++) {
-            if (j < len) {
-                const int i = s + j;
-                const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
-                const double diag = 2.0 * (hr + hl);
-                al[j] = i == a ? diag : diag - hl * gp;
-                g[j] = hr / al[j];
-                gp = g[j];
-            } else {
-                al[j] = 1.0;
-                g[j] = 0.0;
-            }
-        }
-    }
+    const int wa = t.wa, wb = t.wb, s = t.s, len = t.len;
+    double zz[kRun];
     double rv[kRun];
 #pragma unroll
     for (int j = 0; j < kRun; j++) rv[j] = j < len ? rhs(s + j) : 0.0;
@@ -608,6 +623,79 @@ __device__ inline void tridiag_window_wave(const double *x, int n, const RHS &rh
             if (j < len && s + j == n - 2) out[n - 1] += zv[j] * 3.0 / (x[n - 1] - x[n - 2]);
     }
 }
+
+// One window [lo, hi) of natural-spline rows by one wavefront (factors
+// formed in place).
+template <class RHS, bool ADJ_RHS = false>
+__device__ inline void tridiag_window_wave(const double *x, int n, const RHS &rhs, int lo, int hi, double *out)
+{
+    const int lane = threadIdx.x & 63;
+    if (n < 3) {
+        if (!ADJ_RHS)
+            for (int i = lo + lane; i < hi; i += 64) out[i] = 0.0;
+        return;
+    }
+    const TriLane t = tri_lane(n, lo, hi);
+    double g[kRun], al[kRun], gb;
+    tri_factor(x, t, g, al, gb);
+    tri_solve<RHS, ADJ_RHS>(x, n, rhs, lo, hi, out, t, g, al, gb);
+}
+
+// Factors of the knots x[0..n) stored once for every row that shares them:
+// window w at fac + w kFacStride, [j][lane] for gam, then alp, then
+// gam_{s-1} per lane (the same values tri_factor forms in each wavefront).
+constexpr int kFacStride = (2 * kRun + 1) * 64;
+
+__host__ __device__ inline int fac_windows(int n) { return (n + kKeep - 1) / kKeep; }
+
+__device__ inline void tri_factor_store(const double *x, int n, int w, double *fac)
+{
+    const int lane = threadIdx.x & 63;
+    const int lo = w * kKeep, hi = lo + kKeep < n ? lo + kKeep : n;
+    const TriLane t = tri_lane(n, lo, hi);
+    double g[kRun], al[kRun], gb;
+    tri_factor(x, t, g, al, gb);
+    double *f = fac + (size_t)w * kFacStride;
+#pragma unroll
+    for (int j = 0; j < kRun; j++) {
+        f[j * 64 + lane] = g[j];
+        f[(kRun + j) * 64 + lane] = al[j];
+    }
+    f[2 * kRun * 64 + lane] = gb;
+}
+
+// tridiag_window_wave for window w = lo / kKeep with the stored factors.
+template <class RHS, bool ADJ_RHS = false>
+__device__ inline void tridiag_window_wave_fac(const double *x, int n, const RHS &rhs, int lo, int hi, double *out,
+                                               const double *fac)
+{
+    const int lane = threadIdx.x & 63;
+    if (n < 3) {
+        if (!ADJ_RHS)
+            for (int i = lo + lane; i < hi; i += 64) out[i] = 0.0;
+        return;
+    }
+    const TriLane t = tri_lane(n, lo, hi);
+    const double *f = fac + (size_t)(lo / kKeep) * kFacStride;
+    double g[kRun], al[kRun];
+#pragma unroll
+    for (int j = 0; j < kRun; j++) {
+        g[j] = f[j * 64 + lane];
+        al[j] = f[(kRun + j) * 64 + lane];
+    }
+    tri_solve<RHS, ADJ_RHS>(x, n, rhs, lo, hi, out, t, g, al, f[2 * kRun * 64 + lane]);
+}
+
+// Right-hand side of the natural-spline system with the reciprocal knot
+// spacings ih[i] = 1 / (x[i+1] - x[i]) (0 for a zero spacing) stored: the
+// values of SplineRhs, bit for bit.
+struct SplineRhsIh {
+    const double *ih, *y;
+    __device__ __forceinline__ double operator()(int i) const
+    {
+        return 3.0 * ((y[i + 1] - y[i]) * ih[i] - (y[i] - y[i - 1]) * ih[i - 1]);
+    }
+};
 
 // ---- derivatives for the adjoint ----------------------------------------
 
