@@ -188,13 +188,14 @@ __device__ __forceinline__ void gk61_pair(const F &f, double a1, double b1, doub
     const int lane = threadIdx.x & 63, hf = lane >> 5, j = lane & 31;
     const double a = hf ? a2 : a1, b = hf ? b2 : b1;
     const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+    // lane 30 takes the centre (c_xgk[30] = 0: c - 0 is c) in the same
+    // evaluation as the other lanes' left nodes, so a wavefront runs the
+    // integrand twice, not three times
     double f1 = 0.0, f2 = 0.0;
-    if (j < 30) {
+    if (j <= 30) {
         const double dx = h * c_xgk[j];
         f1 = f(c - dx);
-        f2 = f(c + dx);
-    } else if (j == 30) {
-        f1 = f(c);
+        if (j < 30) f2 = f(c + dx);
     }
     const double wk = j <= 30 ? c_wgk[j] : 0.0;
     const double wgv = (j < 30 && (j & 1)) ? c_wg[j >> 1] : 0.0;
