@@ -125,8 +125,8 @@ __device__ inline void fs_node_wave(const HTab &H, int nfs, double x0, double xa
 }
 
 // The fslength nodes in two passes with the result of one QAG per node.
-// Pass 1 (k_fs_first, no LDS): two nodes per wavefront, lanes 0-31 and 32-63
-// each forming the first 61-point Gauss-Kronrod estimate over [x_i, log a]
+// Pass 1 (k_fs_first, no LDS): four nodes per wavefront, 16 lanes each
+// forming the first 61-point Gauss-Kronrod estimate over [x_i, log a]
 // exactly as wave_qag61's first step does, and accepting it by the same test
 // (QUADPACK returns that estimate when it already meets the tolerance); a node
 // that needs bisection is appended to a list.  Pass 2 (k_fs_rest) runs the
@@ -137,37 +137,30 @@ __device__ inline void fs_first_wave(const HTab &H, int nfs, double x0, double x
                                      double *__restrict__ fx, double *__restrict__ fy, int *__restrict__ status,
                                      int wave, const Push &push)
 {
-    const int lane = threadIdx.x & 63, hf = lane >> 5;
-    if (2 * wave >= nfs) return;  // wave-uniform
-    const int nA = 2 * wave, nB = nA + 1;
+    if (4 * wave >= nfs) return;  // wave-uniform
+    const int lane = threadIdx.x & 63;
+    const int node = 4 * wave + (lane >> 4);  // four nodes per wavefront, 16 lanes each
     // delta_tot_table.c:582: log T0 + i (log a - log T0) / (Nfs - 1.)
-    const double xA = x0 + nA * (xa - x0) / (nfs - 1.);
-    const double xB = nB < nfs ? x0 + nB * (xa - x0) / (nfs - 1.) : xa;
+    const double xi = node < nfs ? x0 + node * (xa - x0) / (nfs - 1.) : xa;
     FslIntegrand f;
     f.H = H;
-    GK gA, gB;
-    gk61_pair(f, xA < xa ? xA : xa, xa, xB < xa ? xB : xa, xa, gA, gB);
-    if ((lane & 31) == 0) {
-        const int node = hf ? nB : nA;
-        const double xi = hf ? xB : xA;
-        const GK &g = hf ? gB : gA;
-        if (node < nfs) {
-            double v = 0.0;
-            if (xi < xa) {  // fslength returns 0 when logai >= logaf (:402)
-                const double tol = 1e-6 * fabs(g.res);
-                if (g.err <= 50 * DBL_EPSILON * g.resabs && g.err > tol) {
-                    atomicOr(status, 1);  // roundoff on the first estimate (wave_qag61 returns 18)
-                    v = light * g.res;
-                } else if ((g.err <= tol && g.err != g.resasc) || g.err == 0.0) {
-                    v = light * g.res;
-                } else {
-                    push(node);
-                    v = 0.0;
-                }
+    const GK g = gk61_quad(f, xi < xa ? xi : xa, xa);
+    if ((lane & 15) == 0 && node < nfs) {
+        double v = 0.0;
+        if (xi < xa) {  // fslength returns 0 when logai >= logaf (:402)
+            const double tol = 1e-6 * fabs(g.res);
+            if (g.err <= 50 * DBL_EPSILON * g.resabs && g.err > tol) {
+                atomicOr(status, 1);  // roundoff on the first estimate (wave_qag61 returns 18)
+                v = light * g.res;
+            } else if ((g.err <= tol && g.err != g.resasc) || g.err == 0.0) {
+                v = light * g.res;
+            } else {
+                push(node);
+                v = 0.0;
             }
-            fx[node] = xi;
-            fy[node] = v;
         }
+        fx[node] = xi;
+        fy[node] = v;
     }
 }
 
@@ -201,7 +194,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_fs_rest(HTab H, int nfs
 
 constexpr int kFsRestBlocks = 64;
 
-inline int fs_first_blocks(int nfs) { return (nfs + 7) / 8; }  // 4 waves x 2 nodes per block
+inline int fs_first_blocks(int nfs) { return (nfs + 15) / 16; }  // 4 waves x 4 nodes per block
 
 // Rows of natural-spline coefficients (row r: x[0..n), y + r*ystride,
 // c + r*cstride): one wavefront per (row, window of kKeep rows).

@@ -185,6 +185,7 @@ struct GK {
 template <class F>
 __device__ __forceinline__ void gk61_pair(const F &f, double a1, double b1, double a2, double b2, GK &g1, GK &g2)
 {
+#pragma clang fp contract(off)  // every product and sum rounded as written (gk61_quad's sums match bit for bit)
     const int lane = threadIdx.x & 63, hf = lane >> 5, j = lane & 31;
     const double a = hf ? a2 : a1, b = hf ? b2 : b1;
     const double c = 0.5 * (a + b), h = 0.5 * (b - a);
@@ -218,6 +219,51 @@ __device__ __forceinline__ void gk61_pair(const F &f, double a1, double b1, doub
     g2.err = __shfl(r.err, 32, 64);
     g2.resabs = __shfl(r.resabs, 32, 64);
     g2.resasc = __shfl(r.resasc, 32, 64);
+}
+
+// 61-point Gauss-Kronrod on [a, b] of this lane's 16-lane group (four
+// intervals per wavefront); the result is valid in every lane of the group.
+// Lane j of a group takes the nodes j and j + 16 of gk61_pair's 32-lane
+// layout and adds their terms itself, which is the xor-16 step of half_sum:
+// the sums, and so the results, are gk61_pair's bit for bit.
+__device__ __forceinline__ double quarter_sum(double v)
+{
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <class F>
+__device__ __forceinline__ GK gk61_quad(const F &f, double a, double b)
+{
+#pragma clang fp contract(off)  // as gk61_pair
+    const int j0 = threadIdx.x & 15, j1 = j0 + 16;
+    const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+    const double dx0 = h * c_xgk[j0];
+    const double f01 = f(c - dx0), f02 = f(c + dx0);
+    double f11 = 0.0, f12 = 0.0;
+    if (j1 <= 30) {
+        const double dx1 = h * c_xgk[j1];
+        f11 = f(c - dx1);
+        if (j1 < 30) f12 = f(c + dx1);
+    }
+    const double wk0 = c_wgk[j0], wk1 = j1 <= 30 ? c_wgk[j1] : 0.0;
+    const double wg0 = (j0 & 1) ? c_wg[j0 >> 1] : 0.0;
+    const double wg1 = (j1 < 30 && (j1 & 1)) ? c_wg[j1 >> 1] : 0.0;
+    const double sk = quarter_sum(wk0 * (f01 + f02) + (j1 == 30 ? wk1 * f11 : wk1 * (f11 + f12)));
+    const double sg = quarter_sum(wg0 * (f01 + f02) + wg1 * (f11 + f12));
+    const double sa = quarter_sum(wk0 * (fabs(f01) + fabs(f02)) +
+                                  (j1 == 30 ? wk1 * fabs(f11) : wk1 * (fabs(f11) + fabs(f12))));
+    const double mean = sk * 0.5;
+    const double sasc = quarter_sum(wk0 * (fabs(f01 - mean) + fabs(f02 - mean)) +
+                                    (j1 == 30 ? wk1 * fabs(f11 - mean)
+                                              : (j1 < 30 ? wk1 * (fabs(f11 - mean) + fabs(f12 - mean)) : 0.0)));
+    GK r;
+    r.res = sk * h;
+    r.resabs = sa * fabs(h);
+    r.resasc = sasc * fabs(h);
+    r.err = gk_rescale((sk - sg) * h, r.resabs, r.resasc);
+    return r;
 }
 
 // Interval list of one wavefront (QUADPACK workspace), in LDS.
@@ -511,11 +557,12 @@ __device__ __forceinline__ void tri_factor(const double *x, const TriLane &t, do
 // n >= 3, given the lane's factors.
 // ADJ_RHS: instead of storing the solution z, add its image under the
 // transpose of the natural-spline right-hand side's y-derivative onto out
-// (the second half of the spline adjoint) for the kept rows.
+// (the second half of the spline adjoint) for the kept rows, with the
+// reciprocal knot spacings ih (as SplineRhsIh's).
 template <class RHS, bool ADJ_RHS>
 __device__ __forceinline__ void tri_solve(const double *x, int n, const RHS &rhs, int lo, int hi, double *out,
                                           const TriLane &t, const double (&g)[kRun], const double (&al)[kRun],
-                                          double g_before)
+                                          double g_before, const double *ih)
 {
     const int lane = threadIdx.x & 63;
     const int wa = t.wa, wb = t.wb, s = t.s, len = t.len;
@@ -610,36 +657,36 @@ __device__ __forceinline__ void tri_solve(const double *x, int n, const RHS &rhs
         const int i = s + j;
         if (j < len && i >= lo && i < hi) {
             const double zl = j > 0 ? zv[j - 1] : zp, zr = j + 1 < len ? zv[j + 1] : zn;
+            const double wl = 3.0 * ih[i - 1], wr = 3.0 * ih[i];
             double acc = 0.0;
-            if (i - 1 >= 1) acc += zl * 3.0 / (x[i] - x[i - 1]);
-            acc -= zv[j] * (3.0 / (x[i + 1] - x[i]) + 3.0 / (x[i] - x[i - 1]));
-            if (i + 1 <= n - 2) acc += zr * 3.0 / (x[i + 1] - x[i]);
+            if (i - 1 >= 1) acc += zl * wl;
+            acc -= zv[j] * (wr + wl);
+            if (i + 1 <= n - 2) acc += zr * wr;
             out[i] += acc;
         }
     }
-    if (lane == 0 && lo == 0 && len > 0) out[0] += zv[0] * 3.0 / (x[1] - x[0]);  // row 0: the z_1 term
+    if (lane == 0 && lo == 0 && len > 0) out[0] += zv[0] * (3.0 * ih[0]);  // row 0: the z_1 term
     if (lo <= n - 1 && n - 1 < hi) {                                         // row n-1: the z_{n-2} term
 #pragma unroll
         for (int j = 0; j < kRun; j++)
-            if (j < len && s + j == n - 2) out[n - 1] += zv[j] * 3.0 / (x[n - 1] - x[n - 2]);
+            if (j < len && s + j == n - 2) out[n - 1] += zv[j] * (3.0 * ih[n - 2]);
     }
 }
 
 // One window [lo, hi) of natural-spline rows by one wavefront (factors
 // formed in place).
-template <class RHS, bool ADJ_RHS = false>
+template <class RHS>
 __device__ inline void tridiag_window_wave(const double *x, int n, const RHS &rhs, int lo, int hi, double *out)
 {
     const int lane = threadIdx.x & 63;
     if (n < 3) {
-        if (!ADJ_RHS)
-            for (int i = lo + lane; i < hi; i += 64) out[i] = 0.0;
+        for (int i = lo + lane; i < hi; i += 64) out[i] = 0.0;
         return;
     }
     const TriLane t = tri_lane(n, lo, hi);
     double g[kRun], al[kRun], gb;
     tri_factor(x, t, g, al, gb);
-    tri_solve<RHS, ADJ_RHS>(x, n, rhs, lo, hi, out, t, g, al, gb);
+    tri_solve<RHS, false>(x, n, rhs, lo, hi, out, t, g, al, gb, nullptr);
 }
 
 // Factors of the knots x[0..n) stored once for every row that shares them:
@@ -665,7 +712,8 @@ __device__ inline void tri_factor_store(const double *x, int n, int w, double *f
     f[2 * kRun * 64 + lane] = gb;
 }
 
-// tridiag_window_wave for window w = lo / kKeep with the stored factors.
+// tridiag_window_wave for window w = lo / kKeep with the stored factors
+// (fac_windows(n) windows, then the reciprocal spacings).
 template <class RHS, bool ADJ_RHS = false>
 __device__ inline void tridiag_window_wave_fac(const double *x, int n, const RHS &rhs, int lo, int hi, double *out,
                                                const double *fac)
@@ -684,7 +732,8 @@ __device__ inline void tridiag_window_wave_fac(const double *x, int n, const RHS
         g[j] = f[j * 64 + lane];
         al[j] = f[(kRun + j) * 64 + lane];
     }
-    tri_solve<RHS, ADJ_RHS>(x, n, rhs, lo, hi, out, t, g, al, f[2 * kRun * 64 + lane]);
+    tri_solve<RHS, ADJ_RHS>(x, n, rhs, lo, hi, out, t, g, al, f[2 * kRun * 64 + lane],
+                            fac + (size_t)fac_windows(n) * kFacStride);
 }
 
 // Right-hand side of the natural-spline system with the reciprocal knot
