@@ -446,12 +446,25 @@ struct AdjArgs {
     const int *nint;
     const double *gbar;
     double prefac;
-    double *ybar, *cbar, *mu_bar;
+    double *ybar, *cbar, *mu_bar;  // mu_bar: [item][part]
+    int nparts, plen;              // knots split in parts of plen (see adj_parts)
     IcAdjArgs ica;
 };
 
-// Block `blk` of one history's adjoint: items [0, nk), then the
-// initial-condition piece, 64 k per block.  `lds`: 2 ns doubles.
+// An item's knots [0, ns) are split in nparts parts of plen knots, one block
+// each, so that a block's LDS rows stay small (more wavefronts per CU).
+constexpr int kAdjPartKnots = 1024;
+__host__ __device__ inline int adj_parts(int ns) { return (ns + kAdjPartKnots - 1) / kAdjPartKnots; }
+__host__ __device__ inline int adj_plen(int ns) { return (ns + adj_parts(ns) - 1) / adj_parts(ns); }
+// LDS of one block: ybar and cbar rows of plen doubles, the selected interval list
+inline size_t adj_lds_bytes(int ns) { return sizeof(double) * 2 * (size_t)adj_plen(ns) + sizeof(int) * kLimit; }
+
+// Block `blk` of one history's adjoint: (item, part) blocks [0, nk nparts),
+// then the initial-condition piece, 64 k per block.  The part owning knots
+// [K0, K1) evaluates the points of the recorded intervals that can reach
+// them (a point in knot interval si feeds knots si and si+1) and keeps the
+// terms that land in [K0, K1); a point's d/d(m/kT) term goes to the part
+// owning its si, so every term is counted once.  `lds`: adj_lds_bytes(ns).
 __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
 {
     const int nk = A.nk, ns = A.ns, nfs = A.nfs;
@@ -460,26 +473,52 @@ __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
     const double prefac = A.prefac, vcrit = A.vcrit, sinv_bw = A.sinv_bw;
     const int nsb = A.nsb;
     const HTab &H = A.H;
-    if (blk >= nk) {
-        const int kk = (blk - nk) * 64 + (int)(threadIdx.x & 63);
+    const int P = A.nparts, L = A.plen;
+    if (blk >= nk * P) {
+        const int kk = (blk - nk * P) * 64 + (int)(threadIdx.x & 63);
         if (kk < nk) ic_adjoint_at(kk, A.wavenum, A.gbar, *A.sp, A.ica);
         return;
     }
-    const int lane = threadIdx.x & 63, item = blk, k = item % nk;
+    const int lane = threadIdx.x & 63, item = blk / P, part = blk % P, k = item % nk;
     const int hf = lane >> 5, j = lane & 31;
-    for (int t = lane; t < 2 * ns; t += 64) lds[t] = 0.0;
+    const int K0 = part * L, K1 = K0 + L < ns ? K0 + L : ns;
+    int *sel = (int *)(lds + 2 * L);
+    for (int t = lane; t < 2 * L; t += 64) lds[t] = 0.0;
+    const double *__restrict__ ivl = A.ivl;
+    const int ni = A.nint[item];
+    // the recorded intervals that reach this part's knots, in their order:
+    // x in [sx[K0 - 1], sx[K1]] (clamped to the table)
+    int nsel = 0;
+    {
+        const double xlo = sx[K0 > 0 ? K0 - 1 : 0], xhi = sx[K1 < ns ? K1 : ns - 1];
+        for (int t0 = 0; t0 < ni; t0 += 64) {
+            const int t = t0 + lane;
+            int hit = 0;
+            if (t < ni) {
+                const double a = ivl[((size_t)item * kLimit + t) * 2], b = ivl[((size_t)item * kLimit + t) * 2 + 1];
+                hit = (a <= xhi && b >= xlo) ? 1 : 0;
+            }
+            double sc = hit;  // inclusive prefix count over the wavefront
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const double o = __shfl_up(sc, d, 64);
+                if (lane >= d) sc += o;
+            }
+            if (hit) sel[nsel + (int)sc - 1] = t;
+            nsel += (int)__shfl(sc, 63, 64);
+        }
+    }
     __syncthreads();
     const double *syk = A.Y + (size_t)k * A.ystride, *sck = A.C + (size_t)k * A.ystride;
     const bool linear = ns <= 2;
     const double kk = A.wavenum[k], mnubykT = A.sp->mnubykT, gb = A.gbar[item];
     const JPar &J = A.sp->J;
-    const int ni = A.nint[item];
-    const double *__restrict__ ivl = A.ivl;
     const double fx0 = fx[0], finv_h = (nfs - 1) / (fx[nfs - 1] - fx[0]), sx0 = sx[0];
     double mu_acc = 0.0;
-    for (int t0 = 0; t0 < ni; t0 += 2) {
-        const int t = t0 + hf;
-        if (t < ni && j <= 30) {
+    for (int t0 = 0; t0 < nsel; t0 += 2) {
+        const int ts = t0 + hf;
+        if (ts < nsel && j <= 30) {
+            const int t = sel[ts];
             const double a = ivl[((size_t)item * kLimit + t) * 2], b = ivl[((size_t)item * kLimit + t) * 2 + 1];
             const double c = 0.5 * (a + b), hl = 0.5 * (b - a);
             const int npt = j < 30 ? 2 : 1;
@@ -497,19 +536,25 @@ __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
                 const double Jv = specialJ_d(kx, J, Jx, Jq);
                 const double base = fsl * exp(-x - ln_hubble(H, x));  // fsl / (a H), as the forward
                 const double S = linear ? linear_at(sx, syk, si, x) : cspline_at(sx, syk, sck, si, x);
-                mu_acc += w * base * S * (Jx * (-kx / mnubykT) + Jq * vcrit);  // d/d(m/kT), qc = vcrit m/kT
+                const bool own0 = si >= K0 && si < K1, own1 = si + 1 >= K0 && si + 1 < K1;
+                if (own0)  // d/d(m/kT), qc = vcrit m/kT
+                    mu_acc += w * base * S * (Jx * (-kx / mnubykT) + Jq * vcrit);
                 const double wj = w * base * Jv;
                 if (linear) {
                     const double t1 = (x - sx[si]) / (sx[si + 1] - sx[si]);
-                    atomicAdd(&lds[si], wj * (1.0 - t1));
-                    atomicAdd(&lds[si + 1], wj * t1);
+                    if (own0) atomicAdd(&lds[si - K0], wj * (1.0 - t1));
+                    if (own1) atomicAdd(&lds[si + 1 - K0], wj * t1);
                 } else {
                     double wy0, wy1, wc0, wc1;
                     cspline_grad(sx, si, x, wy0, wy1, wc0, wc1);
-                    atomicAdd(&lds[si], wj * wy0);
-                    atomicAdd(&lds[si + 1], wj * wy1);
-                    atomicAdd(&lds[ns + si], wj * wc0);
-                    atomicAdd(&lds[ns + si + 1], wj * wc1);
+                    if (own0) {
+                        atomicAdd(&lds[si - K0], wj * wy0);
+                        atomicAdd(&lds[L + si - K0], wj * wc0);
+                    }
+                    if (own1) {
+                        atomicAdd(&lds[si + 1 - K0], wj * wy1);
+                        atomicAdd(&lds[L + si + 1 - K0], wj * wc1);
+                    }
                 }
             }
         }
@@ -518,17 +563,18 @@ __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
     for (int m = 32; m >= 1; m >>= 1) mu_acc += __shfl_xor(mu_acc, m, 64);
     __syncthreads();
     double *yrow = A.ybar + (size_t)k * A.ystride, *crow = A.cbar + (size_t)k * A.ystride;
-    for (int t = lane; t < ns; t += 64) {
-        yrow[t] = lds[t];
-        crow[t] = lds[ns + t];
+    for (int t = lane; t < K1 - K0; t += 64) {
+        yrow[K0 + t] = lds[t];
+        crow[K0 + t] = lds[L + t];
     }
-    for (int t = ns + lane; t < A.ystride; t += 64) yrow[t] = 0.0;
-    if (lane == 0) A.mu_bar[item] = mu_acc;
+    if (part == P - 1)
+        for (int t = ns + lane; t < A.ystride; t += 64) yrow[t] = 0.0;
+    if (lane == 0) A.mu_bar[(size_t)item * P + part] = mu_acc;
 }
 
 __global__ __launch_bounds__(64) void k_dnu_adj(AdjArgs A)
 {
-    HIP_DYNAMIC_SHARED(double, lds)  // ybar row [ns], cbar row [ns]
+    HIP_DYNAMIC_SHARED(double, lds)  // adj_lds_bytes(ns): part rows of ybar and cbar, interval list
     dnu_adj_block(A, blockIdx.x, lds);
 }
 
@@ -1010,21 +1056,16 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
             }
             dev->cap_adj = ny;
         }
-        if (dev->cap_adj_k < (size_t)nk) {
+        const int P = adj_parts(Na);
+        if (dev->cap_adj_k < (size_t)nk * P) {  // gbar (nk), mubar (nk x parts)
             for (double **p : {&dev->gbar, &dev->mubar}) {
                 if (*p) KN_CHECK(hipFree(*p));
-                KN_CHECK(hipMalloc(p, sizeof(double) * nk));
+                KN_CHECK(hipMalloc(p, sizeof(double) * nk * P));
             }
-            dev->cap_adj_k = nk;
+            dev->cap_adj_k = (size_t)nk * P;
         }
         KN_CHECK(hipMemcpy(dev->gbar, gbar, sizeof(double) * nk, hipMemcpyHostToDevice));
-        const size_t lds = sizeof(double) * 2 * (size_t)Na;
-        if (lds > 160 * 1024) {
-            kn_set_last_error("kn_get_delta_nu_adjoint: %d stored steps exceed the LDS row budget", Na);
-            return KN_ERR_RANGE;
-        }
-        if (lds > 64 * 1024)
-            KN_CHECK(hipFuncSetAttribute((const void *)k_dnu_adj, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const size_t lds = adj_lds_bytes(Na);
         const int nfs = 16 * Na, nsb = 4 * Na;
         const double sinv_bw = nsb / (d->scalefact[Na - 1] - d->scalefact[0]);
         const double vcrit = s.qc > 0 ? d->omnu->hybnu.vcrit : 0.0;
@@ -1053,20 +1094,23 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
         A.ybar = dev->ybar;
         A.cbar = dev->cbar;
         A.mu_bar = dev->mubar;
+        A.nparts = P;
+        A.plen = adj_plen(Na);
         A.ica = IcAdjArgs{nullptr, 0.0, 0.0, nullptr, nullptr};
-        hipLaunchKernelGGL(k_dnu_adj, dim3(nk), dim3(64), lds, 0, A);
+        hipLaunchKernelGGL(k_dnu_adj, dim3(nk * P), dim3(64), lds, 0, A);
         KN_CHECK(hipGetLastError());
         if (Na > 2) {
             hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, Na, nk, (long long)Na,
                                dev->cbar, dev->ybar, dev->fac);
             KN_CHECK(hipGetLastError());
         }
-        // results through the pinned stage: [mu_bar (nk) | dL/d delta_tot (nk Na)]
-        double *mub = (double *)pinned_stage(sizeof(double) * (nk + (dtot_bar ? ny : 0))), *yb = mub + nk;
-        KN_CHECK(hipMemcpyAsync(mub, dev->mubar, sizeof(double) * nk, hipMemcpyDeviceToHost, 0));
+        // results through the pinned stage: [mu_bar (nk x parts) | dL/d delta_tot (nk Na)]
+        const size_t nmu = (size_t)nk * P;
+        double *mub = (double *)pinned_stage(sizeof(double) * (nmu + (dtot_bar ? ny : 0))), *yb = mub + nmu;
+        KN_CHECK(hipMemcpyAsync(mub, dev->mubar, sizeof(double) * nmu, hipMemcpyDeviceToHost, 0));
         if (dtot_bar) KN_CHECK(hipMemcpyAsync(yb, dev->ybar, sizeof(double) * ny, hipMemcpyDeviceToHost, 0));
         KN_CHECK(hipStreamSynchronize(0));
-        for (int k = 0; k < nk; k++) mb += mub[k];
+        for (size_t i = 0; i < nmu; i++) mb += mub[i];  // k-major, parts in order
         if (dtot_bar) {
             const int parts = ny < (1u << 16) ? 1 : 32;
             host_pool().run(parts, [&](int p) {
@@ -1118,8 +1162,8 @@ __global__ void k_group_ic_adj(const double *__restrict__ wavenum, const double 
 }
 
 struct GroupMeta {
-    long long off_k;
-    int nk, integ;
+    long long off_k, off_mu;
+    int nk, integ, nparts;
     double kBtnu;
 };
 
@@ -1134,7 +1178,8 @@ __global__ __launch_bounds__(64) void k_group_mass_bar(const GroupMeta *__restri
     double s = 0.0;
     for (int k = threadIdx.x; k < g.nk; k += 64) {
         s += mbar_ic[g.off_k + k];
-        if (g.integ) s += mubar[g.off_k + k];
+        if (g.integ)
+            for (int p = 0; p < g.nparts; p++) s += mubar[g.off_mu + (long long)k * g.nparts + p];
     }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
@@ -1163,7 +1208,7 @@ struct GroupHist {
     kn_batch_history spec;
     int na, integ, htab, stream;
     double x0, xa, xmax, fsl0, lin, vcrit;
-    long long off_k, off_y, off_s, off_fs, off_b, off_fac;
+    long long off_k, off_y, off_s, off_fs, off_b, off_fac, off_mu;
 };
 
 // ---- merged launches: one grid per phase for every history of the group.
@@ -1184,10 +1229,11 @@ struct GroupDesc {
     double x0, xa, light, sinv_bw, fsl0, lin, prefac, vcrit;
 };
 
-// phases (rows of the prefix table); the adjoint's item blocks are split in
-// classes of na (LDS rows of 2 na doubles): class c holds na <= 256 << c
+// phases (rows of the prefix table); the adjoint's (item, part) blocks are
+// split in classes of part length (LDS rows of 2 adj_plen(na) doubles):
+// class c holds adj_plen(na) <= 128 << c
 enum { kPhFs, kPhFactor, kPhRows, kPhFsSpline, kPhQag, kPhIc, kPhSolve, kPhAdj0 };
-constexpr int kAdjClasses = 7;
+constexpr int kAdjClasses = 4;
 constexpr int kPhases = kPhAdj0 + kAdjClasses;
 constexpr int kMergedRestBlocks = 1280;
 
@@ -1342,7 +1388,7 @@ __global__ __launch_bounds__(64) void k_m_ic_adj(const GroupDesc *__restrict__ D
 __global__ __launch_bounds__(64) void k_m_dnu_adj(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
                                                   int nhist)
 {
-    HIP_DYNAMIC_SHARED(double, lds)  // 2 na doubles of the class's largest na
+    HIP_DYNAMIC_SHARED(double, lds)  // adj_lds_bytes of the class's longest part
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
     AdjArgs A;
@@ -1370,6 +1416,8 @@ __global__ __launch_bounds__(64) void k_m_dnu_adj(const GroupDesc *__restrict__ 
     A.ybar = d.dbar;
     A.cbar = d.cbar;
     A.mu_bar = d.mubar;
+    A.nparts = adj_parts(d.na);
+    A.plen = adj_plen(d.na);
     A.ica = group_ica(d);
     dnu_adj_block(A, blockIdx.x - pre[h], lds);
 }
@@ -1386,7 +1434,7 @@ __global__ __launch_bounds__(256) void k_m_adj_solve(const GroupDesc *__restrict
 
 struct kn_group {
     int nhist = 0, nstream = 0, cap_err = 0;  // nstream 0: merged launches
-    long long nk_tot = 0, y_tot = 0, s_tot = 0, fs_tot = 0, b_tot = 0, fac_tot = 0;
+    long long nk_tot = 0, y_tot = 0, s_tot = 0, fs_tot = 0, b_tot = 0, fac_tot = 0, mu_tot = 0;
     std::vector<GroupHist> h;
     std::vector<HTab> htabs;
     std::vector<double *> hcoef;
@@ -1472,6 +1520,8 @@ void group_meta_upload(kn_group *g)
     std::vector<GroupMeta> m(g->nhist);
     for (int i = 0; i < g->nhist; i++) {
         m[i].off_k = g->h[i].off_k;
+        m[i].off_mu = g->h[i].off_mu;
+        m[i].nparts = adj_parts(g->h[i].na);
         m[i].nk = g->h[i].spec.nk;
         m[i].integ = g->h[i].integ;
         m[i].kBtnu = g->h[i].spec.kBtnu;
@@ -1513,7 +1563,7 @@ void group_plan(kn_group *g)
         d.ivl = g->ivl + (size_t)q.off_k * kLimit * 2;
         d.dbar = g->dbar + q.off_y;
         d.cbar = g->cbar + q.off_y;
-        d.mubar = g->mubar + q.off_k;
+        d.mubar = g->mubar + q.off_mu;
         d.ibar = g->ibar + q.off_k;
         d.mbic = g->mbic + q.off_k;
         d.fac = g->fac + q.off_fac;
@@ -1544,9 +1594,9 @@ void group_plan(kn_group *g)
             cnt[kPhQag] = blocks_for(d.nk);
             cnt[kPhSolve] = q.na > 2 ? spline_blocks(q.na, d.nk) : 0;
             int c = 0;
-            while (c < kAdjClasses - 1 && q.na > (256 << c)) c++;
-            cnt[kPhAdj0 + c] = d.nk + (d.nk + 63) / 64;
-            const size_t b = sizeof(double) * 2 * (size_t)q.na;
+            while (c < kAdjClasses - 1 && adj_plen(q.na) > (128 << c)) c++;
+            cnt[kPhAdj0 + c] = (long long)d.nk * adj_parts(q.na) + (d.nk + 63) / 64;
+            const size_t b = adj_lds_bytes(q.na);
             lds[c] = b > lds[c] ? b : lds[c];
         } else {
             cnt[kPhIc] = (d.nk + 63) / 64;
@@ -1666,9 +1716,11 @@ void group_adjoint_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
     A.ybar = dbar;  // rows written whole: [0, na) the item's terms, [na, na_max) zeros
     A.cbar = d.cbar;
     A.mu_bar = d.mubar;
+    A.nparts = adj_parts(na);
+    A.plen = adj_plen(na);
     A.ica = ica;
-    // nk item blocks, then the initial-condition piece's blocks
-    hipLaunchKernelGGL(k_dnu_adj, dim3(nk + (nk + 63) / 64), dim3(64), sizeof(double) * 2 * (size_t)na, s, A);
+    // nk x parts item blocks, then the initial-condition piece's blocks
+    hipLaunchKernelGGL(k_dnu_adj, dim3(nk * A.nparts + (nk + 63) / 64), dim3(64), adj_lds_bytes(na), s, A);
     if (na > 2)
         hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(na, nk)), dim3(256), 0, s, d.sx, na, nk, (long long)na_max,
                            d.cbar, dbar, d.fac);
@@ -1808,17 +1860,11 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
             kn_set_last_error("kn_group_create: history %d: bad sizes, range or missing hubble", i);
             return KN_ERR_ARG;
         }
-        if (sizeof(double) * 2 * (size_t)s.na_max > 160 * 1024) {
-            kn_set_last_error("kn_group_create: history %d: na_max %d exceeds the adjoint's LDS row budget", i,
-                              s.na_max);
-            return KN_ERR_RANGE;
-        }
     }
     upload_constants();
     kn_group *g = new kn_group();
     g->nhist = nhist;
     g->h.resize(nhist);
-    int na_big = 0;
     for (int i = 0; i < nhist; i++) {
         GroupHist &q = g->h[i];
         q.spec = hist[i];
@@ -1835,12 +1881,13 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
         q.off_b = g->b_tot;
         q.off_fac = g->fac_tot;
         g->fac_tot += (long long)fac_doubles(q.spec.na_max);
+        q.off_mu = g->mu_tot;
+        g->mu_tot += (long long)q.spec.nk * adj_parts(q.spec.na_max);
         g->nk_tot += q.spec.nk;
         g->y_tot += (long long)q.spec.nk * q.spec.na_max;
         g->s_tot += q.spec.na_max;
         g->fs_tot += 16LL * q.spec.na_max;
         g->b_tot += 4LL * q.spec.na_max + 1;
-        na_big = q.spec.na_max > na_big ? q.spec.na_max : na_big;
         const int rc = group_refresh_ic(q);
         if (rc) {
             kn_group_destroy(g);
@@ -1877,8 +1924,9 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
     g->sx = group_alloc<double>(g->s_tot);
     for (double **p : {&g->Y, &g->C, &g->dbar, &g->cbar}) *p = group_alloc<double>(ny);
     for (double **p : {&g->fx, &g->fy, &g->fc}) *p = group_alloc<double>(g->fs_tot);
-    for (double **p : {&g->wavenum, &g->init, &g->out, &g->gbar, &g->ibar, &g->mbic, &g->mubar})
+    for (double **p : {&g->wavenum, &g->init, &g->out, &g->gbar, &g->ibar, &g->mbic})
         *p = group_alloc<double>(nk);
+    g->mubar = group_alloc<double>(g->mu_tot);  // [k][part] per history
     g->mbar = group_alloc<double>(nhist);
     g->fac = group_alloc<double>(g->fac_tot);
     g->ivl = group_alloc<double>(nk * kLimit * 2);
@@ -1903,10 +1951,7 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
     }
     g->meta = group_alloc<GroupMeta>(nhist);
     group_plan(g);
-    if (sizeof(double) * 2 * (size_t)na_big > 64 * 1024)
-        for (const void *k : {(const void *)k_dnu_adj, (const void *)k_m_dnu_adj})
-            KN_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)(sizeof(double) * 2 * (size_t)na_big)));
+
     for (int s = 0; s < kGroupStreams; s++) {
         KN_CHECK(hipStreamCreateWithFlags(&g->st[s], hipStreamNonBlocking));
         KN_CHECK(hipEventCreateWithFlags(&g->join[s], hipEventDisableTiming));

@@ -554,14 +554,14 @@ __device__ __forceinline__ void tri_factor(const double *x, const TriLane &t, do
 }
 
 // Called by all 64 lanes of a wavefront together, with hi - lo <= kKeep and
-// n >= 3, given the lane's factors.
+// n >= 3, given the lane's factors gam and 1 / alp.
 // ADJ_RHS: instead of storing the solution z, add its image under the
 // transpose of the natural-spline right-hand side's y-derivative onto out
 // (the second half of the spline adjoint) for the kept rows, with the
 // reciprocal knot spacings ih (as SplineRhsIh's).
 template <class RHS, bool ADJ_RHS>
 __device__ __forceinline__ void tri_solve(const double *x, int n, const RHS &rhs, int lo, int hi, double *out,
-                                          const TriLane &t, const double (&g)[kRun], const double (&al)[kRun],
+                                          const TriLane &t, const double (&g)[kRun], const double (&ial)[kRun],
                                           double g_before, const double *ih)
 {
     const int lane = threadIdx.x & 63;
@@ -595,7 +595,7 @@ __device__ __forceinline__ void tri_solve(const double *x, int n, const RHS &rhs
         if (j < len) {
             const double gm = s + j == wa ? 0.0 : (j == 0 ? g_before : g[j - 1]);
             z = rv[j] - gm * z;
-            zz[j] = z / al[j];
+            zz[j] = z * ial[j];
         } else {
             zz[j] = 0.0;
         }
@@ -686,12 +686,14 @@ __device__ inline void tridiag_window_wave(const double *x, int n, const RHS &rh
     const TriLane t = tri_lane(n, lo, hi);
     double g[kRun], al[kRun], gb;
     tri_factor(x, t, g, al, gb);
+#pragma unroll
+    for (int j = 0; j < kRun; j++) al[j] = 1.0 / al[j];
     tri_solve<RHS, false>(x, n, rhs, lo, hi, out, t, g, al, gb, nullptr);
 }
 
 // Factors of the knots x[0..n) stored once for every row that shares them:
-// window w at fac + w kFacStride, [j][lane] for gam, then alp, then
-// gam_{s-1} per lane (the same values tri_factor forms in each wavefront).
+// window w at fac + w kFacStride, [j][lane] for gam, then 1 / alp, then
+// gam_{s-1} per lane (from the values tri_factor forms in each wavefront).
 constexpr int kFacStride = (2 * kRun + 1) * 64;
 
 __host__ __device__ inline int fac_windows(int n) { return (n + kKeep - 1) / kKeep; }
@@ -707,7 +709,7 @@ __device__ inline void tri_factor_store(const double *x, int n, int w, double *f
 #pragma unroll
     for (int j = 0; j < kRun; j++) {
         f[j * 64 + lane] = g[j];
-        f[(kRun + j) * 64 + lane] = al[j];
+        f[(kRun + j) * 64 + lane] = 1.0 / al[j];
     }
     f[2 * kRun * 64 + lane] = gb;
 }
@@ -726,13 +728,13 @@ __device__ inline void tridiag_window_wave_fac(const double *x, int n, const RHS
     }
     const TriLane t = tri_lane(n, lo, hi);
     const double *f = fac + (size_t)(lo / kKeep) * kFacStride;
-    double g[kRun], al[kRun];
+    double g[kRun], ial[kRun];
 #pragma unroll
     for (int j = 0; j < kRun; j++) {
         g[j] = f[j * 64 + lane];
-        al[j] = f[(kRun + j) * 64 + lane];
+        ial[j] = f[(kRun + j) * 64 + lane];
     }
-    tri_solve<RHS, ADJ_RHS>(x, n, rhs, lo, hi, out, t, g, al, f[2 * kRun * 64 + lane],
+    tri_solve<RHS, ADJ_RHS>(x, n, rhs, lo, hi, out, t, g, ial, f[2 * kRun * 64 + lane],
                             fac + (size_t)fac_windows(n) * kFacStride);
 }
 
