@@ -299,12 +299,78 @@ __device__ __forceinline__ void qag_sort(QagWs &w, int n, int limit, int &nrmax,
     nrmax = nr;
 }
 
+// qag_sort with nrmax = 0 (QAG never moves it; only QAGS's extrapolation
+// does), by all 64 lanes of the wavefront: the two linear scans become
+// wavefront searches for the first position where the scan would stop, and
+// the shifts become parallel moves (each lane reads its entry before any
+// lane writes; a wavefront's LDS accesses complete in program order).  The
+// list comes out exactly as qag_sort leaves it.  Returns ord[0].
+__device__ __forceinline__ int qag_sort_wave(QagWs &w, int n, int limit)
+{
+    const int lane = threadIdx.x & 63;
+    const int last = n - 1;
+    const int im = w.ord[0];
+    if (last < 2) {
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+            w.ord[0] = 0;
+            w.ord[1] = 1;
+        }
+        return im;
+    }
+    const double emax = w.e[im];
+    __builtin_amdgcn_wave_barrier();  // every lane has read ord[0] before it moves
+    const int top = last < limit / 2 + 2 ? last : limit - last + 1;
+    // i = first p in [1, top) where !(emax < e[ord[p]]), else top
+    int i = top;
+    for (int p0 = 1; p0 < top; p0 += 64) {
+        const int p = p0 + lane;
+        const unsigned long long m = __ballot(p < top && !(emax < w.e[w.ord[p]]));
+        if (m) {
+            i = p0 + __ffsll((long long)m) - 1;
+            break;
+        }
+    }
+    // ord[p - 1] = ord[p] for p in [1, i), then ord[i - 1] = im
+    for (int p0 = 1; p0 < i; p0 += 64) {
+        const int p = p0 + lane;
+        const int v = p < i ? w.ord[p] : 0;
+        __builtin_amdgcn_wave_barrier();
+        if (p < i) w.ord[p - 1] = v;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) w.ord[i - 1] = im;
+    __builtin_amdgcn_wave_barrier();
+    // k = largest p in [i - 1, top - 1] where !(emin >= e[ord[p]]), else i - 2
+    const double emin = w.e[last];
+    int k = i - 2;
+    for (int p0 = top - 1; p0 > i - 2; p0 -= 64) {
+        const int p = p0 - lane;
+        const unsigned long long m = __ballot(p > i - 2 && !(emin >= w.e[w.ord[p]]));
+        if (m) {
+            k = p0 - (__ffsll((long long)m) - 1);
+            break;
+        }
+    }
+    // ord[p + 1] = ord[p] for p in [k + 1, top - 1], then ord[k + 1] = last
+    for (int p0 = top - 1; p0 > k; p0 -= 64) {
+        const int p = p0 - lane;
+        const int v = p > k ? w.ord[p] : 0;
+        __builtin_amdgcn_wave_barrier();
+        if (p > k) w.ord[p + 1] = v;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) w.ord[k + 1] = last;
+    __builtin_amdgcn_wave_barrier();
+    return w.ord[0];
+}
+
 // QAG (QUADPACK dqage semantics, GSL's qag.c control flow) by one wavefront.
 // Returns 0 on success or the GSL errno (11 iteration limit, 18 roundoff,
 // 21 bad integrand); *result is the sum over the final subintervals.  Every
-// lane keeps the same scalar state; lane 0 alone writes the interval list and
-// keeps it ordered, and the shuffles that broadcast its choice (imax, nrmax)
-// order those writes before the next reads by the other lanes.
+// lane keeps the same scalar state; lane 0 alone writes the intervals, and
+// all lanes keep the error order together (qag_sort_wave), whose wavefront
+// operations order lane 0's writes before the other lanes' reads.
 template <class F>
 __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int limit, QagWs &w, double *result,
                           int *nint)
@@ -322,7 +388,7 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
         w.ord[0] = 0;
         w.lev[0] = 0;
     }
-    int imax = __shfl(0, 0, 64), nrmax = 0;
+    int imax = __shfl(0, 0, 64);
     double tol = epsrel * fabs(g0.res);
     if (g0.err <= 50 * DBL_EPSILON * g0.resabs && g0.err > tol) return 18;
     if ((g0.err <= tol && g0.err != g0.resasc) || g0.err == 0.0) return 0;
@@ -369,11 +435,9 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
             }
             w.lev[im] = lv;
             w.lev[n] = lv;
-            qag_sort(w, n + 1, limit, nrmax, imax);
         }
+        imax = qag_sort_wave(w, n + 1, limit);
         n++;
-        imax = __shfl(imax, 0, 64);
-        nrmax = __shfl(nrmax, 0, 64);
         it++;
     } while (it < limit && !etype && errsum > tol);
     double s = 0.0;

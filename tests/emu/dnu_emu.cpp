@@ -85,3 +85,42 @@ extern "C" int emu_fs_two_pass_check(double (*hub)(double, void *), void *ctx, d
                        gx.data(), full, status.data());
     return listed;
 }
+
+// ---- test hook: the wavefront error-order update against QUADPACK's serial one ----
+namespace {
+__global__ __launch_bounds__(64) void k_qag_sort_test(const double *errs, int nsteps, int limit, int *bad)
+{
+    __shared__ QagWs w1, w2;  // serial (lane 0) and wavefront
+    const int lane = threadIdx.x & 63;
+    if (lane == 0) {
+        w1.e[0] = w2.e[0] = 1.0;
+        w1.ord[0] = w2.ord[0] = 0;
+    }
+    int nrmax = 0, imax1 = 0, imax2 = __shfl(0, 0, 64);
+    for (int t = 0; t < nsteps && t + 1 < limit; t++) {
+        const int n = t + 1;  // intervals before this bisection
+        if (lane == 0) {
+            w1.e[imax1] = errs[2 * t];
+            w1.e[n] = errs[2 * t + 1];
+            w2.e[imax2] = errs[2 * t];
+            w2.e[n] = errs[2 * t + 1];
+            qag_sort(w1, n + 1, limit, nrmax, imax1);
+        }
+        imax2 = qag_sort_wave(w2, n + 1, limit);
+        imax1 = __shfl(imax1, 0, 64);
+        if (lane == 0) {
+            int diff = imax1 != imax2;
+            for (int q = 0; q <= n; q++) diff |= w1.ord[q] != w2.ord[q];
+            bad[t] = diff;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+}  // namespace
+
+// Runs nsteps bisections whose two new error estimates are errs[2t], errs[2t+1]
+// through qag_sort and qag_sort_wave; bad[t] = 1 where their lists differ.
+extern "C" void emu_qag_sort_check(const double *errs, int nsteps, int limit, int *bad)
+{
+    hipLaunchKernelGGL(k_qag_sort_test, dim3(1), dim3(64), 0, 0, errs, nsteps, limit, bad);
+}

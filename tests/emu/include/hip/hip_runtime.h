@@ -101,7 +101,20 @@ inline double __shfl_down(double v, unsigned d, int = 64)
     const int l = (int)(emu_tid & 63);
     return __shfl(v, l + (int)d < 64 ? l + (int)d : l);
 }
-inline void __builtin_amdgcn_wave_barrier() {}
+// a wavefront sync point here (every lane arrives before any goes on)
+inline void __builtin_amdgcn_wave_barrier() { emu_yield(); }
+inline unsigned long long __ballot(int pred)
+{
+    EmuWaveSched &S = *emu_wave;
+    S.x[S.cur] = pred ? 1 : 0;
+    emu_yield();  // every lane has written
+    unsigned long long m = 0;
+    for (int l = 0; l < 64; l++)
+        if (S.x[l]) m |= 1ull << l;
+    emu_yield();  // every lane has read
+    return m;
+}
+inline int __ffsll(long long v) { return __builtin_ffsll(v); }
 inline void __syncthreads()
 {
     if (emu_bdim > 64) {

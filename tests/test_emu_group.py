@@ -250,3 +250,25 @@ def test_emu_group_set_streams_checks(emu):
     assert emu.kn_group_set_streams(g, 8) == 0 and emu.kn_group_streams(g) == 8
     emu.kn_group_destroy(g)
     c.free()
+
+
+@pytest.mark.parametrize("limit", [200, 31, 8])
+def test_emu_wavefront_error_order_equals_quadpack(limit):
+    """QAG's error-ordered interval list kept by all 64 lanes (searches by
+    ballot, parallel shifts) against QUADPACK's serial dqpsrt (qag_sort), over
+    199 bisections whose new error estimates decrease with noise and repeat
+    exactly (ties), so both scans stop at equal keys; every list and every
+    choice of the next interval identical.  limit 200 reaches the branch that
+    maintains fewer than all entries (last >= limit / 2 + 2)."""
+    lib = C.CDLL(DNU_EMU_LIB)
+    lib.emu_qag_sort_check.restype = None
+    lib.emu_qag_sort_check.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(C.c_int)]
+    rng = np.random.default_rng(limit)
+    nsteps = 199
+    levels = np.array([1.0, 0.5, 0.5, 0.25, 0.125, 0.0625])
+    errs = rng.choice(levels, 2 * nsteps) * 0.97 ** np.repeat(np.arange(nsteps), 2)
+    errs[::7] = errs[1::7][:len(errs[::7])]  # more exact repeats
+    e = (C.c_double * len(errs))(*errs)
+    bad = (C.c_int * nsteps)()
+    lib.emu_qag_sort_check(e, nsteps, limit, bad)
+    assert sum(bad[:min(nsteps, limit - 1)]) == 0
