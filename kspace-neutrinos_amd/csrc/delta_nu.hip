@@ -350,19 +350,36 @@ struct QagArgs {
     IcArgs ic;
 };
 
-// Item `item` (species sp, wavenumber k) by one wavefront.
-__device__ inline void dnu_qag_item(const QagArgs &a, const SpeciesDev &sp, int item, int k, QagWs &ws)
+// Histories of at most this many stored steps have their log a values copied
+// to LDS by the QAG kernels (48 KB; a block's four wavefronts share the copy,
+// which every wavefront writes whole before it reads, so no block barrier).
+constexpr int kStageMax = 6144;
+
+// Item `item` (species sp, wavenumber k) by one wavefront; sxs: LDS for the
+// stored log a values (na <= kStageMax), or null to read them from memory.
+__device__ inline void dnu_qag_item(const QagArgs &a, const SpeciesDev &sp, int item, int k, QagWs &ws,
+                                    double *sxs)
 {
     const int lane = threadIdx.x & 63;
+    const double *sx = a.sx;
+    if (sxs) {
+        for (int t = lane; t < a.ns; t += 64) sxs[t] = a.sx[t];
+        __builtin_amdgcn_wave_barrier();  // every lane's entries written before any lane reads
+        sx = sxs;
+    }
     DnuIntegrand f;
     f.H = a.H;
-    f.fx = a.fx;
     f.fy = a.fy;
     f.fc = a.fc;
+    f.fk.x0 = a.x0;  // the fslength knots' own expression (fs_first_wave)
+    f.fk.span = a.xa - a.x0;
+    f.fk.den = a.nfs - 1.;
     f.nfs = a.nfs;
     f.fx0 = a.fx[0];
+    f.fxhi = a.fx[a.nfs - 1];
     f.finv_h = (a.nfs - 1) / (a.fx[a.nfs - 1] - a.fx[0]);
-    f.sx = a.sx;
+    f.sx = sx;
+    f.sxhi = a.sx[a.ns - 1];
     f.sy = a.Y + (size_t)k * a.ystride;
     f.sc = a.C + (size_t)k * a.ystride;
     f.sbucket = a.sb;
@@ -395,10 +412,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_dnu_qag(QagArgs a, cons
                                                                   int nitems)
 {
     __shared__ QagWs ws[kWavesPerBlock];
+    HIP_DYNAMIC_SHARED(double, sxs)  // a.ns doubles when a.ns <= kStageMax
     const int wave = threadIdx.x >> 6;
     const int item = blockIdx.x * kWavesPerBlock + wave;
     if (item >= nitems) return;
-    dnu_qag_item(a, sp[item_species[item / nk]], item, item % nk, ws[wave]);
+    dnu_qag_item(a, sp[item_species[item / nk]], item, item % nk, ws[wave], a.ns <= kStageMax ? sxs : nullptr);
 }
 
 // Adjoint of the initial-condition piece at k: dL/d delta_nu_init[k] and the
@@ -434,6 +452,7 @@ struct AdjArgs {
     HTab H;
     const double *fx, *fy, *fc;
     int nfs;
+    double x0, xa;  // the integration's limits, which set the fslength knots
     const double *sx, *Y, *C;
     const int *sb;
     int ns, nsb, nk;
@@ -514,6 +533,10 @@ __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
     const double kk = A.wavenum[k], mnubykT = A.sp->mnubykT, gb = A.gbar[item];
     const JPar &J = A.sp->J;
     const double fx0 = fx[0], finv_h = (nfs - 1) / (fx[nfs - 1] - fx[0]), sx0 = sx[0];
+    UniformKnots fk;  // the fslength knots (fs_first_wave's expression)
+    fk.x0 = A.x0;
+    fk.span = A.xa - A.x0;
+    fk.den = nfs - 1.;
     double mu_acc = 0.0;
     for (int t0 = 0; t0 < nsel; t0 += 2) {
         const int ts = t0 + hf;
@@ -526,8 +549,9 @@ __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
                 const double x = j < 30 ? (side ? c + hl * c_xgk[j] : c - hl * c_xgk[j]) : c;
                 const double w = c_wgk[j] * hl * prefac * gb;
                 // the pieces of delta_tot_table.c:492-500
-                const int fi = settle(fx, nfs, (int)((x - fx0) * finv_h), x);
-                const double fsl = cspline_at(fx, fy, fc, fi, x);
+                double xl, xr;
+                const int fi = settle_uniform(fk, nfs, (int)((x - fx0) * finv_h), x, xl, xr);
+                const double fsl = cspline_val(xl, xr, fy[fi], fy[fi + 1], fc[fi], fc[fi + 1], x);
                 int q = (int)((x - sx0) * sinv_bw);
                 q = q < 0 ? 0 : (q > nsb ? nsb : q);
                 const int si = settle(sx, ns, sb[q], x);
@@ -990,8 +1014,8 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     qa.ivl = record ? dev->ivl : nullptr;
     qa.ic.init = nullptr;
     qa.ic.fsl0 = qa.ic.lin = qa.ic.prefac = 0.0;
-    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nitems)), dim3(64 * kWavesPerBlock), 0, 0, qa, dev->sp,
-                       dev->item_species, nk, nitems);
+    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nitems)), dim3(64 * kWavesPerBlock),
+                       Na <= kStageMax ? sizeof(double) * Na : 0, 0, qa, dev->sp, dev->item_species, nk, nitems);
     KN_CHECK(hipGetLastError());
     KN_CHECK(hipMemcpyAsync(res, dev->out, sizeof(double) * nitems, hipMemcpyDeviceToHost, 0));
     KN_CHECK(hipMemcpyAsync(nint, dev->nint, sizeof(int) * nitems, hipMemcpyDeviceToHost, 0));
@@ -1075,6 +1099,8 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
         A.fy = dev->fy;
         A.fc = dev->fc;
         A.nfs = nfs;
+        A.x0 = log(d->TimeTransfer);
+        A.xa = log(a);
         A.sx = dev->sx;
         A.Y = dev->Y;
         A.C = dev->C;
@@ -1311,6 +1337,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_m_dnu_qag(const GroupDe
                                                                     const int *__restrict__ pre, int nhist)
 {
     __shared__ QagWs ws[kWavesPerBlock];
+    HIP_DYNAMIC_SHARED(double, sxs)  // min(largest na, kStageMax) doubles
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
     const int wave = threadIdx.x >> 6;
@@ -1341,7 +1368,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_m_dnu_qag(const GroupDe
     a.ic.fsl0 = d.fsl0;
     a.ic.lin = d.lin;
     a.ic.prefac = d.prefac;
-    dnu_qag_item(a, *d.sp, k, k, ws[wave]);
+    dnu_qag_item(a, *d.sp, k, k, ws[wave], d.na <= kStageMax ? sxs : nullptr);
 }
 
 // initial-condition-only histories, 64 k per block
@@ -1397,6 +1424,8 @@ __global__ __launch_bounds__(64) void k_m_dnu_adj(const GroupDesc *__restrict__ 
     A.fy = d.fy;
     A.fc = d.fc;
     A.nfs = d.nfs;
+    A.x0 = d.x0;
+    A.xa = d.xa;
     A.sx = d.sx;
     A.Y = d.Y;
     A.C = d.C;
@@ -1450,6 +1479,7 @@ struct kn_group {
     int *pre = nullptr;      // kPhases x (nhist + 1)
     long long *mlist = nullptr;  // pass-2 fslength list (fs_tot entries)
     std::vector<GroupDesc> desc_h;
+    size_t qag_lds = 0;  // staged log a values of the longest stageable history
     int tot[kPhases] = {};
     size_t adj_lds[kAdjClasses] = {};
     hipStream_t st[kGroupStreams] = {};
@@ -1608,6 +1638,10 @@ void group_plan(kn_group *g)
         }
     }
     for (int p = 0; p < kPhases; p++) g->tot[p] = (int)acc[p];
+    g->qag_lds = 0;
+    for (int i = 0; i < n; i++)
+        if (g->h[i].integ && g->h[i].na <= kStageMax && sizeof(double) * g->h[i].na > g->qag_lds)
+            g->qag_lds = sizeof(double) * g->h[i].na;
     for (int c = 0; c < kAdjClasses; c++) g->adj_lds[c] = lds[c];
     KN_CHECK(hipMemcpy(g->desc, g->desc_h.data(), sizeof(GroupDesc) * n, hipMemcpyHostToDevice));
     KN_CHECK(hipMemcpy(g->pre, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice));
@@ -1668,8 +1702,8 @@ void group_forward_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
     a.ic.fsl0 = q.fsl0;
     a.ic.lin = q.lin;
     a.ic.prefac = q.spec.delta_nu_prefac;
-    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nk)), dim3(64 * kWavesPerBlock), 0, s, a, g->sp + i, g->spec0, nk,
-                       nk);
+    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nk)), dim3(64 * kWavesPerBlock),
+                       na <= kStageMax ? sizeof(double) * na : 0, s, a, g->sp + i, g->spec0, nk, nk);
 }
 
 // Adjoint of one history's last forward on stream s.
@@ -1697,6 +1731,8 @@ void group_adjoint_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
     A.fy = d.fy;
     A.fc = d.fc;
     A.nfs = d.nfs;
+    A.x0 = d.x0;
+    A.xa = d.xa;
     A.sx = d.sx;
     A.Y = d.Y;
     A.C = d.C;
@@ -1745,8 +1781,8 @@ void group_launch_merged(kn_group *g, hipStream_t s, bool forward, bool adjoint)
         if (g->tot[kPhFsSpline])
             hipLaunchKernelGGL(k_m_fs_spline, dim3(g->tot[kPhFsSpline]), dim3(256), 0, s, g->desc, pre(kPhFsSpline), n);
         if (g->tot[kPhQag])
-            hipLaunchKernelGGL(k_m_dnu_qag, dim3(g->tot[kPhQag]), dim3(64 * kWavesPerBlock), 0, s, g->desc,
-                               pre(kPhQag), n);
+            hipLaunchKernelGGL(k_m_dnu_qag, dim3(g->tot[kPhQag]), dim3(64 * kWavesPerBlock), g->qag_lds, s,
+                               g->desc, pre(kPhQag), n);
         if (g->tot[kPhIc]) hipLaunchKernelGGL(k_m_ic, dim3(g->tot[kPhIc]), dim3(64), 0, s, g->desc, pre(kPhIc), n);
     }
     if (adjoint) {

@@ -91,15 +91,20 @@ __device__ __forceinline__ double specialJ(double x, const JPar &P)
 }
 
 // ---- natural cubic spline evaluation, GSL cspline/linear semantics ----
-__device__ __forceinline__ double cspline_at(const double *xa, const double *ya, const double *c, int i, double x)
+__device__ __forceinline__ double cspline_val(double xl, double xr, double yl, double yr, double ci, double ci1,
+                                              double x)
 {
-    const double xl = xa[i], h = xa[i + 1] - xl;
-    const double yl = ya[i], dy = ya[i + 1] - yl;
+    const double h = xr - xl;
+    const double dy = yr - yl;
     const double dx = x - xl;
-    const double ci = c[i], ci1 = c[i + 1];
     const double b = (dy / h) - h * (ci1 + 2.0 * ci) / 3.0;
     const double d = (ci1 - ci) / (3.0 * h);
     return yl + dx * (b + dx * (ci + dx * d));
+}
+
+__device__ __forceinline__ double cspline_at(const double *xa, const double *ya, const double *c, int i, double x)
+{
+    return cspline_val(xa[i], xa[i + 1], ya[i], ya[i + 1], c[i], c[i + 1], x);
 }
 
 __device__ __forceinline__ double linear_at(const double *xa, const double *ya, int i, double x)
@@ -107,6 +112,28 @@ __device__ __forceinline__ double linear_at(const double *xa, const double *ya, 
     const double xl = xa[i], h = xa[i + 1] - xl;
     if (h > 0.0) return ya[i] + (x - xl) / h * (ya[i + 1] - ya[i]);
     return ya[i];
+}
+
+// The fslength table's knots are x0 + i (xa - x0) / (n - 1.) (delta_tot_table.c:582),
+// the expression the fslength kernels store them with: knot i recomputed here
+// equals the stored one, so the spline is evaluated without loading knots.
+struct UniformKnots {
+    double x0, span, den;  // x0, xa - x0, n - 1.
+    __device__ __forceinline__ double at(int i) const { return x0 + i * span / den; }
+};
+
+// settle() on those knots: i and its interval [xl, xr]
+__device__ __forceinline__ int settle_uniform(const UniformKnots &u, int n, int i, double x, double &xl, double &xr)
+{
+    i = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
+    xl = u.at(i);
+    while (i > 0 && xl > x) xl = u.at(--i);
+    xr = u.at(i + 1);
+    while (i < n - 2 && xr <= x) {
+        xl = xr;
+        xr = u.at(++i + 1);
+    }
+    return i;
 }
 
 // Largest i <= n-2 with xa[i] <= x, starting from a guess (GSL bsearch result).
@@ -126,24 +153,26 @@ struct FslIntegrand {  // delta_tot_table.c:378-383: 1 / a / (a H(a)) = exp(-2 l
 
 struct DnuIntegrand {  // delta_tot_table.c:492-500
     HTab H;
-    const double *fx, *fy, *fc;  // fslength spline, uniform nodes
+    const double *fy, *fc;  // fslength spline on the uniform knots fk
+    UniformKnots fk;
     int nfs;
-    double fx0, finv_h;
+    double fx0, fxhi, finv_h;    // first and last knot, 1 / spacing
     const double *sx, *sy, *sc;  // delta_tot spline of this k
     const int *sbucket;
     int ns, nsb, linear;
-    double sx0, sinv_bw;
+    double sx0, sxhi, sinv_bw;   // first and last stored step
     double k, mnubykT;
     const JPar *J;  // in device memory: the series weights are read where used
     int *edom;
     __device__ __forceinline__ double operator()(double x) const
     {
-        if (x < fx[0] || x > fx[nfs - 1] || x < sx[0] || x > sx[ns - 1]) {
+        if (x < fx0 || x > fxhi || x < sx0 || x > sxhi) {
             atomicOr(edom, 1);  // GSL_EDOM in gsl_interp_eval -> terminate(2001)
             return 0.0;
         }
-        const int fi = settle(fx, nfs, (int)((x - fx0) * finv_h), x);
-        const double fsl = cspline_at(fx, fy, fc, fi, x);
+        double xl, xr;
+        const int fi = settle_uniform(fk, nfs, (int)((x - fx0) * finv_h), x, xl, xr);
+        const double fsl = cspline_val(xl, xr, fy[fi], fy[fi + 1], fc[fi], fc[fi + 1], x);
         int q = (int)((x - sx0) * sinv_bw);
         q = q < 0 ? 0 : (q > nsb ? nsb : q);
         const int si = settle(sx, ns, sbucket[q], x);
