@@ -955,13 +955,20 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, 0, dev->fx, dev->fy, dev->fc, nfs, 1, 0LL,
                        0LL);
     KN_CHECK(hipGetLastError());
-    // meanwhile the history rows are packed to stride Na and uploaded on the copy stream
+    // meanwhile the history rows are packed to stride Na and uploaded on the
+    // copy stream, in chunks: a chunk's upload runs while the next is packed
     {
-        const int parts = nY < (1u << 16) ? 1 : 32;
-        host_pool().run(parts, [&](int p) {
-            const int k0 = (int)((long long)nk * p / parts), k1 = (int)((long long)nk * (p + 1) / parts);
-            for (int k = k0; k < k1; k++) memcpy(stY + (size_t)k * Na, d->delta_tot[k], sizeof(double) * Na);
-        });
+        const int chunks = nY < (1u << 16) ? 1 : 4, parts = nY < (1u << 16) ? 1 : 8;
+        for (int c = 0; c < chunks; c++) {
+            const int c0 = (int)((long long)nk * c / chunks), c1 = (int)((long long)nk * (c + 1) / chunks);
+            host_pool().run(parts, [&](int p) {
+                const int k0 = c0 + (int)((long long)(c1 - c0) * p / parts);
+                const int k1 = c0 + (int)((long long)(c1 - c0) * (p + 1) / parts);
+                for (int k = k0; k < k1; k++) memcpy(stY + (size_t)k * Na, d->delta_tot[k], sizeof(double) * Na);
+            });
+            KN_CHECK(hipMemcpyAsync(dev->Y + (size_t)c0 * Na, stY + (size_t)c0 * Na, sizeof(double) * (c1 - c0) * Na,
+                                    hipMemcpyHostToDevice, g_copy));
+        }
     }
     memcpy(stX, d->scalefact, sizeof(double) * Na);
     memcpy(stK, wavenum, sizeof(double) * nk);
@@ -975,7 +982,6 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
         for (int n = 1; n < 20; n++) sd[i].J.en[n] = exp(-n * q.qc);
         ispec[i] = (int)i;
     }
-    KN_CHECK(hipMemcpyAsync(dev->Y, stY, sizeof(double) * nY, hipMemcpyHostToDevice, g_copy));
     KN_CHECK(hipMemcpyAsync(dev->sx, stX, sizeof(double) * Na, hipMemcpyHostToDevice, g_copy));
     KN_CHECK(hipMemcpyAsync(dev->wavenum, stK, sizeof(double) * nk, hipMemcpyHostToDevice, g_copy));
     KN_CHECK(hipMemcpyAsync(dev->sp, sd, sizeof(SpeciesDev) * need.size(), hipMemcpyHostToDevice, g_copy));
