@@ -1140,14 +1140,24 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
         const size_t nmu = (size_t)nk * P;
         double *mub = (double *)pinned_stage(sizeof(double) * (nmu + (dtot_bar ? ny : 0))), *yb = mub + nmu;
         KN_CHECK(hipMemcpyAsync(mub, dev->mubar, sizeof(double) * nmu, hipMemcpyDeviceToHost, 0));
-        if (dtot_bar) KN_CHECK(hipMemcpyAsync(yb, dev->ybar, sizeof(double) * ny, hipMemcpyDeviceToHost, 0));
-        KN_CHECK(hipStreamSynchronize(0));
+        // dL/d delta_tot comes back in chunks: the host copies one out while the next is in flight
+        static hipEvent_t done[4];
+        const int chunks = dtot_bar ? (ny < (1u << 16) ? 1 : 4) : 0;
+        for (int c = 0; c < chunks; c++) {
+            if (!done[c]) KN_CHECK(hipEventCreateWithFlags(&done[c], hipEventDisableTiming));
+            const size_t i0 = ny * c / chunks, i1 = ny * (c + 1) / chunks;
+            KN_CHECK(hipMemcpyAsync(yb + i0, dev->ybar + i0, sizeof(double) * (i1 - i0), hipMemcpyDeviceToHost, 0));
+            KN_CHECK(hipEventRecord(done[c], 0));
+        }
+        KN_CHECK(chunks ? hipEventSynchronize(done[0]) : hipStreamSynchronize(0));  // mu_bar went first
         for (size_t i = 0; i < nmu; i++) mb += mub[i];  // k-major, parts in order
-        if (dtot_bar) {
-            const int parts = ny < (1u << 16) ? 1 : 32;
+        for (int c = 0; c < chunks; c++) {
+            KN_CHECK(hipEventSynchronize(done[c]));
+            const size_t i0 = ny * c / chunks, i1 = ny * (c + 1) / chunks;
+            const int parts = chunks > 1 ? 8 : 1;
             host_pool().run(parts, [&](int p) {
-                const size_t i0 = ny * p / parts, i1 = ny * (p + 1) / parts;
-                memcpy(dtot_bar + i0, yb + i0, sizeof(double) * (i1 - i0));
+                const size_t j0 = i0 + (i1 - i0) * p / parts, j1 = i0 + (i1 - i0) * (p + 1) / parts;
+                memcpy(dtot_bar + j0, yb + j0, sizeof(double) * (j1 - j0));
             });
         }
     }

@@ -49,6 +49,7 @@ def dnu_emu(kn):
     dict(name="noisy", masses=[0.3, 0, 0], Na=150, nk=6, amax=1.0, hybrid=None, noise=0.1),
     dict(name="massless", masses=[0.15, 0, 0], Na=20, nk=8, amax=0.2, hybrid=None, m=0.0),
     dict(name="long Na=2300 (3 adjoint parts)", masses=[0.2, 0, 0], Na=2300, nk=3, amax=0.9, hybrid=None),
+    dict(name="Na=1100 nk=64 (chunked transfers)", masses=[0.2, 0, 0], Na=1100, nk=64, amax=0.9, hybrid=None),
 ], ids=lambda c: c["name"])
 def test_emu_plain_adjoint_matches_oracle(dnu_emu, case):
     """The per-call path (csrc/delta_nu.hip: k_dnu_qag with recorded

@@ -199,6 +199,8 @@ def test_reproduce_linear_theory_on_gpu(kn, torch_cuda):
     dict(name="hybrid", masses=[0.3, 0.06, 0.06], Na=80, nk=8, amax=0.9, hybrid=(500.0, 0.3333)),
     dict(name="noisy", masses=[0.3, 0, 0], Na=150, nk=6, amax=1.0, hybrid=None, noise=0.1),
     dict(name="C1-like", masses=[0.15, 0, 0], Na=100, nk=64, amax=0.5, hybrid=None),
+    # 76800 row entries: chunked upload and download, two knot parts per item
+    dict(name="long Na=1200 nk=64", masses=[0.2, 0, 0], Na=1200, nk=64, amax=0.9, hybrid=None),
 ], ids=lambda c: c["name"])
 def test_get_delta_nu_adjoint_per_call_matches_oracle(kn, torch_cuda, case):
     """kn_get_delta_nu_adjoint (csrc/delta_nu.hip) against the oracle's adjoint:
