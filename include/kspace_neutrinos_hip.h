@@ -160,6 +160,12 @@ double fslength(const double logai, const double logaf, const double light);
 /* delta_tot_table.h:155 */
 double get_delta_tot(const double delta_nu_curr, const double delta_cdm_curr, const double OmegaNua3,
                      const double Omeganonu, const double Omeganu1, const double partnu);
+/* delta_tot_table.h:121-135: state files in the reference's text formats (host only).
+ * A NULL file name means "delta_tot_nu.txt" in the working directory. */
+void save_delta_tot(const _delta_tot_table *const d_tot, const int iia, char *savedir);
+void save_all_nu_state(const _delta_tot_table *const d_tot, char *savedir);
+int save_nu_power(const _delta_tot_table *const d_tot, const double Time, const int snapnum, const char *OutputDir);
+void read_all_nu_state(_delta_tot_table *const d_tot, char *savedir);
 
 /* ---------------------------------------- Omega_nu (host, no kernel) */
 
@@ -214,6 +220,9 @@ void allocate_kspace_memory(const int nk_in, const int ThisTask, const double Bo
 /* interface_common.h:80 (code-agnostic entry: pre-binned P(k)) */
 _delta_pow compute_neutrino_power_from_cdm(const double Time, const double keff_in[], const double P_cdm[],
                                            const long int Nmodes[], const int nk_in, MPI_Comm MYMPI_COMM_WORLD);
+/* interface_common.h:86, :117: state and P_nu(k) files of the module table */
+void save_nu_state(char *savedir);
+int save_neutrino_power(const double Time, const int snapnum, const char *OutputDir);
 /* interface_common.h:97, :108 */
 void get_nu_state(double **scalefact, double **delta_tot, size_t *nk, size_t *ia);
 void set_nu_state(double *scalefact, double *delta_tot, const size_t nk, const size_t ia, MPI_Comm MYMPI_COMM_WORLD);

@@ -11,6 +11,7 @@
  * (mass, hybrid cut, tolerance) exactly as the reference derives them.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -260,4 +261,88 @@ void get_delta_nu_update(_delta_tot_table *const d, const double a, const int nk
                          delta_cdm_curr[k], keff[k]);
         if (delta_nu_curr[k] < 0) delta_nu_curr[k] = 0;
     }
+}
+
+/* ---- state files (delta_tot_table.c:254-374 of the reference): the same text
+ * formats, so a run resumes from files the reference wrote and the reference
+ * resumes from files written here.  One line per stored step:
+ * "# <log a> <delta_tot[0]> ... <delta_tot[nk-1]>" (%le, 7 significant digits). */
+
+static const char *state_path(const char *f) { return f ? f : "delta_tot_nu.txt"; }
+
+void read_all_nu_state(_delta_tot_table *const d, char *savefile)
+{
+    const char *path = state_path(savefile);
+    FILE *fd = fopen(path, "r");
+    int row;
+    if (!fd) return; /* nothing to resume */
+    for (row = 0; row < d->namax; row++) {
+        double la;
+        int k;
+        if (fscanf(fd, "# %lg ", &la) != 1) break;
+        d->scalefact[row] = la;
+        for (k = 0; k < d->nk; k++) {
+            if (fscanf(fd, "%lg ", &d->delta_tot[k][row]) == 1) continue;
+            /* a short first line sets nk; a short later line is an error */
+            if (row != 0)
+                kn_terminate(2006, "Expected %d k values, got %d for delta_tot in %s; a=%g\n", d->nk, k, path,
+                             exp(d->scalefact[row]));
+            d->nk = k;
+            break;
+        }
+    }
+    /* the stored history must start at the transfer function's time */
+    if (fabs(d->scalefact[0] - log(d->TimeTransfer)) > 1e-4)
+        kn_terminate(2007, "%s starts with a=%g, transfer function is at a=%g\n", path, exp(d->scalefact[0]),
+                     d->TimeTransfer);
+    if (row > 0) d->ia = row;
+    if (d->debug) kn_message(1, "Read %d stored power spectra from %s\n", row, path);
+    fclose(fd);
+}
+
+void save_delta_tot(const _delta_tot_table *const d, const int iia, char *savefile)
+{
+    const char *path = state_path(savefile);
+    FILE *fd = fopen(path, "a");
+    int k;
+    if (!fd) kn_terminate(2012, "Could not open %s for writing!\n", path);
+    fprintf(fd, "# %le ", d->scalefact[iia]);
+    for (k = 0; k < d->nk; k++) fprintf(fd, "%le ", d->delta_tot[k][iia]);
+    fprintf(fd, "\n");
+    fclose(fd);
+}
+
+void save_all_nu_state(const _delta_tot_table *const d, char *savefile)
+{
+    const char *path = state_path(savefile);
+    FILE *old = fopen(path, "r");
+    int row;
+    if (old) { /* keep the previous file as <name>.bak */
+        const size_t n = strlen(path);
+        char *bak = kn_malloc("kspace_state_bak", n + 5);
+        fclose(old);
+        memcpy(bak, path, n);
+        memcpy(bak + n, ".bak", 5);
+        rename(path, bak);
+        kn_free(bak);
+    }
+    for (row = 0; row < d->ia; row++) save_delta_tot(d, row, (char *)path);
+}
+
+int save_nu_power(const _delta_tot_table *const d, const double Time, const int snapnum, const char *OutputDir)
+{
+    char path[1000];
+    FILE *fd;
+    int k;
+    snprintf(path, sizeof path, "%s/powerspec_nu_%03d.txt", OutputDir, snapnum);
+    if (!(fd = fopen(path, "w"))) {
+        fprintf(stderr, "can't open file `%s` for writing\n", path);
+        return -1;
+    }
+    fprintf(fd, "# k P_nu(k)\n");
+    fprintf(fd, "# a = %g\n", Time);
+    fprintf(fd, "# nbins = %d\n", d->nk);
+    for (k = 0; k < d->nk; k++) fprintf(fd, "%g %g\n", d->wavenum[k], d->delta_nu_last[k] * d->delta_nu_last[k]);
+    fclose(fd);
+    return 0;
 }

@@ -46,7 +46,6 @@ void allocate_kspace_memory(const int nk_in, const int ThisTask, const double Bo
                             const double UnitLength_in_cm, const double Omega0, char *snapdir, const double TimeMax,
                             MPI_Comm comm)
 {
-    (void)snapdir;
     (void)comm;
     if (kspace_params.hybrid_neutrinos_on)
         init_hybrid_nu(&omeganu_table.hybnu, kspace_params.MNu, kspace_params.vcrit, KN_LIGHTCGS / 1e5,
@@ -56,7 +55,21 @@ void allocate_kspace_memory(const int nk_in, const int ThisTask, const double Bo
     delta_tot_table.ThisTask = ThisTask;
     allocate_delta_tot_table(&delta_tot_table, nk_in, kspace_params.TimeTransfer, TimeMax, Omega0, &omeganu_table,
                              UnitTime_in_s, UnitLength_in_cm, 0);
+    /* resume from a saved history when the caller names one (interface_common.c:93-95) */
+    if (ThisTask == 0 && snapdir != NULL) read_all_nu_state(&delta_tot_table, snapdir);
     delta_cdm_curr = kn_malloc("temp_power_spectrum", 3 * (size_t)nk_in * sizeof(double));
+}
+
+/* interface_common.c:41-52: the module table's state and P_nu(k) files (rank 0 writes) */
+void save_nu_state(char *savefile)
+{
+    if (delta_tot_table.ThisTask == 0) save_all_nu_state(&delta_tot_table, savefile);
+}
+
+int save_neutrino_power(const double Time, const int snapnum, const char *OutputDir)
+{
+    if (delta_tot_table.ThisTask != 0) return 0;
+    return save_nu_power(&delta_tot_table, Time, snapnum, OutputDir);
 }
 
 /* interface_common.c:125-148 */

@@ -103,6 +103,12 @@ SIGNATURES = {
     "compute_neutrino_power_from_cdm": (DeltaPow, [C.c_double, dp, dp, C.POINTER(C.c_long), C.c_int, C.c_int]),
     "get_nu_state": (None, [C.POINTER(dp), C.POINTER(dp), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "set_nu_state": (None, [dp, dp, C.c_size_t, C.c_size_t, C.c_int]),
+    "save_delta_tot": (None, [C.POINTER(DeltaTotTable), C.c_int, C.c_char_p]),
+    "save_all_nu_state": (None, [C.POINTER(DeltaTotTable), C.c_char_p]),
+    "save_nu_power": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, C.c_int, C.c_char_p]),
+    "read_all_nu_state": (None, [C.POINTER(DeltaTotTable), C.c_char_p]),
+    "save_nu_state": (None, [C.c_char_p]),
+    "save_neutrino_power": (C.c_int, [C.c_double, C.c_int, C.c_char_p]),
     "particle_nu_active": (C.c_int, [C.c_double]),
     "add_nu_power_to_rhogrid": (None, [C.c_double, C.c_double, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "kn_set_hubble_function": (None, [C.c_void_p]),
