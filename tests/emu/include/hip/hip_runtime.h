@@ -4,7 +4,8 @@
 // source with g++ and runs every kernel on the host.  Each lane is a fiber
 // (its own stack, switched by emu_ctx_switch on one host thread); a wavefront
 // runs its 64 lanes round-robin up to the next wavefront operation
-// (__shfl, __shfl_xor, readfirstlane, __syncthreads), where every lane must
+// (__shfl and its variants, __ballot, readfirstlane, wave_barrier,
+// __syncthreads), where every lane must
 // arrive, as converged hardware code does; otherwise the emulation stops
 // with a message.  Waves and blocks run one after another, so __syncthreads
 // is supported in one-wave blocks only.  "Device" memory is host memory
