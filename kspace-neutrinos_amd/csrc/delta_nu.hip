@@ -390,6 +390,7 @@ __device__ inline void dnu_qag_item(const QagArgs &a, const SpeciesDev &sp, int 
     f.sinv_bw = a.sinv_bw;
     f.k = a.wavenum[k];
     f.mnubykT = sp.mnubykT;
+    f.k_by_m = f.k / sp.mnubykT;
     f.J = &sp.J;
     f.edom = a.status + 1;
     double r;

@@ -116,6 +116,7 @@ inline unsigned long long __ballot(int pred)
     return m;
 }
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }
+inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }  // the hardware estimate, exactly
 inline void __syncthreads()
 {
     if (emu_bdim > 64) {
