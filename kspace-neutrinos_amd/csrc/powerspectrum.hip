@@ -41,6 +41,7 @@ constexpr int kBlock = 256;  // geometry pass
 constexpr int kWaves = kBlock / 64;
 constexpr int kPowWaves = 8;  // power pass: 8 waves, 4 consecutive modes per lane per step, next step prefetched
 constexpr int kPowRun = 4;
+constexpr int kPowSlots = 2;  // power pass blocks per resident slot (1.57 ms vs 1.64 ms at one, 1024^3)
 
 struct BinGeom {
     int dims, nrbins, startslab, nslab;
@@ -123,8 +124,19 @@ __global__ __launch_bounds__(64 * kRedWaves) void k_reduce_partials(const double
     const int b = blockIdx.x * 64 + lane;
     const int g0 = nblk * wave / kRedWaves, g1 = nblk * (wave + 1) / kRedWaves;
     double s = 0.0;
-    if (b < nrbins)
-        for (int g = g0; g < g1; g++) s += partial[(size_t)g * nrbins + b];
+    if (b < nrbins) {
+        // eight loads in flight, added in block order
+        constexpr int U = 8;
+        int g = g0;
+        for (; g + U <= g1; g += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = partial[(size_t)(g + u) * nrbins + b];
+#pragma unroll
+            for (int u = 0; u < U; u++) s += v[u];
+        }
+        for (; g < g1; g++) s += partial[(size_t)g * nrbins + b];
+    }
     acc[wave][lane] = s;
     __syncthreads();
     if (wave == 0 && b < nrbins) {
@@ -210,7 +222,7 @@ BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
     // round of blocks measured slower once the doubled partial histograms are
     // reduced)
     const long long pgrid = (long long)resident_blocks(kn_bin::k_bin_power_runs_pf<kPowWaves, kPowRun>, 64 * kPowWaves,
-                                                       power_smem(dims, nrbins)) * num_cus();
+                                                       power_smem(dims, nrbins)) * num_cus() * kPowSlots;
     g->grid = (int)(rows < pgrid ? rows : pgrid);
     g->ggrid = (int)(rows < 4LL * num_cus() ? rows : 4LL * num_cus());
     const int pmax = g->grid > g->ggrid ? g->grid : g->ggrid;
