@@ -291,7 +291,7 @@ struct BatchIntegrand {
         const int si = settle(sx, ns, sbucket[q], x);
         const double dtot = linear ? linear_at(sx, sy, si, x) : cspline_at(sx, sy, sc, si, x);
         const double specJ = specialJ(k * fsl / mnubykT, hd->J);
-        return fsl * exp(-x - ln_hubble(hd->H, x)) * specJ * dtot;  // fsl / (a H) specJ dtot
+        return fsl * inv_ah(hd->H, x) * specJ * dtot;  // fsl / (a H) specJ dtot
     }
 };
 
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(64 * kBW, 2) void k_b_adj_nodes(BatchDev B, const H
                         const double kx = f.k * fsl / f.mnubykT;
                         double Jx, Jq;
                         const double J = specialJ_d(kx, d.J, Jx, Jq);
-                        const double base = fsl * exp(-x - ln_hubble(d.H, x));  // fsl / (a H)
+                        const double base = fsl * inv_ah(d.H, x);  // fsl / (a H)
                         const double S = f.linear ? linear_at(f.sx, f.sy, si, x) : cspline_at(f.sx, f.sy, f.sc, si, x);
                         // d/dmu of J(k fsl/mu, qc = vcrit mu)
                         mu_acc += w * base * S * (Jx * (-kx / f.mnubykT) + Jq * d.vcrit);
@@ -568,20 +568,10 @@ void build_htable(const kn_batch_history &h, std::vector<double> &coef, double &
     x0 = log(h.time_transfer);
     const double x1 = log(h.a_max > h.a ? h.a_max : h.a);
     npanel = (int)ceil((x1 - x0) / kHWidth) + 1;
-    coef.assign((size_t)npanel * kHnc, 0.0);
-    for (int p = 0; p < npanel; p++) {
-        const double half = 0.5 * kHWidth, mid = x0 + p * kHWidth + half;
-        double g[kHnc];
-        for (int jn = 0; jn < kHnc; jn++) {
-            const double t = cos(M_PI * (jn + 0.5) / kHnc);
-            g[jn] = log(h.hubble(exp(mid + half * t), h.hubble_ctx));
-        }
-        for (int kk = 0; kk < kHnc; kk++) {
-            double s = 0.0;
-            for (int jn = 0; jn < kHnc; jn++) s += g[jn] * cos(M_PI * kk * (jn + 0.5) / kHnc);
-            coef[(size_t)p * kHnc + kk] = (kk == 0 ? 1.0 : 2.0) * s / kHnc;
-        }
-    }
+    coef.assign((size_t)npanel * kHStride, 0.0);
+    for (int p = 0; p < npanel; p++)
+        h_cheb_panel(x0 + p * kHWidth, kHWidth, [&](double a) { return h.hubble(a, h.hubble_ctx); },
+                     &coef[(size_t)p * kHStride]);
 }
 
 void upload_consts()

@@ -16,8 +16,9 @@
 //   k_dnu_qag    one wavefront per (species, k): QAG-GK61 of the integrand
 //                fsl/(a H) * specialJ * S_k over [log T0, log a] at the
 //                species' tolerance, as the reference does per k.
-// H(a) on the device is a Chebyshev interpolant of ln H(ln a) sampled from the
-// host's hubble_function on fixed panels (built once, extended on demand).
+// H(a) enters the device integrands as Chebyshev interpolants of 1/(a^2 H) and
+// 1/(a H) in ln a, sampled from the host's hubble_function on fixed panels
+// (built once, extended on demand).
 #include <math.h>
 #include <string.h>
 #include <algorithm>
@@ -60,32 +61,22 @@ void h_reset(double origin)
     g_h.origin = origin;
     g_h.x0 = origin - 20.0;
     g_h.npanel = (int)(28.0 / g_h.width);
-    g_h.coef.assign((size_t)g_h.npanel * kHnc, 0.0);
+    g_h.coef.assign((size_t)g_h.npanel * kHStride, 0.0);
     g_h.built.assign(g_h.npanel, 0);
     g_h.dirty = true;
 }
 
-// Chebyshev coefficients of ln H(e^x) on the panel [lo, lo + width].
+// Chebyshev coefficients of 1/(a^2 H) and 1/(a H) on the panel [lo, lo + width] of ln a.
 void cheb_panel(double lo, double width, double (*hub)(double, void *), void *ctx, double *coef)
 {
-    const double half = 0.5 * width, mid = lo + half;
-    double g[kHnc];
-    for (int j = 0; j < kHnc; j++) {
-        const double t = cos(M_PI * (j + 0.5) / kHnc);
-        g[j] = log(hub(exp(mid + half * t), ctx));
-    }
-    for (int k = 0; k < kHnc; k++) {
-        double s = 0.0;
-        for (int j = 0; j < kHnc; j++) s += g[j] * cos(M_PI * k * (j + 0.5) / kHnc);
-        coef[k] = (k == 0 ? 1.0 : 2.0) * s / kHnc;
-    }
+    h_cheb_panel(lo, width, [&](double a) { return hub(a, ctx); }, coef);
 }
 
 double hub_process(double a, void *) { return kn_hubble(a); }
 
 void h_build_panel(int p)
 {
-    cheb_panel(g_h.x0 + p * g_h.width, g_h.width, hub_process, nullptr, &g_h.coef[(size_t)p * kHnc]);
+    cheb_panel(g_h.x0 + p * g_h.width, g_h.width, hub_process, nullptr, &g_h.coef[(size_t)p * kHStride]);
     g_h.built[p] = 1;
 }
 
@@ -559,7 +550,7 @@ __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
                 const double kx = kk * fsl / mnubykT;
                 double Jx, Jq;
                 const double Jv = specialJ_d(kx, J, Jx, Jq);
-                const double base = fsl * exp(-x - ln_hubble(H, x));  // fsl / (a H), as the forward
+                const double base = fsl * inv_ah(H, x);  // fsl / (a H), as the forward
                 const double S = linear ? linear_at(sx, syk, si, x) : cspline_at(sx, syk, sck, si, x);
                 const bool own0 = si >= K0 && si < K1, own1 = si + 1 >= K0 && si + 1 < K1;
                 if (own0)  // d/d(m/kT), qc = vcrit m/kT
@@ -827,20 +818,10 @@ extern "C" void kn_htable_ensure(double origin, double xlo, double xhi)
     }
 }
 
+// H(e^x) from the table's 1/(a H) panels
 extern "C" double kn_htable_eval_host(double x)
 {
-    const double u = (x - g_h.x0) / g_h.width;
-    int p = (int)floor(u);
-    p = p < 0 ? 0 : (p >= g_h.npanel ? g_h.npanel - 1 : p);
-    const double t = 2.0 * (u - p) - 1.0;
-    const double *c = g_h.coef.data() + (size_t)p * kHnc;
-    double b1 = 0.0, b2 = 0.0;
-    for (int k = kHnc - 1; k >= 1; k--) {
-        const double b0 = fma(2.0 * t, b1, c[k] - b2);
-        b2 = b1;
-        b1 = b0;
-    }
-    return exp(fma(t, b1, c[0] - b2));
+    return exp(-x) / h_cheb(g_h.coef.data(), g_h.x0, 1.0 / g_h.width, g_h.npanel, x, 1);
 }
 
 extern "C" int kn_dnu_last_max_intervals(void) { return g_last_max_intervals; }
@@ -1182,7 +1163,7 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
 namespace {
 
 constexpr int kGroupStreams = 8;
-constexpr double kGroupHWidth = 1.0 / 256.0;  // ln H panel width (the per-call table's)
+constexpr double kGroupHWidth = 1.0 / 256.0;  // H panel width (the per-call table's)
 
 // delta_nu[k] = IC(k) of an IC-only history (delta_tot_table.c:522-535,
 // :543 and :551: hybrid with all neutrinos in particles, massless, or one
@@ -1946,7 +1927,7 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
             kn_group_destroy(g);
             return rc;
         }
-        // one ln H table per (H, T0): panel edges on log T0, up to log max(a, a_max)
+        // one H table per (H, T0): panel edges on log T0, up to log max(a, a_max)
         int found = -1;
         for (int j = 0; j < i && found < 0; j++) {
             const GroupHist &p = g->h[j];
@@ -1956,10 +1937,10 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
         }
         if (found < 0) {
             const int npanel = (int)ceil((q.xmax - q.x0) / kGroupHWidth) + 1;
-            std::vector<double> coef((size_t)npanel * kHnc);
+            std::vector<double> coef((size_t)npanel * kHStride);
             for (int p = 0; p < npanel; p++)
                 cheb_panel(q.x0 + p * kGroupHWidth, kGroupHWidth, q.spec.hubble, q.spec.hubble_ctx,
-                           &coef[(size_t)p * kHnc]);
+                           &coef[(size_t)p * kHStride]);
             double *d = group_alloc<double>(coef.size());
             KN_CHECK(hipMemcpy(d, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice));
             HTab H;

@@ -22,26 +22,57 @@
 namespace kn_dnu {
 
 constexpr int kLimit = 200;   // QAG subinterval limit (GSL_VAL)
-constexpr int kHnc = 10;      // Chebyshev coefficients per ln H panel
+constexpr int kHnc = 10;      // Chebyshev coefficients per function per H panel
+constexpr int kHStride = 2 * kHnc;  // one panel: 1/(a^2 H) coefficients, then 1/(a H)
 
 __constant__ double c_xgk[31];
 __constant__ double c_wgk[31];
 __constant__ double c_wg[15];
 
-// ---- ln H(x), x = ln a: Chebyshev panels of width 1/inv_w from origin x0 ----
+// ---- H(a) factors as functions of x = ln a: Chebyshev panels of width 1/inv_w from origin x0 ----
+// Each panel holds the coefficients of 1/(a^2 H) (the fslength integrand,
+// delta_tot_table.c:378-383) and of 1/(a H) (the delta_nu integrand's factor,
+// :492-500), so neither integrand forms an exponential.
 struct HTab {
-    const double *coef;  // npanel * kHnc
+    const double *coef;  // npanel * kHStride
     double x0, inv_w;
     int npanel;
 };
 
-__device__ __forceinline__ double ln_hubble(const HTab &H, double x)
+// Host: the coefficients of both functions on the panel [lo, lo + width],
+// interpolating at the kHnc Chebyshev points; hub(a) = H(a).
+template <typename Hub>
+inline void h_cheb_panel(double lo, double width, Hub hub, double *coef)
 {
-    double u = (x - H.x0) * H.inv_w;
+    const double half = 0.5 * width, mid = lo + half;
+    double g2[kHnc], g1[kHnc];
+    for (int j = 0; j < kHnc; j++) {
+        const double x = mid + half * cos(M_PI * (j + 0.5) / kHnc);
+        const double h = hub(exp(x));
+        g2[j] = exp(-2.0 * x) / h;
+        g1[j] = exp(-x) / h;
+    }
+    for (int k = 0; k < kHnc; k++) {
+        double s2 = 0.0, s1 = 0.0;
+        for (int j = 0; j < kHnc; j++) {
+            const double c = cos(M_PI * k * (j + 0.5) / kHnc);
+            s2 += g2[j] * c;
+            s1 += g1[j] * c;
+        }
+        coef[k] = (k == 0 ? 1.0 : 2.0) * s2 / kHnc;
+        coef[kHnc + k] = (k == 0 ? 1.0 : 2.0) * s1 / kHnc;
+    }
+}
+
+// Clenshaw sum of one function's panel (which = 0: 1/(a^2 H), 1: 1/(a H)).
+__host__ __device__ __forceinline__ double h_cheb(const double *coef, double x0, double inv_w, int npanel, double x,
+                                                  int which)
+{
+    const double u = (x - x0) * inv_w;
     int p = (int)floor(u);
-    p = p < 0 ? 0 : (p >= H.npanel ? H.npanel - 1 : p);
+    p = p < 0 ? 0 : (p >= npanel ? npanel - 1 : p);
     const double t = 2.0 * (u - p) - 1.0;
-    const double *c = H.coef + (size_t)p * kHnc;
+    const double *c = coef + (size_t)p * kHStride + which * kHnc;
     double b1 = 0.0, b2 = 0.0;
 #pragma unroll
     for (int k = kHnc - 1; k >= 1; k--) {
@@ -51,6 +82,9 @@ __device__ __forceinline__ double ln_hubble(const HTab &H, double x)
     }
     return fma(t, b1, c[0] - b2);
 }
+
+__device__ __forceinline__ double inv_a2h(const HTab &H, double x) { return h_cheb(H.coef, H.x0, H.inv_w, H.npanel, x, 0); }
+__device__ __forceinline__ double inv_ah(const HTab &H, double x) { return h_cheb(H.coef, H.x0, H.inv_w, H.npanel, x, 1); }
 
 // 1/d from the hardware estimate refined by one Newton step (relative error
 // a few units in the last place, against ten instructions for a division);
@@ -155,9 +189,9 @@ __device__ __forceinline__ int settle(const double *xa, int n, int i, double x)
 }
 
 // ---- integrands ----
-struct FslIntegrand {  // delta_tot_table.c:378-383: 1 / a / (a H(a)) = exp(-2 ln a - ln H)
+struct FslIntegrand {  // delta_tot_table.c:378-383: 1 / a / (a H(a))
     HTab H;
-    __device__ __forceinline__ double operator()(double x) const { return exp(-2.0 * x - ln_hubble(H, x)); }
+    __device__ __forceinline__ double operator()(double x) const { return inv_a2h(H, x); }
 };
 
 struct DnuIntegrand {  // delta_tot_table.c:492-500
@@ -187,7 +221,7 @@ struct DnuIntegrand {  // delta_tot_table.c:492-500
         const int si = settle(sx, ns, sbucket[q], x);
         const double dtot = linear ? linear_at(sx, sy, si, x) : cspline_at(sx, sy, sc, si, x);
         const double specJ = specialJ(k_by_m * fsl, *J);
-        return fsl * exp(-x - ln_hubble(H, x)) * specJ * dtot;  // fsl / (a H) specJ dtot
+        return fsl * inv_ah(H, x) * specJ * dtot;  // fsl / (a H) specJ dtot
     }
 };
 

@@ -52,11 +52,11 @@ double kn_linear_eval(const double *x, const double *y, int n, double xv, int *e
 size_t kn_bsearch(const double *xa, double x, size_t lo, size_t hi);
 
 /* ---- device Hubble table (delta_nu.hip) ---- */
-/* Make sure the device interpolant of ln H covers [xlo, xhi] (x = ln a), with a
+/* Make sure the device interpolants of 1/(a^2 H) and 1/(a H) cover [xlo, xhi] (x = ln a), with a
  * panel edge at origin (the integration's lower limit). */
 void kn_htable_ensure(double origin, double xlo, double xhi);
 void kn_htable_build_host(double origin, double xlo, double xhi);
-/* Host evaluation of the same interpolant (tests). */
+/* H(e^x) from the same 1/(a H) interpolant, on the host (tests). */
 double kn_htable_eval_host(double x);
 
 /* ---- device solver for get_delta_nu (delta_nu.hip) ---- */

