@@ -83,8 +83,28 @@ __host__ __device__ __forceinline__ double h_cheb(const double *coef, double x0,
     return fma(t, b1, c[0] - b2);
 }
 
-__device__ __forceinline__ double inv_a2h(const HTab &H, double x) { return h_cheb(H.coef, H.x0, H.inv_w, H.npanel, x, 0); }
-__device__ __forceinline__ double inv_ah(const HTab &H, double x) { return h_cheb(H.coef, H.x0, H.inv_w, H.npanel, x, 1); }
+// Device form: the table lives in global memory, so its loads are global
+// loads even where the pointer came from a descriptor in memory.
+typedef const __attribute__((address_space(1))) double gconst_double;
+__device__ __forceinline__ double h_cheb_dev(const HTab &H, double x, int which)
+{
+    const double u = (x - H.x0) * H.inv_w;
+    int p = (int)floor(u);
+    p = p < 0 ? 0 : (p >= H.npanel ? H.npanel - 1 : p);
+    const double t = 2.0 * (u - p) - 1.0;
+    gconst_double *c = (gconst_double *)(H.coef + (size_t)p * kHStride + which * kHnc);
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int k = kHnc - 1; k >= 1; k--) {
+        const double b0 = fma(2.0 * t, b1, c[k] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return fma(t, b1, c[0] - b2);
+}
+
+__device__ __forceinline__ double inv_a2h(const HTab &H, double x) { return h_cheb_dev(H, x, 0); }
+__device__ __forceinline__ double inv_ah(const HTab &H, double x) { return h_cheb_dev(H, x, 1); }
 
 // 1/d from the hardware estimate refined by one Newton step (relative error
 // a few units in the last place, against ten instructions for a division);
