@@ -27,7 +27,7 @@ def rel(a, b):
 
 
 def test_hubble_interpolant_matches_host(kn, host, torch_cuda):
-    """ln H panels anchored at ln T0.  Away from the host's own branch switch
+    """H factor panels anchored at ln T0 (H read back from 1/(a H)).  Away from the host's own branch switch
     (rho_nu's table -> series at a = 100 kT/m, a ~1e-8 jump of H) the
     interpolant is exact to ~1e-14; the panel holding the jump smears it."""
     P = Pair([0.1, 0.1, 0.1], 0.3, 8, 1.0)
