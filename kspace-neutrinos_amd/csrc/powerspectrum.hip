@@ -201,7 +201,7 @@ BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
 {
     for (BinGeom *g : g_geoms)
         if (g->dims == dims && g->nrbins == nrbins && g->startslab == startslab && g->nslab == nslab) return g;
-    if (dims < 2 || (dims & 1) || nrbins < 1 || nslab < 1 || startslab < 0 || startslab + nslab > dims)
+    if (dims < 2 || nrbins < 1 || nslab < 1 || startslab < 0 || startslab + nslab > dims)
         kn_terminate(KN_ERR_ARG, "total_powerspectrum: bad geometry dims=%d nrbins=%d slabs=[%d,%d)\n", dims,
                      nrbins, startslab, startslab + nslab);
     if (power_smem(dims, nrbins) > 160 * 1024)

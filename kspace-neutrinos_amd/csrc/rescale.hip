@@ -120,7 +120,7 @@ int launch_grid(long long work)
 extern "C" int kn_rescale_device(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
                                  const double *d_logkk, const double *d_ratio, int nbins, double norm, void *stream)
 {
-    if (nbins < 1 || dims < 2 || nslab < 1 || (dims & 1)) return KN_ERR_ARG;
+    if (nbins < 1 || dims < 2 || nslab < 1) return KN_ERR_ARG;
     const int nbuckets = kn_res::kBucketsPerBin * nbins;
     const size_t smem = kn_res::smem_bytes(nbins, nbuckets);
     if (smem > 160 * 1024) return KN_ERR_ARG;

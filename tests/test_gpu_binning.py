@@ -61,7 +61,10 @@ def test_reference_4cubed_known_answer(kn, torch_cuda):
     assert abs(p[0] - 0.254834) < 4e-7 and abs(p[1] - 0.00212722) < 5e-8 and abs(p[2] - 0.00323766) < 3e-8
 
 
-@pytest.mark.parametrize("N,nrbins", [(8, 7), (16, 15), (16, 64), (32, 33), (64, 100), (128, 64), (96, 300)])
+# odd N: the z = N/2 plane keeps multiplicity 1 as in the reference (counts
+# then total N^2 (N - 1) - 1); one bin: binsperunit = 0, every mode in bin 0
+@pytest.mark.parametrize("N,nrbins", [(8, 7), (16, 15), (16, 64), (32, 33), (64, 100), (128, 64), (96, 300),
+                                      (9, 5), (15, 16), (7, 3), (12, 1)])
 def test_binning_matches_oracle(kn, torch_cuda, N, nrbins):
     check_against_oracle(kn, torch_cuda, _rand_grid(N, N * 7 + nrbins), nrbins)
 
@@ -130,7 +133,7 @@ def test_binning_full_size_invariants(kn, torch_cuda, N, nrbins):
 
 # ------------------------------------------------------------------ rescale
 
-@pytest.mark.parametrize("N,nb", [(16, 12), (64, 64), (128, 300)])
+@pytest.mark.parametrize("N,nb", [(16, 12), (64, 64), (128, 300), (9, 4), (15, 7)])
 def test_rescale_matches_oracle(kn, torch_cuda, N, nb):
     torch = torch_cuda
     F = _rand_grid(N, N + nb)
