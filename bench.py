@@ -244,7 +244,30 @@ def plain_vs_group(kn, torch, keep, hists, data, reps=3, seed=5):
             "max_rel_diff_group_vs_plain": diff}
 
 
-def c4_batching(kn, torch, nhist=256, seed=1000, reps=3):
+def c4_cpu_baseline(data, nsample=4):
+    """The oracle (C restatement, 1 core) on a bounded sample of the same
+    workload: nsample whole histories (every k bin) spread over the set, each
+    with its own cosmology; k.step/s of the forward integral."""
+    from batch_common import Cosmo
+    tc, ksteps, names = 0.0, 0, []
+    for h in np.linspace(0, len(data) - 1, nsample).astype(int):
+        dd = data[h]
+        sel = np.arange(dd["nk"])
+        cz = Cosmo([dd["m"], 0.0, 0.0], omega0=dd["om0"])
+        T = cz.oracle(dd["nk"], dd["logs"], dd["rows"][sel], dd["init"][sel])
+        t0 = time.perf_counter()
+        rc, _ = T.get_delta_nu(1.0, dd["k"][sel], dd["m"])
+        tc += time.perf_counter() - t0
+        assert rc == 0
+        ksteps += dd["nk"] * dd["na"]
+        names.append("%d (nk %d, Na %d)" % (h, dd["nk"], dd["na"]))
+        del T
+        cz.free()
+    return {"kstep_per_s": round(ksteps / tc, 1), "cores": 1, "kind": "port",
+            "sample": "whole histories %s of the %d, oracle get_delta_nu %.2fs" % (", ".join(names), len(data), tc)}
+
+
+def c4_batching(kn, torch, nhist=256, seed=1000, reps=3, no_cpu=False):
     """256 independent histories: Na ~ U[100, 3000], nk ~ U[64, 512], one
     species m ~ U[0.06, 0.6] eV, Omega_m ~ U[0.25, 0.35], each with its own
     H(a) (noise drawn per k: drawn per (k, a) entry it makes the reference's own
@@ -253,6 +276,8 @@ def c4_batching(kn, torch, nhist=256, seed=1000, reps=3):
                       float(r.uniform(0.25, 0.35)))
     keep, hists, data = build_histories(kn, nhist, np.random.default_rng(seed), draw)
     res = plain_vs_group(kn, torch, keep, hists, data, reps)
+    if not no_cpu:
+        res["cpu_baseline"] = c4_cpu_baseline(data)
     res["workload"] = ("%d histories, Na~U[100,3000], nk~U[64,512], m~U[0.06,0.6] eV, Omega_m~U[0.25,0.35]"
                        % nhist)
     res["persistent_batch"] = "kn_batch (one persistent launch) switched off in this build: returns 3005"
@@ -438,7 +463,7 @@ def main():
         del grid
         torch.cuda.empty_cache()
         out["delta_nu"] = dnu_bench(kn, n, no_cpu=args.no_cpu)
-        out["delta_nu_batch"] = c4_batching(kn, torch)
+        out["delta_nu_batch"] = c4_batching(kn, torch, no_cpu=args.no_cpu)
         out["adjoints"] = c5_adjoints(kn, torch)
         out["adjoints"]["group_64x1024x2000"] = c5_group(kn, torch)
     print(json.dumps(out))
