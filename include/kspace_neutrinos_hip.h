@@ -271,12 +271,14 @@ const char *kn_last_error(void);
 int kn_bin_power_adjoint(const fftw_complex *d_field, int dims, int nrbins, int startslab, int nslab,
                          const double *gbar, int nout, fftw_complex *d_field_bar, void *stream);
 
-/* ---- (b) batched: many get_delta_nu integrals in one launch set ----
- * One history is what one get_delta_nu call integrates
+/* ---- (b) one history of a group: what one get_delta_nu call integrates ----
  * (delta_tot_table.c:507-611): its stored log a values and delta_tot rows,
- * its own H(a), mass and tolerance.  All per-history state lives in device
- * memory; na grows on the device by kn_batch_append, so a captured graph
- * (append + solve [+ adjoint]) can be replayed step after step. */
+ * its own H(a), mass and hybrid setting.  qc, relerr and ic_only are what
+ * get_delta_nu derives from a, the stored step count and the mass
+ * (delta_tot_table.c:513-555); kn_batch_history_from_table fills them, and a
+ * group derives them again at create, append and set_mass from a, na, mnu and
+ * the hybrid fields (hybrid, nu_crit_time, nufrac_low, vcrit), so values
+ * passed in are not used. */
 typedef struct kn_batch_history {
     int nk;                 /* k bins */
     int na;                 /* stored steps now (>= 1) */
@@ -287,55 +289,28 @@ typedef struct kn_batch_history {
     double mnu, kBtnu;      /* neutrino mass (eV) and k_B T_nu (eV) */
     double light;           /* speed of light in internal units */
     double delta_nu_prefac; /* 1.5 Omega0 H0^2 / c */
-    double qc, relerr;      /* hybrid critical momentum (0: off) and QAG tolerance */
+    double qc, relerr;      /* derived: hybrid critical momentum (0: off) and QAG tolerance */
     double nufrac_low;      /* hybnu.nufrac_low[0] */
-    double vcrit;           /* hybnu.vcrit (d qc / d (m/kT)) */
-    int ic_only;            /* 1: only the initial-condition piece */
+    double vcrit;           /* hybnu.vcrit (qc = vcrit m / kT once particles are active) */
+    int hybrid;             /* hybnu.enabled */
+    double nu_crit_time;    /* hybnu.nu_crit_time: particles active for a > nu_crit_time */
+    int ic_only;            /* derived: 1 when only the initial-condition piece is computed */
     double (*hubble)(double a, void *ctx);
     void *hubble_ctx;
 } kn_batch_history;
 
-typedef struct kn_batch kn_batch;
-
-/* Returns 3005 (switched off) in this build: the batched kernels have not
- * completed a device run on gfx950 and stay disabled until they do; they are
- * exercised in a CPU SIMT emulation build only. */
-int kn_batch_create(kn_batch **out, int nhist, const kn_batch_history *hist);
-void kn_batch_destroy(kn_batch *b);
-/* Device arrays of the batch (any pointer may be NULL): scalefact (log a) per
- * history [na_max], delta_tot per history [nk][na_max], wavenum/
- * delta_nu_init/delta_nu/gbar per item, dtot_bar like delta_tot, init_bar per
- * item, mnu_bar per history. */
-int kn_batch_views(kn_batch *b, double **scalefact, double **delta_tot, double **wavenum, double **delta_nu_init,
-                   double **delta_nu, double **gbar, double **dtot_bar, double **init_bar, double **mnu_bar);
-/* Offsets of each history (nhist+1 entries): per-item arrays, delta_tot block, scalefact. */
-int kn_batch_layout(const kn_batch *b, long long *off_k, long long *off_y, long long *off_s);
-/* delta_nu of every history (synchronous; returns 0 or an error code). */
-int kn_batch_solve(kn_batch *b, void *stream);
-/* Adjoint of the last solve for the gbar seeds (synchronous). */
-int kn_batch_adjoint(kn_batch *b, void *stream);
-/* Append one step per history: d_loga[nhist], d_rows[items] (device). */
-int kn_batch_append(kn_batch *b, const double *d_loga, const double *d_rows, void *stream);
-/* Capture [append +] solve [+ adjoint] into one graph; launch replays it (async). */
-int kn_batch_graph_capture(kn_batch *b, int with_adjoint, const double *d_loga, const double *d_rows, void *stream);
-int kn_batch_graph_launch(kn_batch *b, void *stream);
-/* Error state of the last launches (0, 2001 QAG/interpolation failure, 3004 capacity). */
-int kn_batch_status(kn_batch *b);
-/* New neutrino mass (eV) of history h; qc follows it for a hybrid history. */
-int kn_batch_set_mass(kn_batch *b, int h, double mnu);
 /* The history get_delta_nu(d, a, ..., mnu) integrates (delta_tot_table.c:507-555:
- * tolerance, hybrid qc, IC-only cases), with na = na_max = d->ia. */
+ * tolerance, hybrid qc, IC-only cases), with na = na_max = d->ia and the
+ * process's hubble_function as its H (kn_hubble_ctx). */
 int kn_batch_history_from_table(const _delta_tot_table *d, double a, double mnu, kn_batch_history *out);
 /* The process's hubble_function (or the one registered with
  * kn_set_hubble_function) in the kn_batch_history callback form. */
 double kn_hubble_ctx(double a, void *ctx);
-/* Pick persistent-grid sizes by timing the forward. */
-int kn_batch_autotune(kn_batch *b, void *stream);
 
 /* ---- (b) grouped: many get_delta_nu integrals on the per-call kernels ----
  * No reference counterpart (the reference evaluates one table per
- * get_delta_nu call, delta_tot_table.c:507-611); additive.  The same
- * histories, device views and layout as kn_batch, solved with the kernels of
+ * get_delta_nu call, delta_tot_table.c:507-611); additive.  Histories
+ * (kn_batch_history) live in device memory and are solved with the kernels of
  * get_delta_nu, either in merged launches (the default: one grid per phase
  * covering every history, so the launch count does not grow with the number
  * of histories) or with each history's launch sequence on one of up to 8 HIP
@@ -363,6 +338,11 @@ int kn_group_append(kn_group *g, const double *d_loga, const double *d_rows, voi
 int kn_group_graph_capture(kn_group *g, int with_adjoint, const double *d_loga, const double *d_rows, void *stream);
 int kn_group_graph_launch(kn_group *g, void *stream);
 int kn_group_status(kn_group *g);
+/* History h's final QAG subdivisions of the last solve (held fixed by the
+ * adjoint), in QUADPACK's interval-list order: ivl[(k * 200 + t) * 2 + {0,1}]
+ * for t < nint[k]; nint is 0 for initial-condition-only items.  Host arrays of
+ * nk * 400 doubles and nk ints. */
+int kn_group_subdivision(kn_group *g, int h, double *ivl, int *nint);
 int kn_group_set_mass(kn_group *g, int h, double mnu);
 /* Pick the launch strategy by timing the forward: merged launches (0) or
  * per-history sequences on 1, 2, 4 or 8 streams. */
@@ -378,8 +358,7 @@ int kn_group_set_streams(kn_group *g, int nstream);
  * delta_nu_init[nk] and dL/d mnu (the function's mnu argument; the hybrid
  * state of the table held fixed).  The QAG subdivision is that of the
  * forward evaluation, so the result is the exact derivative of the computed
- * quadrature.  Built on the batch kernels: returns 3005 while those are
- * switched off. */
+ * quadrature.  Returns 0, or 2001 when the forward's QAG fails. */
 int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, const double *wavenum, double mnu,
                             const double *gbar, double *delta_nu, double *dtot_bar, double *init_bar,
                             double *mnu_bar);

@@ -1253,6 +1253,13 @@ struct GroupDesc {
     double x0, xa, light, sinv_bw, fsl0, lin, prefac, vcrit;
 };
 
+// The last stored log a of every history (append checks the new step against it).
+__global__ void k_group_last_loga(const GroupDesc *__restrict__ D, int nhist, double *__restrict__ out)
+{
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < nhist) out[h] = D[h].sx[D[h].na - 1];
+}
+
 // phases (rows of the prefix table); the adjoint's (item, part) blocks are
 // split in classes of part length (LDS rows of 2 adj_plen(na) doubles):
 // class c holds adj_plen(na) <= 128 << c
@@ -1468,7 +1475,8 @@ struct kn_group {
     std::vector<int> order;
     double *sx = nullptr, *Y = nullptr, *C = nullptr, *dbar = nullptr, *cbar = nullptr, *fx = nullptr,
            *fy = nullptr, *fc = nullptr, *wavenum = nullptr, *init = nullptr, *out = nullptr, *gbar = nullptr,
-           *ibar = nullptr, *mbic = nullptr, *mubar = nullptr, *mbar = nullptr, *ivl = nullptr, *fac = nullptr;
+           *ibar = nullptr, *mbic = nullptr, *mubar = nullptr, *mbar = nullptr, *ivl = nullptr, *fac = nullptr,
+           *lastla = nullptr;
     int *bucket = nullptr, *nint = nullptr, *status = nullptr, *zeros = nullptr, *spec0 = nullptr, *fs_list = nullptr;
     SpeciesDev *sp = nullptr;
     GroupMeta *meta = nullptr;
@@ -1502,24 +1510,49 @@ SpeciesDev species_dev(const kn_batch_history &s)
     return d;
 }
 
-// fslength(log T0, log a) with the history's own H (delta_tot_table.c:394-407),
-// and the initial-condition factor 1 + deriv_prefac fsl.
-int group_refresh_ic(GroupHist &q)
+// fslength(log T0, log xa) with the history's own H (delta_tot_table.c:394-407),
+// and the initial-condition factor 1 + deriv_prefac fsl (:522-535).
+int group_ic_factors(const GroupHist &q, double xa, double *fsl0, double *lin)
 {
     HubCtx hc = {q.spec.hubble, q.spec.hubble_ctx};
     double v = 0.0, e = 0.0;
-    if (q.x0 < q.xa && kn_qag61(fsl_integrand_ctx, &hc, q.x0, q.xa, 0, 1e-6, KN_GSL_VAL, &v, &e)) {
-        kn_set_last_error("kn_group: QAG failed in fslength(%g, %g)", q.x0, q.xa);
+    if (q.x0 < xa && kn_qag61(fsl_integrand_ctx, &hc, q.x0, xa, 0, 1e-6, KN_GSL_VAL, &v, &e)) {
+        kn_set_last_error("kn_group: QAG failed in fslength(%g, %g)", q.x0, xa);
         return 2001;
     }
-    q.fsl0 = q.spec.light * v;
+    *fsl0 = q.spec.light * v;
     const double T0 = q.spec.time_transfer;
     const double deriv_prefac = T0 * (q.spec.hubble(T0, q.spec.hubble_ctx) / q.spec.light) * T0;
-    q.lin = 1. + deriv_prefac * q.fsl0;
+    *lin = 1. + deriv_prefac * *fsl0;
     return 0;
 }
 
+int group_refresh_ic(GroupHist &q) { return group_ic_factors(q, q.xa, &q.fsl0, &q.lin); }
+
 void group_set_integ(GroupHist &q) { q.integ = !q.spec.ic_only && q.spec.mnu > 0 && q.na >= 2; }
+
+// What get_delta_nu derives from a, the stored step count and the mass
+// (delta_tot_table.c:513-555): with particle neutrinos active (hybrid and
+// a > nu_crit_time, fraction nufrac_low[0]) the integral is skipped when
+// they are nearly all the neutrinos, otherwise qc = vcrit m/kT and the
+// tolerance is loosened; one stored step or no mass leaves the
+// initial-condition piece alone.
+void group_derive(GroupHist &q)
+{
+    kn_batch_history &s = q.spec;
+    const double partnu = (s.hybrid && s.a > s.nu_crit_time) ? s.nufrac_low : 0.0;
+    s.qc = 0.0;
+    s.relerr = 1e-6;
+    s.ic_only = 0;
+    if (partnu > 0) {
+        if (1 - partnu < 1e-3) s.ic_only = 1;
+        s.qc = s.vcrit * (s.mnu / s.kBtnu);
+        s.relerr /= (1. + 1e-5 - partnu);
+    }
+    if (!(q.na > 1 && s.mnu > 0)) s.ic_only = 1;
+    q.vcrit = s.qc > 0 ? s.vcrit : 0.0;
+    group_set_integ(q);
+}
 
 // Longest-first issue order, each history to the least-loaded stream.
 void group_schedule(kn_group *g)
@@ -1873,7 +1906,7 @@ extern "C" void kn_group_destroy(kn_group *g)
     void *ptrs[] = {g->sx,   g->Y,    g->C,      g->dbar, g->cbar,   g->fx,     g->fy,    g->fc,
                     g->wavenum, g->init, g->out,  g->gbar,   g->ibar, g->mbic,   g->mubar,  g->mbar,  g->ivl,
                     g->bucket, g->nint, g->status, g->zeros, g->spec0, g->sp,    g->fs_list, g->meta, g->desc,
-                    g->pre,  g->mlist, g->fac};
+                    g->pre,  g->mlist, g->fac,  g->lastla};
     for (void *p : ptrs)
         if (p) KN_CHECK(hipFree(p));
     for (double *p : g->hcoef) KN_CHECK(hipFree(p));
@@ -1906,8 +1939,7 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
         q.x0 = log(q.spec.time_transfer);
         q.xa = log(q.spec.a);
         q.xmax = log(q.spec.a_max > q.spec.a ? q.spec.a_max : q.spec.a);
-        q.vcrit = q.spec.qc > 0 ? q.spec.vcrit : 0.0;
-        group_set_integ(q);
+        group_derive(q);
         q.off_k = g->nk_tot;
         q.off_y = g->y_tot;
         q.off_s = g->s_tot;
@@ -1962,6 +1994,7 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
         *p = group_alloc<double>(nk);
     g->mubar = group_alloc<double>(g->mu_tot);  // [k][part] per history
     g->mbar = group_alloc<double>(nhist);
+    g->lastla = group_alloc<double>(nhist);
     g->fac = group_alloc<double>(g->fac_tot);
     g->ivl = group_alloc<double>(nk * kLimit * 2);
     g->bucket = group_alloc<int>(g->b_tot);
@@ -2044,16 +2077,24 @@ extern "C" int kn_group_adjoint(kn_group *g, void *stream)
 
 // One more stored step per history: d_loga[nhist] and d_rows[items] in device
 // memory.  The host reads the new log a values (they set each history's upper
-// limit and its launch sizes), checks capacity and the H table range for all
-// histories before writing any, then copies scalefact[na] and column na of
-// delta_tot on the device.
+// limit and its launch sizes) and each history's last stored log a, and
+// checks every history before anything changes: capacity, the H table range
+// and a strictly increasing log a (the reference's spline would reject equal
+// or decreasing knots, GSL_EINVAL -> 2001).  The new initial-condition factors
+// are computed for all histories first; only then are scalefact[na] and
+// column na of delta_tot copied on the device and the stored-step count,
+// upper limit and the derived tolerance / hybrid / IC-only state updated.
 extern "C" int kn_group_append(kn_group *g, const double *d_loga, const double *d_rows, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
-    std::vector<double> la(g->nhist);
-    KN_CHECK(hipMemcpyAsync(la.data(), d_loga, sizeof(double) * g->nhist, hipMemcpyDeviceToHost, st));
+    const int n = g->nhist;
+    std::vector<double> la(n), last(n), fsl0(n), lin(n);
+    KN_CHECK(hipMemcpyAsync(la.data(), d_loga, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_group_last_loga, dim3((n + 63) / 64), dim3(64), 0, st, g->desc, n, g->lastla);
+    KN_CHECK(hipGetLastError());
+    KN_CHECK(hipMemcpyAsync(last.data(), g->lastla, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     KN_CHECK(hipStreamSynchronize(st));
-    for (int i = 0; i < g->nhist; i++) {
+    for (int i = 0; i < n; i++) {
         const GroupHist &q = g->h[i];
         if (q.na >= q.spec.na_max || !(la[i] <= q.xmax) || !(la[i] > q.x0)) {
             g->cap_err = 1;
@@ -2061,8 +2102,18 @@ extern "C" int kn_group_append(kn_group *g, const double *d_loga, const double *
                               q.spec.na_max);
             return KN_ERR_RANGE;
         }
+        if (!(la[i] > last[i])) {
+            kn_set_last_error("kn_group_append: history %d: log a %.17g does not follow the last stored %.17g", i,
+                              la[i], last[i]);
+            return KN_ERR_ARG;
+        }
     }
-    for (int i = 0; i < g->nhist; i++) {
+    for (int i = 0; i < n; i++) {
+        const int rc = group_ic_factors(g->h[i], la[i], &fsl0[i], &lin[i]);
+        if (rc) return rc;
+    }
+    std::vector<SpeciesDev> sd(n);
+    for (int i = 0; i < n; i++) {
         GroupHist &q = g->h[i];
         KN_CHECK(hipMemcpyAsync(g->sx + q.off_s + q.na, d_loga + i, sizeof(double), hipMemcpyDeviceToDevice, st));
         KN_CHECK(hipMemcpy2DAsync(g->Y + q.off_y + q.na, sizeof(double) * q.spec.na_max, d_rows + q.off_k,
@@ -2070,14 +2121,16 @@ extern "C" int kn_group_append(kn_group *g, const double *d_loga, const double *
         q.na++;
         q.xa = la[i];
         q.spec.a = exp(la[i]);
-        group_set_integ(q);
-        const int rc = group_refresh_ic(q);
-        if (rc) return rc;
+        q.fsl0 = fsl0[i];
+        q.lin = lin[i];
+        group_derive(q);
+        sd[i] = species_dev(q.spec);
     }
+    KN_CHECK(hipMemcpyAsync(g->sp, sd.data(), sizeof(SpeciesDev) * n, hipMemcpyHostToDevice, st));
+    KN_CHECK(hipStreamSynchronize(st));
     group_plan(g);
     group_schedule(g);
     g->gen++;
-    KN_CHECK(hipStreamSynchronize(st));
     return 0;
 }
 
@@ -2120,16 +2173,46 @@ extern "C" int kn_group_graph_launch(kn_group *g, void *stream)
 
 extern "C" int kn_group_status(kn_group *g) { return group_status(g); }
 
-// New neutrino mass (eV) of history h; qc follows it for a hybrid history.
-// Only the device species record changes, so a captured graph stays valid.
+// History h's final QAG subdivisions of the last solve (the ones the adjoint
+// holds fixed), in the interval-list order of QUADPACK's workspace:
+// ivl[(k * 200 + t) * 2 + {0, 1}] for t < nint[k] (host arrays of nk * 400
+// doubles and nk ints).  Items that only take the initial-condition piece
+// have nint 0.
+extern "C" int kn_group_subdivision(kn_group *g, int h, double *ivl, int *nint)
+{
+    if (!g || h < 0 || h >= g->nhist || !ivl || !nint) return KN_ERR_ARG;
+    const GroupHist &q = g->h[h];
+    const int nk = q.spec.nk;
+    if (!q.integ) {
+        memset(nint, 0, sizeof(int) * nk);
+        return 0;
+    }
+    KN_CHECK(hipMemcpy(ivl, g->ivl + (size_t)q.off_k * kLimit * 2, sizeof(double) * (size_t)nk * kLimit * 2,
+                       hipMemcpyDeviceToHost));
+    KN_CHECK(hipMemcpy(nint, g->nint + q.off_k, sizeof(int) * nk, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// New neutrino mass (eV) of history h; qc, the tolerance and the IC-only
+// state follow it as get_delta_nu derives them.  When only the device species
+// record changes a captured graph stays valid; when the history starts or
+// stops integrating (a massless history given a mass) the launch plan changes
+// and a captured graph is stale.
 extern "C" int kn_group_set_mass(kn_group *g, int h, double mnu)
 {
     if (h < 0 || h >= g->nhist || !(mnu > 0)) return KN_ERR_ARG;
     GroupHist &q = g->h[h];
+    const int integ0 = q.integ;
+    const double vcrit0 = q.vcrit;
     q.spec.mnu = mnu;
-    if (q.spec.qc > 0) q.spec.qc = q.spec.vcrit * (mnu / q.spec.kBtnu);
+    group_derive(q);
     const SpeciesDev sd = species_dev(q.spec);
     KN_CHECK(hipMemcpy(g->sp + h, &sd, sizeof sd, hipMemcpyHostToDevice));
+    if (q.integ != integ0 || q.vcrit != vcrit0) {
+        group_plan(g);
+        group_schedule(g);
+        g->gen++;
+    }
     return 0;
 }
 
@@ -2175,5 +2258,38 @@ extern "C" int kn_group_set_streams(kn_group *g, int nstream)
     g->nstream = nstream;
     if (nstream) group_schedule(g);
     g->gen++;  // a captured graph is stale
+    return 0;
+}
+
+// The parameters get_delta_nu (delta_tot_table.c:507-555) derives from a table,
+// a and mnu, as one group history (na = na_max = ia; hubble = kn_hubble_ctx).
+extern "C" double kn_hubble_ctx(double a, void *) { return kn_hubble(a); }
+
+extern "C" int kn_batch_history_from_table(const _delta_tot_table *d, double a, double mnu, kn_batch_history *out)
+{
+    if (!d || !out || d->ia < 1) return KN_ERR_ARG;
+    kn_batch_history hh;
+    memset(&hh, 0, sizeof hh);
+    hh.nk = d->nk;
+    hh.na = d->ia;
+    hh.na_max = d->ia;
+    hh.a = a;
+    hh.a_max = a;
+    hh.time_transfer = d->TimeTransfer;
+    hh.mnu = mnu;
+    hh.kBtnu = d->omnu->kBtnu;
+    hh.light = d->light;
+    hh.delta_nu_prefac = d->delta_nu_prefac;
+    hh.nufrac_low = d->omnu->hybnu.nufrac_low[0];
+    hh.vcrit = d->omnu->hybnu.vcrit;
+    hh.hybrid = d->omnu->hybnu.enabled;
+    hh.nu_crit_time = d->omnu->hybnu.nu_crit_time;
+    kn_species s;
+    kn_species_for(d, a, mnu, &s);
+    hh.qc = s.qc;
+    hh.relerr = s.relerr;
+    hh.ic_only = s.ic_only;
+    hh.hubble = kn_hubble_ctx;
+    *out = hh;
     return 0;
 }

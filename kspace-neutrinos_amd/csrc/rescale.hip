@@ -135,6 +135,7 @@ extern "C" int kn_rescale_device(fftw_complex *d_field, int dims, int startslab,
     a.norm = norm;
     a.logkk = d_logkk;
     a.ratio = d_ratio;
+    a.bucket = nullptr;
     a.nan_flag = nan_flag_ptr();
     hipLaunchKernelGGL((kn_res::k_rescale_rows_pf<kResWaves, kResUnroll>), dim3(res_grid(a.rows, smem)),
                        dim3(64 * kResWaves), smem, (hipStream_t)stream, a);

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "kn_mem.h"
+
 namespace kn_res {
 
 __host__ __device__ inline int kval(int n, int dims) { return n <= dims / 2 ? n : n - dims; }
@@ -19,6 +21,7 @@ struct ResArgs {
     long long rows;
     double boxsize, norm;
     const double *logkk, *ratio;
+    const int *bucket;  // nbuckets + 1 segment indices (k_res_buckets), or null: each block builds its own
     int *nan_flag;
 };
 
@@ -60,9 +63,36 @@ __device__ __forceinline__ Tab load_tab(const ResArgs &a, int nthreads)
     T.norm = a.norm;
     T.log_kunit = log(2 * M_PI / a.boxsize);
     // bucket[q]: the GSL segment index (largest i <= n-2 with lk[i] <= x) at the bucket's left edge
-    for (int q = threadIdx.x; q <= a.nbuckets; q += nthreads) {
-        const double edge = T.x0 + q / T.inv_bw;
-        int lo = 0, hi = a.nbins - 1;
+    if (a.bucket) {
+        for (int q = threadIdx.x; q <= a.nbuckets; q += nthreads) bucket[q] = a.bucket[q];
+    } else {
+        for (int q = threadIdx.x; q <= a.nbuckets; q += nthreads) {
+            const double edge = T.x0 + q / T.inv_bw;
+            int lo = 0, hi = a.nbins - 1;
+            while (hi > lo + 1) {
+                const int mid = (lo + hi) >> 1;
+                if (lk[mid] > edge) hi = mid;
+                else lo = mid;
+            }
+            bucket[q] = lo;
+        }
+    }
+    __syncthreads();
+    return T;
+}
+
+// The bucket index of load_tab, once for all blocks of a sweep (one block).
+__global__ __launch_bounds__(256) void k_res_buckets(const double *__restrict__ lk, int nbins, int nbuckets,
+                                                      int *__restrict__ bucket)
+{
+    if (nbins < 2) {
+        for (int q = threadIdx.x; q <= nbuckets; q += 256) bucket[q] = 0;
+        return;
+    }
+    const double x0 = lk[0], inv_bw = nbuckets / (lk[nbins - 1] - x0);
+    for (int q = threadIdx.x; q <= nbuckets; q += 256) {
+        const double edge = x0 + q / inv_bw;
+        int lo = 0, hi = nbins - 1;
         while (hi > lo + 1) {
             const int mid = (lo + hi) >> 1;
             if (lk[mid] > edge) hi = mid;
@@ -70,8 +100,6 @@ __device__ __forceinline__ Tab load_tab(const ResArgs &a, int nthreads)
         }
         bucket[q] = lo;
     }
-    __syncthreads();
-    return T;
 }
 
 // 1 + get_dnudcdm_powerspec(log(|k|)) for k2 > 0, with |k| = sqrt(k2) 2 pi / BoxSize.
@@ -200,14 +228,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_rescale_runs(ResArgs a)
 }
 
 // Software-pipelined coalesced sweep: the next UNROLL modes are loaded
-// before the current ones are rescaled and stored.
-template <int WAVES, int UNROLL>
+// before the current ones are rescaled and stored; NTL / NTS put the
+// non-temporal hint on the loads / stores.
+template <int WAVES, int UNROLL, bool NTL = false, bool NTS = false, bool REV = false>
 __global__ __launch_bounds__(64 * WAVES) void k_rescale_rows_pf(ResArgs a)
 {
     constexpr int B = 64 * WAVES;
     const Tab T = load_tab(a, B);
     const int rowlen = a.dims / 2 + 1;
-    const long long r0 = a.rows * blockIdx.x / gridDim.x, r1 = a.rows * (blockIdx.x + 1) / gridDim.x;
+    // REV: the first blocks dispatched take the last rows
+    const long long blk = REV ? (long long)gridDim.x - 1 - blockIdx.x : blockIdx.x;
+    const long long r0 = a.rows * blk / gridDim.x, r1 = a.rows * (blk + 1) / gridDim.x;
     const long long m1 = r1 * rowlen;
     const long long S = (long long)B * UNROLL;
     Pos p = pos_of(threadIdx.x, r0, rowlen, a.dims);
@@ -217,14 +248,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_rescale_rows_pf(ResArgs a)
 #pragma unroll
     for (int u = 0; u < UNROLL; u++) {
         const long long mu = m + (long long)B * u;
-        cur[u] = mu < m1 ? a.F[mu] : make_double2(0.0, 0.0);
+        cur[u] = mu < m1 ? kn_mem::ld<NTL>(a.F + mu) : make_double2(0.0, 0.0);
     }
     for (; m < m1; m += S) {
         double2 nxt[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
             const long long mu = m + S + (long long)B * u;
-            nxt[u] = mu < m1 ? a.F[mu] : make_double2(0.0, 0.0);
+            nxt[u] = mu < m1 ? kn_mem::ld<NTL>(a.F + mu) : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
@@ -240,7 +271,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_rescale_rows_pf(ResArgs a)
             double2 f = cur[u];
             f.x *= s;
             f.y *= s;
-            a.F[mu] = f;
+            kn_mem::st<NTS>(a.F + mu, f);
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) cur[u] = nxt[u];

@@ -54,12 +54,12 @@ class KspaceParams(C.Structure):
 
 
 class BatchHistory(C.Structure):
-    """kn_batch_history: one get_delta_nu integral of a batch (hubble: C function address)."""
+    """kn_batch_history: one get_delta_nu integral of a group (hubble: C function address)."""
     _fields_ = [("nk", C.c_int), ("na", C.c_int), ("na_max", C.c_int), ("a", C.c_double), ("a_max", C.c_double),
                 ("time_transfer", C.c_double), ("mnu", C.c_double), ("kBtnu", C.c_double), ("light", C.c_double),
                 ("delta_nu_prefac", C.c_double), ("qc", C.c_double), ("relerr", C.c_double),
-                ("nufrac_low", C.c_double), ("vcrit", C.c_double), ("ic_only", C.c_int), ("hubble", C.c_void_p),
-                ("hubble_ctx", C.c_void_p)]
+                ("nufrac_low", C.c_double), ("vcrit", C.c_double), ("hybrid", C.c_int), ("nu_crit_time", C.c_double),
+                ("ic_only", C.c_int), ("hubble", C.c_void_p), ("hubble_ctx", C.c_void_p)]
 
 
 vpp = C.POINTER(C.c_void_p)
@@ -121,18 +121,6 @@ SIGNATURES = {
     "kn_last_error": (C.c_char_p, []),
     "kn_bin_power_adjoint": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_void_p,
                                        C.c_void_p]),
-    "kn_batch_create": (C.c_int, [vpp, C.c_int, C.POINTER(BatchHistory)]),
-    "kn_batch_destroy": (None, [C.c_void_p]),
-    "kn_batch_views": (C.c_int, [C.c_void_p] + [vpp] * 9),
-    "kn_batch_layout": (C.c_int, [C.c_void_p, llp, llp, llp]),
-    "kn_batch_solve": (C.c_int, [C.c_void_p, C.c_void_p]),
-    "kn_batch_adjoint": (C.c_int, [C.c_void_p, C.c_void_p]),
-    "kn_batch_append": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
-    "kn_batch_graph_capture": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
-    "kn_batch_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
-    "kn_batch_status": (C.c_int, [C.c_void_p]),
-    "kn_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
-    "kn_batch_set_mass": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "kn_batch_history_from_table": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, C.c_double,
                                               C.POINTER(BatchHistory)]),
     "kn_hubble_ctx": (C.c_double, [C.c_double, C.c_void_p]),
@@ -146,14 +134,13 @@ SIGNATURES = {
     "kn_group_graph_capture": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kn_group_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kn_group_status": (C.c_int, [C.c_void_p]),
+    "kn_group_subdivision": (C.c_int, [C.c_void_p, C.c_int, dp, C.POINTER(C.c_int)]),
     "kn_group_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kn_group_set_mass": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "kn_group_streams": (C.c_int, [C.c_void_p]),
     "kn_group_set_streams": (C.c_int, [C.c_void_p, C.c_int]),
     "kn_get_delta_nu_adjoint": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, dp, C.c_double, dp, dp, dp, dp,
                                           dp]),
-    "kn_batch_get_delta_nu_adjoint": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, dp, C.c_double, dp, dp, dp,
-                                                dp, dp]),
 }
 
 _lib = None
@@ -216,8 +203,8 @@ def bind(L):
 
 
 class Batch:
-    """Owner of a kn_batch (api "kn_batch") or kn_group (api "kn_group"): loads
-    histories into its device arrays and reads results back.
+    """Owner of a kn_group: loads histories into its device arrays and reads
+    results back.
 
     hists: list of BatchHistory (their hubble_ctx objects must outlive the batch).
     L, memcpy: the library build and its device copy (defaults: this library, hipMemcpy).
@@ -226,7 +213,7 @@ class Batch:
     VIEWS = ("scalefact", "delta_tot", "wavenum", "delta_nu_init", "delta_nu", "gbar", "dtot_bar", "init_bar",
              "mnu_bar")
 
-    def __init__(self, hists, keep=(), L=None, memcpy=None, api="kn_batch"):
+    def __init__(self, hists, keep=(), L=None, memcpy=None, api="kn_group"):
         L = L or lib()
         self.L = L
         self.api = api
@@ -295,6 +282,16 @@ class Batch:
     def status(self):
         return self._f("status")(self.h)
 
+    def subdivision(self, i):
+        """History i's final QAG subdivisions: (ivl[nk, 200, 2], nint[nk])."""
+        import numpy as np
+        nk = self.hists[i].nk
+        ivl = np.zeros((nk, 200, 2))
+        nint = np.zeros(nk, dtype=np.int32)
+        _check(self._f("subdivision")(self.h, i, ivl.ctypes.data_as(dp), nint.ctypes.data_as(C.POINTER(C.c_int))),
+               self.api + "_subdivision", self.L)
+        return ivl, nint
+
     def delta_nu(self, i=None):
         if i is None:
             return self._get("delta_nu", 0, self.off_k[self.n])
@@ -314,7 +311,7 @@ class Batch:
         _check(self._f("set_mass")(self.h, i, mnu), self.api + "_set_mass", self.L)
 
     def set_streams(self, n):
-        """kn_group only: 0 merged launches, n > 0 per-history sequences on n streams."""
+        """0 merged launches, n > 0 per-history sequences on n streams."""
         _check(self._f("set_streams")(self.h, n), self.api + "_set_streams", self.L)
 
     def streams(self):
@@ -370,7 +367,7 @@ def delta_nu_function():
     """torch.autograd.Function over get_delta_nu of one table:
     apply(delta_tot[nk, na], delta_nu_init[nk], mnu[()], state) -> delta_nu[nk],
     where state = DeltaNuState(table, a, wavenum).  Gradients come from the
-    C-ABI adjoint (kn_batch_adjoint) with the forward's QAG subdivision."""
+    C-ABI adjoint (kn_group_adjoint) with the forward's QAG subdivision."""
     import torch
 
     class DeltaNuFunction(torch.autograd.Function):
@@ -442,7 +439,7 @@ def delta_nu_plain_function():
 
 
 class DeltaNuState:
-    """A one-history batch for get_delta_nu(table, a, wavenum, .) whose inputs
+    """A one-history group for get_delta_nu(table, a, wavenum, .) whose inputs
     come from torch tensors (device memory, float64)."""
 
     def __init__(self, table, a, wavenum, mnu, L=None, memcpy=None):

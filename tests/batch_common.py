@@ -89,8 +89,8 @@ def compare_adjoint(name, gpu, orc):
     assert e_f < RTOL and e_y < RTOL and e_i < RTOL and e_m < RTOL
 
 
-def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None, api="kn_batch", streams=None):
-    """Histories over three cosmologies with mixed lengths and k counts (a kn_batch or a kn_group)."""
+def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None, api="kn_group", streams=None):
+    """Histories over three cosmologies with mixed lengths and k counts, in one kn_group."""
     rng = np.random.default_rng(seed)
     cosmos = [Cosmo([0.15, 0, 0]), Cosmo([0.1, 0.1, 0.1]), Cosmo([0.3, 0.06, 0.06], hybrid=(500.0, 0.3333))]
     specs = []
@@ -112,3 +112,96 @@ def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None, api="kn_b
     return cosmos, specs, B
 
 
+
+
+def append_flow(L=None, memcpy=None, nk=6, steps=6):
+    """Histories whose derived state changes while steps are appended, each
+    step compared with the oracle's get_delta_nu at the new a:
+      0: created with one stored step (initial-condition piece only), then integrates;
+      1: hybrid (vcrit 500 km/s, particles from a = 0.3333): qc and the tolerance switch on;
+      2: hybrid with vcrit 1e5 km/s: past a = 0.3333 particles are nearly all
+         the neutrinos and only the initial-condition piece remains;
+      3: massless, then given a mass by set_mass before the last step.
+    Returns the worst relative error over all steps."""
+    cosmos = [Cosmo([0.2, 0, 0]), Cosmo([0.3, 0, 0], hybrid=(500.0, 0.3333)),
+              Cosmo([0.3, 0, 0], hybrid=(1e5, 0.3333))]
+    rng = np.random.default_rng(17)
+    a_start = [0.0101, 0.25, 0.25, 0.25]
+    specs = []
+    for i, c in enumerate([0, 1, 2, 0]):
+        m = cosmos[c].masses[0] if i != 3 else 0.0
+        if i == 0:
+            logs = np.array([math.log(0.01)])
+            k = np.geomspace(2e-4, 5e-2, nk)
+            rows = np.outer(np.linspace(1.0, 2.0, nk), [1.0])
+            init = 0.4 * rows[:, 0]
+        else:
+            logs, rows, k, init = synthetic_history(20, nk, a_start[i], seed=40 + i, noise=0.02)
+        specs.append(dict(c=c, m=m, logs=logs, rows=rows, k=k, init=init, a=a_start[i]))
+    hists = [cosmos[s["c"]].history(nk, len(s["logs"]), len(s["logs"]) + steps + 1, s["a"], 0.6, s["m"]) for s in specs]
+    B = knhip.Batch(hists, keep=cosmos, L=L, memcpy=memcpy)
+    for i, s in enumerate(specs):
+        B.load(i, s["logs"], s["rows"], s["k"], s["init"])
+    worst = 0.0
+    for step in range(steps + 1):
+        if step:
+            loga, rows = np.zeros(B.n), np.zeros(B.off_k[B.n])
+            for i, s in enumerate(specs):
+                s["a"] = 0.25 + 0.05 * step if i else math.exp(s["logs"][-1]) * 1.5
+                r = s["rows"][:, -1] * (1.05 + 0.01 * rng.standard_normal(nk))
+                s["logs"] = np.append(s["logs"], math.log(s["a"]))
+                s["rows"] = np.concatenate([s["rows"], r[:, None]], axis=1)
+                loga[i] = math.log(s["a"])
+                rows[B.off_k[i]:B.off_k[i] + nk] = r
+            B.append(loga, rows)
+        if step == steps:
+            specs[3]["m"] = 0.15
+            B.set_mass(3, 0.15)
+        B.solve()
+        for i, s in enumerate(specs):
+            T = cosmos[s["c"]].oracle(nk, s["logs"], s["rows"], s["init"])
+            rc, o = T.get_delta_nu(s["a"], s["k"], s["m"])
+            assert rc == 0
+            e = rel(B.delta_nu(i), o)
+            print("  step %d hist %d (Na=%d a=%.4f m=%g): rel err %.2e" % (step, i, len(s["logs"]), s["a"], s["m"], e))
+            worst = max(worst, e)
+    # a step that does not follow the last stored one is rejected before anything changes
+    sf = B._get("scalefact", 0, B.off_s[B.n])
+    bad = np.array([s["logs"][-1] for s in specs])
+    with pytest_raises_code(3002):
+        B.append(bad, np.zeros(B.off_k[B.n]))
+    assert np.array_equal(B._get("scalefact", 0, B.off_s[B.n]), sf)
+    B.solve()
+    for i, s in enumerate(specs):
+        T = cosmos[s["c"]].oracle(nk, s["logs"], s["rows"], s["init"])
+        rc, o = T.get_delta_nu(s["a"], s["k"], s["m"])
+        worst = max(worst, rel(B.delta_nu(i), o))
+    B.close()
+    for c in cosmos:
+        c.free()
+    return worst
+
+
+def pytest_raises_code(code):
+    import pytest
+    return pytest.raises(RuntimeError, match="code %d" % code)
+
+
+def compare_subdivisions(B, cosmos, specs):
+    """Each history's final QAG subdivisions (interval lists in QUADPACK's
+    workspace order) against the oracle's recorded ones: the same number of
+    intervals and the same end points bit for bit, i.e. the same sequence of
+    bisection decisions.  Returns (items compared, total intervals)."""
+    items = total = 0
+    for i, s in enumerate(specs):
+        T = cosmos[s["c"]].oracle(s["nk"], s["logs"], s["rows"], s["init"])
+        rc, _, oivl, onint = T.get_delta_nu_record(s["amax"], s["k"], s["m"])
+        assert rc == 0
+        ivl, nint = B.subdivision(i)
+        np.testing.assert_array_equal(nint, onint, err_msg="history %d: interval counts" % i)
+        for k in range(s["nk"]):
+            n = int(nint[k])
+            assert np.array_equal(ivl[k, :n], oivl[k, :n]), "history %d k %d: interval lists differ" % (i, k)
+            items += n > 0
+            total += n
+    return items, total

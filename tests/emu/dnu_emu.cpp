@@ -124,3 +124,9 @@ extern "C" void emu_qag_sort_check(const double *errs, int nsteps, int limit, in
 {
     hipLaunchKernelGGL(k_qag_sort_test, dim3(1), dim3(64), 0, 0, errs, nsteps, limit, bad);
 }
+
+// The device rule's error rescaling (dnu_device.h gk_rescale), for tests.
+extern "C" double emu_gk_rescale(double err, double resabs, double resasc)
+{
+    return kn_dnu::gk_rescale(err, resabs, resasc);
+}

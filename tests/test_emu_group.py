@@ -272,3 +272,29 @@ def test_emu_wavefront_error_order_equals_quadpack(limit):
     bad = (C.c_int * nsteps)()
     lib.emu_qag_sort_check(e, nsteps, limit, bad)
     assert sum(bad[:min(nsteps, limit - 1)]) == 0
+
+
+def test_emu_group_append_rederives_hybrid_and_ic_state(emu):
+    """Appends that change what get_delta_nu derives (delta_tot_table.c:513-555):
+    a group created at one stored step, hybrid histories crossing nu_crit_time
+    (one of them into initial-condition-only), a massless history given a mass,
+    and a non-increasing step rejected with nothing changed."""
+    from batch_common import append_flow
+    worst = append_flow(L=emu, memcpy=host_memcpy)
+    print("  worst rel err %.3e" % worst)
+    assert worst < RTOL
+
+
+def test_emu_group_subdivisions_equal_the_oracle(emu):
+    """The device QAG (run by the emulation) makes the oracle's bisection
+    decisions: per item the same interval count and the same interval list,
+    bit for bit, over three cosmologies (one hybrid) and Na 2..150."""
+    from batch_common import compare_subdivisions
+    cosmos, specs, B = mixed_batch(4, seed=3, L=emu, memcpy=host_memcpy)
+    B.solve()
+    items, total = compare_subdivisions(B, cosmos, specs)
+    print("  %d items, %d intervals identical" % (items, total))
+    assert items > 0
+    B.close()
+    for c in cosmos:
+        c.free()

@@ -303,3 +303,30 @@ def test_group_merged_equals_streams_bit_for_bit(kn, torch_cuda):
         _free(cosmos)
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+def test_group_append_rederives_hybrid_and_ic_state(kn, torch_cuda):
+    """On the device: a group created at one stored step, hybrid histories
+    crossing nu_crit_time (one into initial-condition-only), a massless
+    history given a mass, a non-increasing step rejected with nothing changed;
+    every step against the oracle (delta_tot_table.c:513-555)."""
+    from batch_common import append_flow
+    print("group append flow", flush=True)
+    worst = append_flow()
+    print("  worst rel err %.3e" % worst)
+    assert worst < RTOL
+
+
+def test_group_subdivisions_equal_the_oracle(kn, torch_cuda):
+    """The device QAG makes the oracle's bisection decisions: per item the
+    same interval count and interval list, bit for bit (48 histories, three
+    cosmologies, one hybrid, Na 2..150)."""
+    from batch_common import compare_subdivisions
+    print("group subdivisions vs oracle", flush=True)
+    cosmos, specs, B = mixed_batch(16, seed=3)
+    B.solve()
+    items, total = compare_subdivisions(B, cosmos, specs)
+    print("  %d items, %d intervals identical" % (items, total))
+    assert items > 0
+    B.close()
+    _free(cosmos)
