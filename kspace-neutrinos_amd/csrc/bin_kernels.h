@@ -414,10 +414,24 @@ __global__ __launch_bounds__(64 * WAVES) void k_bin_power_seg(BinArgs a)
 }
 
 // Slot of mode m (0 <= m < 64*RUN) of a wavefront's staging buffer.  Lanes
-// write consecutive modes and read back modes 4 apart (runs of 4); XOR-ing
-// bits 3-4 of m into bits 0-1 spreads both patterns over distinct 16-byte
-// bank groups.  Stays inside m's aligned group of 8 slots.
-__device__ __forceinline__ int stage_slot(int m) { return m ^ ((m >> 3) & 3); }
+// write consecutive modes (m = 64u + l) and read back runs (m = RUN l + r).
+// A 16-byte access pass serves 8 or 16 lanes; for the reads, the lanes of a
+// pass differ in the bits of l above the run length, so those bits (folded
+// so that both pass widths see a one-to-one map) are XOR-ed into the slot's
+// low bits; the writes of a pass keep distinct low bits since the XOR term is
+// constant across each aligned group of 8 lanes.  The slot stays inside m's
+// aligned group of 2, 4 or 8 slots.
+template <int RUN>
+__device__ __forceinline__ int stage_slot(int m)
+{
+    static_assert(RUN == 2 || RUN == 4 || RUN == 8, "runs of 2, 4 or 8");
+    if constexpr (RUN == 2)  // l bits l0..l3 = m1..m4: XOR l2 ^ l3 into m0
+        return m ^ (((m >> 3) ^ (m >> 4)) & 1);
+    else if constexpr (RUN == 4)  // l bits l0..l3 = m2..m5: XOR (l1 ^ l3, l2) into m0, m1
+        return m ^ ((((m >> 3) ^ (m >> 5)) & 1) | (((m >> 4) & 1) << 1));
+    else  // l bits l0..l3 = m3..m6: XOR (l0 ^ l3, l1, l2) into m0..m2
+        return m ^ ((((m >> 3) ^ (m >> 6)) & 1) | (((m >> 4) & 1) << 1) | (((m >> 5) & 1) << 2));
+}
 
 // Coalesced loads, per-lane runs: wave w takes chunks of 64*RUN consecutive
 // modes (w, w + WAVES, ... within the block's row range).  Load u of a chunk
@@ -439,7 +453,6 @@ template <int WAVES, int RUN, int NH, bool NT, int PF = 1>
 __global__ __launch_bounds__(64 * WAVES) void k_bin_power_lds(BinArgs a)
 {
     static_assert(WAVES % NH == 0, "histogram copies must divide the wavefronts");
-    static_assert(RUN == 4, "the staging swizzle assumes runs of 4");
     extern __shared__ double smem[];
     double2 *stage_all = (double2 *)smem;  // WAVES * 64 * RUN modes, 16-byte aligned
     double *hist = (double *)(stage_all + (size_t)WAVES * 64 * RUN);
@@ -475,15 +488,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_bin_power_lds(BinArgs a)
             const long long mu = c + PF * S + 64 * u + lane;
             buf[PF][u] = mu < m1 ? kn_mem::ld<NT>(a.F + mu) : make_double2(0.0, 0.0);
         }
-        // transpose through LDS: lane l wrote modes l, 64+l, ...; reads 4l .. 4l+3
+        // transpose through LDS: lane l wrote modes l, 64+l, ...; reads RUN l .. RUN l + RUN - 1
 #pragma unroll
-        for (int u = 0; u < RUN; u++) stage[stage_slot(64 * u + lane)] = buf[0][u];
+        for (int u = 0; u < RUN; u++) stage[stage_slot<RUN>(64 * u + lane)] = buf[0][u];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         double2 f[RUN];
 #pragma unroll
-        for (int r = 0; r < RUN; r++) f[r] = stage[stage_slot(RUN * lane + r)];
+        for (int r = 0; r < RUN; r++) f[r] = stage[stage_slot<RUN>(RUN * lane + r)];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

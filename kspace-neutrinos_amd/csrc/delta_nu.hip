@@ -840,8 +840,11 @@ extern "C" void kn_dnu_release(const _delta_tot_table *d)
     g_dev.erase(it);
 }
 
-static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum, int nspecies,
-                      const kn_species *sp, double *out, bool record);
+// Returns 0, or 2001 when the QAG or the interpolation fails: fatal
+// (terminate 2001, as the reference's GSL error handler) unless report is
+// set, when the code is returned with the message in kn_last_error.
+static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum, int nspecies,
+                     const kn_species *sp, double *out, bool record, bool report = false);
 
 extern "C" void kn_dnu_solve(const _delta_tot_table *d, double a, const double *wavenum, int nspecies,
                              const kn_species *sp, double *out)
@@ -849,8 +852,8 @@ extern "C" void kn_dnu_solve(const _delta_tot_table *d, double a, const double *
     dnu_solve(d, a, wavenum, nspecies, sp, out, false);
 }
 
-static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum, int nspecies,
-                      const kn_species *sp, double *out, bool record)
+static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum, int nspecies,
+                     const kn_species *sp, double *out, bool record, bool report)
 {
     const int Na = d->ia, nk = d->nk;
     const double x0 = log(d->TimeTransfer), xa = log(a);
@@ -867,8 +870,8 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     std::vector<int> need;
     for (int s = 0; s < nspecies; s++)
         if (!sp[s].ic_only) need.push_back(s);
-    if (need.empty()) return;
-    if (Na < 2) return;
+    if (need.empty()) return 0;
+    if (Na < 2) return 0;
 
     upload_constants();
     kn_htable_ensure(x0, x0, xa);
@@ -1009,13 +1012,20 @@ static void dnu_solve(const _delta_tot_table *d, double a, const double *wavenum
     KN_CHECK(hipMemcpyAsync(nint, dev->nint, sizeof(int) * nitems, hipMemcpyDeviceToHost, 0));
     KN_CHECK(hipMemcpyAsync(status, dev->status, sizeof(int) * 2, hipMemcpyDeviceToHost, 0));
     KN_CHECK(hipStreamSynchronize(0));
-    if (status[1]) kn_terminate(2001, "GSL_ERROR-equivalent: interpolation outside the stored table (GSL_EDOM)\n");
-    if (status[0]) kn_terminate(2001, "GSL_ERROR-equivalent: QAG failed in get_delta_nu (status %d)\n", status[0]);
+    if (status[1] || status[0]) {
+        if (status[1])
+            kn_set_last_error("GSL_ERROR-equivalent: interpolation outside the stored table (GSL_EDOM)");
+        else
+            kn_set_last_error("GSL_ERROR-equivalent: QAG failed in get_delta_nu (status %d)", status[0]);
+        if (report) return 2001;
+        kn_terminate(2001, "%s\n", kn_last_error());
+    }
     g_last_max_intervals = 0;
     for (int i = 0; i < nitems; i++) g_last_max_intervals = nint[i] > g_last_max_intervals ? nint[i] : g_last_max_intervals;
     for (size_t i = 0; i < need.size(); i++)
         for (int k = 0; k < nk; k++)
             out[(size_t)need[i] * nk + k] += d->delta_nu_prefac * res[i * nk + k];
+    return 0;
 }
 
 // d specialJ / dx of the rational fit (delta_tot_table.c:417-428), the branch
@@ -1044,7 +1054,8 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
     kn_species_for(d, a, mnu, &s);
     const int nk = d->nk, Na = d->ia;
     std::vector<double> out(nk);
-    dnu_solve(d, a, wavenum, 1, &s, out.data(), true);
+    const int rc = dnu_solve(d, a, wavenum, 1, &s, out.data(), true, true);
+    if (rc) return rc;
     if (delta_nu) memcpy(delta_nu, out.data(), sizeof(double) * nk);
     // initial-condition piece (delta_tot_table.c:522-535)
     const double fsl_A0a = fslength(log(d->TimeTransfer), log(a), d->light);

@@ -997,17 +997,17 @@ int orc_psindex(long long k2, int dims, int nrbins)
     return (int)floor(binsperunit * log(sqrt((double)k2)));
 }
 
-/* powerspectrum.c:33-117, single rank (the allreduces are identities) */
-int orc_total_powerspectrum(int dims, const double *field, int nrbins, int startslab, int nslab,
-                            double *power, long long *count, double *keffs)
+/* The accumulation loop of powerspectrum.c:56-89 on one rank's slabs: per bin
+ * the sums of c |F|^2 W^-4, c and c |k| (c the multiplicity), before the
+ * allreduce and normalisation. */
+void orc_bin_sums(int dims, const double *field, int nrbins, int startslab, int nslab, double *power,
+                  long long *count, double *keffs)
 {
 #define KVAL(n) ((n) <= dims / 2 ? (n) : ((n)-dims))
 #define RE(ix) field[2 * (size_t)(ix)]
 #define IM(ix) field[2 * (size_t)(ix) + 1]
     const double binsperunit = (nrbins - 1) / log(sqrt(3) * dims / 2.0);
-    double total_mass2 = 0;
-    int i, nonzero;
-    if (startslab == 0) total_mass2 = RE(0) * RE(0) + IM(0) * IM(0);
+    int i;
     memset(power, 0, sizeof(double) * (size_t)nrbins);
     memset(count, 0, sizeof(long long) * (size_t)nrbins);
     memset(keffs, 0, sizeof(double) * (size_t)nrbins);
@@ -1044,6 +1044,19 @@ int orc_total_powerspectrum(int dims, const double *field, int nrbins, int start
             }
         }
     }
+#undef RE
+#undef IM
+#undef KVAL
+}
+
+/* powerspectrum.c:33-117, single rank (the allreduces are identities) */
+int orc_total_powerspectrum(int dims, const double *field, int nrbins, int startslab, int nslab,
+                            double *power, long long *count, double *keffs)
+{
+    double total_mass2 = 0;
+    int i, nonzero;
+    if (startslab == 0) total_mass2 = field[0] * field[0] + field[1] * field[1];  /* powerspectrum.c:45-47 */
+    orc_bin_sums(dims, field, nrbins, startslab, nslab, power, count, keffs);
     for (i = 0; i < nrbins; i++) {
         power[i] /= total_mass2;
         if (count[i]) {
@@ -1062,9 +1075,6 @@ int orc_total_powerspectrum(int dims, const double *field, int nrbins, int start
         }
     }
     return nonzero;
-#undef RE
-#undef IM
-#undef KVAL
 }
 
 /* ======================= delta_pow + rescale ======================= */

@@ -132,6 +132,9 @@ int orc_get_delta_nu_adjoint(orc_dtt *d, double a, const double wavenum[], doubl
 double orc_specialJ_dmu(double kfsl, double mu, double vcrit, int hybrid, double nufrac_low);
 
 /* ---- k-shell binning and rescale (powerspectrum.c:7-117, interface_gadget.c:158-194) ---- */
+/* powerspectrum.c:56-89: raw per-bin sums of one rank's slabs (no normalisation, no compaction). */
+void orc_bin_sums(int dims, const double *field, int nrbins, int startslab, int nslab, double *power,
+                  long long *count, double *keffs);
 int orc_total_powerspectrum(int dims, const double *field /* re,im interleaved */, int nrbins,
                             int startslab, int nslab, double *power, long long *count, double *keffs);
 /* Bin index of an integer |k|^2 exactly as powerspectrum.c:40,67 evaluates it. */

@@ -85,6 +85,7 @@ def lib():
             "orc_get_delta_nu_update": (C.c_int, [C.POINTER(OrcDtt), C.c_double, C.c_int, dp, dp, dp,
                                                   C.POINTER(OrcTransfer)]),
             "orc_total_powerspectrum": (C.c_int, [C.c_int, dp, C.c_int, C.c_int, C.c_int, dp, llp, dp]),
+            "orc_bin_sums": (None, [C.c_int, dp, C.c_int, C.c_int, C.c_int, dp, llp, dp]),
             "orc_psindex": (C.c_int, [C.c_longlong, C.c_int, C.c_int]),
             "orc_get_dnudcdm": (C.c_double, [C.POINTER(OrcDeltaPow), C.c_double]),
             "orc_rescale": (C.c_int, [C.c_int, dp, C.c_int, C.c_int, C.c_double, C.POINTER(OrcDeltaPow)]),
@@ -139,6 +140,17 @@ def bin_power_adjoint(field_c, nrbins, gbar, startslab=0):
     out = np.zeros_like(f)
     rc = lib().orc_bin_power_adjoint(dims, ptr(f), nrbins, startslab, field_c.shape[0], ptr(g), len(g), ptr(out))
     return rc, out.view(np.complex128).reshape(field_c.shape)
+
+
+def bin_sums(field_c, nrbins, startslab=0):
+    """Raw per-bin sums (power, count, keff) of the slabs in field_c [nslab][N][N/2+1]."""
+    dims = field_c.shape[1]
+    f = np.ascontiguousarray(field_c).view(np.float64).ravel()
+    power, keffs = np.zeros(nrbins), np.zeros(nrbins)
+    count = np.zeros(nrbins, dtype=np.int64)
+    lib().orc_bin_sums(dims, ptr(f), nrbins, startslab, field_c.shape[0], ptr(power), count.ctypes.data_as(llp),
+                       ptr(keffs))
+    return power, count, keffs
 
 
 def rescale(field_c, boxsize, logkk, ratio, norm, startslab=0):

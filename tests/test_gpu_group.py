@@ -192,16 +192,19 @@ def test_group_at_batching_workload_shape(kn, torch_cuda):
         B.load(i, s["logs"], s["rows"], s["k"], s["init"])
     print("group at the batching workload shape: %d histories" % B.n, flush=True)
     B.solve()
-    g = rng.uniform(-1, 1, B.off_k[B.n])
+    # dL/d delta_nu on 3 sampled k per history and zero elsewhere: d/d mnu is
+    # then the sum over those k alone, which the oracle gives from their rows
+    idxs = [np.array(sorted(rng.choice(s["nk"], 3, replace=False))) for s in specs]
+    g = np.zeros(B.off_k[B.n])
+    for i, idx in enumerate(idxs):
+        g[B.off_k[i] + idx] = rng.uniform(-1, 1, len(idx))
     B.adjoint(g)
     mb = B.mnu_bar()
-    worst_f, worst_b = 0.0, 0.0
+    worst_f, worst_b, worst_m = 0.0, 0.0, 0.0
     for i, s in enumerate(specs):
-        idx = np.array(sorted(rng.choice(s["nk"], 3, replace=False)))
+        idx = idxs[i]
         T = cosmos[i].oracle(len(idx), s["logs"], s["rows"][idx], s["init"][idx])
-        gi = np.zeros(len(idx))
-        gi[:] = g[B.off_k[i] + idx]
-        orc = T.get_delta_nu_adjoint(1.0, s["k"][idx], s["m"], gi)
+        orc = T.get_delta_nu_adjoint(1.0, s["k"][idx], s["m"], g[B.off_k[i] + idx])
         assert orc[0] == 0
         dnu = B.delta_nu(i)[idx]
         worst_f = max(worst_f, rel(dnu, orc[1]))
@@ -209,9 +212,9 @@ def test_group_at_batching_workload_shape(kn, torch_cuda):
         scale = np.maximum(np.max(np.abs(orc[2]), axis=1, keepdims=True), 1e-300)
         worst_b = max(worst_b, float(np.max(np.abs(db - orc[2]) / scale)),
                       float(np.max(np.abs(B.init_bar(i)[idx] - orc[3]) / np.max(np.abs(orc[3])))))
-    print("  sampled k: forward %.3e, d/d delta_tot and d/d init %.3e" % (worst_f, worst_b))
-    assert worst_f < RTOL and worst_b < RTOL
-    assert np.all(np.isfinite(mb))
+        worst_m = max(worst_m, abs(mb[i] / orc[4] - 1))
+    print("  sampled k: forward %.3e, d/d delta_tot and d/d init %.3e, d/d mnu %.3e" % (worst_f, worst_b, worst_m))
+    assert worst_f < RTOL and worst_b < RTOL and worst_m < RTOL
     merged = _group_arrays(B)
     B.set_streams(4)
     print("  the same on 4 streams", flush=True)
