@@ -8,7 +8,14 @@ mode (16 B read by the bin sweep + 32 B read/write by the rescale sweep) times
 the mode count, divided by the time of the two sweeps (HIP events on the null
 stream both sweeps run on).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--no-cpu]
+The same line carries the stream ceilings this part sustains for the two
+sweeps' access patterns (plain kernels of the grid's size,
+libkn_stream_ceiling.so) and the two-sweep bound they imply, and the CPU
+baseline: the oracle (the C restatement of the reference's path) built with
+the reference's compile flags (-O2 -ffast-math), timed on one core and on all
+usable cores of the host (at most 16).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--no-cpu] [--no-ceiling]
 """
 import argparse
 import ctypes as C
@@ -53,20 +60,56 @@ def bbks_amplitude(N, box, rms=1e-2, ns=0.96, gamma=0.2 / 1000.0):
     return rms ** 2 * float(N) ** 6 / shape
 
 
-def cpu_baseline(N, nrbins, box, d_grid, torch, nslab):
-    """The oracle (C restatement, 1 core) on nslab slabs of the same grid."""
-    import oracle_py as O
+def cpu_baseline(N, nrbins, box, d_grid, nslab, one_core_slabs=32):
+    """The oracle built with the reference's flags on nslab slabs of the same
+    grid (bin sums + rescale): one core on the first one_core_slabs slabs, then
+    all usable cores (at most 16) on all nslab slabs, contiguous slab ranges
+    per thread as the reference's MPI ranks split the slabs."""
+    import cpu_timing as CT
+    info = CT.host_info()
     F = d_grid[:nslab].cpu().numpy()
-    modes = F.shape[0] * F.shape[1] * F.shape[2]
-    t0 = time.perf_counter()
-    n, p, c, k = O.total_powerspectrum(F, nrbins, 0, nslab)
-    t1 = time.perf_counter()
-    logkk = np.log(k[:n] * 2 * math.pi / box)
-    O.rescale(F, box, logkk, np.full(n, 0.3), 0.05)
-    t2 = time.perf_counter()
-    return {"value": round(48.0 * modes / (t2 - t0) / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": "%d of %d slabs of the %d^3 grid (%d modes): oracle total_powerspectrum %.2fs + rescale %.2fs"
-                      % (nslab, N, N, modes, t1 - t0, t2 - t1)}
+    logkk = np.log(np.geomspace(2 * math.pi / box, math.sqrt(3) * math.pi * N / box, 256))
+    r = CT.grid_sweeps(F, nrbins, box, logkk, np.full(256, 0.3), 0.05, info["workers"], min(one_core_slabs, nslab))
+    del F
+    return {"value": round(r["all_cores_GBs"], 4), "unit": "GB/s", "cores": info["workers"], "kind": "port",
+            "sample": "%d of %d slabs of the %d^3 grid, bin sums + rescale, oracle built -O2 -ffast-math, %d threads "
+                      "over slab ranges, %.2fs" % (nslab, N, N, info["workers"], r["all_cores_s"]),
+            "one_core": {"value": round(r["one_core_GBs"], 4), "cores": 1,
+                         "sample": "%d slabs, %.2fs" % (r["one_core_slabs"], r["one_core_s"])},
+            "host": {"nproc": info["nproc"], "usable": info["usable"], "cpu_model": info["model"]}}
+
+
+def stream_ceiling(grid, modes, reps=10):
+    """GB/s of a plain read-only stream (16 B/mode) and an in-place read+write
+    stream (32 B/mode) over the grid's bytes, with and without the
+    non-temporal hint (libkn_stream_ceiling.so, null stream, HIP events),
+    and the bound they put on two dependent sweeps of 48 B/mode."""
+    import torch
+    L = C.CDLL(os.path.join(ROOT, "kspace-neutrinos_amd", "libkn_stream_ceiling.so"))
+    L.kn_stream_ceiling.restype = C.c_int
+    L.kn_stream_ceiling.argtypes = [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    sink = torch.zeros(1, dtype=torch.float64, device="cuda")
+    out = {}
+    for kind, name, nbytes, slots in ((0, "read", 16, 2), (1, "read_write", 32, 8)):
+        for nt in (0, 1):
+            assert L.kn_stream_ceiling(grid.data_ptr(), modes, kind, nt, slots, sink.data_ptr()) == 0
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                L.kn_stream_ceiling(grid.data_ptr(), modes, kind, nt, slots, sink.data_ptr())
+                e1.record()
+                e1.synchronize()
+                ts.append(e0.elapsed_time(e1) * 1e-3)
+            out["%s%s_GBs" % (name, "_nt" if nt else "")] = round(nbytes * modes / float(np.median(ts)) / 1e9, 1)
+    t2 = 16.0 * modes / (out["read_nt_GBs"] * 1e9) + 32.0 * modes / (out["read_write_nt_GBs"] * 1e9)
+    out["two_sweep_bound_GBs"] = round(48.0 * modes / t2 / 1e9, 1)
+    out["two_sweep_bound_frac_of_peak"] = round(out["two_sweep_bound_GBs"] / HBM_PEAK_GBS, 4)
+    out["note"] = ("plain kernels of the grid's size: read-only and in-place read+write streams, 512 threads per "
+                   "block, contiguous ranges per block; the bound is 48 B/mode over 16 B at the read ceiling plus "
+                   "32 B at the read+write ceiling, both non-temporal")
+    return out
 
 
 def dnu_bench(kn, nk, na=2000, calls=3, cpu_k=16, no_cpu=False):
@@ -94,16 +137,19 @@ def dnu_bench(kn, nk, na=2000, calls=3, cpu_k=16, no_cpu=False):
            "ms_per_call": round(tg * 1e3, 3), "kstep_per_s": round(ksteps / tg, 1),
            "note": "k.step = one (k bin, stored step) pair per distinct mass; plain C-ABI call, host in/out"}
     if not no_cpu:
+        import cpu_timing as CT
         sel = np.linspace(0, nk - 1, cpu_k).astype(int)
         Q = Pair([0.06, 0.06, 0.3], 0.3, cpu_k, 0.01 + na / 100.0 + 0.02, hybrid=(500.0, 0.3333))
         Q.set_history(logs, rows[sel], init[sel])
-        t0 = time.perf_counter()
-        rc, o = Q.T.get_delta_nu_combined(1.0, k[sel])
-        tc = time.perf_counter() - t0
-        res["cpu_baseline"] = {"kstep_per_s": round(2.0 * cpu_k * na / tc, 1), "cores": 1, "kind": "port",
-                               "sample": "%d of the %d k bins, oracle get_delta_nu_combined %.2fs" % (cpu_k, nk, tc)}
+        rc, o = Q.T.get_delta_nu_combined(1.0, k[sel])       # the strict oracle, for the error
         res["max_rel_err_vs_oracle"] = float(np.max(np.abs(out[sel] / o - 1)))
         Q.free()
+        amp = rows[sel, -1]                                   # delta_tot = amp a, a_max = 1
+        r = CT.dnu_one_core([([0.06, 0.06, 0.3], 0.3, (500.0, 0.3333), na, amp, k[sel], 1.0, None)])
+        res["cpu_baseline"] = {"kstep_per_s": round(r["kstep_per_s"], 1), "cores": 1, "kind": "port",
+                               "sample": "%d of the %d k bins, oracle built -O2 -ffast-math, get_delta_nu_combined "
+                                         "%.2fs (the reference computes this on every rank, one core each)"
+                                         % (cpu_k, nk, r["s"])}
     P.free()
     return res
 
@@ -115,9 +161,10 @@ class _HCtx(C.Structure):
 
 def build_histories(kn, nhist, rng, draw):
     """nhist tables with their own Omega_nu and H(a): draw(rng) -> (na, nk, m, Omega_m).
-    delta_tot = a sqrt(P(k)) (1 + 0.1 N(0,1)) with the noise drawn per k, log-spaced a
-    in [0.01, 1] and k in [2 pi/512, sqrt(3) pi 1024/512] h/Mpc; returns
-    (keep, hists, data) with the kn_batch_history of each table."""
+    delta_tot = D(a) sqrt(P(k)) (1 + 0.1 N(0,1)) with D(a) = a and the noise drawn
+    per k (smooth along a), log-spaced a in [0.01, 1] and k in
+    [2 pi/512, sqrt(3) pi 1024/512] h/Mpc; returns (keep, hists, data) with the
+    kn_batch_history of each table."""
     import conftest
     host = conftest.load_testhost()
     import knhip
@@ -149,7 +196,7 @@ def build_histories(kn, nhist, rng, draw):
         keep.append((om, d, ctx))
         hists.append(hh)
         data.append(dict(na=na, nk=nk, m=m, om0=om0, logs=logs, rows=np.array(rows[:nk, :na]), k=k,
-                         init=np.array(init)))
+                         init=np.array(init), amp=amp))
     return keep, hists, data
 
 
@@ -244,27 +291,29 @@ def plain_vs_group(kn, torch, keep, hists, data, reps=3, seed=5):
             "max_rel_diff_group_vs_plain": diff}
 
 
-def c4_cpu_baseline(data, nsample=4):
-    """The oracle (C restatement, 1 core) on a bounded sample of the same
-    workload: nsample whole histories (every k bin) spread over the set, each
-    with its own cosmology; k.step/s of the forward integral."""
-    from batch_common import Cosmo
-    tc, ksteps, names = 0.0, 0, []
-    for h in np.linspace(0, len(data) - 1, nsample).astype(int):
+def c4_cpu_baseline(data, nsample=4, nall=64):
+    """The oracle built with the reference's flags on whole histories of the
+    same workload (every k bin, each history its own cosmology): one core on
+    nsample histories spread over the set, all usable cores (at most 16) on
+    nall histories, one history per job."""
+    import cpu_timing as CT
+    info = CT.host_info()
+
+    def job(h):
         dd = data[h]
-        sel = np.arange(dd["nk"])
-        cz = Cosmo([dd["m"], 0.0, 0.0], omega0=dd["om0"])
-        T = cz.oracle(dd["nk"], dd["logs"], dd["rows"][sel], dd["init"][sel])
-        t0 = time.perf_counter()
-        rc, _ = T.get_delta_nu(1.0, dd["k"][sel], dd["m"])
-        tc += time.perf_counter() - t0
-        assert rc == 0
-        ksteps += dd["nk"] * dd["na"]
-        names.append("%d (nk %d, Na %d)" % (h, dd["nk"], dd["na"]))
-        del T
-        cz.free()
-    return {"kstep_per_s": round(ksteps / tc, 1), "cores": 1, "kind": "port",
-            "sample": "whole histories %s of the %d, oracle get_delta_nu %.2fs" % (", ".join(names), len(data), tc)}
+        return ([dd["m"], 0.0, 0.0], dd["om0"], None, dd["na"], dd["amp"], dd["k"], 1.0, dd["m"])
+    one = np.linspace(0, len(data) - 1, nsample).astype(int)
+    allh = np.linspace(0, len(data) - 1, min(nall, len(data))).astype(int)
+    r1 = CT.dnu_one_core([job(h) for h in one])
+    rn = CT.dnu_all_cores([job(h) for h in allh], info["workers"])
+    return {"kstep_per_s": round(rn["kstep_per_s"], 1), "cores": info["workers"], "kind": "port",
+            "sample": "%d whole histories of the %d (every %dth), oracle built -O2 -ffast-math get_delta_nu, one "
+                      "history per process, %.2fs" % (len(allh), len(data), max(1, len(data) // len(allh)), rn["s"]),
+            "one_core": {"kstep_per_s": round(r1["kstep_per_s"], 1), "cores": 1,
+                         "sample": "%d whole histories (%s), %.2fs"
+                                   % (nsample, ", ".join("%d: nk %d Na %d" % (h, data[h]["nk"], data[h]["na"])
+                                                         for h in one), r1["s"])},
+            "host": {"nproc": info["nproc"], "usable": info["usable"], "cpu_model": info["model"]}}
 
 
 def c4_batching(kn, torch, nhist=256, seed=1000, reps=3, no_cpu=False):
@@ -278,9 +327,9 @@ def c4_batching(kn, torch, nhist=256, seed=1000, reps=3, no_cpu=False):
     res = plain_vs_group(kn, torch, keep, hists, data, reps)
     if not no_cpu:
         res["cpu_baseline"] = c4_cpu_baseline(data)
-    res["workload"] = ("%d histories, Na~U[100,3000], nk~U[64,512], m~U[0.06,0.6] eV, Omega_m~U[0.25,0.35]"
-                       % nhist)
-    res["persistent_batch"] = "kn_batch (one persistent launch) switched off in this build: returns 3005"
+    res["workload"] = ("%d histories, Na~U[100,3000], nk~U[64,512], m~U[0.06,0.6] eV, Omega_m~U[0.25,0.35]; "
+                       "delta_tot = D(a) sqrt(P(k)) (1 + 0.1 N(0,1)) with D(a) = a and the noise drawn per k "
+                       "(drawn per (k, a) entry it makes the reference's own QAG fail, terminate 2001)" % nhist)
     return res
 
 
@@ -350,7 +399,8 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--nrbins", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-slabs", type=int, default=192)
+    ap.add_argument("--no-ceiling", action="store_true", help="skip the stream-ceiling measurement")
+    ap.add_argument("--cpu-slabs", type=int, default=256)
     ap.add_argument("--no-dnu", action="store_true", help="skip the free-streaming integral timings")
     ap.add_argument("--only-dnu", action="store_true",
                     help="only the get_delta_nu_combined timing (757 k x 2000 steps), for profiling")
@@ -457,8 +507,39 @@ def main():
         "rescale_sweep": {"ms": round(t_res.mean() * 1e3, 4), "GB/s": round(res_gbs, 1)},
         "wall_s": round(wall, 3),
     }
+    # the rescale fused with the PM Green's-function multiply that follows it
+    # in a force pass, against the two separate sweeps (rescale, then Green's
+    # function alone), on the same grid
+    def timed(fn, reps=5):
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return float(np.median(ts))
+    asmth = 1.25 * box / N
+    t_res = timed(lambda: kn.kn_rescale_device(grid.data_ptr(), N, 0, N, box, tab.data_ptr(), tab.data_ptr() + 8 * n,
+                                               n, norm, None))
+    t_green = timed(lambda: kn.kn_rescale_green_device(grid.data_ptr(), N, 0, N, box, None, None, 0, 0.0, asmth, None))
+    t_fused = timed(lambda: kn.kn_rescale_green_device(grid.data_ptr(), N, 0, N, box, tab.data_ptr(),
+                                                       tab.data_ptr() + 8 * n, n, norm, asmth, None))
+    out["rescale_green_fused"] = {
+        "workload": "rescale + PM Green's function (CiC deconvolution, r_s = 1.25 cells), %d^3" % N,
+        "fused_ms": round(t_fused * 1e3, 4), "two_sweeps_ms": round((t_res + t_green) * 1e3, 4),
+        "rescale_ms": round(t_res * 1e3, 4), "green_ms": round(t_green * 1e3, 4),
+        "fused_GBs": round(32.0 * modes / t_fused / 1e9, 1),
+        "note": "one read+write sweep (32 B/mode) in place of two (64 B/mode)"}
+    # the Green's factors shrink the field (high-k modes towards denormals): regenerate it
+    assert kn.kn_synth_grid(grid.data_ptr(), N, 0, N, box, amp, 0.96, 0.2 / 1000.0, 3) == 0
+    if not args.no_ceiling:
+        sc = stream_ceiling(grid, modes)
+        sc["frac_of_bound"] = round(value / sc["two_sweep_bound_GBs"], 4)
+        out["stream_ceiling"] = sc
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(N, nrbins, box, grid, torch, min(args.cpu_slabs, N))
+        out["cpu_baseline"] = cpu_baseline(N, nrbins, box, grid, min(args.cpu_slabs, N))
     if not args.no_dnu:
         del grid
         torch.cuda.empty_cache()

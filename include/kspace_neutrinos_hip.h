@@ -258,6 +258,27 @@ int kn_bin_power_device(const fftw_complex *d_field, int dims, int nrbins, int s
 int kn_rescale_device(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
                       const double *d_logkk, const double *d_ratio, int nbins, double norm, void *stream);
 
+/* get_delta_nu keeps each table's delta_tot history in device memory at the
+ * host's row stride and compares the host rows with its copy on every call:
+ * only columns from a row's first changed entry on are uploaded (an appended
+ * or overwritten step is one or two columns; any host write is seen).  This
+ * returns how many columns the last call uploaded (diagnostics, tests). */
+int kn_dnu_last_upload_columns(void);
+
+/* (c) fused with the next step of a PM force pass: no reference counterpart.
+ * The reference rescales the grid in add_nu_power_to_rhogrid and the caller
+ * then multiplies it by the potential's Green's function (Gadget-2's
+ * pmforce_periodic, right after the hook of 0002-Add-hooks-to-kspace-neutrino-code.patch:116-125):
+ * two read + write sweeps.  This does both in one: for k2 > 0,
+ * F *= 1 + norm lininterp(log k) (as kn_rescale_device), then
+ * F *= -exp(-k2 (2 pi asmth / boxsize)^2) / k2 * (W^-1_x W^-1_y W^-1_z)^4 (W^-1
+ * the per-axis CiC inverse window); the k = 0 mode of slab 0 is zeroed.
+ * nbins 0 (logkk, ratio NULL) applies the Green's function alone.
+ * asmth: the PM force split scale in box length units.  No host sync. */
+int kn_rescale_green_device(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
+                            const double *d_logkk, const double *d_ratio, int nbins, double norm, double asmth,
+                            void *stream);
+
 /* Last GPU error text (empty if none). */
 const char *kn_last_error(void);
 

@@ -66,17 +66,17 @@ void h_reset(double origin)
     g_h.dirty = true;
 }
 
-// Chebyshev coefficients of 1/(a^2 H) and 1/(a H) on the panel [lo, lo + width] of ln a.
-void cheb_panel(double lo, double width, double (*hub)(double, void *), void *ctx, double *coef)
-{
-    h_cheb_panel(lo, width, [&](double a) { return hub(a, ctx); }, coef);
-}
-
-double hub_process(double a, void *) { return kn_hubble(a); }
-
+// Panel p of the process table (a split panel's right piece appended).
 void h_build_panel(int p)
 {
-    cheb_panel(g_h.x0 + p * g_h.width, g_h.width, hub_process, nullptr, &g_h.coef[(size_t)p * kHStride]);
+    double right[kHStride];
+    const double lo = g_h.x0 + p * g_h.width;
+    if (h_panel(lo, g_h.x0 + (p + 1) * g_h.width, [](double a) { return kn_hubble(a); },
+                &g_h.coef[(size_t)p * kHStride], right)) {
+        const size_t idx = g_h.coef.size() / kHStride;
+        g_h.coef.insert(g_h.coef.end(), right, right + kHStride);
+        g_h.coef[(size_t)p * kHStride + kHRight] = (double)idx;
+    }
     g_h.built[p] = 1;
 }
 
@@ -638,9 +638,17 @@ struct DnuDev {
     size_t cap_fac = 0;
     double *fac = nullptr;  // spline factors of scalefact (fac_doubles(Na))
     double *gbar = nullptr, *ybar = nullptr, *cbar = nullptr, *mubar = nullptr;  // adjoint
+    // the delta_tot rows held in Y (row stride namax, the host table's), and a
+    // host copy of what Y holds: each call uploads a row only from its first
+    // entry that differs from that copy (an appended or overwritten step is
+    // one or two columns), whoever wrote the host table
+    int ystride = 0, have_nk = 0, have_na = 0;
+    std::vector<double> shadow;
+    std::vector<int> first;
 };
 std::unordered_map<const void *, DnuDev *> g_dev;
 int g_last_max_intervals = 0;
+int g_last_upload_cols = 0;
 
 template <typename T>
 void grow(T *&p, size_t &cap, size_t need)
@@ -826,6 +834,10 @@ extern "C" double kn_htable_eval_host(double x)
 
 extern "C" int kn_dnu_last_max_intervals(void) { return g_last_max_intervals; }
 
+// Columns of the delta_tot history the last get_delta_nu uploaded (the rest
+// was already on the device).
+extern "C" int kn_dnu_last_upload_columns(void) { return g_last_upload_cols; }
+
 extern "C" void kn_dnu_release(const _delta_tot_table *d)
 {
     auto it = g_dev.find(d);
@@ -890,12 +902,21 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
         dev->cap_fs = nfs;
     }
     grow(dev->sx, dev->cap_x, (size_t)Na);
-    if (dev->cap_y < (size_t)nk * Na) {  // Y, C
+    const int namax = d->namax;
+    if (dev->ystride != namax || dev->have_nk != nk) {  // a new layout: nothing of the history is on the device
+        dev->ystride = namax;
+        dev->have_nk = nk;
+        dev->have_na = 0;
+        dev->shadow.assign((size_t)nk * namax, 0.0);
+        dev->first.assign(nk, 0);
+    }
+    if (dev->cap_y < (size_t)nk * namax) {  // Y, C (their contents follow the shadow: start again)
         for (double **p : {&dev->Y, &dev->C}) {
             if (*p) KN_CHECK(hipFree(*p));
-            KN_CHECK(hipMalloc(p, sizeof(double) * (size_t)nk * Na));
+            KN_CHECK(hipMalloc(p, sizeof(double) * (size_t)nk * namax));
         }
-        dev->cap_y = (size_t)nk * Na;
+        dev->cap_y = (size_t)nk * namax;
+        dev->have_na = 0;
     }
     grow(dev->wavenum, dev->cap_k, (size_t)nk);
     grow(dev->bucket, dev->cap_b, (size_t)nsb + 1);
@@ -917,8 +938,37 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
         KN_CHECK(hipMalloc(&dev->ivl, sizeof(double) * (size_t)nitems * kLimit * 2));
         dev->cap_ivl = nitems;
     }
-    // pinned staging: inputs [Y packed | scalefact | wavenum | species | species index], then outputs
-    const size_t nY = (size_t)nk * Na;
+    // which columns of the history the device lacks: per row, the first entry
+    // that differs from the shadow (entries past the shadow's count differ)
+    {
+        const int have = dev->have_na < Na ? dev->have_na : Na;
+        const int parts = (size_t)nk * Na < (1u << 16) ? 1 : 8;
+        host_pool().run(parts, [&](int p) {
+            const int k0 = (int)((long long)nk * p / parts), k1 = (int)((long long)nk * (p + 1) / parts);
+            for (int k = k0; k < k1; k++) {
+                const double *src = d->delta_tot[k];
+                double *sh = &dev->shadow[(size_t)k * namax];
+                int f = 0;
+                while (f < have) {  // 32-entry blocks by memcmp, then the entry within the first differing block
+                    const int n = have - f < 32 ? have - f : 32;
+                    if (memcmp(src + f, sh + f, sizeof(double) * n)) {
+                        while (!memcmp(src + f, sh + f, sizeof(double))) f++;
+                        break;
+                    }
+                    f += n;
+                }
+                memcpy(sh + f, src + f, sizeof(double) * (Na - f));
+                dev->first[k] = f;
+            }
+        });
+    }
+    int col0 = Na;
+    for (int k = 0; k < nk; k++) col0 = dev->first[k] < col0 ? dev->first[k] : col0;
+    dev->have_na = Na;
+    const int ncol = Na - col0;  // columns [col0, Na) of every row go up
+    g_last_upload_cols = ncol;
+    // pinned staging: inputs [Y columns col0..Na | scalefact | wavenum | species | species index], then outputs
+    const size_t nY = (size_t)nk * ncol;
     const size_t b_sp = sizeof(double) * (nY + Na + nk);
     const size_t b_in = align64(b_sp + sizeof(SpeciesDev) * NUSPECIES + sizeof(int) * NUSPECIES);
     char *stage = pinned_stage(b_in + align64(sizeof(double) * nitems) + sizeof(int) * ((size_t)nitems + 2));
@@ -940,19 +990,23 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
     hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, 0, dev->fx, dev->fy, dev->fc, nfs, 1, 0LL,
                        0LL);
     KN_CHECK(hipGetLastError());
-    // meanwhile the history rows are packed to stride Na and uploaded on the
-    // copy stream, in chunks: a chunk's upload runs while the next is packed
-    {
+    // meanwhile the missing columns are packed (ncol per row) and uploaded on
+    // the copy stream into rows of stride namax, in chunks of rows: a chunk's
+    // upload runs while the next is packed
+    if (ncol > 0) {
         const int chunks = nY < (1u << 16) ? 1 : 4, parts = nY < (1u << 16) ? 1 : 8;
         for (int c = 0; c < chunks; c++) {
             const int c0 = (int)((long long)nk * c / chunks), c1 = (int)((long long)nk * (c + 1) / chunks);
             host_pool().run(parts, [&](int p) {
                 const int k0 = c0 + (int)((long long)(c1 - c0) * p / parts);
                 const int k1 = c0 + (int)((long long)(c1 - c0) * (p + 1) / parts);
-                for (int k = k0; k < k1; k++) memcpy(stY + (size_t)k * Na, d->delta_tot[k], sizeof(double) * Na);
+                for (int k = k0; k < k1; k++)
+                    memcpy(stY + (size_t)k * ncol, &dev->shadow[(size_t)k * namax + col0], sizeof(double) * ncol);
             });
-            KN_CHECK(hipMemcpyAsync(dev->Y + (size_t)c0 * Na, stY + (size_t)c0 * Na, sizeof(double) * (c1 - c0) * Na,
-                                    hipMemcpyHostToDevice, g_copy));
+            if (c1 > c0)
+                KN_CHECK(hipMemcpy2DAsync(dev->Y + (size_t)c0 * namax + col0, sizeof(double) * namax,
+                                          stY + (size_t)c0 * ncol, sizeof(double) * ncol, sizeof(double) * ncol,
+                                          c1 - c0, hipMemcpyHostToDevice, g_copy));
         }
     }
     memcpy(stX, d->scalefact, sizeof(double) * Na);
@@ -980,7 +1034,7 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
         hipLaunchKernelGGL(k_factor, dim3(factor_blocks(Na)), dim3(256), 0, 0, dev->sx, Na, dev->fac,
                            dev->fac + (size_t)fac_windows(Na) * kFacStride);
         hipLaunchKernelGGL(k_rows_prep, dim3(rows_prep_blocks(bsp, nsb)), dim3(256), 0, 0, dev->sx, dev->Y, dev->C, Na,
-                           nk, (long long)Na, bsp, nsb, sinv_bw, dev->bucket, dev->fac);
+                           nk, (long long)namax, bsp, nsb, sinv_bw, dev->bucket, dev->fac);
     }
     QagArgs qa;
     qa.H = H;
@@ -994,7 +1048,7 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
     qa.sb = dev->bucket;
     qa.ns = Na;
     qa.nsb = nsb;
-    qa.ystride = Na;
+    qa.ystride = namax;
     qa.sinv_bw = sinv_bw;
     qa.wavenum = dev->wavenum;
     qa.x0 = x0;
@@ -1071,13 +1125,14 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
     if (dtot_bar && (s.ic_only || Na < 2)) memset(dtot_bar, 0, sizeof(double) * (size_t)nk * Na);
     if (!s.ic_only && Na >= 2) {
         DnuDev *dev = g_dev[d];
-        const size_t ny = (size_t)nk * Na;
-        if (dev->cap_adj < ny) {
+        const int namax = dev->ystride;  // rows of Y, C, ybar and cbar are namax doubles apart
+        const size_t ny = (size_t)nk * Na, nyd = (size_t)nk * namax;
+        if (dev->cap_adj < nyd) {
             for (double **p : {&dev->ybar, &dev->cbar}) {
                 if (*p) KN_CHECK(hipFree(*p));
-                KN_CHECK(hipMalloc(p, sizeof(double) * ny));
+                KN_CHECK(hipMalloc(p, sizeof(double) * nyd));
             }
-            dev->cap_adj = ny;
+            dev->cap_adj = nyd;
         }
         const int P = adj_parts(Na);
         if (dev->cap_adj_k < (size_t)nk * P) {  // gbar (nk), mubar (nk x parts)
@@ -1107,7 +1162,7 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
         A.ns = Na;
         A.nsb = nsb;
         A.nk = nk;
-        A.ystride = Na;
+        A.ystride = namax;
         A.sinv_bw = sinv_bw;
         A.wavenum = dev->wavenum;
         A.sp = dev->sp;
@@ -1125,8 +1180,8 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
         hipLaunchKernelGGL(k_dnu_adj, dim3(nk * P), dim3(64), lds, 0, A);
         KN_CHECK(hipGetLastError());
         if (Na > 2) {
-            hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, Na, nk, (long long)Na,
-                               dev->cbar, dev->ybar, dev->fac);
+            hipLaunchKernelGGL(k_adj_solve, dim3(spline_blocks(Na, nk)), dim3(256), 0, 0, dev->sx, Na, nk,
+                               (long long)namax, dev->cbar, dev->ybar, dev->fac);
             KN_CHECK(hipGetLastError());
         }
         // results through the pinned stage: [mu_bar (nk x parts) | dL/d delta_tot (nk Na)]
@@ -1136,17 +1191,20 @@ extern "C" int kn_get_delta_nu_adjoint(const _delta_tot_table *d, double a, cons
         // dL/d delta_tot comes back in chunks: the host copies one out while the next is in flight
         static hipEvent_t done[4];
         const int chunks = dtot_bar ? (ny < (1u << 16) ? 1 : 4) : 0;
-        for (int c = 0; c < chunks; c++) {
+        for (int c = 0; c < chunks; c++) {  // chunks of rows, each row's [0, Na) to stride Na
             if (!done[c]) KN_CHECK(hipEventCreateWithFlags(&done[c], hipEventDisableTiming));
-            const size_t i0 = ny * c / chunks, i1 = ny * (c + 1) / chunks;
-            KN_CHECK(hipMemcpyAsync(yb + i0, dev->ybar + i0, sizeof(double) * (i1 - i0), hipMemcpyDeviceToHost, 0));
+            const int r0 = (int)((long long)nk * c / chunks), r1 = (int)((long long)nk * (c + 1) / chunks);
+            if (r1 > r0)
+                KN_CHECK(hipMemcpy2DAsync(yb + (size_t)r0 * Na, sizeof(double) * Na, dev->ybar + (size_t)r0 * namax,
+                                          sizeof(double) * namax, sizeof(double) * Na, r1 - r0, hipMemcpyDeviceToHost,
+                                          0));
             KN_CHECK(hipEventRecord(done[c], 0));
         }
         KN_CHECK(chunks ? hipEventSynchronize(done[0]) : hipStreamSynchronize(0));  // mu_bar went first
         for (size_t i = 0; i < nmu; i++) mb += mub[i];  // k-major, parts in order
         for (int c = 0; c < chunks; c++) {
             KN_CHECK(hipEventSynchronize(done[c]));
-            const size_t i0 = ny * c / chunks, i1 = ny * (c + 1) / chunks;
+            const size_t i0 = (size_t)((long long)nk * c / chunks) * Na, i1 = (size_t)((long long)nk * (c + 1) / chunks) * Na;
             const int parts = chunks > 1 ? 8 : 1;
             host_pool().run(parts, [&](int p) {
                 const size_t j0 = i0 + (i1 - i0) * p / parts, j1 = i0 + (i1 - i0) * (p + 1) / parts;
@@ -1980,10 +2038,10 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
         }
         if (found < 0) {
             const int npanel = (int)ceil((q.xmax - q.x0) / kGroupHWidth) + 1;
-            std::vector<double> coef((size_t)npanel * kHStride);
-            for (int p = 0; p < npanel; p++)
-                cheb_panel(q.x0 + p * kGroupHWidth, kGroupHWidth, q.spec.hubble, q.spec.hubble_ctx,
-                           &coef[(size_t)p * kHStride]);
+            std::vector<double> coef;
+            double (*hub)(double, void *) = q.spec.hubble;
+            void *ctx = q.spec.hubble_ctx;
+            h_table(q.x0, kGroupHWidth, npanel, [hub, ctx](double a) { return hub(a, ctx); }, coef);
             double *d = group_alloc<double>(coef.size());
             KN_CHECK(hipMemcpy(d, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice));
             HTab H;

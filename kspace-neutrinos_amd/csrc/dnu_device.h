@@ -17,13 +17,23 @@
 #include <float.h>
 #include <math.h>
 
+#include <vector>
+
 #include "kn_gk61.h"
 
 namespace kn_dnu {
 
 constexpr int kLimit = 200;   // QAG subinterval limit (GSL_VAL)
 constexpr int kHnc = 10;      // Chebyshev coefficients per function per H panel
-constexpr int kHStride = 2 * kHnc;  // one panel: 1/(a^2 H) coefficients, then 1/(a H)
+// One panel block: the coefficients of 1/(a^2 H), then of 1/(a H), then the
+// block's centre and scale (t = (x - centre) * scale on [-1, 1]), the split
+// point (+inf for a panel in one piece) and the index of the block that holds
+// the piece right of the split.
+constexpr int kHMid = 2 * kHnc, kHScale = kHMid + 1, kHSplit = kHMid + 2, kHRight = kHMid + 3;
+constexpr int kHStride = 2 * kHnc + 4;
+// A panel whose fit misses H by more than this (relative, at test points
+// between the interpolation nodes) holds a jump or a kink of H: it is split there.
+constexpr double kHSplitTol = 1e-12;
 
 __constant__ double c_xgk[31];
 __constant__ double c_wgk[31];
@@ -32,19 +42,24 @@ __constant__ double c_wg[15];
 // ---- H(a) factors as functions of x = ln a: Chebyshev panels of width 1/inv_w from origin x0 ----
 // Each panel holds the coefficients of 1/(a^2 H) (the fslength integrand,
 // delta_tot_table.c:378-383) and of 1/(a H) (the delta_nu integrand's factor,
-// :492-500), so neither integrand forms an exponential.
+// :492-500), so neither integrand forms an exponential.  H may be only
+// piecewise smooth (the host's neutrino density switches from its spline
+// table to the non-relativistic series at a = 100 kT/m, omega_nu_single.c:
+// 171-183, where H jumps by ~1e-8 relative): a panel that holds such a point
+// is split there into two fits, the right one in a block after the npanel
+// regular ones.
 struct HTab {
-    const double *coef;  // npanel * kHStride
+    const double *coef;  // npanel blocks of kHStride, then the right pieces of split panels
     double x0, inv_w;
     int npanel;
 };
 
-// Host: the coefficients of both functions on the panel [lo, lo + width],
-// interpolating at the kHnc Chebyshev points; hub(a) = H(a).
+// Host: both functions' interpolants at the kHnc Chebyshev points of
+// [lo, hi] into one block (unsplit); hub(a) = H(a).
 template <typename Hub>
-inline void h_cheb_panel(double lo, double width, Hub hub, double *coef)
+inline void h_fit(double lo, double hi, Hub hub, double *blk)
 {
-    const double half = 0.5 * width, mid = lo + half;
+    const double half = 0.5 * (hi - lo), mid = lo + half;
     double g2[kHnc], g1[kHnc];
     for (int j = 0; j < kHnc; j++) {
         const double x = mid + half * cos(M_PI * (j + 0.5) / kHnc);
@@ -59,48 +74,124 @@ inline void h_cheb_panel(double lo, double width, Hub hub, double *coef)
             s2 += g2[j] * c;
             s1 += g1[j] * c;
         }
-        coef[k] = (k == 0 ? 1.0 : 2.0) * s2 / kHnc;
-        coef[kHnc + k] = (k == 0 ? 1.0 : 2.0) * s1 / kHnc;
+        blk[k] = (k == 0 ? 1.0 : 2.0) * s2 / kHnc;
+        blk[kHnc + k] = (k == 0 ? 1.0 : 2.0) * s1 / kHnc;
+    }
+    blk[kHMid] = mid;
+    blk[kHScale] = half > 0.0 ? 1.0 / half : 0.0;
+    blk[kHSplit] = INFINITY;
+    blk[kHRight] = 0.0;
+}
+
+// Clenshaw sum of one function of one block (which = 0: 1/(a^2 H), 1: 1/(a H)).
+template <typename P>
+__host__ __device__ __forceinline__ double h_block(P blk, int which, double x)
+{
+    const double t = (x - blk[kHMid]) * blk[kHScale];
+    P c = blk + which * kHnc;
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int k = kHnc - 1; k >= 1; k--) {
+        const double b0 = fma(2.0 * t, b1, c[k] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return fma(t, b1, c[0] - b2);
+}
+
+// Host: the largest relative miss of a block's two fits on [lo, hi], at the
+// ends and half-way between adjacent interpolation nodes.
+template <typename Hub>
+inline double h_fit_err(const double *blk, double lo, double hi, Hub hub)
+{
+    const double half = 0.5 * (hi - lo), mid = lo + half;
+    double worst = 0.0;
+    for (int j = -1; j < kHnc; j++) {
+        const double t = j < 0 ? -1.0 : (j == kHnc - 1 ? 1.0 : cos(M_PI * (j + 1) / kHnc));
+        const double x = j < 0 ? lo : (j == kHnc - 1 ? hi : mid + half * t);
+        const double h = hub(exp(x));
+        const double e2 = fabs(h_block(blk, 0, x) / (exp(-2.0 * x) / h) - 1.0);
+        const double e1 = fabs(h_block(blk, 1, x) / (exp(-x) / h) - 1.0);
+        worst = fmax(worst, fmax(e2, e1));
+    }
+    return worst;
+}
+
+// Host: the panel [lo, hi] into blk.  Where one fit misses H by more than
+// kHSplitTol the panel is split at the point that bisection finds (the
+// half whose own fit misses more is kept, until both halves fit or the
+// bracket is a few units in the last place wide): blk then holds [lo, xs]
+// and right[] the block of [xs, hi].  Returns whether it split.
+template <typename Hub>
+inline bool h_panel(double lo, double hi, Hub hub, double *blk, double *right)
+{
+    h_fit(lo, hi, hub, blk);
+    if (h_fit_err(blk, lo, hi, hub) <= kHSplitTol) return false;
+    double l = lo, r = hi, tmp[kHStride];
+    for (;;) {
+        const double m = l + 0.5 * (r - l);
+        if (!(m > l && m < r)) break;
+        h_fit(l, m, hub, tmp);
+        const double el = h_fit_err(tmp, l, m, hub);
+        h_fit(m, r, hub, tmp);
+        const double er = h_fit_err(tmp, m, r, hub);
+        if (el <= kHSplitTol && er <= kHSplitTol) {
+            l = r = m;
+            break;
+        }
+        if (el >= er) r = m;
+        else l = m;
+    }
+    const double xs = l + 0.5 * (r - l);
+    h_fit(lo, xs, hub, blk);
+    h_fit(xs, hi, hub, right);
+    blk[kHSplit] = xs;
+    return true;
+}
+
+// Host: the whole table of npanel panels of width w from x0 (split panels'
+// right pieces appended, blk[kHRight] pointing at them).
+template <typename Hub>
+inline void h_table(double x0, double w, int npanel, Hub hub, std::vector<double> &coef)
+{
+    coef.assign((size_t)npanel * kHStride, 0.0);
+    double right[kHStride];
+    for (int p = 0; p < npanel; p++) {
+        double *blk = &coef[(size_t)p * kHStride];
+        if (h_panel(x0 + p * w, x0 + (p + 1) * w, hub, blk, right)) {
+            const size_t idx = coef.size() / kHStride;
+            coef.insert(coef.end(), right, right + kHStride);
+            coef[(size_t)p * kHStride + kHRight] = (double)idx;
+        }
     }
 }
 
-// Clenshaw sum of one function's panel (which = 0: 1/(a^2 H), 1: 1/(a H)).
+// Evaluation of one function (which = 0: 1/(a^2 H), 1: 1/(a H)) at x.
 __host__ __device__ __forceinline__ double h_cheb(const double *coef, double x0, double inv_w, int npanel, double x,
                                                   int which)
 {
     const double u = (x - x0) * inv_w;
     int p = (int)floor(u);
     p = p < 0 ? 0 : (p >= npanel ? npanel - 1 : p);
-    const double t = 2.0 * (u - p) - 1.0;
-    const double *c = coef + (size_t)p * kHStride + which * kHnc;
-    double b1 = 0.0, b2 = 0.0;
-#pragma unroll
-    for (int k = kHnc - 1; k >= 1; k--) {
-        const double b0 = fma(2.0 * t, b1, c[k] - b2);
-        b2 = b1;
-        b1 = b0;
-    }
-    return fma(t, b1, c[0] - b2);
+    const double *c = coef + (size_t)p * kHStride;
+    if (x >= c[kHSplit]) c = coef + (size_t)c[kHRight] * kHStride;
+    return h_block(c, which, x);
 }
 
 // Device form: the table lives in global memory, so its loads are global
-// loads even where the pointer came from a descriptor in memory.
+// loads even where the pointer came from a descriptor in memory.  The
+// regular block's sum is formed first (its loads, split point included, go
+// out together); only x right of a split point sums a second block.
 typedef const __attribute__((address_space(1))) double gconst_double;
 __device__ __forceinline__ double h_cheb_dev(const HTab &H, double x, int which)
 {
     const double u = (x - H.x0) * H.inv_w;
     int p = (int)floor(u);
     p = p < 0 ? 0 : (p >= H.npanel ? H.npanel - 1 : p);
-    const double t = 2.0 * (u - p) - 1.0;
-    gconst_double *c = (gconst_double *)(H.coef + (size_t)p * kHStride + which * kHnc);
-    double b1 = 0.0, b2 = 0.0;
-#pragma unroll
-    for (int k = kHnc - 1; k >= 1; k--) {
-        const double b0 = fma(2.0 * t, b1, c[k] - b2);
-        b2 = b1;
-        b1 = b0;
-    }
-    return fma(t, b1, c[0] - b2);
+    gconst_double *c = (gconst_double *)(H.coef + (size_t)p * kHStride);
+    double v = h_block(c, which, x);
+    if (x >= c[kHSplit]) v = h_block((gconst_double *)(H.coef + (size_t)c[kHRight] * kHStride), which, x);
+    return v;
 }
 
 __device__ __forceinline__ double inv_a2h(const HTab &H, double x) { return h_cheb_dev(H, x, 0); }

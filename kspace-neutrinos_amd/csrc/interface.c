@@ -26,11 +26,10 @@ void kn_bin_power_host_sums(const fftw_complex *d_field, int dims, int nrbins, i
                             double *power, double *total_mass2);
 void kn_bin_geometry_host(int dims, int nrbins, int startslab, int nslab, long long *count, double *keff_sum);
 int kn_finish_powerspectrum(int nrbins, double total_mass2, double *power, long long *count, double *keffs);
-int kn_rescale_nan_flag(void);
 fftw_complex *kn_grid_to_device(fftw_complex *grid, size_t nmodes, int *staged);
 void kn_grid_release(fftw_complex *host, fftw_complex *dev, size_t nmodes, int staged, int copy_back);
-void kn_rescale_from_host_table(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
-                                const double *logkk, const double *ratio, int nbins, double norm);
+int kn_rescale_from_host_table(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
+                               const double *logkk, const double *ratio, int nbins, double norm);
 
 double OmegaNu(double a) { return get_omega_nu(&omeganu_table, a); }
 
@@ -143,9 +142,9 @@ void add_nu_power_to_rhogrid(const double Time, const double BoxSize, fftw_compl
     _delta_pow d_pow;
     (void)comm;
     d_pow = compute_neutrino_power_spectrum(Time, BoxSize, d_grid, pmgrid, slabstart_y, nslab_y);
-    kn_rescale_from_host_table(d_grid, pmgrid, slabstart_y, nslab_y, BoxSize, d_pow.logkk, d_pow.delta_ratio,
-                               d_pow.nbins, d_pow.norm);
-    if (kn_rescale_nan_flag()) kn_terminate(5, "delta_nu or delta_cdm is nan\n");
+    if (kn_rescale_from_host_table(d_grid, pmgrid, slabstart_y, nslab_y, BoxSize, d_pow.logkk, d_pow.delta_ratio,
+                                   d_pow.nbins, d_pow.norm))
+        kn_terminate(5, "delta_nu or delta_cdm is nan\n");
     kn_grid_release(fft_of_rhogrid, d_grid, nmodes, staged, 1);
     kn_message(0, "Done adding neutrinos to grid on all processors\n");
     free_d_pow(&d_pow);

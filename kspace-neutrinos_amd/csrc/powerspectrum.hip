@@ -12,16 +12,19 @@
 //    (powerspectrum.c:97-116).
 //
 // MI355X design: the pass streams the grid once from HBM (16 B per mode,
-// coalesced dwordx4 loads, several loads in flight per lane).  The bin index
-// is integer-exact: the host evaluates the reference expression once per
-// integer |k|^2 and stores, per bin, the smallest |k|^2 that reaches it; the
-// kernel takes an fp32 estimate of the bin and corrects it against that
-// threshold table in LDS, so counts are bit-exact with the reference's libm.
-// Each wavefront accumulates into its own LDS histogram (ds_add_f64), blocks
-// write partial histograms and a second kernel sums them in fixed order, so
-// results are deterministic run to run.  Mode counts and <|k|> depend only on
+// coalesced non-temporal dwordx4 loads, transposed through LDS so that each
+// lane bins a run of consecutive modes; see bin_kernels.h).  The bin index is
+// integer-exact: the host evaluates the reference expression once per integer
+// |k|^2 and stores, per bin, the smallest |k|^2 that reaches it; the kernel
+// takes an fp32 estimate of the bin and corrects it against that threshold
+// table, so counts are bit-exact with the reference's libm.  The wavefronts of
+// a block add into one set of bin sums in LDS (ds_add_f64; the order of the
+// adds between wavefronts is not fixed, so a sum can differ in its last bit
+// from run to run), blocks write partial sums and a second kernel adds them in
+// a fixed order.  Tables go to LDS when they fit, else are read from global
+// memory, so any number of bins works.  Mode counts and <|k|> depend only on
 // the geometry; they are computed once per (dims, nrbins, slab range) by a
-// read-free kernel and cached.
+// read-free kernel and cached (bit-exact, deterministic).
 #include <math.h>
 #include <string.h>
 #include <vector>
@@ -39,9 +42,8 @@ using kn_bin::kval;
 
 constexpr int kBlock = 256;  // geometry pass
 constexpr int kWaves = kBlock / 64;
-constexpr int kPowWaves = 8;  // power pass: 8 waves, 4 consecutive modes per lane per step, next step prefetched
-constexpr int kPowRun = 4;
-constexpr int kPowSlots = 2;  // power pass blocks per resident slot (1.57 ms vs 1.64 ms at one, 1024^3)
+constexpr int kSweepSlots = 2;  // bin sweep blocks per resident slot (1.41-1.44 ms at 1024^3 against 1.45 at three)
+constexpr size_t kLdsMax = 160 * 1024;
 
 struct BinGeom {
     int dims, nrbins, startslab, nslab;
@@ -51,6 +53,8 @@ struct BinGeom {
     std::vector<long long> count;
     std::vector<double> keff_sum;
     int grid, ggrid;        // power-pass and geometry-pass grids
+    int place;              // where the sweep keeps its tables (kn_bin::TablePlace)
+    size_t smem;            // the sweep's LDS bytes
     double *d_partial;      // max(grid, ggrid) * nrbins
     unsigned *d_cpartial;   // ggrid * nrbins (geometry pass)
     double *d_sum;          // nrbins + 1 (last: |F0|^2)
@@ -61,21 +65,39 @@ struct BinGeom {
 std::vector<BinGeom *> g_geoms;
 
 // Geometry pass (no grid reads): multiplicity counts and sum of multiplicity*|k|.
+// GLOBAL: the per-block sums are added with global atomics into the block's
+// partial rows and the thresholds read from global memory (any number of bins).
+template <bool GLOBAL>
 __global__ __launch_bounds__(kBlock) void k_bin_geometry(int dims, int startslab, long long rows, int nrbins,
                                                           const int *__restrict__ thr_g, float half_bpu,
                                                           unsigned *__restrict__ cpartial,
                                                           double *__restrict__ kpartial)
 {
     extern __shared__ double smem[];
-    double *khist = smem;                                      // kWaves * nrbins
-    unsigned *chist = (unsigned *)(khist + (size_t)kWaves * nrbins);  // kWaves * nrbins
-    int *thr = (int *)(chist + (size_t)kWaves * nrbins);
+    double *khist;
+    unsigned *chist;
+    const int *thr;
     const int tid = threadIdx.x, wave = tid >> 6;
-    for (int t = tid; t < kWaves * nrbins; t += kBlock) {
-        khist[t] = 0.0;
-        chist[t] = 0u;
+    if constexpr (GLOBAL) {
+        khist = kpartial + (size_t)blockIdx.x * nrbins;
+        chist = cpartial + (size_t)blockIdx.x * nrbins;
+        thr = thr_g;
+        for (int t = tid; t < nrbins; t += kBlock) {
+            khist[t] = 0.0;
+            chist[t] = 0u;
+        }
+    } else {
+        khist = smem;                                          // kWaves * nrbins
+        chist = (unsigned *)(khist + (size_t)kWaves * nrbins);  // kWaves * nrbins
+        int *thr_l = (int *)(chist + (size_t)kWaves * nrbins);
+        for (int t = tid; t < kWaves * nrbins; t += kBlock) {
+            khist[t] = 0.0;
+            chist[t] = 0u;
+        }
+        for (int t = tid; t <= nrbins; t += kBlock) thr_l[t] = thr_g[t];
+        thr = thr_l;
     }
-    for (int t = tid; t <= nrbins; t += kBlock) thr[t] = thr_g[t];
+    const size_t hoff = GLOBAL ? 0 : (size_t)wave * nrbins;
     __syncthreads();
     const int rowlen = dims / 2 + 1, half = dims / 2;
     const long long r0 = rows * blockIdx.x / gridDim.x, r1 = rows * (blockIdx.x + 1) / gridDim.x;
@@ -95,11 +117,12 @@ __global__ __launch_bounds__(kBlock) void k_bin_geometry(int dims, int startslab
             const int b = find_bin(k2, half_bpu, thr, nrbins);
             const unsigned c = (kz != 0 && kz != half) ? 2u : 1u;
             const double kk = sqrt((double)k2);
-            atomicAdd(&chist[(size_t)wave * nrbins + b], c);
-            atomicAdd(&khist[(size_t)wave * nrbins + b], c == 2u ? 2.0 * kk : kk);
+            atomicAdd(&chist[hoff + b], c);
+            atomicAdd(&khist[hoff + b], c == 2u ? 2.0 * kk : kk);
         }
         advance(p, kBlock, rowlen, dims);
     }
+    if constexpr (GLOBAL) return;
     __syncthreads();
     for (int b = tid; b < nrbins; b += kBlock) {
         double s = khist[b];
@@ -184,7 +207,7 @@ int num_cus()
     return n;
 }
 
-size_t power_smem(int dims, int nrbins) { return kn_bin::smem_bytes<kPowWaves>(dims, nrbins); }
+size_t geometry_smem(int nrbins) { return (sizeof(double) + sizeof(unsigned)) * (size_t)kWaves * nrbins + sizeof(int) * (nrbins + 1); }
 
 // Resident blocks per CU of a kernel at a given dynamic LDS size.
 template <typename K>
@@ -204,8 +227,6 @@ BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
     if (dims < 2 || nrbins < 1 || nslab < 1 || startslab < 0 || startslab + nslab > dims)
         kn_terminate(KN_ERR_ARG, "total_powerspectrum: bad geometry dims=%d nrbins=%d slabs=[%d,%d)\n", dims,
                      nrbins, startslab, startslab + nslab);
-    if (power_smem(dims, nrbins) > 160 * 1024)
-        kn_terminate(KN_ERR_ARG, "total_powerspectrum: nrbins=%d dims=%d exceed the LDS budget\n", nrbins, dims);
     BinGeom *g = new BinGeom();
     g->dims = dims;
     g->nrbins = nrbins;
@@ -218,11 +239,29 @@ BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
     kn_bin::build_invwindow(dims, iw);
     g->half_bpu = (float)(0.5 * (nrbins - 1) / log(sqrt(3) * dims / 2.0));
     const long long rows = (long long)nslab * dims;
-    // one resident wave of row-range blocks: no tail of late blocks (a second
-    // round of blocks measured slower once the doubled partial histograms are
-    // reduced)
-    const long long pgrid = (long long)resident_blocks(kn_bin::k_bin_power_runs_pf<kPowWaves, kPowRun>, 64 * kPowWaves,
-                                                       power_smem(dims, nrbins)) * num_cus() * kPowSlots;
+    // the first table placement whose LDS leaves room for two blocks per CU,
+    // else the first that fits at all (kTablesGlobal always does)
+    g->place = kn_bin::kTablesGlobal;
+    for (int pl : {kn_bin::kTablesLds, kn_bin::kWindowGlobal})
+        if (kn_bin::sweep_smem_bytes(pl, dims, nrbins) <= kLdsMax / 2) {
+            g->place = pl;
+            break;
+        }
+    if (g->place == kn_bin::kTablesGlobal)
+        for (int pl : {kn_bin::kTablesLds, kn_bin::kWindowGlobal})
+            if (kn_bin::sweep_smem_bytes(pl, dims, nrbins) <= kLdsMax) {
+                g->place = pl;
+                break;
+            }
+    g->smem = kn_bin::sweep_smem_bytes(g->place, dims, nrbins);
+    const int per_cu = g->place == kn_bin::kTablesLds
+                           ? resident_blocks(kn_bin::k_bin_sweep<kn_bin::kTablesLds>, 64 * kn_bin::kSweepWaves, g->smem)
+                       : g->place == kn_bin::kWindowGlobal
+                           ? resident_blocks(kn_bin::k_bin_sweep<kn_bin::kWindowGlobal>, 64 * kn_bin::kSweepWaves,
+                                             g->smem)
+                           : resident_blocks(kn_bin::k_bin_sweep<kn_bin::kTablesGlobal>, 64 * kn_bin::kSweepWaves,
+                                             g->smem);
+    const long long pgrid = (long long)per_cu * num_cus() * kSweepSlots;
     g->grid = (int)(rows < pgrid ? rows : pgrid);
     g->ggrid = (int)(rows < 4LL * num_cus() ? rows : 4LL * num_cus());
     const int pmax = g->grid > g->ggrid ? g->grid : g->ggrid;
@@ -238,9 +277,17 @@ BinGeom *get_geom(int dims, int nrbins, int startslab, int nslab)
     double *d_ksum;
     KN_CHECK(hipMalloc(&d_count, sizeof(unsigned long long) * nrbins));
     KN_CHECK(hipMalloc(&d_ksum, sizeof(double) * nrbins));
-    const size_t gsmem = (sizeof(double) + sizeof(unsigned)) * (size_t)kWaves * nrbins + sizeof(int) * (nrbins + 1);
-    hipLaunchKernelGGL(k_bin_geometry, dim3(g->ggrid), dim3(kBlock), gsmem, 0, dims, startslab, rows, nrbins,
-                       g->d_thr, g->half_bpu, g->d_cpartial, g->d_partial);
+    const size_t gsmem = geometry_smem(nrbins);
+    if (gsmem <= kLdsMax) {
+        if (gsmem > 64 * 1024)
+            KN_CHECK(hipFuncSetAttribute((const void *)k_bin_geometry<false>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)gsmem));
+        hipLaunchKernelGGL(k_bin_geometry<false>, dim3(g->ggrid), dim3(kBlock), gsmem, 0, dims, startslab, rows,
+                           nrbins, g->d_thr, g->half_bpu, g->d_cpartial, g->d_partial);
+    } else {
+        hipLaunchKernelGGL(k_bin_geometry<true>, dim3(g->ggrid), dim3(kBlock), 0, 0, dims, startslab, rows, nrbins,
+                           g->d_thr, g->half_bpu, g->d_cpartial, g->d_partial);
+    }
     KN_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_reduce_counts, dim3((nrbins + 255) / 256), dim3(256), 0, 0, g->d_cpartial, g->d_partial,
                        g->ggrid, nrbins, d_count, d_ksum);
@@ -273,8 +320,13 @@ void launch_power(BinGeom *g, const double2 *F, hipStream_t st, double *sum_out 
     a.iw = g->d_iw;
     a.half_bpu = g->half_bpu;
     a.partial = g->d_partial;
-    hipLaunchKernelGGL((kn_bin::k_bin_power_runs_pf<kPowWaves, kPowRun>), dim3(g->grid), dim3(64 * kPowWaves),
-                       power_smem(g->dims, g->nrbins), st, a);
+    const dim3 grid(g->grid), block(64 * kn_bin::kSweepWaves);
+    if (g->place == kn_bin::kTablesLds)
+        hipLaunchKernelGGL(kn_bin::k_bin_sweep<kn_bin::kTablesLds>, grid, block, g->smem, st, a);
+    else if (g->place == kn_bin::kWindowGlobal)
+        hipLaunchKernelGGL(kn_bin::k_bin_sweep<kn_bin::kWindowGlobal>, grid, block, g->smem, st, a);
+    else
+        hipLaunchKernelGGL(kn_bin::k_bin_sweep<kn_bin::kTablesGlobal>, grid, block, g->smem, st, a);
     KN_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_reduce_partials, dim3((g->nrbins + 63) / 64), dim3(64 * kRedWaves), 0, st, g->d_partial,
                        g->grid, g->nrbins, sum_out);
@@ -289,6 +341,9 @@ void launch_power(BinGeom *g, const double2 *F, hipStream_t st, double *sum_out 
 constexpr int kAdjWaves = 8;
 constexpr int kAdjUnroll = 2;
 
+// GLOBAL: the tables are read from global memory (through the caches) when
+// they do not fit in LDS.
+template <bool GLOBAL>
 __global__ __launch_bounds__(64 * kAdjWaves) void k_bin_adjoint(const double2 *__restrict__ F,
                                                                 double2 *__restrict__ Fb, int dims, int startslab,
                                                                 long long rows, int nrbins,
@@ -298,14 +353,21 @@ __global__ __launch_bounds__(64 * kAdjWaves) void k_bin_adjoint(const double2 *_
                                                                 double beta)
 {
     extern __shared__ double smem[];
-    double *alpha = smem;
-    double *iw = alpha + nrbins;
-    int *thr = (int *)(iw + dims);
+    const double *alpha = alpha_g, *iw = iw_g;
+    const int *thr = thr_g;
     constexpr int B = 64 * kAdjWaves;
-    for (int t = threadIdx.x; t < nrbins; t += B) alpha[t] = alpha_g[t];
-    for (int t = threadIdx.x; t < dims; t += B) iw[t] = iw_g[t];
-    for (int t = threadIdx.x; t <= nrbins; t += B) thr[t] = thr_g[t];
-    __syncthreads();
+    if constexpr (!GLOBAL) {
+        double *alpha_l = smem;
+        double *iw_l = alpha_l + nrbins;
+        int *thr_l = (int *)(iw_l + dims);
+        for (int t = threadIdx.x; t < nrbins; t += B) alpha_l[t] = alpha_g[t];
+        for (int t = threadIdx.x; t < dims; t += B) iw_l[t] = iw_g[t];
+        for (int t = threadIdx.x; t <= nrbins; t += B) thr_l[t] = thr_g[t];
+        alpha = alpha_l;
+        iw = iw_l;
+        thr = thr_l;
+        __syncthreads();
+    }
     const int rowlen = dims / 2 + 1, half = dims / 2;
     const long long r0 = rows * blockIdx.x / gridDim.x, r1 = rows * (blockIdx.x + 1) / gridDim.x;
     const long long m1 = r1 * rowlen;
@@ -465,17 +527,25 @@ extern "C" int kn_bin_power_adjoint(const fftw_complex *d_field, int dims, int n
         kn_set_last_error("kn_bin_power_adjoint: nout=%d but the forward has %d non-empty bins", nout, nonzero);
         return KN_ERR_ARG;
     }
+    const bool adj_global = adjoint_smem(dims, nrbins) > kLdsMax;
+    const size_t asmem = adj_global ? 0 : adjoint_smem(dims, nrbins);
     if (!g->d_alpha) {
         KN_CHECK(hipMalloc(&g->d_alpha, sizeof(double) * nrbins));
         const long long rows = (long long)nslab * dims;
-        const long long ag = (long long)resident_blocks(k_bin_adjoint, 64 * kAdjWaves, adjoint_smem(dims, nrbins)) *
-                             num_cus();
+        const int per_cu = adj_global ? resident_blocks(k_bin_adjoint<true>, 64 * kAdjWaves, 0)
+                                      : resident_blocks(k_bin_adjoint<false>, 64 * kAdjWaves, asmem);
+        const long long ag = (long long)per_cu * num_cus();
         g->agrid = (int)(rows < ag ? rows : ag);
     }
     KN_CHECK(hipMemcpyAsync(g->d_alpha, alpha.data(), sizeof(double) * nrbins, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_bin_adjoint, dim3(g->agrid), dim3(64 * kAdjWaves), adjoint_smem(dims, nrbins), st,
-                       (const double2 *)d_field, (double2 *)d_field_bar, dims, startslab, (long long)nslab * dims,
-                       nrbins, g->d_thr, g->d_iw, g->d_alpha, g->half_bpu, -2.0 * gp / M);
+    if (adj_global)
+        hipLaunchKernelGGL(k_bin_adjoint<true>, dim3(g->agrid), dim3(64 * kAdjWaves), 0, st, (const double2 *)d_field,
+                           (double2 *)d_field_bar, dims, startslab, (long long)nslab * dims, nrbins, g->d_thr,
+                           g->d_iw, g->d_alpha, g->half_bpu, -2.0 * gp / M);
+    else
+        hipLaunchKernelGGL(k_bin_adjoint<false>, dim3(g->agrid), dim3(64 * kAdjWaves), asmem, st,
+                           (const double2 *)d_field, (double2 *)d_field_bar, dims, startslab, (long long)nslab * dims,
+                           nrbins, g->d_thr, g->d_iw, g->d_alpha, g->half_bpu, -2.0 * gp / M);
     KN_CHECK(hipGetLastError());
     KN_CHECK(hipStreamSynchronize(st));
     return 0;

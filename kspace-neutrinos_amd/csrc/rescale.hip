@@ -8,22 +8,27 @@
 //    clamped too; the reference also prints a warning for them);
 //    smth = 1 + norm * (linear interpolation of ratio in logk);
 //    NaN smth is fatal (code 5); F *= smth.
-// MI355X design: one coalesced read+write sweep (32 B per mode).  The ratio
-// table and a uniform bucket index over logk live in LDS, so the segment
-// search is a bucket lookup plus a short forward probe instead of a binary
-// search.  All arithmetic is FP64 and follows the reference's expression order.
+// MI355X design: one coalesced read+write sweep (32 B per mode, non-temporal
+// loads and stores).  The ratio table and a uniform bucket index over logk
+// live in LDS, so the segment search is a bucket lookup plus a short forward
+// probe instead of a binary search.  All arithmetic is FP64 and follows the
+// reference's expression order.
 #include <math.h>
+#include <string.h>
+
+#include <vector>
 
 #include "kn_hip.h"
 
+#include "bin_kernels.h"
 #include "rescale_kernels.h"
 
 namespace {
 
 constexpr int kBlock = 256;   // synthetic-field generator
-constexpr int kResWaves = 8;  // rescale: 8 waves per block, 2 modes per lane per step, next step prefetched
-constexpr int kResUnroll = 2;
-constexpr int kResBlocksPerSlot = 8;
+constexpr int kResWaves = 8;  // rescale: 8 waves per block, 1 mode per lane per step, next step prefetched
+constexpr int kResUnroll = 1;
+constexpr int kResBlocksPerSlot = 16;
 
 int *g_nan_flag = nullptr;    // device flag raised by the rescale on a NaN factor
 
@@ -36,6 +41,7 @@ int *nan_flag_ptr()
     return g_nan_flag;
 }
 
+template <bool GREEN>
 int res_grid(long long rows, size_t smem)
 {
     static int cus = 0;
@@ -45,13 +51,13 @@ int res_grid(long long rows, size_t smem)
         KN_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
     int per_cu = 0;
-    if (smem > 64 * 1024)
-        KN_CHECK(hipFuncSetAttribute((const void *)kn_res::k_rescale_rows_pf<kResWaves, kResUnroll>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-    KN_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kn_res::k_rescale_rows_pf<kResWaves, kResUnroll>,
+    const void *k = (const void *)kn_res::k_rescale_rows_pf<kResWaves, kResUnroll, GREEN>;
+    if (smem > 64 * 1024) KN_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    KN_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kn_res::k_rescale_rows_pf<kResWaves, kResUnroll, GREEN>,
                                                           64 * kResWaves, smem));
-    // eight blocks per resident slot: smaller contiguous row ranges per block
-    // balance the sweep's tail across the channels (3.76 -> 3.43 ms at 1024^3)
+    // sixteen blocks per resident slot: smaller contiguous row ranges per block
+    // balance the sweep's tail across the channels (1024^3: 3.76 ms at one,
+    // 3.43 at eight; with non-temporal loads and stores 3.35-3.40 at sixteen)
     const long long g = (long long)(per_cu > 0 ? per_cu : 1) * cus * kResBlocksPerSlot;
     return (int)(rows < g ? rows : g);
 }
@@ -135,21 +141,65 @@ extern "C" int kn_rescale_device(fftw_complex *d_field, int dims, int startslab,
     a.norm = norm;
     a.logkk = d_logkk;
     a.ratio = d_ratio;
-    a.bucket = nullptr;
     a.nan_flag = nan_flag_ptr();
-    hipLaunchKernelGGL((kn_res::k_rescale_rows_pf<kResWaves, kResUnroll>), dim3(res_grid(a.rows, smem)),
+    a.iw = nullptr;
+    a.asmth2 = 0.0;
+    hipLaunchKernelGGL((kn_res::k_rescale_rows_pf<kResWaves, kResUnroll>), dim3(res_grid<false>(a.rows, smem)),
                        dim3(64 * kResWaves), smem, (hipStream_t)stream, a);
     KN_CHECK(hipGetLastError());
     return 0;
 }
 
-// Host-side check of the NaN flag raised by the rescale (clears it).
-extern "C" int kn_rescale_nan_flag(void)
+// The neutrino rescale and the PM Green's-function multiply that follows it
+// in one read + write sweep (no reference counterpart; see the header).
+extern "C" int kn_rescale_green_device(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
+                                       const double *d_logkk, const double *d_ratio, int nbins, double norm,
+                                       double asmth, void *stream)
 {
-    int v = 0, z = 0;
-    KN_CHECK(hipMemcpy(&v, nan_flag_ptr(), sizeof(int), hipMemcpyDeviceToHost));
-    if (v) KN_CHECK(hipMemcpy(nan_flag_ptr(), &z, sizeof(int), hipMemcpyHostToDevice));
-    return v;
+    if (nbins < 0 || dims < 2 || nslab < 1 || startslab < 0 || startslab + nslab > dims || !(boxsize > 0) ||
+        !(asmth >= 0) || (nbins > 0 && (!d_logkk || !d_ratio))) {
+        kn_set_last_error("kn_rescale_green_device: bad arguments");
+        return KN_ERR_ARG;
+    }
+    const int nbuckets = kn_res::kBucketsPerBin * nbins;
+    const size_t smem = kn_res::smem_bytes(nbins, nbuckets, dims);
+    if (smem > 160 * 1024) {
+        kn_set_last_error("kn_rescale_green_device: nbins=%d dims=%d exceed the LDS budget", nbins, dims);
+        return KN_ERR_RANGE;
+    }
+    // the per-axis inverse window of this dims, kept for the process
+    static std::vector<double> iw_h;
+    static double *iw_d = nullptr;
+    static int iw_dims = 0, iw_cap = 0;
+    if (iw_dims != dims) {
+        kn_bin::build_invwindow(dims, iw_h);
+        if (dims > iw_cap) {
+            if (iw_d) KN_CHECK(hipFree(iw_d));
+            KN_CHECK(hipMalloc(&iw_d, sizeof(double) * dims));
+            iw_cap = dims;
+        }
+        KN_CHECK(hipMemcpy(iw_d, iw_h.data(), sizeof(double) * dims, hipMemcpyHostToDevice));
+        iw_dims = dims;
+    }
+    kn_res::ResArgs a;
+    a.F = (double2 *)d_field;
+    a.dims = dims;
+    a.startslab = startslab;
+    a.nbins = nbins;
+    a.nbuckets = nbuckets;
+    a.rows = (long long)nslab * dims;
+    a.boxsize = boxsize;
+    a.norm = norm;
+    a.logkk = d_logkk;
+    a.ratio = d_ratio;
+    a.nan_flag = nan_flag_ptr();
+    a.iw = iw_d;
+    const double s = 2 * M_PI * asmth / boxsize;
+    a.asmth2 = s * s;
+    hipLaunchKernelGGL((kn_res::k_rescale_rows_pf<kResWaves, kResUnroll, true>), dim3(res_grid<true>(a.rows, smem)),
+                       dim3(64 * kResWaves), smem, (hipStream_t)stream, a);
+    KN_CHECK(hipGetLastError());
+    return 0;
 }
 
 extern "C" int kn_synth_grid(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
@@ -190,15 +240,34 @@ extern "C" void kn_grid_release(fftw_complex *host, fftw_complex *dev, size_t nm
     KN_CHECK(hipFree(dev));
 }
 
-extern "C" void kn_rescale_from_host_table(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
-                                           const double *logkk, const double *ratio, int nbins, double norm)
+// The rescale of add_nu_power_to_rhogrid with the ratio table on the host:
+// the table goes through a pinned staging buffer into a device buffer, both
+// kept for the process and grown when a longer table arrives, so a PM step
+// allocates nothing; the copy, the sweep and the read-back of the NaN flag
+// are ordered on the null stream and the one synchronisation is the flag's.
+// Returns the flag (1: a NaN factor, terminate 5 in the caller).
+extern "C" int kn_rescale_from_host_table(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
+                                          const double *logkk, const double *ratio, int nbins, double norm)
 {
-    double *d_tab = nullptr;
-    KN_CHECK(hipMalloc(&d_tab, sizeof(double) * 2 * (nbins > 0 ? nbins : 1)));
-    KN_CHECK(hipMemcpy(d_tab, logkk, sizeof(double) * nbins, hipMemcpyHostToDevice));
-    KN_CHECK(hipMemcpy(d_tab + nbins, ratio, sizeof(double) * nbins, hipMemcpyHostToDevice));
+    static double *d_tab = nullptr, *h_tab = nullptr;
+    static int *h_flag = nullptr;
+    static int cap = 0;
+    if (nbins > cap) {
+        if (d_tab) KN_CHECK(hipFree(d_tab));
+        if (h_tab) KN_CHECK(hipHostFree(h_tab));
+        KN_CHECK(hipMalloc(&d_tab, sizeof(double) * 2 * nbins));
+        KN_CHECK(hipHostMalloc((void **)&h_tab, sizeof(double) * 2 * nbins, hipHostMallocDefault));
+        cap = nbins;
+    }
+    if (!h_flag) KN_CHECK(hipHostMalloc((void **)&h_flag, sizeof(int), hipHostMallocDefault));
+    memcpy(h_tab, logkk, sizeof(double) * nbins);
+    memcpy(h_tab + nbins, ratio, sizeof(double) * nbins);
+    KN_CHECK(hipMemcpyAsync(d_tab, h_tab, sizeof(double) * 2 * nbins, hipMemcpyHostToDevice, 0));
     const int rc = kn_rescale_device(d_field, dims, startslab, nslab, boxsize, d_tab, d_tab + nbins, nbins, norm, 0);
     if (rc) kn_terminate(rc, "rescale: bad table (nbins=%d)\n", nbins);
-    KN_CHECK(hipDeviceSynchronize());
-    KN_CHECK(hipFree(d_tab));
+    KN_CHECK(hipMemcpyAsync(h_flag, nan_flag_ptr(), sizeof(int), hipMemcpyDeviceToHost, 0));
+    KN_CHECK(hipStreamSynchronize(0));
+    const int nan = *h_flag;
+    if (nan) KN_CHECK(hipMemset(nan_flag_ptr(), 0, sizeof(int)));
+    return nan;
 }

@@ -21,15 +21,19 @@ struct ResArgs {
     long long rows;
     double boxsize, norm;
     const double *logkk, *ratio;
-    const int *bucket;  // nbuckets + 1 segment indices (k_res_buckets), or null: each block builds its own
     int *nan_flag;
+    // Green's function of the PM potential (GREEN sweeps): the per-axis inverse
+    // CiC window (dims doubles) and (2 pi r_s / BoxSize)^2
+    const double *iw;
+    double asmth2;
 };
 
 constexpr int kBucketsPerBin = 4;
 
-__host__ __device__ inline size_t smem_bytes(int nbins, int nbuckets)
+__host__ __device__ inline size_t smem_bytes(int nbins, int nbuckets, int green_dims = 0)
 {
-    return sizeof(double) * 2 * (size_t)nbins + sizeof(int) * (size_t)(nbuckets + 1);
+    return sizeof(double) * (2 * (size_t)nbins + (size_t)green_dims) +
+           sizeof(int) * (size_t)(nbuckets + 1 + ((nbuckets + 1) & 1));
 }
 
 struct Tab {
@@ -63,36 +67,9 @@ __device__ __forceinline__ Tab load_tab(const ResArgs &a, int nthreads)
     T.norm = a.norm;
     T.log_kunit = log(2 * M_PI / a.boxsize);
     // bucket[q]: the GSL segment index (largest i <= n-2 with lk[i] <= x) at the bucket's left edge
-    if (a.bucket) {
-        for (int q = threadIdx.x; q <= a.nbuckets; q += nthreads) bucket[q] = a.bucket[q];
-    } else {
-        for (int q = threadIdx.x; q <= a.nbuckets; q += nthreads) {
-            const double edge = T.x0 + q / T.inv_bw;
-            int lo = 0, hi = a.nbins - 1;
-            while (hi > lo + 1) {
-                const int mid = (lo + hi) >> 1;
-                if (lk[mid] > edge) hi = mid;
-                else lo = mid;
-            }
-            bucket[q] = lo;
-        }
-    }
-    __syncthreads();
-    return T;
-}
-
-// The bucket index of load_tab, once for all blocks of a sweep (one block).
-__global__ __launch_bounds__(256) void k_res_buckets(const double *__restrict__ lk, int nbins, int nbuckets,
-                                                      int *__restrict__ bucket)
-{
-    if (nbins < 2) {
-        for (int q = threadIdx.x; q <= nbuckets; q += 256) bucket[q] = 0;
-        return;
-    }
-    const double x0 = lk[0], inv_bw = nbuckets / (lk[nbins - 1] - x0);
-    for (int q = threadIdx.x; q <= nbuckets; q += 256) {
-        const double edge = x0 + q / inv_bw;
-        int lo = 0, hi = nbins - 1;
+    for (int q = threadIdx.x; q <= a.nbuckets; q += nthreads) {
+        const double edge = T.x0 + q / T.inv_bw;
+        int lo = 0, hi = a.nbins - 1;
         while (hi > lo + 1) {
             const int mid = (lo + hi) >> 1;
             if (lk[mid] > edge) hi = mid;
@@ -100,15 +77,16 @@ __global__ __launch_bounds__(256) void k_res_buckets(const double *__restrict__ 
         }
         bucket[q] = lo;
     }
+    __syncthreads();
+    return T;
 }
 
 // 1 + get_dnudcdm_powerspec(log(|k|)) for k2 > 0, with |k| = sqrt(k2) 2 pi / BoxSize.
 // log|k| is evaluated as 0.5 log(k2) + log(2 pi / BoxSize): the same value as
 // the reference's log(sqrt(k2)*2*M_PI/BoxSize) to about one ulp, with one
 // transcendental per mode instead of sqrt, two divides and a log.
-__device__ __forceinline__ double smth_of(int k2i, double boxsize, const Tab &T)
+__device__ __forceinline__ double smth_of(int k2i, const Tab &T)
 {
-    (void)boxsize;
     double kk = 0.5 * log((double)k2i) + T.log_kunit;
     if (kk < T.x0 && kk > T.lower_clamp) kk = T.x0;
     if (kk < T.lower_clamp) kk = T.x0;
@@ -153,92 +131,42 @@ __device__ __forceinline__ Pos pos_of(long long off, long long row0, int rowlen,
     return p;
 }
 
-// Coalesced sweep over a contiguous row range per block, UNROLL loads in flight.
-template <int WAVES, int UNROLL>
-__global__ __launch_bounds__(64 * WAVES) void k_rescale_rows(ResArgs a)
+// The PM Green's function with CiC deconvolution at k2 > 0, as the potential
+// pass of a PM step applies it after the neutrino rescale (Gadget-2's
+// pmforce_periodic, where the hook of 0002-Add-hooks-to-kspace-neutrino-code.patch:116-125
+// calls add_nu_power_to_rhogrid): -exp(-k2 asmth2) / k2 times ff^4, ff the
+// product of the three per-axis inverse windows.
+__device__ __forceinline__ double green_of(int k2, double iwx, double iwy, double iwz, double asmth2)
 {
-    constexpr int B = 64 * WAVES;
-    const Tab T = load_tab(a, B);
-    const int rowlen = a.dims / 2 + 1;
-    const long long r0 = a.rows * blockIdx.x / gridDim.x, r1 = a.rows * (blockIdx.x + 1) / gridDim.x;
-    const long long m1 = r1 * rowlen;
-    Pos p = pos_of(threadIdx.x, r0, rowlen, a.dims);
-    bool saw_nan = false;
-    for (long long m = r0 * rowlen + threadIdx.x; m < m1; m += (long long)B * UNROLL) {
-        double2 f[UNROLL];
-        Pos q[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; u++) {
-            q[u] = p;
-            const long long mu = m + (long long)B * u;
-            if (mu < m1) f[u] = a.F[mu];
-            advance(p, B, rowlen, a.dims);
-        }
-#pragma unroll
-        for (int u = 0; u < UNROLL; u++) {
-            const long long mu = m + (long long)B * u;
-            if (mu >= m1) break;
-            const int kx = kval(a.startslab + q[u].i, a.dims), ky = kval(q[u].j, a.dims);
-            const int k2 = kx * kx + ky * ky + q[u].z * q[u].z;
-            if (k2 <= 0) continue;
-            const double s = smth_of(k2, a.boxsize, T);
-            saw_nan |= isnan(s);
-            f[u].x *= s;
-            f[u].y *= s;
-            a.F[mu] = f[u];
-        }
-    }
-    if (saw_nan) atomicOr(a.nan_flag, 1);
+    const double ff = iwx * iwy * iwz;
+    double g = -exp(-k2 * asmth2) / k2;
+    return g * (ff * ff * ff * ff);
 }
 
-// Each lane rescales RUN consecutive modes (one row mostly); the wave's RUN
-// accesses cover one contiguous span.
-template <int WAVES, int RUN>
-__global__ __launch_bounds__(64 * WAVES) void k_rescale_runs(ResArgs a)
-{
-    constexpr int B = 64 * WAVES;
-    const Tab T = load_tab(a, B);
-    const int rowlen = a.dims / 2 + 1;
-    const long long r0 = a.rows * blockIdx.x / gridDim.x, r1 = a.rows * (blockIdx.x + 1) / gridDim.x;
-    const long long m1 = r1 * rowlen;
-    Pos p = pos_of((long long)threadIdx.x * RUN, r0, rowlen, a.dims);
-    bool saw_nan = false;
-    for (long long m = r0 * rowlen + (long long)threadIdx.x * RUN; m < m1; m += (long long)B * RUN) {
-        double2 f[RUN];
-#pragma unroll
-        for (int r = 0; r < RUN; r++)
-            if (m + r < m1) f[r] = a.F[m + r];
-        Pos q = p;
-#pragma unroll
-        for (int r = 0; r < RUN; r++) {
-            if (m + r >= m1) break;
-            if (r) advance(q, 1, rowlen, a.dims);
-            const int kx = kval(a.startslab + q.i, a.dims), ky = kval(q.j, a.dims);
-            const int k2 = kx * kx + ky * ky + q.z * q.z;
-            if (k2 <= 0) continue;
-            const double s = smth_of(k2, a.boxsize, T);
-            saw_nan |= isnan(s);
-            f[r].x *= s;
-            f[r].y *= s;
-            a.F[m + r] = f[r];
-        }
-        advance(p, B * RUN, rowlen, a.dims);
-    }
-    if (saw_nan) atomicOr(a.nan_flag, 1);
-}
-
-// Software-pipelined coalesced sweep: the next UNROLL modes are loaded
-// before the current ones are rescaled and stored; NTL / NTS put the
-// non-temporal hint on the loads / stores.
-template <int WAVES, int UNROLL, bool NTL = false, bool NTS = false, bool REV = false>
+// The rescale sweep: each block a contiguous row range, lane l of the block
+// takes modes l, l + B, ... (coalesced), the next UNROLL modes loaded before
+// the current ones are rescaled and stored; loads and stores carry the
+// non-temporal hint (each mode is read and written once per sweep).
+// GREEN: the same sweep also applies the Green's function (F s_nu, then times
+// G, the two roundings of the two passes it replaces) and zeroes the k = 0
+// mode; with nbins 0 it applies the Green's function alone.
+template <int WAVES, int UNROLL, bool GREEN = false>
 __global__ __launch_bounds__(64 * WAVES) void k_rescale_rows_pf(ResArgs a)
 {
     constexpr int B = 64 * WAVES;
-    const Tab T = load_tab(a, B);
+    Tab T;
+    const double *iw = nullptr;
+    if (!GREEN || a.nbins > 0) T = load_tab(a, B);
+    if constexpr (GREEN) {
+        extern __shared__ double sm[];
+        int *after_buckets = (int *)(sm + 2 * (size_t)a.nbins) + a.nbuckets + 1 + ((a.nbuckets + 1) & 1);
+        double *iw_l = (double *)after_buckets;  // 8-byte aligned
+        for (int t = threadIdx.x; t < a.dims; t += B) iw_l[t] = a.iw[t];
+        __syncthreads();
+        iw = iw_l;
+    }
     const int rowlen = a.dims / 2 + 1;
-    // REV: the first blocks dispatched take the last rows
-    const long long blk = REV ? (long long)gridDim.x - 1 - blockIdx.x : blockIdx.x;
-    const long long r0 = a.rows * blk / gridDim.x, r1 = a.rows * (blk + 1) / gridDim.x;
+    const long long r0 = a.rows * blockIdx.x / gridDim.x, r1 = a.rows * (blockIdx.x + 1) / gridDim.x;
     const long long m1 = r1 * rowlen;
     const long long S = (long long)B * UNROLL;
     Pos p = pos_of(threadIdx.x, r0, rowlen, a.dims);
@@ -248,14 +176,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_rescale_rows_pf(ResArgs a)
 #pragma unroll
     for (int u = 0; u < UNROLL; u++) {
         const long long mu = m + (long long)B * u;
-        cur[u] = mu < m1 ? kn_mem::ld<NTL>(a.F + mu) : make_double2(0.0, 0.0);
+        cur[u] = mu < m1 ? kn_mem::ld<true>(a.F + mu) : make_double2(0.0, 0.0);
     }
     for (; m < m1; m += S) {
         double2 nxt[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
             const long long mu = m + S + (long long)B * u;
-            nxt[u] = mu < m1 ? kn_mem::ld<NTL>(a.F + mu) : make_double2(0.0, 0.0);
+            nxt[u] = mu < m1 ? kn_mem::ld<true>(a.F + mu) : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
@@ -265,13 +193,23 @@ __global__ __launch_bounds__(64 * WAVES) void k_rescale_rows_pf(ResArgs a)
             if (mu >= m1) continue;
             const int kx = kval(a.startslab + q.i, a.dims), ky = kval(q.j, a.dims);
             const int k2 = kx * kx + ky * ky + q.z * q.z;
-            if (k2 <= 0) continue;
-            const double s = smth_of(k2, a.boxsize, T);
-            saw_nan |= isnan(s);
+            if (k2 <= 0) {
+                if (GREEN) kn_mem::st<true>(a.F + mu, make_double2(0.0, 0.0));
+                continue;
+            }
             double2 f = cur[u];
-            f.x *= s;
-            f.y *= s;
-            kn_mem::st<NTS>(a.F + mu, f);
+            if (!GREEN || a.nbins > 0) {
+                const double s = smth_of(k2, T);
+                saw_nan |= isnan(s);
+                f.x *= s;
+                f.y *= s;
+            }
+            if constexpr (GREEN) {
+                const double g = green_of(k2, iw[a.startslab + q.i], iw[q.j], iw[q.z], a.asmth2);
+                f.x *= g;
+                f.y *= g;
+            }
+            kn_mem::st<true>(a.F + mu, f);
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) cur[u] = nxt[u];

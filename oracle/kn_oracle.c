@@ -1077,6 +1077,45 @@ int orc_total_powerspectrum(int dims, const double *field, int nrbins, int start
     return nonzero;
 }
 
+/* The Green's-function multiply of a PM potential pass, the step that follows
+ * the neutrino rescale in Gadget-2's pmforce_periodic (the hook of
+ * 0002-Add-hooks-to-kspace-neutrino-code.patch:116-125 sits right before it):
+ * for k2 > 0, F *= -exp(-k2 asmth2)/k2 * ff^4 with ff = 1/(fx fy fz),
+ * f = sin(pi k/N)/(pi k/N) per axis (1 at k = 0); the k = 0 mode is zeroed on
+ * the rank holding slab 0.  asmth2 = (2 pi asmth / BoxSize)^2.  Restated from
+ * the published Gadget-2 source (not part of the reference tree). */
+void orc_green(int dims, double *field, int startslab, int nslab, double asmth2)
+{
+    int i, j, k;
+    for (i = startslab; i < startslab + nslab; i++)
+        for (j = 0; j < dims; j++)
+            for (k = 0; k < dims / 2 + 1; k++) {
+                const int ky = i > dims / 2 ? i - dims : i, kx = j > dims / 2 ? j - dims : j, kz = k;
+                const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
+                const size_t ip = ((size_t)(i - startslab) * dims + j) * (size_t)(dims / 2 + 1) + k;
+                double smth, fx = 1, fy = 1, fz = 1, ff;
+                if (k2 <= 0) continue;
+                smth = -exp(-k2 * asmth2) / k2;
+                if (kx != 0) {
+                    fx = (M_PI * kx) / dims;
+                    fx = sin(fx) / fx;
+                }
+                if (ky != 0) {
+                    fy = (M_PI * ky) / dims;
+                    fy = sin(fy) / fy;
+                }
+                if (kz != 0) {
+                    fz = (M_PI * kz) / dims;
+                    fz = sin(fz) / fz;
+                }
+                ff = 1 / (fx * fy * fz);
+                smth *= ff * ff * ff * ff;
+                field[2 * ip] *= smth;
+                field[2 * ip + 1] *= smth;
+            }
+    if (startslab == 0) field[0] = field[1] = 0.0;
+}
+
 /* ======================= delta_pow + rescale ======================= */
 
 /* delta_pow.c:19-37 */

@@ -132,6 +132,8 @@ int orc_get_delta_nu_adjoint(orc_dtt *d, double a, const double wavenum[], doubl
 double orc_specialJ_dmu(double kfsl, double mu, double vcrit, int hybrid, double nufrac_low);
 
 /* ---- k-shell binning and rescale (powerspectrum.c:7-117, interface_gadget.c:158-194) ---- */
+/* The PM Green's-function multiply that follows the neutrino rescale (Gadget-2 pmforce_periodic). */
+void orc_green(int dims, double *field, int startslab, int nslab, double asmth2);
 /* powerspectrum.c:56-89: raw per-bin sums of one rank's slabs (no normalisation, no compaction). */
 void orc_bin_sums(int dims, const double *field, int nrbins, int startslab, int nslab, double *power,
                   long long *count, double *keffs);

@@ -66,8 +66,8 @@ extern "C" int emu_fs_two_pass_check(double (*hub)(double, void *), void *ctx, d
     upload_constants();
     const double w = 1.0 / 256.0;
     const int npanel = (int)ceil((xa - x0) / w) + 1;
-    std::vector<double> coef((size_t)npanel * kHStride);
-    for (int p = 0; p < npanel; p++) cheb_panel(x0 + p * w, w, hub, ctx, &coef[(size_t)p * kHStride]);
+    std::vector<double> coef;
+    h_table(x0, w, npanel, [hub, ctx](double a) { return hub(a, ctx); }, coef);
     HTab H;
     H.coef = coef.data();
     H.x0 = x0;

@@ -8,7 +8,11 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libkn_oracle.so")
+# KN_ORACLE_BUILD=fast selects the build with the reference's compile flags
+# (-O2 -ffast-math), used only by the CPU-baseline timings.
+LIB_PATH = os.path.join(ROOT, "oracle", "_build",
+                        "libkn_oracle_fast.so" if os.environ.get("KN_ORACLE_BUILD") == "fast" else "libkn_oracle.so")
+FAST_LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libkn_oracle_fast.so")
 
 dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)
@@ -49,14 +53,30 @@ class OrcDeltaPow(C.Structure):
 
 
 _lib = None
+_lib_fast = None
+
+
+def lib_fast():
+    """The fast-math build (stateless binning/rescale functions only: the
+    integrator state of this process lives in lib())."""
+    global _lib_fast
+    if _lib_fast is None:
+        _lib_fast = _bind(FAST_LIB_PATH)
+    return _lib_fast
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        _lib = _bind(LIB_PATH)
+    return _lib
+
+
+def _bind(path):
+    if True:  # one indentation level with the signature table below
+        if not os.path.exists(path):
             raise RuntimeError("oracle not built: run `make -C oracle`")
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         sig = {
             "orc_init_omega_nu": (None, [C.POINTER(OrcOmegaNu), dp, C.c_double, C.c_double, C.c_double]),
             "orc_get_omega_nu": (C.c_double, [C.POINTER(OrcOmegaNu), C.c_double]),
@@ -86,6 +106,7 @@ def lib():
                                                   C.POINTER(OrcTransfer)]),
             "orc_total_powerspectrum": (C.c_int, [C.c_int, dp, C.c_int, C.c_int, C.c_int, dp, llp, dp]),
             "orc_bin_sums": (None, [C.c_int, dp, C.c_int, C.c_int, C.c_int, dp, llp, dp]),
+            "orc_green": (None, [C.c_int, dp, C.c_int, C.c_int, C.c_double]),
             "orc_psindex": (C.c_int, [C.c_longlong, C.c_int, C.c_int]),
             "orc_get_dnudcdm": (C.c_double, [C.POINTER(OrcDeltaPow), C.c_double]),
             "orc_rescale": (C.c_int, [C.c_int, dp, C.c_int, C.c_int, C.c_double, C.POINTER(OrcDeltaPow)]),
@@ -101,8 +122,7 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+        return L
 
 
 def ptr(a):
@@ -142,26 +162,36 @@ def bin_power_adjoint(field_c, nrbins, gbar, startslab=0):
     return rc, out.view(np.complex128).reshape(field_c.shape)
 
 
-def bin_sums(field_c, nrbins, startslab=0):
+def bin_sums(field_c, nrbins, startslab=0, L=None):
     """Raw per-bin sums (power, count, keff) of the slabs in field_c [nslab][N][N/2+1]."""
     dims = field_c.shape[1]
     f = np.ascontiguousarray(field_c).view(np.float64).ravel()
     power, keffs = np.zeros(nrbins), np.zeros(nrbins)
     count = np.zeros(nrbins, dtype=np.int64)
-    lib().orc_bin_sums(dims, ptr(f), nrbins, startslab, field_c.shape[0], ptr(power), count.ctypes.data_as(llp),
-                       ptr(keffs))
+    (L or lib()).orc_bin_sums(dims, ptr(f), nrbins, startslab, field_c.shape[0], ptr(power),
+                              count.ctypes.data_as(llp), ptr(keffs))
     return power, count, keffs
 
 
-def rescale(field_c, boxsize, logkk, ratio, norm, startslab=0):
+def rescale(field_c, boxsize, logkk, ratio, norm, startslab=0, L=None, in_place=False):
     dims = field_c.shape[1]
-    f = np.ascontiguousarray(field_c.copy())
+    f = field_c if in_place else np.ascontiguousarray(field_c.copy())
+    assert f.flags.c_contiguous
     fv = f.view(np.float64).ravel()
     lk = np.ascontiguousarray(logkk, dtype=np.float64)
     rt = np.ascontiguousarray(ratio, dtype=np.float64)
     dpow = OrcDeltaPow(ptr(lk), ptr(rt), len(lk), norm)
-    rc = lib().orc_rescale(dims, ptr(fv), startslab, f.shape[0], boxsize, C.byref(dpow))
+    rc = (L or lib()).orc_rescale(dims, ptr(fv), startslab, f.shape[0], boxsize, C.byref(dpow))
     return rc, f
+
+
+def green(field_c, boxsize, asmth, startslab=0):
+    """The PM Green's-function multiply (a copy of field_c)."""
+    dims = field_c.shape[1]
+    f = np.ascontiguousarray(field_c.copy())
+    s = 2 * np.pi * asmth / boxsize
+    lib().orc_green(dims, ptr(f.view(np.float64).ravel()), startslab, f.shape[0], s * s)
+    return f
 
 
 def get_dnudcdm(logkk, ratio, norm, x):

@@ -181,6 +181,37 @@ def test_emu_fslength_two_passes_equal_one_qag_per_node(emu):
         assert (listed > 0) == want_listed
 
 
+def test_emu_fslength_across_the_density_switch_matches_the_oracle():
+    """The host's neutrino density switches from its spline table to the
+    non-relativistic series at a = 100 kT/m (omega_nu_single.c:171-183), where
+    H jumps by ~1e-8 relative.  For m = 0.379 eV that point, ln a = -3.115,
+    lies inside the integration range: the H panel that holds it is split
+    there, so the fslength nodes (two passes over the panel table) match the
+    oracle's per-node QAG on the exact H to rounding.  Without the split the
+    nodes whose Gauss-Kronrod points fall in that panel were 2e-10 off, and
+    the adjoint's d/d delta_tot near the switch 1e-9 off."""
+    import oracle_py as O
+    lib = C.CDLL(DNU_EMU_LIB)
+    lib.emu_fs_two_pass_check.restype = C.c_int
+    lib.emu_fs_two_pass_check.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    m, om0 = 0.379, 0.3
+    c = Cosmo([m, 0.0, 0.0], omega0=om0)
+    x_switch = math.log(100 * c.P.om.kBtnu / m)
+    x0, nfs = math.log(0.01), 2400
+    assert x0 < x_switch < 0.0
+    two, full = (C.c_double * nfs)(), (C.c_double * nfs)()
+    lib.emu_fs_two_pass_check(c.hfn, C.addressof(c.ctx), x0, 0.0, nfs, 1.0, two, full)
+    O.lib().orc_init_hubble(C.byref(c.P.oom), om0, 3.085678e21 / 1e5)
+    xs = x0 + np.arange(nfs) * (0.0 - x0) / (nfs - 1.0)
+    orc = np.array([O.lib().orc_fslength(float(v), 0.0, 1.0) for v in xs[:-1]])
+    err = float(np.max(np.abs(np.array(two[:-1]) / orc - 1)))
+    print("  fslength at %d nodes across ln a = %.4f: max rel err %.2e" % (nfs, x_switch, err))
+    assert err < 1e-13
+    assert np.array_equal(np.array(two[:]), np.array(full[:]))
+    c.free()
+
+
 def _group_arrays(B):
     n_y = B.off_y[B.n]
     return (B.delta_nu(), B._get("dtot_bar", 0, n_y), B._get("init_bar", 0, B.off_k[B.n]), B.mnu_bar())

@@ -31,7 +31,8 @@ def gpu_adjoint(kn, torch, F, nrbins, g):
     return fb.cpu().numpy()
 
 
-@pytest.mark.parametrize("N,nrbins", [(8, 12), (16, 20), (32, 40), (48, 64), (9, 5), (15, 16)])
+# (32, 20000): the adjoint sweep reads its tables from global memory
+@pytest.mark.parametrize("N,nrbins", [(8, 12), (16, 20), (32, 40), (48, 64), (9, 5), (15, 16), (32, 20000)])
 def test_bin_adjoint_matches_oracle(kn, torch_cuda, N, nrbins):
     F = _rand_grid(N, seed=N)
     F[0, 0, 0] = N ** 3 * (1 - 0.2j)

@@ -282,3 +282,38 @@ def test_get_delta_nu_at_configuration_shapes(kn, torch_cuda, cfg):
     print("  %s: max rel err vs oracle %.3e" % (cfg, e))
     assert rc == 0 and e < RTOL
     P.free()
+
+
+def test_history_stays_on_the_device_and_sees_host_edits(kn, torch_cuda):
+    """get_delta_nu keeps the delta_tot history on the device and uploads only
+    the columns that differ from what it holds: a first call uploads every
+    stored step, an appended step one column, an overwritten last step one,
+    an interior edit written straight into the host table (no library call)
+    the columns from the edit on, a dropped step none; every call against the
+    oracle on the same table."""
+    masses = [0.15, 0.0, 0.0]
+    P = Pair(masses, 0.3, 12, 0.01 + 80 / 100.0 + 0.02)
+    logs, rows, k, init = synthetic_history(80, 12, 0.8, noise=0.02)
+    n0 = 60
+
+    def check(na, expect_cols, what):
+        P.set_history(logs[:na], rows[:, :na], init)
+        a = math.exp(logs[na - 1])
+        g = P.gpu_get_delta_nu(a, k, 0.15)
+        cols = kn.kn_dnu_last_upload_columns()
+        rc, o = P.T.get_delta_nu(a, k, 0.15)
+        e = rel(g, o)
+        print("  %-34s Na=%d: %d columns uploaded, rel err %.2e" % (what, na, cols, e))
+        assert rc == 0 and e < RTOL
+        assert cols == expect_cols, (what, cols, expect_cols)
+
+    print("device-resident history", flush=True)
+    check(n0, n0, "first call")
+    check(n0 + 1, 1, "one step appended")
+    rows[:, n0] *= 1.01
+    check(n0 + 1, 1, "last step overwritten")
+    rows[3, 10] *= 1.5
+    check(n0 + 1, n0 + 1 - 10, "interior entry edited")
+    check(n0, 0, "last step dropped")
+    check(n0 + 2, 2, "two steps appended")
+    P.free()
