@@ -258,6 +258,18 @@ int kn_bin_power_device(const fftw_complex *d_field, int dims, int nrbins, int s
 int kn_rescale_device(fftw_complex *d_field, int dims, int startslab, int nslab, double boxsize,
                       const double *d_logkk, const double *d_ratio, int nbins, double norm, void *stream);
 
+/* kn_bin_power_device with no allocation: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * outputs; the sweep's bin thresholds, inverse window and per-block partial
+ * sums live in a 32 MiB device array of the library, one per GPU, shared by
+ * every call on that GPU (calls that overlap on one GPU race).  Limits:
+ * dims <= 65536, nrbins <= 262144 (3004 beyond them).  A new (dims, nrbins)
+ * copies its tables from the host synchronously on stream; otherwise the call
+ * only launches on stream. */
+int kn_bin_power_device_noalloc(const fftw_complex *d_field, int dims, int nrbins, int startslab, int nslab,
+                                double *d_power, double *d_total_mass2, void *stream);
+
 /* get_delta_nu keeps each table's delta_tot history in device memory at the
  * host's row stride and compares the host rows with its copy on every call:
  * only columns from a row's first changed entry on are uploaded (an appended

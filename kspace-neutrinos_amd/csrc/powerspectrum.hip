@@ -423,6 +423,147 @@ extern "C" int kn_bin_power_device(const fftw_complex *d_field, int dims, int nr
     return 0;
 }
 
+// ---- the bin sweep without allocation ---------------------------------
+// kn_bin_power_device keeps its bin thresholds, inverse window and per-block
+// partial sums in device buffers it allocates when it first meets a geometry.
+// kn_bin_power_device_noalloc keeps them instead in `scratch`, a file-scope
+// device array of kScratchBytes: every GPU the library runs on holds its own
+// copy until the process ends, and every call on that GPU shares it (the
+// tables of the last geometry stay there; the partial sums are rewritten
+// whole by every sweep, so nothing has to start at zero).  Calls that overlap
+// on one GPU race on it.
+constexpr size_t kScratchBytes = (size_t)32 << 20;  // per GPU
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kScratchMaxDims = 65536, kScratchMaxBins = 1 << 18;
+constexpr int kScratchMaxDevices = 64;
+
+struct ScratchDev {
+    unsigned char *ptr = nullptr;  // this device's copy of scratch (hipGetSymbolAddress)
+    int dims = 0, nrbins = 0;      // the geometry whose tables the copy holds
+};
+ScratchDev g_scratch[kScratchMaxDevices];
+
+// host copies of the last geometry's tables (no allocation per call)
+int g_sthr[kScratchMaxBins + 1];
+double g_siw[kScratchMaxDims];
+int g_sdims = 0, g_snrbins = 0;
+
+inline size_t round256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+}  // namespace
+
+// kn_bin_power_device (same arguments and outputs) with no allocation: the
+// sweep's tables and partial sums live in scratch.  Limits: dims <= 65536,
+// nrbins <= 262144, and the tables plus at least one block's partial sums
+// within 32 MiB; the sweep's grid is capped so that its partial sums fit.
+// Returns 0, 3002 for bad arguments, 3004 for sizes beyond those limits.
+extern "C" int kn_bin_power_device_noalloc(const fftw_complex *d_field, int dims, int nrbins, int startslab,
+                                           int nslab, double *d_power, double *d_total_mass2, void *stream)
+{
+    if (!d_field || !d_power || !d_total_mass2 || dims < 2 || nrbins < 1 || nslab < 1 || startslab < 0 ||
+        startslab + nslab > dims) {
+        kn_set_last_error("kn_bin_power_device_noalloc: bad geometry dims=%d nrbins=%d slabs=[%d,%d)", dims, nrbins,
+                          startslab, startslab + nslab);
+        return KN_ERR_ARG;
+    }
+    const size_t off_iw = round256(sizeof(int) * ((size_t)nrbins + 1));
+    const size_t off_part = off_iw + round256(sizeof(double) * (size_t)dims);
+    if (dims > kScratchMaxDims || nrbins > kScratchMaxBins || off_part + sizeof(double) * nrbins > kScratchBytes) {
+        kn_set_last_error("kn_bin_power_device_noalloc: dims=%d nrbins=%d beyond the %zu-byte scratch (dims <= %d, "
+                          "nrbins <= %d)", dims, nrbins, kScratchBytes, kScratchMaxDims, kScratchMaxBins);
+        return KN_ERR_RANGE;
+    }
+    int dev = 0;
+    KN_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchMaxDevices) {
+        kn_set_last_error("kn_bin_power_device_noalloc: device %d beyond %d", dev, kScratchMaxDevices);
+        return KN_ERR_RANGE;
+    }
+    ScratchDev &sd = g_scratch[dev];
+    if (!sd.ptr) {
+        const hipError_t e = hipGetSymbolAddress((void **)&sd.ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            kn_set_last_error("kn_bin_power_device_noalloc: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+            sd.ptr = nullptr;
+            return KN_ERR_HIP;
+        }
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (sd.dims != dims || sd.nrbins != nrbins) {
+        if (g_sdims != dims || g_snrbins != nrbins) {
+            const long long bad = kn_bin::build_thresholds(dims, nrbins, g_sthr);
+            if (bad) {
+                g_sdims = g_snrbins = 0;
+                kn_set_last_error("kn_bin_power_device_noalloc: bin index of |k|^2 = %lld is not monotone or "
+                                  "beyond %d bins", bad, nrbins);
+                return KN_ERR_RANGE;
+            }
+            kn_bin::build_invwindow(dims, g_siw);
+            g_sdims = dims;
+            g_snrbins = nrbins;
+        }
+        KN_CHECK(hipMemcpyAsync(sd.ptr, g_sthr, sizeof(int) * ((size_t)nrbins + 1), hipMemcpyHostToDevice, st));
+        KN_CHECK(hipMemcpyAsync(sd.ptr + off_iw, g_siw, sizeof(double) * dims, hipMemcpyHostToDevice, st));
+        KN_CHECK(hipStreamSynchronize(st));  // the host tables may be rebuilt by the next call
+        sd.dims = dims;
+        sd.nrbins = nrbins;
+    }
+    // placement and grid as get_geom chooses them, the grid capped by the scratch
+    int place = kn_bin::kTablesGlobal;
+    for (int pl : {kn_bin::kTablesLds, kn_bin::kWindowGlobal})
+        if (kn_bin::sweep_smem_bytes(pl, dims, nrbins) <= kLdsMax / 2) {
+            place = pl;
+            break;
+        }
+    if (place == kn_bin::kTablesGlobal)
+        for (int pl : {kn_bin::kTablesLds, kn_bin::kWindowGlobal})
+            if (kn_bin::sweep_smem_bytes(pl, dims, nrbins) <= kLdsMax) {
+                place = pl;
+                break;
+            }
+    const size_t smem = kn_bin::sweep_smem_bytes(place, dims, nrbins);
+    const int per_cu = place == kn_bin::kTablesLds
+                           ? resident_blocks(kn_bin::k_bin_sweep<kn_bin::kTablesLds>, 64 * kn_bin::kSweepWaves, smem)
+                       : place == kn_bin::kWindowGlobal
+                           ? resident_blocks(kn_bin::k_bin_sweep<kn_bin::kWindowGlobal>, 64 * kn_bin::kSweepWaves,
+                                             smem)
+                           : resident_blocks(kn_bin::k_bin_sweep<kn_bin::kTablesGlobal>, 64 * kn_bin::kSweepWaves,
+                                             smem);
+    const long long rows = (long long)nslab * dims;
+    long long grid = (long long)per_cu * num_cus() * kSweepSlots;
+    const long long fit = (long long)((kScratchBytes - off_part) / (sizeof(double) * (size_t)nrbins));
+    if (grid > fit) grid = fit;
+    if (grid > rows) grid = rows;
+    kn_bin::BinArgs a;
+    a.F = (const double2 *)d_field;
+    a.dims = dims;
+    a.startslab = startslab;
+    a.nrbins = nrbins;
+    a.rows = rows;
+    a.thr = (const int *)sd.ptr;
+    a.iw = (const double *)(sd.ptr + off_iw);
+    a.half_bpu = (float)(0.5 * (nrbins - 1) / log(sqrt(3) * dims / 2.0));
+    a.partial = (double *)(sd.ptr + off_part);
+    const dim3 gr((unsigned)grid), block(64 * kn_bin::kSweepWaves);
+    if (place == kn_bin::kTablesLds)
+        hipLaunchKernelGGL(kn_bin::k_bin_sweep<kn_bin::kTablesLds>, gr, block, smem, st, a);
+    else if (place == kn_bin::kWindowGlobal)
+        hipLaunchKernelGGL(kn_bin::k_bin_sweep<kn_bin::kWindowGlobal>, gr, block, smem, st, a);
+    else
+        hipLaunchKernelGGL(kn_bin::k_bin_sweep<kn_bin::kTablesGlobal>, gr, block, smem, st, a);
+    KN_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_partials, dim3((nrbins + 63) / 64), dim3(64 * kRedWaves), 0, st, a.partial,
+                       (int)grid, nrbins, d_power);
+    KN_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_total_mass2, dim3(1), dim3(1), 0, st, a.F, startslab == 0 ? 1 : 0, d_total_mass2);
+    KN_CHECK(hipGetLastError());
+    return 0;
+}
+
 // Geometry sums for callers that already hold the power sums (add_nu_power_to_rhogrid).
 extern "C" void kn_bin_geometry_host(int dims, int nrbins, int startslab, int nslab, long long *count,
                                      double *keff_sum)

@@ -118,6 +118,8 @@ SIGNATURES = {
                                       C.c_void_p]),
     "kn_rescale_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                     C.c_int, C.c_double, C.c_void_p]),
+    "kn_bin_power_device_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                              C.c_void_p, C.c_void_p]),
     "kn_dnu_last_upload_columns": (C.c_int, []),
     "kn_rescale_green_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p,
                                           C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p]),
