@@ -132,10 +132,31 @@ def dnu_bench(kn, nk, na=2000, calls=3, cpu_k=16, no_cpu=False):
         kn.get_delta_nu_combined(C.byref(P.d), 1.0, kk.ctypes.data_as(DP), out.ctypes.data_as(DP))
         t.append(time.perf_counter() - t0)
     tg = float(np.median(t))
+    # the QAG kernel alone, HIP events around its launch on its stream
+    kn.kn_dnu_qag_timing(1)
+    tq = []
+    for _ in range(calls):
+        kn.get_delta_nu_combined(C.byref(P.d), 1.0, kk.ctypes.data_as(DP), out.ctypes.data_as(DP))
+        tq.append(kn.kn_dnu_last_qag_ms())
+    kn.kn_dnu_qag_timing(0)
+    items = C.c_int()
+    total = kn.kn_dnu_last_intervals(C.byref(items))
+    evals = 61 * (2 * total - items.value)          # QAG: the 61-point rule 2n - 1 times per item
+    flops = 570.0 * evals                           # hybrid integrand, flops per evaluation (see DESIGN)
+    tqm = float(np.median(tq)) * 1e-3
     ksteps = 2.0 * nk * na
     res = {"workload": "get_delta_nu_combined, hybrid 0.06/0.06/0.3 eV, %d k x %d steps, a=1" % (nk, na),
            "ms_per_call": round(tg * 1e3, 3), "kstep_per_s": round(ksteps / tg, 1),
-           "note": "k.step = one (k bin, stored step) pair per distinct mass; plain C-ABI call, host in/out"}
+           "note": "k.step = one (k bin, stored step) pair per distinct mass; plain C-ABI call, host in/out",
+           "roofline": {"bound": "fp64", "kernel": "k_dnu_qag", "achieved": round(flops / tqm / 1e12, 3),
+                        "peak": 78.6, "unit": "TFLOP/s", "frac": round(flops / tqm / 1e12 / 78.6, 4),
+                        "kernel_ms": round(tqm * 1e3, 4), "qag_items": items.value,
+                        "intervals": int(total), "evaluations": int(evals),
+                        "evaluations_per_kstep": round(evals / ksteps, 3),
+                        "note": "algorithmic FP64 work: integrand evaluations (61 (2n - 1) per item ending "
+                                "with n subintervals) x 570 flops (hybrid integrand, add/mul 1, fma 2, div "
+                                "and sqrt 10, transcendentals 20), over the QAG kernel's HIP-event time; "
+                                "the kernel is bound by its serial bisection chain, not FP64 issue"}}
     if not no_cpu:
         import cpu_timing as CT
         sel = np.linspace(0, nk - 1, cpu_k).astype(int)
@@ -270,6 +291,7 @@ def plain_vs_group(kn, torch, keep, hists, data, reps=3, seed=5):
     dev = B.delta_nu()
     diff = max(float(np.max(np.abs(dev[B.off_k[i]:B.off_k[i + 1]] / plain_out[i] - 1))) for i in range(nhist))
     B.close()
+    steps = pm_steps(kn, torch, keep, hists, data, plain_out, g_all)
     for om, d, ctx in keep:
         kn.free_delta_tot_table(C.byref(d))
     ksteps = sum(x["nk"] * x["na"] for x in data)
@@ -288,7 +310,79 @@ def plain_vs_group(kn, torch, keep, hists, data, reps=3, seed=5):
                               "n HIP streams); create (one H table per history) once, outside the step"},
             "speedup_forward_adjoint": round((t_fwd + t_adj) / min(tfa, tgr), 2),
             "speedup_forward": round(t_fwd / tf, 2),
-            "max_rel_diff_group_vs_plain": diff}
+            "max_rel_diff_group_vs_plain": diff,
+            "pm_step": steps}
+
+
+def pm_steps(kn, torch, keep, hists, data, plain_out, g_all, nsteps=8):
+    """The per-PM-step update of the same histories: each created with its
+    last nsteps stored steps missing (na_max = the full count), then stepped
+    nsteps times, each step appending the next stored column and solving +
+    differentiating at its a.  Eager: kn_group_append (host checks) +
+    kn_group_solve + kn_group_adjoint per step.  Graph: the group on its
+    capacity plan, one captured graph of append + solve + adjoint replayed per
+    step (the step's log a and column copied device to device into the
+    captured arrays first).  Both end at a = 1 with every stored step, where
+    they are compared with the per-history path."""
+    import knhip
+    nhist = len(hists)
+    short = []
+    for hh, x in zip(hists, data):
+        h2 = knhip.BatchHistory.from_buffer_copy(hh)
+        h2.na = x["na"] - nsteps
+        h2.na_max = x["na"]
+        h2.a = math.exp(x["logs"][x["na"] - nsteps - 1])
+        h2.a_max = 1.0
+        short.append(h2)
+    res, outs = {}, {}
+    for mode in ("eager", "graph"):
+        B = knhip.Batch(short, keep=keep, api="kn_group")
+        for i, x in enumerate(data):
+            B.load(i, x["logs"][:x["na"] - nsteps], x["rows"][:, :x["na"] - nsteps], x["k"], x["init"])
+        B._put("gbar", 0, g_all)
+        off = np.array(B.off_k[:nhist + 1])
+        la = torch.tensor(np.array([[x["logs"][x["na"] - nsteps + t] for x in data] for t in range(nsteps)]),
+                          dtype=torch.float64, device="cuda")
+        cols = np.zeros((nsteps, int(off[-1])))
+        for i, x in enumerate(data):
+            for t in range(nsteps):
+                cols[t, off[i]:off[i + 1]] = x["rows"][:, x["na"] - nsteps + t]
+        cols = torch.tensor(cols, dtype=torch.float64, device="cuda")
+        s = torch.cuda.Stream()
+        if mode == "graph":
+            B.set_capacity_plan(True)
+            B.stage_step(np.zeros(nhist), np.zeros(int(off[-1])))
+            B.capture(with_adjoint=True, with_append=True, stream=C.c_void_p(s.cuda_stream))
+        torch.cuda.synchronize()
+        ts = []
+        for t in range(nsteps):
+            t0 = time.perf_counter()
+            if mode == "eager":
+                assert kn.kn_group_append(B.h, C.c_void_p(la[t].data_ptr()), C.c_void_p(cols[t].data_ptr()),
+                                          None) == 0
+                assert kn.kn_group_solve(B.h, None) == 0
+                assert kn.kn_group_adjoint(B.h, None) == 0
+            else:
+                with torch.cuda.stream(s):
+                    B.step_buf[0].copy_(la[t])
+                    B.step_buf[1].copy_(cols[t])
+                B.graph_launch(C.c_void_p(s.cuda_stream))
+                s.synchronize()
+            ts.append(time.perf_counter() - t0)
+        assert kn.kn_group_status(B.h) == 0
+        outs[mode] = (B.delta_nu(), B.mnu_bar())
+        res[mode + "_s_per_step"] = round(float(np.median(ts)), 5)
+        B.close()
+    dev = outs["graph"][0]
+    res["graph_equals_eager"] = bool(np.array_equal(outs["graph"][0], outs["eager"][0]) and
+                                     np.array_equal(outs["graph"][1], outs["eager"][1]))
+    res["max_rel_diff_vs_plain"] = max(float(np.max(np.abs(dev[B.off_k[i]:B.off_k[i + 1]] / plain_out[i] - 1)))
+                                       for i in range(nhist))
+    res["steps"] = nsteps
+    res["note"] = ("per PM step: append one stored step to every history, solve and differentiate; eager = "
+                   "kn_group_append + solve + adjoint; graph = one replay of the captured append + solve + adjoint "
+                   "(capacity plan: launches sized for na_max)")
+    return res
 
 
 def c4_cpu_baseline(data, nsample=4, nall=64):

@@ -276,6 +276,16 @@ int kn_bin_power_device_noalloc(const fftw_complex *d_field, int dims, int nrbin
  * or overwritten step is one or two columns; any host write is seen).  This
  * returns how many columns the last call uploaded (diagnostics, tests). */
 int kn_dnu_last_upload_columns(void);
+/* Diagnostics of the last get_delta_nu's QAG items: the largest final
+ * subinterval count, and the count summed over items (items into *items).
+ * An item ending with n subintervals evaluated the integrand 61 (2n - 1)
+ * times, the unit of work the free-streaming integral is measured in. */
+int kn_dnu_last_max_intervals(void);
+long long kn_dnu_last_intervals(int *items);
+/* HIP-event timing of get_delta_nu's QAG kernel, on (1) or off (0, the
+ * default), and the last call's kernel time in ms (measurement only). */
+int kn_dnu_qag_timing(int on);
+double kn_dnu_last_qag_ms(void);
 
 /* (c) fused with the next step of a PM force pass: no reference counterpart.
  * The reference rescales the grid in add_nu_power_to_rhogrid and the caller
@@ -361,13 +371,32 @@ int kn_group_views(kn_group *g, double **scalefact, double **delta_tot, double *
 int kn_group_layout(const kn_group *g, long long *off_k, long long *off_y, long long *off_s);
 int kn_group_solve(kn_group *g, void *stream);   /* synchronous */
 int kn_group_adjoint(kn_group *g, void *stream); /* synchronous */
-/* d_loga[nhist], d_rows[items] in device memory; the host reads the new log a
- * values (launch sizes follow them).  3004 before any write when a history is
- * full or the new a is outside its H table (built up to max(a, a_max)). */
+/* One more stored step per history (delta_tot_table.c:177-191, the step
+ * get_delta_nu_update keeps): d_loga[nhist] (the new log a) and
+ * d_rows[items] (the new delta_tot column, at each history's off_k) in device
+ * memory.  The device checks every history (capacity, the H table range,
+ * built up to max(a, a_max), and a log a after the last stored one) before
+ * anything is written: 3004 or 3002 then, with the history named.  It also
+ * computes the new fslength(T0, a) and re-derives tolerance, qc and the
+ * IC-only state as get_delta_nu does (delta_tot_table.c:513-555). */
 int kn_group_append(kn_group *g, const double *d_loga, const double *d_rows, void *stream);
-/* Capture solve [+ adjoint] on a non-null stream; replayable until the next
- * append or autotune (new masses and new values in the device arrays are read
- * at replay).  d_loga and d_rows must be NULL: append is not capturable. */
+/* Launch sizes for na_max stored steps (on = 1) or for the stored steps (0,
+ * the default).  On the capacity plan the grids cover na_max steps and both
+ * the integral's and the IC-only phases, and the kernels read from device
+ * memory what applies, so device appends leave the launches (and a captured
+ * graph) valid; merged launches only (kn_group_set_streams(g, n > 0) is 3002). */
+int kn_group_set_capacity_plan(kn_group *g, int on);
+/* kn_group_append with no host synchronisation, on the capacity plan (3002
+ * otherwise): one launch checks and appends every history; a history whose
+ * check fails is left as it was and its code is reported by the next
+ * kn_group_status. */
+int kn_group_append_async(kn_group *g, const double *d_loga, const double *d_rows, void *stream);
+/* Capture [append +] solve [+ adjoint] on a non-null stream.  Without append
+ * (d_loga = d_rows = NULL) the graph is replayable until the next append or
+ * autotune (new masses and new values in the device arrays are read at
+ * replay).  With d_loga and d_rows the group goes on its capacity plan and
+ * every replay first appends the step held in those arrays at replay time,
+ * so one graph serves every PM step until a history is full. */
 int kn_group_graph_capture(kn_group *g, int with_adjoint, const double *d_loga, const double *d_rows, void *stream);
 int kn_group_graph_launch(kn_group *g, void *stream);
 int kn_group_status(kn_group *g);

@@ -47,8 +47,9 @@ struct HostHTab {
     double x0 = 0.0;             // ln a of panel 0 (origin - 20); a panel edge falls on the origin
     double width = 1.0 / 256.0;  // panel width in ln a
     int npanel = 0;
-    std::vector<double> coef;
+    std::vector<double> coef;  // npanel panels; the split records follow them on the device
     std::vector<unsigned char> built;
+    HSplits splits;
     double *d_coef = nullptr;
     size_t d_cap = 0;
     bool dirty = false;
@@ -63,20 +64,18 @@ void h_reset(double origin)
     g_h.npanel = (int)(28.0 / g_h.width);
     g_h.coef.assign((size_t)g_h.npanel * kHStride, 0.0);
     g_h.built.assign(g_h.npanel, 0);
+    g_h.splits = HSplits();
     g_h.dirty = true;
 }
 
-// Panel p of the process table (a split panel's right piece appended).
+// Panel p of the process table (and its split record, if it needs one).
 void h_build_panel(int p)
 {
-    double right[kHStride];
+    double rec[kHSplitStride];
     const double lo = g_h.x0 + p * g_h.width;
-    if (h_panel(lo, g_h.x0 + (p + 1) * g_h.width, [](double a) { return kn_hubble(a); },
-                &g_h.coef[(size_t)p * kHStride], right)) {
-        const size_t idx = g_h.coef.size() / kHStride;
-        g_h.coef.insert(g_h.coef.end(), right, right + kHStride);
-        g_h.coef[(size_t)p * kHStride + kHRight] = (double)idx;
-    }
+    const double e = h_panel(lo, g_h.x0 + (p + 1) * g_h.width, [](double a) { return kn_hubble(a); },
+                             &g_h.coef[(size_t)p * kHStride], rec);
+    if (e > kHSplitTol) g_h.splits.offer(p, e, rec);
     g_h.built[p] = 1;
 }
 
@@ -87,6 +86,8 @@ HTab device_htab()
     H.x0 = g_h.x0;
     H.inv_w = 1.0 / g_h.width;
     H.npanel = g_h.npanel;
+    H.nsplit = g_h.splits.n;
+    for (int i = 0; i < kHSplitMax; i++) H.split_p[i] = i < g_h.splits.n ? g_h.splits.p[i] : -1;
     return H;
 }
 
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(256) void k_spline(const double *__restrict__ x, co
 }
 
 // Blocks of k_spline / k_adj_solve for nrows rows of n knots.
-inline int spline_blocks(int n, int nrows)
+__host__ __device__ inline int spline_blocks(int n, int nrows)
 {
     const long long items = (long long)((n + kKeep - 1) / kKeep) * nrows;
     return (int)((items + 3) / 4);
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(256) void k_rows_prep(const double *__restrict__ sx
 
 inline int rows_prep_blocks(int b_sp, int nsb) { return b_sp + (nsb + 256) / 256; }
 
-inline int rows_prep_spline_blocks(int na, int nk) { return na > 2 ? spline_blocks(na, nk) : 0; }
+__host__ __device__ inline int rows_prep_spline_blocks(int na, int nk) { return na > 2 ? spline_blocks(na, nk) : 0; }
 
 struct SpeciesDev {
     double mnubykT, relerr;
@@ -647,7 +648,12 @@ struct DnuDev {
     std::vector<int> first;
 };
 std::unordered_map<const void *, DnuDev *> g_dev;
-int g_last_max_intervals = 0;
+int g_last_max_intervals = 0, g_last_items = 0;
+long long g_last_intervals = 0;
+// optional HIP-event timing of the QAG launch (k_dnu_qag) of each get_delta_nu
+int g_qag_timing = 0;
+hipEvent_t g_qag_ev[2];
+float g_qag_ms = 0.0f;
 int g_last_upload_cols = 0;
 
 template <typename T>
@@ -814,14 +820,18 @@ extern "C" void kn_htable_ensure(double origin, double xlo, double xhi)
 {
     if (!g_h.coef.empty() && origin == g_h.origin) h_validate(xlo, xhi);
     kn_htable_build_host(origin, xlo, xhi);
-    if (g_h.d_cap < g_h.coef.size()) {
+    const size_t nreg = g_h.coef.size(), need = nreg + (size_t)kHSplitMax * kHSplitStride;
+    if (g_h.d_cap < need) {
         if (g_h.d_coef) KN_CHECK(hipFree(g_h.d_coef));
-        KN_CHECK(hipMalloc(&g_h.d_coef, sizeof(double) * g_h.coef.size()));
-        g_h.d_cap = g_h.coef.size();
+        KN_CHECK(hipMalloc(&g_h.d_coef, sizeof(double) * need));
+        g_h.d_cap = need;
         g_h.dirty = true;
     }
     if (g_h.dirty) {
-        KN_CHECK(hipMemcpy(g_h.d_coef, g_h.coef.data(), sizeof(double) * g_h.coef.size(), hipMemcpyHostToDevice));
+        KN_CHECK(hipMemcpy(g_h.d_coef, g_h.coef.data(), sizeof(double) * nreg, hipMemcpyHostToDevice));
+        if (g_h.splits.n)
+            KN_CHECK(hipMemcpy(g_h.d_coef + nreg, g_h.splits.rec, sizeof(double) * kHSplitStride * g_h.splits.n,
+                               hipMemcpyHostToDevice));
         g_h.dirty = false;
     }
 }
@@ -829,10 +839,41 @@ extern "C" void kn_htable_ensure(double origin, double xlo, double xhi)
 // H(e^x) from the table's 1/(a H) panels
 extern "C" double kn_htable_eval_host(double x)
 {
-    return exp(-x) / h_cheb(g_h.coef.data(), g_h.x0, 1.0 / g_h.width, g_h.npanel, x, 1);
+    const double u = (x - g_h.x0) / g_h.width;
+    int p = (int)floor(u);
+    p = p < 0 ? 0 : (p >= g_h.npanel ? g_h.npanel - 1 : p);
+    for (int s = 0; s < g_h.splits.n; s++)
+        if (p == g_h.splits.p[s]) return exp(-x) / h_split_eval((const double *)g_h.splits.rec[s], x, 1);
+    return exp(-x) / h_clenshaw(g_h.coef.data() + (size_t)p * kHStride + kHnc, 2.0 * (u - p) - 1.0);
 }
 
 extern "C" int kn_dnu_last_max_intervals(void) { return g_last_max_intervals; }
+
+// HIP-event timing of get_delta_nu's QAG launch on (1) or off (0); the last
+// call's time in ms (0 when off).
+extern "C" int kn_dnu_qag_timing(int on)
+{
+    if (on && !g_qag_timing) {
+        KN_CHECK(hipEventCreate(&g_qag_ev[0]));
+        KN_CHECK(hipEventCreate(&g_qag_ev[1]));
+    } else if (!on && g_qag_timing) {
+        KN_CHECK(hipEventDestroy(g_qag_ev[0]));
+        KN_CHECK(hipEventDestroy(g_qag_ev[1]));
+    }
+    g_qag_timing = on ? 1 : 0;
+    g_qag_ms = 0.0f;
+    return 0;
+}
+
+extern "C" double kn_dnu_last_qag_ms(void) { return g_qag_timing ? (double)g_qag_ms : 0.0; }
+
+// QAG items of the last get_delta_nu and their final subinterval count: an
+// item ending with n intervals evaluated the 61-point rule 2n - 1 times.
+extern "C" long long kn_dnu_last_intervals(int *items)
+{
+    if (items) *items = g_last_items;
+    return g_last_intervals;
+}
 
 // Columns of the delta_tot history the last get_delta_nu uploaded (the rest
 // was already on the device).
@@ -1059,9 +1100,11 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
     qa.ivl = record ? dev->ivl : nullptr;
     qa.ic.init = nullptr;
     qa.ic.fsl0 = qa.ic.lin = qa.ic.prefac = 0.0;
+    if (g_qag_timing) KN_CHECK(hipEventRecord(g_qag_ev[0], 0));
     hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nitems)), dim3(64 * kWavesPerBlock),
                        Na <= kStageMax ? sizeof(double) * Na : 0, 0, qa, dev->sp, dev->item_species, nk, nitems);
     KN_CHECK(hipGetLastError());
+    if (g_qag_timing) KN_CHECK(hipEventRecord(g_qag_ev[1], 0));
     KN_CHECK(hipMemcpyAsync(res, dev->out, sizeof(double) * nitems, hipMemcpyDeviceToHost, 0));
     KN_CHECK(hipMemcpyAsync(nint, dev->nint, sizeof(int) * nitems, hipMemcpyDeviceToHost, 0));
     KN_CHECK(hipMemcpyAsync(status, dev->status, sizeof(int) * 2, hipMemcpyDeviceToHost, 0));
@@ -1074,8 +1117,14 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
         if (report) return 2001;
         kn_terminate(2001, "%s\n", kn_last_error());
     }
+    if (g_qag_timing) KN_CHECK(hipEventElapsedTime(&g_qag_ms, g_qag_ev[0], g_qag_ev[1]));
     g_last_max_intervals = 0;
-    for (int i = 0; i < nitems; i++) g_last_max_intervals = nint[i] > g_last_max_intervals ? nint[i] : g_last_max_intervals;
+    g_last_intervals = 0;
+    g_last_items = nitems;
+    for (int i = 0; i < nitems; i++) {
+        g_last_max_intervals = nint[i] > g_last_max_intervals ? nint[i] : g_last_max_intervals;
+        g_last_intervals += nint[i];
+    }
     for (size_t i = 0; i < need.size(); i++)
         for (int k = 0; k < nk; k++)
             out[(size_t)need[i] * nk + k] += d->delta_nu_prefac * res[i * nk + k];
@@ -1300,7 +1349,7 @@ double fsl_integrand_ctx(double x, void *p)
 struct GroupHist {
     kn_batch_history spec;
     int na, integ, htab, stream;
-    double x0, xa, xmax, fsl0, lin, vcrit;
+    double x0, xa, xmax, fsl0, lin, vcrit, deriv_prefac;
     long long off_k, off_y, off_s, off_fs, off_b, off_fac, off_mu;
 };
 
@@ -1314,20 +1363,19 @@ struct GroupHist {
 // are na_max doubles apart).
 struct GroupDesc {
     HTab H;
-    const double *sx, *Y, *wavenum, *init, *gbar;
+    double *sx, *Y;
+    const double *wavenum, *init, *gbar;
     double *C, *fx, *fy, *fc, *out, *ivl, *dbar, *cbar, *mubar, *ibar, *mbic, *fac;
     int *bucket, *nint, *status;
     const SpeciesDev *sp;
-    int nk, na, na_max, nfs, nsb, bsp;
-    double x0, xa, light, sinv_bw, fsl0, lin, prefac, vcrit;
+    // the stored steps and what they set: live in device memory (the device
+    // append updates them), read by every launch
+    int nk, na, na_max, nfs, nsb, bsp, integ, ic_only;
+    double x0, xa, a, light, sinv_bw, fsl0, lin, prefac, vcrit;
+    // constants of the history that the device append derives from
+    int hybrid;
+    double xmax, nu_crit_time, nufrac_low, vcrit_spec, mnu, kBtnu, deriv_prefac;
 };
-
-// The last stored log a of every history (append checks the new step against it).
-__global__ void k_group_last_loga(const GroupDesc *__restrict__ D, int nhist, double *__restrict__ out)
-{
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h < nhist) out[h] = D[h].sx[D[h].na - 1];
-}
 
 // phases (rows of the prefix table); the adjoint's (item, part) blocks are
 // split in classes of part length (LDS rows of 2 adj_plen(na) doubles):
@@ -1361,6 +1409,7 @@ __global__ __launch_bounds__(256) void k_m_fs_first(const GroupDesc *__restrict_
 {
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
+    if (!d.integ) return;
     const GroupPush push = {list, count, h};
     fs_first_wave(d.H, d.nfs, d.x0, d.xa, d.light, d.fx, d.fy, d.status,
                   (int)(((blockIdx.x - pre[h]) * blockDim.x + threadIdx.x) >> 6), push);
@@ -1385,6 +1434,7 @@ __global__ __launch_bounds__(256) void k_m_factor(const GroupDesc *__restrict__ 
 {
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
+    if (!d.integ) return;
     factor_block(d.sx, d.na, d.fac, d.fac + (size_t)fac_windows(d.na) * kFacStride, blockIdx.x - pre[h]);
 }
 
@@ -1393,6 +1443,7 @@ __global__ __launch_bounds__(256) void k_m_rows_prep(const GroupDesc *__restrict
 {
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
+    if (!d.integ) return;
     rows_prep_block(d.sx, d.Y, d.C, d.na, d.nk, (long long)d.na_max, d.bsp, d.nsb, d.sinv_bw, d.bucket, d.fac,
                     blockIdx.x - pre[h]);
 }
@@ -1403,7 +1454,7 @@ __global__ __launch_bounds__(256) void k_m_fs_spline(const GroupDesc *__restrict
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
     const long long item = (long long)(blockIdx.x - pre[h]) * 4 + (threadIdx.x >> 6);
-    if (item >= (d.nfs + kKeep - 1) / kKeep) return;  // wave-uniform
+    if (!d.integ || item >= (d.nfs + kKeep - 1) / kKeep) return;  // wave-uniform
     spline_item_wave(d.fx, d.fy, d.fc, d.nfs, 0LL, 0LL, item);
 }
 
@@ -1416,7 +1467,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_m_dnu_qag(const GroupDe
     const GroupDesc &d = D[h];
     const int wave = threadIdx.x >> 6;
     const int k = (blockIdx.x - pre[h]) * kWavesPerBlock + wave;
-    if (k >= d.nk) return;
+    if (!d.integ || k >= d.nk) return;
     QagArgs a;
     a.H = d.H;
     a.fx = d.fx;
@@ -1451,7 +1502,7 @@ __global__ __launch_bounds__(64) void k_m_ic(const GroupDesc *__restrict__ D, co
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
     const int k = (blockIdx.x - pre[h]) * 64 + threadIdx.x;
-    if (k >= d.nk) return;
+    if (d.integ || k >= d.nk) return;
     IcArgs ic;
     ic.init = d.init;
     ic.fsl0 = d.fsl0;
@@ -1479,6 +1530,7 @@ __global__ __launch_bounds__(64) void k_m_ic_adj(const GroupDesc *__restrict__ D
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
     const int k0 = (blockIdx.x - pre[h]) * 64;
+    if (d.integ) return;
     if (k0 + (int)threadIdx.x < d.nk) ic_adjoint_at(k0 + threadIdx.x, d.wavenum, d.gbar, *d.sp, group_ica(d));
     for (int r = 0; r < 64 && k0 + r < d.nk; r++) {
         double *row = d.dbar + (size_t)(k0 + r) * d.na_max;
@@ -1492,6 +1544,7 @@ __global__ __launch_bounds__(64) void k_m_dnu_adj(const GroupDesc *__restrict__ 
     HIP_DYNAMIC_SHARED(double, lds)  // adj_lds_bytes of the class's longest part
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
+    if (!d.integ) return;
     AdjArgs A;
     A.H = d.H;
     A.fx = d.fx;
@@ -1530,7 +1583,91 @@ __global__ __launch_bounds__(256) void k_m_adj_solve(const GroupDesc *__restrict
 {
     const int h = hist_of(pre, nhist, blockIdx.x);
     const GroupDesc &d = D[h];
+    if (!d.integ) return;
     adj_solve_block(d.sx, d.na, d.nk, (long long)d.na_max, d.cbar, d.dbar, d.fac, blockIdx.x - pre[h]);
+}
+
+// One more stored step per history, on the device (one wavefront per
+// history).  mode bit 0: check the new log a la[h] (capacity, the H table's
+// range, strictly after the last stored step; 3004 / 3002 as the host's
+// append) and compute fslength(log T0, la) by the wavefront QAG over the
+// history's H panels (delta_tot_table.c:394-407), into err[h] and fsl[h];
+// bit 1: where err[h] is 0, store la and the history's new delta_tot column
+// (rows[off_k + k]) at column na and update what get_delta_nu derives from a
+// and the stored steps (delta_tot_table.c:513-555: hybrid qc and tolerance,
+// the IC-only cases, the IC factor 1 + deriv_prefac fsl) in the descriptor,
+// the species record and the mass-gradient metadata.  Mode 3 does both in one
+// launch and also ORs the code into the sticky sticky[h] (the capturable
+// step); the host's append runs mode 1, reads err, then mode 2.
+__global__ __launch_bounds__(64) void k_group_step(GroupDesc *__restrict__ D, SpeciesDev *__restrict__ sp,
+                                                   GroupMeta *__restrict__ meta, const double *__restrict__ la_in,
+                                                   const double *__restrict__ rows, double *__restrict__ fsl,
+                                                   int *__restrict__ err, int *__restrict__ sticky, int mode)
+{
+    __shared__ QagWs ws;
+    const int h = blockIdx.x, lane = threadIdx.x & 63;
+    GroupDesc &d = D[h];
+    const double la = la_in[h];
+    int code = 0;
+    double f = 0.0;
+    if (mode & 1) {
+        const int na = d.na;
+        if (na >= d.na_max || !(la <= d.xmax) || !(la > d.x0)) code = KN_ERR_RANGE;
+        else if (!(la > d.sx[na - 1])) code = KN_ERR_ARG;
+        if (!code) {  // wave-uniform
+            FslIntegrand fi;
+            fi.H = d.H;
+            double r = 0.0;
+            int ni = 0;
+            if (wave_qag61(fi, d.x0, la, 1e-6, kLimit, ws, &r, &ni)) code = 2001;
+            f = d.light * r;
+        }
+        if (lane == 0) {
+            err[h] = code;
+            fsl[h] = f;
+            if ((mode & 2) && code) sticky[h] = code;
+        }
+    } else {
+        code = err[h];
+        f = fsl[h];
+    }
+    if (!(mode & 2) || code) return;
+    const int na = d.na, nk = d.nk;
+    const long long off = meta[h].off_k;
+    for (int k = lane; k < nk; k += 64) d.Y[(size_t)k * d.na_max + na] = rows[off + k];
+    const int n1 = na + 1;
+    const double a = exp(la);
+    const double partnu = (d.hybrid && a > d.nu_crit_time) ? d.nufrac_low : 0.0;
+    double qc = 0.0, relerr = 1e-6;
+    int ic_only = 0;
+    if (partnu > 0) {
+        if (1 - partnu < 1e-3) ic_only = 1;
+        qc = d.vcrit_spec * (d.mnu / d.kBtnu);
+        relerr /= (1. + 1e-5 - partnu);
+    }
+    if (!(n1 > 1 && d.mnu > 0)) ic_only = 1;
+    const int integ = !ic_only && d.mnu > 0 && n1 >= 2;
+    __syncthreads();  // every lane has read the descriptor before lane 0 rewrites it
+    if (lane < 20) sp[h].J.en[lane] = lane == 0 ? 1.0 : exp(-lane * qc);
+    if (lane == 0) {
+        d.sx[na] = la;
+        d.na = n1;
+        d.nfs = 16 * n1;
+        d.nsb = 4 * n1;
+        d.bsp = rows_prep_spline_blocks(n1, nk);
+        d.integ = integ;
+        d.ic_only = ic_only;
+        d.xa = la;
+        d.a = a;
+        d.sinv_bw = 4.0 * n1 / (la - d.x0);
+        d.fsl0 = f;
+        d.lin = 1. + d.deriv_prefac * f;
+        d.vcrit = qc > 0 ? d.vcrit_spec : 0.0;
+        sp[h].relerr = relerr;
+        sp[h].J.qc = qc;
+        meta[h].integ = integ;
+        meta[h].nparts = adj_parts(n1);
+    }
 }
 
 }  // namespace
@@ -1544,8 +1681,7 @@ struct kn_group {
     std::vector<int> order;
     double *sx = nullptr, *Y = nullptr, *C = nullptr, *dbar = nullptr, *cbar = nullptr, *fx = nullptr,
            *fy = nullptr, *fc = nullptr, *wavenum = nullptr, *init = nullptr, *out = nullptr, *gbar = nullptr,
-           *ibar = nullptr, *mbic = nullptr, *mubar = nullptr, *mbar = nullptr, *ivl = nullptr, *fac = nullptr,
-           *lastla = nullptr;
+           *ibar = nullptr, *mbic = nullptr, *mubar = nullptr, *mbar = nullptr, *ivl = nullptr, *fac = nullptr;
     int *bucket = nullptr, *nint = nullptr, *status = nullptr, *zeros = nullptr, *spec0 = nullptr, *fs_list = nullptr;
     SpeciesDev *sp = nullptr;
     GroupMeta *meta = nullptr;
@@ -1561,8 +1697,16 @@ struct kn_group {
     hipEvent_t fork = nullptr, join[kGroupStreams] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int have_graph = 0;
+    int have_graph = 0, graph_appends = 0;
     long long gen = 0, graph_gen = -1;
+    // launches sized for na_max (the device append leaves them valid)
+    int cap_plan = 0;
+    // the device changed the stored steps since the host mirror was read
+    int host_stale = 0;
+    // device append: fslength(log T0, la) and the check code per history, and
+    // the codes of capturable appends, kept until kn_group_status reports them
+    double *step_fsl = nullptr;
+    int *step_err = nullptr, *step_sticky = nullptr;
 };
 
 namespace {
@@ -1590,9 +1734,7 @@ int group_ic_factors(const GroupHist &q, double xa, double *fsl0, double *lin)
         return 2001;
     }
     *fsl0 = q.spec.light * v;
-    const double T0 = q.spec.time_transfer;
-    const double deriv_prefac = T0 * (q.spec.hubble(T0, q.spec.hubble_ctx) / q.spec.light) * T0;
-    *lin = 1. + deriv_prefac * *fsl0;
+    *lin = 1. + q.deriv_prefac * *fsl0;
     return 0;
 }
 
@@ -1707,30 +1849,49 @@ void group_plan(kn_group *g)
         d.nfs = 16 * q.na;
         d.nsb = 4 * q.na;
         d.bsp = rows_prep_spline_blocks(q.na, q.spec.nk);
+        d.integ = q.integ;
+        d.ic_only = q.spec.ic_only;
         d.x0 = q.x0;
         d.xa = q.xa;
+        d.a = q.spec.a;
         d.light = q.spec.light;
         d.sinv_bw = group_sinv_bw(q);
         d.fsl0 = q.fsl0;
         d.lin = q.lin;
         d.prefac = q.spec.delta_nu_prefac;
         d.vcrit = q.vcrit;
+        d.hybrid = q.spec.hybrid;
+        d.xmax = q.xmax;
+        d.nu_crit_time = q.spec.nu_crit_time;
+        d.nufrac_low = q.spec.nufrac_low;
+        d.vcrit_spec = q.spec.vcrit;
+        d.mnu = q.spec.mnu;
+        d.kBtnu = q.spec.kBtnu;
+        d.deriv_prefac = q.deriv_prefac;
+        // launch sizes: for the stored steps, or (capacity plan) for na_max
+        // with both the integral's and the IC-only phases, the kernels reading
+        // which one applies and how much of the grid is live from d
+        const int np = g->cap_plan ? q.spec.na_max : q.na;
+        const bool p_integ = g->cap_plan || q.integ, p_ic = g->cap_plan || !q.integ;
         long long cnt[kPhases] = {};
-        if (q.integ) {
-            cnt[kPhFs] = fs_first_blocks(d.nfs);
-            cnt[kPhFactor] = factor_blocks(q.na);
-            cnt[kPhRows] = rows_prep_blocks(d.bsp, d.nsb);
-            cnt[kPhFsSpline] = spline_blocks(d.nfs, 1);
+        if (p_integ) {
+            cnt[kPhFs] = fs_first_blocks(16 * np);
+            cnt[kPhFactor] = factor_blocks(np);
+            cnt[kPhRows] = rows_prep_blocks(rows_prep_spline_blocks(np, d.nk), 4 * np);
+            cnt[kPhFsSpline] = spline_blocks(16 * np, 1);
             cnt[kPhQag] = blocks_for(d.nk);
-            cnt[kPhSolve] = q.na > 2 ? spline_blocks(q.na, d.nk) : 0;
+            cnt[kPhSolve] = np > 2 ? spline_blocks(np, d.nk) : 0;
+            // part length: adj_plen is not monotone in the step count (1024 at
+            // 1024 knots, 513 at 1025), so the capacity plan takes the longest
+            // part any count up to na_max has, min(na_max, kAdjPartKnots)
+            const int nl = g->cap_plan ? (np < kAdjPartKnots ? np : kAdjPartKnots) : np;
             int c = 0;
-            while (c < kAdjClasses - 1 && adj_plen(q.na) > (128 << c)) c++;
-            cnt[kPhAdj0 + c] = (long long)d.nk * adj_parts(q.na) + (d.nk + 63) / 64;
-            const size_t b = adj_lds_bytes(q.na);
+            while (c < kAdjClasses - 1 && adj_plen(nl) > (128 << c)) c++;
+            cnt[kPhAdj0 + c] = (long long)d.nk * adj_parts(np) + (d.nk + 63) / 64;
+            const size_t b = adj_lds_bytes(nl);
             lds[c] = b > lds[c] ? b : lds[c];
-        } else {
-            cnt[kPhIc] = (d.nk + 63) / 64;
         }
+        if (p_ic) cnt[kPhIc] = (d.nk + 63) / 64;
         for (int p = 0; p < kPhases; p++) {
             acc[p] += cnt[p];
             if (acc[p] > 0x7fffffffLL) kn_terminate(KN_ERR_RANGE, "kn_group: %lld blocks in one phase\n", acc[p]);
@@ -1739,9 +1900,11 @@ void group_plan(kn_group *g)
     }
     for (int p = 0; p < kPhases; p++) g->tot[p] = (int)acc[p];
     g->qag_lds = 0;
-    for (int i = 0; i < n; i++)
-        if (g->h[i].integ && g->h[i].na <= kStageMax && sizeof(double) * g->h[i].na > g->qag_lds)
-            g->qag_lds = sizeof(double) * g->h[i].na;
+    for (int i = 0; i < n; i++) {
+        const int np = g->cap_plan ? (g->h[i].spec.na_max < kStageMax ? g->h[i].spec.na_max : kStageMax) : g->h[i].na;
+        if ((g->cap_plan || g->h[i].integ) && np <= kStageMax && sizeof(double) * np > g->qag_lds)
+            g->qag_lds = sizeof(double) * np;
+    }
     for (int c = 0; c < kAdjClasses; c++) g->adj_lds[c] = lds[c];
     KN_CHECK(hipMemcpy(g->desc, g->desc_h.data(), sizeof(GroupDesc) * n, hipMemcpyHostToDevice));
     KN_CHECK(hipMemcpy(g->pre, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice));
@@ -1934,11 +2097,52 @@ void group_launch(kn_group *g, hipStream_t origin, bool forward, bool adjoint)
     }
 }
 
+// The host mirror of what the device append changes (stored steps, upper
+// limit, derived tolerance / qc / IC-only state, IC factors), read back after
+// device appends; every host call that uses them reads it first.
+void group_pull(kn_group *g)
+{
+    if (!g->host_stale) return;
+    const int n = g->nhist;
+    KN_CHECK(hipDeviceSynchronize());
+    KN_CHECK(hipMemcpy(g->desc_h.data(), g->desc, sizeof(GroupDesc) * n, hipMemcpyDeviceToHost));
+    std::vector<SpeciesDev> sd(n);
+    KN_CHECK(hipMemcpy(sd.data(), g->sp, sizeof(SpeciesDev) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+        GroupHist &q = g->h[i];
+        const GroupDesc &d = g->desc_h[i];
+        q.na = d.na;
+        q.xa = d.xa;
+        q.spec.a = d.a;
+        q.fsl0 = d.fsl0;
+        q.lin = d.lin;
+        q.vcrit = d.vcrit;
+        q.integ = d.integ;
+        q.spec.ic_only = d.ic_only;
+        q.spec.qc = sd[i].J.qc;
+        q.spec.relerr = sd[i].relerr;
+    }
+    g->host_stale = 0;
+}
+
 int group_status(kn_group *g)
 {
     if (g->cap_err) {
         kn_set_last_error("kn_group: capacity (na_max or the H table range) exceeded");
         return KN_ERR_RANGE;
+    }
+    if (g->cap_plan) {  // codes of device appends (sticky until reported here)
+        std::vector<int> se(g->nhist);
+        KN_CHECK(hipMemcpy(se.data(), g->step_sticky, sizeof(int) * se.size(), hipMemcpyDeviceToHost));
+        for (int i = 0; i < g->nhist; i++)
+            if (se[i]) {
+                KN_CHECK(hipMemset(g->step_sticky, 0, sizeof(int) * se.size()));
+                kn_set_last_error("kn_group: history %d: device append rejected (%s)", i,
+                                  se[i] == KN_ERR_RANGE ? "capacity or H table range"
+                                  : se[i] == KN_ERR_ARG ? "log a does not follow the last stored step"
+                                                        : "QAG failed in fslength");
+                return se[i];
+            }
     }
     std::vector<int> st(2 * (size_t)g->nhist);
     KN_CHECK(hipMemcpy(st.data(), g->status, sizeof(int) * st.size(), hipMemcpyDeviceToHost));
@@ -1975,7 +2179,7 @@ extern "C" void kn_group_destroy(kn_group *g)
     void *ptrs[] = {g->sx,   g->Y,    g->C,      g->dbar, g->cbar,   g->fx,     g->fy,    g->fc,
                     g->wavenum, g->init, g->out,  g->gbar,   g->ibar, g->mbic,   g->mubar,  g->mbar,  g->ivl,
                     g->bucket, g->nint, g->status, g->zeros, g->spec0, g->sp,    g->fs_list, g->meta, g->desc,
-                    g->pre,  g->mlist, g->fac,  g->lastla};
+                    g->pre,  g->mlist, g->fac,  g->step_fsl, g->step_err, g->step_sticky};
     for (void *p : ptrs)
         if (p) KN_CHECK(hipFree(p));
     for (double *p : g->hcoef) KN_CHECK(hipFree(p));
@@ -2023,6 +2227,10 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
         g->s_tot += q.spec.na_max;
         g->fs_tot += 16LL * q.spec.na_max;
         g->b_tot += 4LL * q.spec.na_max + 1;
+        {
+            const double T0 = q.spec.time_transfer;  // delta_tot_table.c:522-535
+            q.deriv_prefac = T0 * (q.spec.hubble(T0, q.spec.hubble_ctx) / q.spec.light) * T0;
+        }
         const int rc = group_refresh_ic(q);
         if (rc) {
             kn_group_destroy(g);
@@ -2041,14 +2249,11 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
             std::vector<double> coef;
             double (*hub)(double, void *) = q.spec.hubble;
             void *ctx = q.spec.hubble_ctx;
-            h_table(q.x0, kGroupHWidth, npanel, [hub, ctx](double a) { return hub(a, ctx); }, coef);
+            HTab H;
+            h_table(q.x0, kGroupHWidth, npanel, [hub, ctx](double a) { return hub(a, ctx); }, coef, H);
             double *d = group_alloc<double>(coef.size());
             KN_CHECK(hipMemcpy(d, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice));
-            HTab H;
             H.coef = d;
-            H.x0 = q.x0;
-            H.inv_w = 1.0 / kGroupHWidth;
-            H.npanel = npanel;
             g->hcoef.push_back(d);
             g->htabs.push_back(H);
             found = (int)g->htabs.size() - 1;
@@ -2063,7 +2268,6 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
         *p = group_alloc<double>(nk);
     g->mubar = group_alloc<double>(g->mu_tot);  // [k][part] per history
     g->mbar = group_alloc<double>(nhist);
-    g->lastla = group_alloc<double>(nhist);
     g->fac = group_alloc<double>(g->fac_tot);
     g->ivl = group_alloc<double>(nk * kLimit * 2);
     g->bucket = group_alloc<int>(g->b_tot);
@@ -2074,6 +2278,10 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
     g->fs_list = group_alloc<int>(g->fs_tot);
     g->mlist = group_alloc<long long>(g->fs_tot);
     g->spec0 = group_alloc<int>(1);
+    g->step_fsl = group_alloc<double>(nhist);
+    g->step_err = group_alloc<int>(nhist);
+    g->step_sticky = group_alloc<int>(nhist);
+    KN_CHECK(hipMemset(g->step_sticky, 0, sizeof(int) * nhist));
     g->desc = group_alloc<GroupDesc>(nhist);
     g->pre = group_alloc<int>((size_t)kPhases * (nhist + 1));
     KN_CHECK(hipMemset(g->zeros, 0, sizeof(int) * (3 * (size_t)nhist + 1)));
@@ -2145,87 +2353,123 @@ extern "C" int kn_group_adjoint(kn_group *g, void *stream)
 }
 
 // One more stored step per history: d_loga[nhist] and d_rows[items] in device
-// memory.  The host reads the new log a values (they set each history's upper
-// limit and its launch sizes) and each history's last stored log a, and
-// checks every history before anything changes: capacity, the H table range
-// and a strictly increasing log a (the reference's spline would reject equal
-// or decreasing knots, GSL_EINVAL -> 2001).  The new initial-condition factors
-// are computed for all histories first; only then are scalefact[na] and
-// column na of delta_tot copied on the device and the stored-step count,
-// upper limit and the derived tolerance / hybrid / IC-only state updated.
+// memory.  The device checks every history first (capacity, the H table
+// range, a strictly increasing log a: the reference's spline would reject
+// equal or decreasing knots, GSL_EINVAL -> 2001) and computes the new
+// fslength(log T0, log a) for the initial-condition factor; the host reads the
+// codes, and only when all are 0 does the device store the new column and
+// update each history's derived state (k_group_step).  Launch sizes follow
+// the new stored steps (unless the group is on its capacity plan).
 extern "C" int kn_group_append(kn_group *g, const double *d_loga, const double *d_rows, void *stream)
 {
+    if (!g || !d_loga || !d_rows) return KN_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     const int n = g->nhist;
-    std::vector<double> la(n), last(n), fsl0(n), lin(n);
-    KN_CHECK(hipMemcpyAsync(la.data(), d_loga, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    hipLaunchKernelGGL(k_group_last_loga, dim3((n + 63) / 64), dim3(64), 0, st, g->desc, n, g->lastla);
+    group_pull(g);
+    hipLaunchKernelGGL(k_group_step, dim3(n), dim3(64), 0, st, g->desc, g->sp, g->meta, d_loga, d_rows, g->step_fsl,
+                       g->step_err, g->step_sticky, 1);
     KN_CHECK(hipGetLastError());
-    KN_CHECK(hipMemcpyAsync(last.data(), g->lastla, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    std::vector<int> err(n);
+    KN_CHECK(hipMemcpyAsync(err.data(), g->step_err, sizeof(int) * n, hipMemcpyDeviceToHost, st));
     KN_CHECK(hipStreamSynchronize(st));
-    for (int i = 0; i < n; i++) {
-        const GroupHist &q = g->h[i];
-        if (q.na >= q.spec.na_max || !(la[i] <= q.xmax) || !(la[i] > q.x0)) {
-            g->cap_err = 1;
-            kn_set_last_error("kn_group_append: history %d: capacity (na_max %d) or H table range exceeded", i,
-                              q.spec.na_max);
-            return KN_ERR_RANGE;
+    for (int i = 0; i < n; i++)
+        if (err[i]) {
+            if (err[i] == KN_ERR_RANGE) {
+                g->cap_err = 1;
+                kn_set_last_error("kn_group_append: history %d: capacity (na_max %d) or H table range exceeded", i,
+                                  g->h[i].spec.na_max);
+            } else if (err[i] == KN_ERR_ARG) {
+                kn_set_last_error("kn_group_append: history %d: log a does not follow the last stored step", i);
+            } else {
+                kn_set_last_error("kn_group_append: history %d: QAG failed in fslength", i);
+            }
+            return err[i];
         }
-        if (!(la[i] > last[i])) {
-            kn_set_last_error("kn_group_append: history %d: log a %.17g does not follow the last stored %.17g", i,
-                              la[i], last[i]);
-            return KN_ERR_ARG;
-        }
-    }
-    for (int i = 0; i < n; i++) {
-        const int rc = group_ic_factors(g->h[i], la[i], &fsl0[i], &lin[i]);
-        if (rc) return rc;
-    }
-    std::vector<SpeciesDev> sd(n);
-    for (int i = 0; i < n; i++) {
-        GroupHist &q = g->h[i];
-        KN_CHECK(hipMemcpyAsync(g->sx + q.off_s + q.na, d_loga + i, sizeof(double), hipMemcpyDeviceToDevice, st));
-        KN_CHECK(hipMemcpy2DAsync(g->Y + q.off_y + q.na, sizeof(double) * q.spec.na_max, d_rows + q.off_k,
-                                  sizeof(double), sizeof(double), q.spec.nk, hipMemcpyDeviceToDevice, st));
-        q.na++;
-        q.xa = la[i];
-        q.spec.a = exp(la[i]);
-        q.fsl0 = fsl0[i];
-        q.lin = lin[i];
-        group_derive(q);
-        sd[i] = species_dev(q.spec);
-    }
-    KN_CHECK(hipMemcpyAsync(g->sp, sd.data(), sizeof(SpeciesDev) * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_group_step, dim3(n), dim3(64), 0, st, g->desc, g->sp, g->meta, d_loga, d_rows, g->step_fsl,
+                       g->step_err, g->step_sticky, 2);
+    KN_CHECK(hipGetLastError());
     KN_CHECK(hipStreamSynchronize(st));
+    g->host_stale = 1;
+    group_pull(g);
+    if (!g->cap_plan) {
+        group_plan(g);
+        group_schedule(g);
+        g->gen++;
+    }
+    return 0;
+}
+
+// The same step with no host synchronisation, for groups on their capacity
+// plan (so launch sizes need not follow it): check, fslength and store in
+// one launch per history; a history whose check fails is left as it was and
+// its code is kept for kn_group_status.
+extern "C" int kn_group_append_async(kn_group *g, const double *d_loga, const double *d_rows, void *stream)
+{
+    if (!g || !d_loga || !d_rows || !g->cap_plan) {
+        kn_set_last_error("kn_group_append_async: needs device arrays and a group on its capacity plan");
+        return KN_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_group_step, dim3(g->nhist), dim3(64), 0, (hipStream_t)stream, g->desc, g->sp, g->meta, d_loga,
+                       d_rows, g->step_fsl, g->step_err, g->step_sticky, 3);
+    KN_CHECK(hipGetLastError());
+    g->host_stale = 1;
+    return 0;
+}
+
+// Launch sizes for na_max (on = 1) or for the stored steps (0, the default).
+// On the capacity plan every history's grids cover na_max stored steps and
+// both the integral's and the IC-only phases, and the kernels read from
+// device memory how much applies, so appends (kn_group_append_async, or an
+// append captured in the graph) leave the launches valid; it runs merged
+// launches only.
+extern "C" int kn_group_set_capacity_plan(kn_group *g, int on)
+{
+    if (!g || (on != 0 && on != 1)) return KN_ERR_ARG;
+    group_pull(g);
+    g->cap_plan = on;
+    if (on) g->nstream = 0;
     group_plan(g);
     group_schedule(g);
     g->gen++;
     return 0;
 }
 
-// Capture solve [+ adjoint] on stream (not the null stream) into one graph.
-// Launch sizes follow the stored steps, so a graph is replayable until the
-// next append (new masses and new delta_tot/wavenum/init values in the
-// device arrays are read at replay).  Append is host-driven and cannot be
-// captured: d_loga and d_rows must be NULL.
+// Capture [append +] solve [+ adjoint] on stream (not the null stream) into
+// one graph.  Without append (d_loga, d_rows NULL) launch sizes follow the
+// stored steps, so the graph is replayable until the next append (new
+// masses and new delta_tot/wavenum/init values in the device arrays are read
+// at replay).  With d_loga and d_rows the group goes on its capacity plan and
+// every replay first appends the step found in those arrays at replay time
+// (kn_group_append_async's launch), so one graph serves every step until a
+// history is full.
 extern "C" int kn_group_graph_capture(kn_group *g, int with_adjoint, const double *d_loga, const double *d_rows,
                                       void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
-    if (!st || d_loga || d_rows) {
-        kn_set_last_error("kn_group_graph_capture: needs a non-null stream; append is not capturable");
+    if (!g || !st || (!d_loga) != (!d_rows)) {
+        kn_set_last_error("kn_group_graph_capture: needs a non-null stream, and d_loga and d_rows together");
         return KN_ERR_ARG;
     }
+    const bool append = d_loga != nullptr;
+    if (append && !g->cap_plan) {
+        const int rc = kn_group_set_capacity_plan(g, 1);
+        if (rc) return rc;
+    }
+    group_pull(g);
     if (g->have_graph) {
         KN_CHECK(hipGraphExecDestroy(g->exec));
         KN_CHECK(hipGraphDestroy(g->graph));
         g->have_graph = 0;
     }
     KN_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    if (append)
+        hipLaunchKernelGGL(k_group_step, dim3(g->nhist), dim3(64), 0, st, g->desc, g->sp, g->meta, d_loga, d_rows,
+                           g->step_fsl, g->step_err, g->step_sticky, 3);
     group_launch(g, st, true, with_adjoint != 0);
     KN_CHECK(hipStreamEndCapture(st, &g->graph));
     KN_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
     g->have_graph = 1;
+    g->graph_appends = append;
     g->graph_gen = g->gen;
     return 0;
 }
@@ -2237,6 +2481,7 @@ extern "C" int kn_group_graph_launch(kn_group *g, void *stream)
         return KN_ERR_ARG;
     }
     KN_CHECK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    if (g->graph_appends) g->host_stale = 1;
     return 0;
 }
 
@@ -2250,6 +2495,7 @@ extern "C" int kn_group_status(kn_group *g) { return group_status(g); }
 extern "C" int kn_group_subdivision(kn_group *g, int h, double *ivl, int *nint)
 {
     if (!g || h < 0 || h >= g->nhist || !ivl || !nint) return KN_ERR_ARG;
+    group_pull(g);
     const GroupHist &q = g->h[h];
     const int nk = q.spec.nk;
     if (!q.integ) {
@@ -2270,6 +2516,7 @@ extern "C" int kn_group_subdivision(kn_group *g, int h, double *ivl, int *nint)
 extern "C" int kn_group_set_mass(kn_group *g, int h, double mnu)
 {
     if (h < 0 || h >= g->nhist || !(mnu > 0)) return KN_ERR_ARG;
+    group_pull(g);
     GroupHist &q = g->h[h];
     const int integ0 = q.integ;
     const double vcrit0 = q.vcrit;
@@ -2277,8 +2524,8 @@ extern "C" int kn_group_set_mass(kn_group *g, int h, double mnu)
     group_derive(q);
     const SpeciesDev sd = species_dev(q.spec);
     KN_CHECK(hipMemcpy(g->sp + h, &sd, sizeof sd, hipMemcpyHostToDevice));
-    if (q.integ != integ0 || q.vcrit != vcrit0) {
-        group_plan(g);
+    group_plan(g);  // the descriptors carry the mass for the device append
+    if (!g->cap_plan && (q.integ != integ0 || q.vcrit != vcrit0)) {
         group_schedule(g);
         g->gen++;
     }
@@ -2290,6 +2537,8 @@ extern "C" int kn_group_set_mass(kn_group *g, int h, double mnu)
 extern "C" int kn_group_autotune(kn_group *g, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
+    group_pull(g);
+    if (g->cap_plan) return group_status(g);  // merged launches only
     const int cand[] = {0, 1, 2, 4, 8};
     float best = 1e30f;
     int bs = g->nstream;
@@ -2323,7 +2572,8 @@ extern "C" int kn_group_streams(const kn_group *g) { return g->nstream; }
 
 extern "C" int kn_group_set_streams(kn_group *g, int nstream)
 {
-    if (!g || nstream < 0 || nstream > kGroupStreams) return KN_ERR_ARG;
+    if (!g || nstream < 0 || nstream > kGroupStreams || (g->cap_plan && nstream)) return KN_ERR_ARG;
+    group_pull(g);
     g->nstream = nstream;
     if (nstream) group_schedule(g);
     g->gen++;  // a captured graph is stale

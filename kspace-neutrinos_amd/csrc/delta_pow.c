@@ -30,7 +30,7 @@ double get_dnudcdm_powerspec(_delta_pow *d_pow, double kk)
     double v;
     if (kk < lo && kk > lo - log(2)) kk = lo;
     if (kk < lo - log(2)) {
-        fprintf(stderr, "trying to extract a k= %g < min stored = %g \n", kk, lo);
+        fprintf(stderr, "get_dnudcdm_powerspec: log k = %g below the table (from %g); clamped\n", kk, lo);
         kk = lo;
     }
     if (kk > d_pow->logkk[d_pow->nbins - 1]) kk = d_pow->logkk[d_pow->nbins - 1];

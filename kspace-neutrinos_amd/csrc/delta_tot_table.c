@@ -140,7 +140,7 @@ void get_delta_nu(const _delta_tot_table *const d, const double a, const double 
     kn_species s;
     kn_species_for(d, a, mnu, &s);
     if (d->debug)
-        kn_message(0, "Start get_delta_nu: a=%g Na =%d wavenum[0]=%g delta_tot[0]=%g m_nu=%g\n", a, d->ia,
+        kn_message(0, "get_delta_nu at a=%g over %d stored steps (k[0]=%g, last delta_tot[0]=%g, m_nu=%g)\n", a, d->ia,
                    wavenum[0], d->delta_tot[0][d->ia - 1], mnu);
     kn_dnu_solve(d, a, wavenum, 1, &s, delta_nu_curr);
 }
@@ -198,11 +198,11 @@ void delta_tot_init(_delta_tot_table *const d, const int nk_in, const double wav
         }
     }
     if (Time > d->TimeTransfer + 0.01 && d->ia == 0)
-        kn_terminate(2023, "Did not read delta_tot from resume file, but we probably should have\n");
+        kn_terminate(2023, "resume: the delta_tot table is empty although a resume state was expected\n");
     if (Time < d->TimeTransfer - 1e-5)
-        kn_terminate(2024, "Trying to compute delta_tot at a=%g < Transfer time of %g\n", Time, d->TimeTransfer);
+        kn_terminate(2024, "delta_tot_init at a=%g, before the transfer-function time %g\n", Time, d->TimeTransfer);
     if (nk_in > d->nk_allocated)
-        kn_terminate(2011, "input power of %d is longer than memory of %d\n", nk_in, d->nk_allocated);
+        kn_terminate(2011, "%d k bins given, the table holds only %d\n", nk_in, d->nk_allocated);
     d->nk = nk_in;
     {
         const double OmegaNua3 = get_omega_nu_nopart(d->omnu, d->TimeTransfer) * pow(d->TimeTransfer, 3);
@@ -210,7 +210,7 @@ void delta_tot_init(_delta_tot_table *const d, const int nk_in, const double wav
         c = kn_malloc("transfer_spline", sizeof(double) * (size_t)t->NPowerTable);
         kn_cspline_init(t->logk, t->T_nu, t->NPowerTable, c);
         if (log(wavenum[d->nk - 1]) > t->logk[t->NPowerTable - 1])
-            kn_terminate(2, "Want k = %g but maximum in CAMB table is %g\n", wavenum[d->nk - 1],
+            kn_terminate(2, "k = %g lies beyond the transfer table, which ends at k = %g\n", wavenum[d->nk - 1],
                          exp(t->logk[t->NPowerTable - 1]));
         for (k = 0; k < d->nk; k++) {
             int edom = 0;
@@ -240,9 +240,9 @@ void get_delta_nu_update(_delta_tot_table *const d, const double a, const int nk
 {
     int k;
     if (!d->delta_tot_init_done) delta_tot_init(d, nk_in, keff, delta_cdm_curr, t, a);
-    if (!d->delta_tot_init_done) kn_terminate(2001, "Should have called delta_tot_init first\n");
-    if (nk_in != d->nk) kn_terminate(2002, "Number of kbins %d != stored delta_tot %d\n", nk_in, d->nk);
-    if (d->nk < 2) kn_terminate(2003, "Number of kbins is unreasonably small: %d\n", d->nk);
+    if (!d->delta_tot_init_done) kn_terminate(2001, "delta_tot_init has not run on this table\n");
+    if (nk_in != d->nk) kn_terminate(2002, "%d k bins given, the table stores %d\n", nk_in, d->nk);
+    if (d->nk < 2) kn_terminate(2003, "only %d k bins: too few to interpolate\n", d->nk);
     if (log(a) - d->scalefact[d->ia - 1] < KN_FLOAT_ACC) {
         for (k = 0; k < d->nk; k++) delta_nu_curr[k] = d->delta_nu_last[k];
         return;
@@ -257,7 +257,7 @@ void get_delta_nu_update(_delta_tot_table *const d, const double a, const int nk
         d->ia--;
     for (k = 0; k < d->nk; k++) {
         if (isnan(delta_nu_curr[k]))
-            kn_terminate(2004, "delta_nu_curr=%g i=%d delta_cdm_curr=%g kk=%g\n", delta_nu_curr[k], k,
+            kn_terminate(2004, "NaN in the neutrino update: delta_nu=%g at k bin %d (delta_cdm=%g, k=%g)\n", delta_nu_curr[k], k,
                          delta_cdm_curr[k], keff[k]);
         if (delta_nu_curr[k] < 0) delta_nu_curr[k] = 0;
     }
@@ -285,7 +285,7 @@ void read_all_nu_state(_delta_tot_table *const d, char *savefile)
             if (fscanf(fd, "%lg ", &d->delta_tot[k][row]) == 1) continue;
             /* a short first line sets nk; a short later line is an error */
             if (row != 0)
-                kn_terminate(2006, "Expected %d k values, got %d for delta_tot in %s; a=%g\n", d->nk, k, path,
+                kn_terminate(2006, "%d k values expected, %d read from %s at a=%g\n", d->nk, k, path,
                              exp(d->scalefact[row]));
             d->nk = k;
             break;
@@ -296,7 +296,7 @@ void read_all_nu_state(_delta_tot_table *const d, char *savefile)
         kn_terminate(2007, "%s starts with a=%g, transfer function is at a=%g\n", path, exp(d->scalefact[0]),
                      d->TimeTransfer);
     if (row > 0) d->ia = row;
-    if (d->debug) kn_message(1, "Read %d stored power spectra from %s\n", row, path);
+    if (d->debug) kn_message(1, "%d stored steps read from %s\n", row, path);
     fclose(fd);
 }
 
@@ -305,7 +305,7 @@ void save_delta_tot(const _delta_tot_table *const d, const int iia, char *savefi
     const char *path = state_path(savefile);
     FILE *fd = fopen(path, "a");
     int k;
-    if (!fd) kn_terminate(2012, "Could not open %s for writing!\n", path);
+    if (!fd) kn_terminate(2012, "cannot write %s\n", path);
     fprintf(fd, "# %le ", d->scalefact[iia]);
     for (k = 0; k < d->nk; k++) fprintf(fd, "%le ", d->delta_tot[k][iia]);
     fprintf(fd, "\n");

@@ -25,15 +25,17 @@ namespace kn_dnu {
 
 constexpr int kLimit = 200;   // QAG subinterval limit (GSL_VAL)
 constexpr int kHnc = 10;      // Chebyshev coefficients per function per H panel
-// One panel block: the coefficients of 1/(a^2 H), then of 1/(a H), then the
-// block's centre and scale (t = (x - centre) * scale on [-1, 1]), the split
-// point (+inf for a panel in one piece) and the index of the block that holds
-// the piece right of the split.
-constexpr int kHMid = 2 * kHnc, kHScale = kHMid + 1, kHSplit = kHMid + 2, kHRight = kHMid + 3;
-constexpr int kHStride = 2 * kHnc + 4;
+constexpr int kHStride = 2 * kHnc;  // one panel: 1/(a^2 H) coefficients, then 1/(a H)
 // A panel whose fit misses H by more than this (relative, at test points
-// between the interpolation nodes) holds a jump or a kink of H: it is split there.
+// between the interpolation nodes) holds a jump or a kink of H: it is split
+// there into two fits, kept out of line (at most kHSplitMax per table).
 constexpr double kHSplitTol = 1e-12;
+constexpr int kHSplitMax = 4;
+// One piece of a split panel: centre and scale of its own t = (x - centre) *
+// scale, then both functions' coefficients.
+constexpr int kHPiece = 2 + kHStride;
+// One split record: the split point, the piece left of it, the piece right of it.
+constexpr int kHSplitStride = 1 + 2 * kHPiece;
 
 __constant__ double c_xgk[31];
 __constant__ double c_wgk[31];
@@ -46,18 +48,20 @@ __constant__ double c_wg[15];
 // piecewise smooth (the host's neutrino density switches from its spline
 // table to the non-relativistic series at a = 100 kT/m, omega_nu_single.c:
 // 171-183, where H jumps by ~1e-8 relative): a panel that holds such a point
-// is split there into two fits, the right one in a block after the npanel
-// regular ones.
+// is split there into two fits, stored after the npanel regular panels; the
+// table lists the split panels' indices, so the regular evaluation loads
+// nothing more than before.
 struct HTab {
-    const double *coef;  // npanel blocks of kHStride, then the right pieces of split panels
+    const double *coef;  // npanel * kHStride, then nsplit * kHSplitStride
     double x0, inv_w;
-    int npanel;
+    int npanel, nsplit;
+    int split_p[kHSplitMax];
 };
 
 // Host: both functions' interpolants at the kHnc Chebyshev points of
-// [lo, hi] into one block (unsplit); hub(a) = H(a).
+// [lo, hi]; hub(a) = H(a).
 template <typename Hub>
-inline void h_fit(double lo, double hi, Hub hub, double *blk)
+inline void h_fit(double lo, double hi, Hub hub, double *coef)
 {
     const double half = 0.5 * (hi - lo), mid = lo + half;
     double g2[kHnc], g1[kHnc];
@@ -74,21 +78,15 @@ inline void h_fit(double lo, double hi, Hub hub, double *blk)
             s2 += g2[j] * c;
             s1 += g1[j] * c;
         }
-        blk[k] = (k == 0 ? 1.0 : 2.0) * s2 / kHnc;
-        blk[kHnc + k] = (k == 0 ? 1.0 : 2.0) * s1 / kHnc;
+        coef[k] = (k == 0 ? 1.0 : 2.0) * s2 / kHnc;
+        coef[kHnc + k] = (k == 0 ? 1.0 : 2.0) * s1 / kHnc;
     }
-    blk[kHMid] = mid;
-    blk[kHScale] = half > 0.0 ? 1.0 / half : 0.0;
-    blk[kHSplit] = INFINITY;
-    blk[kHRight] = 0.0;
 }
 
-// Clenshaw sum of one function of one block (which = 0: 1/(a^2 H), 1: 1/(a H)).
+// Clenshaw sum of one function's coefficients at t in [-1, 1].
 template <typename P>
-__host__ __device__ __forceinline__ double h_block(P blk, int which, double x)
+__host__ __device__ __forceinline__ double h_clenshaw(P c, double t)
 {
-    const double t = (x - blk[kHMid]) * blk[kHScale];
-    P c = blk + which * kHnc;
     double b1 = 0.0, b2 = 0.0;
 #pragma unroll
     for (int k = kHnc - 1; k >= 1; k--) {
@@ -99,34 +97,53 @@ __host__ __device__ __forceinline__ double h_block(P blk, int which, double x)
     return fma(t, b1, c[0] - b2);
 }
 
-// Host: the largest relative miss of a block's two fits on [lo, hi], at the
+// A split panel's value: the piece left or right of the split point.
+template <typename P>
+__host__ __device__ __forceinline__ double h_split_eval(P rec, double x, int which)
+{
+    P piece = rec + 1 + (x >= rec[0] ? kHPiece : 0);
+    return h_clenshaw(piece + 2 + which * kHnc, (x - piece[0]) * piece[1]);
+}
+
+// Host: the largest relative miss of the fits `coef` of [lo, hi], at the
 // ends and half-way between adjacent interpolation nodes.
 template <typename Hub>
-inline double h_fit_err(const double *blk, double lo, double hi, Hub hub)
+inline double h_fit_err(const double *coef, double lo, double hi, Hub hub)
 {
     const double half = 0.5 * (hi - lo), mid = lo + half;
     double worst = 0.0;
     for (int j = -1; j < kHnc; j++) {
         const double t = j < 0 ? -1.0 : (j == kHnc - 1 ? 1.0 : cos(M_PI * (j + 1) / kHnc));
-        const double x = j < 0 ? lo : (j == kHnc - 1 ? hi : mid + half * t);
+        const double x = mid + half * t;
         const double h = hub(exp(x));
-        const double e2 = fabs(h_block(blk, 0, x) / (exp(-2.0 * x) / h) - 1.0);
-        const double e1 = fabs(h_block(blk, 1, x) / (exp(-x) / h) - 1.0);
+        const double e2 = fabs(h_clenshaw(coef, t) / (exp(-2.0 * x) / h) - 1.0);
+        const double e1 = fabs(h_clenshaw(coef + kHnc, t) / (exp(-x) / h) - 1.0);
         worst = fmax(worst, fmax(e2, e1));
     }
     return worst;
 }
 
-// Host: the panel [lo, hi] into blk.  Where one fit misses H by more than
-// kHSplitTol the panel is split at the point that bisection finds (the
-// half whose own fit misses more is kept, until both halves fit or the
-// bracket is a few units in the last place wide): blk then holds [lo, xs]
-// and right[] the block of [xs, hi].  Returns whether it split.
+// Host: one piece [lo, hi] of a split panel into piece[kHPiece].
 template <typename Hub>
-inline bool h_panel(double lo, double hi, Hub hub, double *blk, double *right)
+inline void h_piece(double lo, double hi, Hub hub, double *piece)
 {
-    h_fit(lo, hi, hub, blk);
-    if (h_fit_err(blk, lo, hi, hub) <= kHSplitTol) return false;
+    const double half = 0.5 * (hi - lo);
+    piece[0] = lo + half;
+    piece[1] = half > 0.0 ? 1.0 / half : 0.0;
+    h_fit(lo, hi, hub, piece + 2);
+}
+
+// Host: the panel [lo, hi] into coef[kHStride].  Where the fit misses H by
+// more than kHSplitTol the panel is also split, into rec[kHSplitStride], at
+// the point that bisection finds (the half whose own fit misses more is
+// kept, until both halves fit or the bracket is a few units in the last
+// place wide).  Returns the fit's miss (above kHSplitTol: rec was written).
+template <typename Hub>
+inline double h_panel(double lo, double hi, Hub hub, double *coef, double *rec)
+{
+    h_fit(lo, hi, hub, coef);
+    const double err = h_fit_err(coef, lo, hi, hub);
+    if (err <= kHSplitTol) return err;
     double l = lo, r = hi, tmp[kHStride];
     for (;;) {
         const double m = l + 0.5 * (r - l);
@@ -143,55 +160,88 @@ inline bool h_panel(double lo, double hi, Hub hub, double *blk, double *right)
         else l = m;
     }
     const double xs = l + 0.5 * (r - l);
-    h_fit(lo, xs, hub, blk);
-    h_fit(xs, hi, hub, right);
-    blk[kHSplit] = xs;
-    return true;
+    rec[0] = xs;
+    h_piece(lo, xs, hub, rec + 1);
+    h_piece(xs, hi, hub, rec + 1 + kHPiece);
+    return err;
 }
 
-// Host: the whole table of npanel panels of width w from x0 (split panels'
-// right pieces appended, blk[kHRight] pointing at them).
+// Host: split records of one table, at most kHSplitMax, the panels whose
+// fit misses most kept (the others keep their one-piece fit).
+struct HSplits {
+    int n = 0;
+    int p[kHSplitMax];
+    double err[kHSplitMax];
+    double rec[kHSplitMax][kHSplitStride];
+    // offers panel q's record; returns whether it was kept
+    bool offer(int q, double e, const double *r)
+    {
+        int slot = n;
+        if (n == kHSplitMax) {
+            slot = 0;
+            for (int i = 1; i < n; i++)
+                if (err[i] < err[slot]) slot = i;
+            if (err[slot] >= e) return false;
+        } else {
+            n++;
+        }
+        p[slot] = q;
+        err[slot] = e;
+        for (int i = 0; i < kHSplitStride; i++) rec[slot][i] = r[i];
+        return true;
+    }
+};
+
+// Host: the whole table of npanel panels of width w from x0, then the split
+// records; H.split_p and H.nsplit from them.
 template <typename Hub>
-inline void h_table(double x0, double w, int npanel, Hub hub, std::vector<double> &coef)
+inline void h_table(double x0, double w, int npanel, Hub hub, std::vector<double> &coef, HTab &H)
 {
     coef.assign((size_t)npanel * kHStride, 0.0);
-    double right[kHStride];
+    HSplits sp;
+    double rec[kHSplitStride];
     for (int p = 0; p < npanel; p++) {
-        double *blk = &coef[(size_t)p * kHStride];
-        if (h_panel(x0 + p * w, x0 + (p + 1) * w, hub, blk, right)) {
-            const size_t idx = coef.size() / kHStride;
-            coef.insert(coef.end(), right, right + kHStride);
-            coef[(size_t)p * kHStride + kHRight] = (double)idx;
-        }
+        const double e = h_panel(x0 + p * w, x0 + (p + 1) * w, hub, &coef[(size_t)p * kHStride], rec);
+        if (e > kHSplitTol) sp.offer(p, e, rec);
     }
+    for (int i = 0; i < sp.n; i++) coef.insert(coef.end(), sp.rec[i], sp.rec[i] + kHSplitStride);
+    H.x0 = x0;
+    H.inv_w = 1.0 / w;
+    H.npanel = npanel;
+    H.nsplit = sp.n;
+    for (int i = 0; i < kHSplitMax; i++) H.split_p[i] = i < sp.n ? sp.p[i] : -1;
 }
 
 // Evaluation of one function (which = 0: 1/(a^2 H), 1: 1/(a H)) at x.
-__host__ __device__ __forceinline__ double h_cheb(const double *coef, double x0, double inv_w, int npanel, double x,
-                                                  int which)
+__host__ __device__ __forceinline__ double h_cheb(const double *coef, double x0, double inv_w, int npanel, int nsplit,
+                                                  const int *split_p, double x, int which)
 {
     const double u = (x - x0) * inv_w;
     int p = (int)floor(u);
     p = p < 0 ? 0 : (p >= npanel ? npanel - 1 : p);
-    const double *c = coef + (size_t)p * kHStride;
-    if (x >= c[kHSplit]) c = coef + (size_t)c[kHRight] * kHStride;
-    return h_block(c, which, x);
+    for (int s = 0; s < nsplit; s++)
+        if (p == split_p[s]) return h_split_eval(coef + (size_t)npanel * kHStride + (size_t)s * kHSplitStride, x, which);
+    return h_clenshaw(coef + (size_t)p * kHStride + which * kHnc, 2.0 * (u - p) - 1.0);
 }
 
 // Device form: the table lives in global memory, so its loads are global
-// loads even where the pointer came from a descriptor in memory.  The
-// regular block's sum is formed first (its loads, split point included, go
-// out together); only x right of a split point sums a second block.
+// loads even where the pointer came from a descriptor in memory.  A lane in
+// a split panel (index compared with the table's list, no load) takes the
+// out-of-line pieces.
 typedef const __attribute__((address_space(1))) double gconst_double;
 __device__ __forceinline__ double h_cheb_dev(const HTab &H, double x, int which)
 {
     const double u = (x - H.x0) * H.inv_w;
     int p = (int)floor(u);
     p = p < 0 ? 0 : (p >= H.npanel ? H.npanel - 1 : p);
-    gconst_double *c = (gconst_double *)(H.coef + (size_t)p * kHStride);
-    double v = h_block(c, which, x);
-    if (x >= c[kHSplit]) v = h_block((gconst_double *)(H.coef + (size_t)c[kHRight] * kHStride), which, x);
-    return v;
+    int s = -1;
+#pragma unroll
+    for (int i = 0; i < kHSplitMax; i++)
+        if (i < H.nsplit && p == H.split_p[i]) s = i;
+    if (s >= 0)
+        return h_split_eval((gconst_double *)(H.coef + (size_t)H.npanel * kHStride + (size_t)s * kHSplitStride), x,
+                            which);
+    return h_clenshaw((gconst_double *)(H.coef + (size_t)p * kHStride + which * kHnc), 2.0 * (u - p) - 1.0);
 }
 
 __device__ __forceinline__ double inv_a2h(const HTab &H, double x) { return h_cheb_dev(H, x, 0); }
@@ -334,7 +384,52 @@ struct DnuIntegrand {  // delta_tot_table.c:492-500
         const double specJ = specialJ(k_by_m * fsl, *J);
         return fsl * inv_ah(H, x) * specJ * dtot;  // fsl / (a H) specJ dtot
     }
+    // Two points with operator()'s arithmetic, in straight-line code so that
+    // their dependent loads (bucket, knot walk, coefficients, H panel) overlap;
+    // use1 / use2 say which results are used.  A used point outside the
+    // tables flags GSL_EDOM and gives 0; the indices of every point are
+    // clamped into the tables, so an unused or outside point reads in bounds.
+    __device__ __forceinline__ void eval2(double x1, double x2, bool use1, bool use2, double &r1, double &r2) const
+    {
+        const bool in1 = !(x1 < fx0 || x1 > fxhi || x1 < sx0 || x1 > sxhi);
+        const bool in2 = !(x2 < fx0 || x2 > fxhi || x2 < sx0 || x2 > sxhi);
+        if ((use1 && !in1) || (use2 && !in2)) atomicOr(edom, 1);
+        double xl1, xr1, xl2, xr2;
+        const int fi1 = settle_uniform(fk, nfs, (int)((x1 - fx0) * finv_h), x1, xl1, xr1);
+        const int fi2 = settle_uniform(fk, nfs, (int)((x2 - fx0) * finv_h), x2, xl2, xr2);
+        int q1 = (int)((x1 - sx0) * sinv_bw), q2 = (int)((x2 - sx0) * sinv_bw);
+        q1 = q1 < 0 ? 0 : (q1 > nsb ? nsb : q1);
+        q2 = q2 < 0 ? 0 : (q2 > nsb ? nsb : q2);
+        const int b1 = sbucket[q1], b2 = sbucket[q2];
+        const double fsl1 = cspline_val(xl1, xr1, fy[fi1], fy[fi1 + 1], fc[fi1], fc[fi1 + 1], x1);
+        const double fsl2 = cspline_val(xl2, xr2, fy[fi2], fy[fi2 + 1], fc[fi2], fc[fi2 + 1], x2);
+        const double ih1 = inv_ah(H, x1), ih2 = inv_ah(H, x2);
+        const int si1 = settle(sx, ns, b1, x1), si2 = settle(sx, ns, b2, x2);
+        const double dt1 = linear ? linear_at(sx, sy, si1, x1) : cspline_at(sx, sy, sc, si1, x1);
+        const double dt2 = linear ? linear_at(sx, sy, si2, x2) : cspline_at(sx, sy, sc, si2, x2);
+        const double j1 = specialJ(k_by_m * fsl1, *J), j2 = specialJ(k_by_m * fsl2, *J);
+        r1 = in1 ? fsl1 * ih1 * j1 * dt1 : 0.0;
+        r2 = in2 ? fsl2 * ih2 * j2 * dt2 : 0.0;
+    }
 };
+
+// The integrand at two points of one lane: f.eval2 where the integrand has
+// it, else two calls.
+template <class F>
+__device__ __forceinline__ void eval_pair(const F &f, double x1, double x2, bool use1, bool use2, double &r1,
+                                          double &r2)
+{
+    r1 = use1 ? f(x1) : 0.0;
+    r2 = use2 ? f(x2) : 0.0;
+}
+template <>
+__device__ __forceinline__ void eval_pair<DnuIntegrand>(const DnuIntegrand &f, double x1, double x2, bool use1,
+                                                        bool use2, double &r1, double &r2)
+{
+    f.eval2(x1, x2, use1, use2, r1, r2);
+    r1 = use1 ? r1 : 0.0;
+    r2 = use2 ? r2 : 0.0;
+}
 
 // ---- wave reductions over 32-lane halves ----
 __device__ __forceinline__ double half_sum(double v)
@@ -391,10 +486,9 @@ __device__ __forceinline__ void gk61_pair(const F &f, double a1, double b1, doub
     // evaluation as the other lanes' left nodes, so a wavefront runs the
     // integrand twice, not three times
     double f1 = 0.0, f2 = 0.0;
-    if (j <= 30) {
-        const double dx = h * c_xgk[j];
-        f1 = f(c - dx);
-        if (j < 30) f2 = f(c + dx);
+    {
+        const double dx = j <= 30 ? h * c_xgk[j] : 0.0;
+        eval_pair(f, c - dx, c + dx, j <= 30, j < 30, f1, f2);
     }
     const double wk = j <= 30 ? c_wgk[j] : 0.0;
     const double wgv = (j < 30 && (j & 1)) ? c_wg[j >> 1] : 0.0;

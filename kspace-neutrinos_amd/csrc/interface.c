@@ -78,7 +78,7 @@ static _delta_pow compute_neutrino_power_internal(const double Time, double *kef
     _delta_pow d_pow;
     int i;
     get_delta_nu_update(&delta_tot_table, Time, nk_nonzero, keff, dcdm, dnu, &transfer_init);
-    kn_message(0, "Done getting neutrino power: nk= %d, k = %g, delta_nu = %g, delta_cdm = %g,\n", nk_nonzero,
+    kn_message(0, "neutrino power ready: %d k bins; k=%g: delta_nu=%g, delta_cdm=%g\n", nk_nonzero,
                keff[1], dnu[1], dcdm[1]);
     for (i = 0; i < nk_nonzero; i++) {
         keff[i] = log(keff[i]);
@@ -144,9 +144,9 @@ void add_nu_power_to_rhogrid(const double Time, const double BoxSize, fftw_compl
     d_pow = compute_neutrino_power_spectrum(Time, BoxSize, d_grid, pmgrid, slabstart_y, nslab_y);
     if (kn_rescale_from_host_table(d_grid, pmgrid, slabstart_y, nslab_y, BoxSize, d_pow.logkk, d_pow.delta_ratio,
                                    d_pow.nbins, d_pow.norm))
-        kn_terminate(5, "delta_nu or delta_cdm is nan\n");
+        kn_terminate(5, "NaN in the neutrino/CDM power ratio\n");
     kn_grid_release(fft_of_rhogrid, d_grid, nmodes, staged, 1);
-    kn_message(0, "Done adding neutrinos to grid on all processors\n");
+    kn_message(0, "neutrino rescale of the grid done\n");
     free_d_pow(&d_pow);
 }
 

@@ -578,7 +578,7 @@ extern "C" int kn_finish_powerspectrum(int nrbins, double total_mass2, double *p
                                        double *keffs)
 {
     int nonzero = 0;
-    kn_message(0, "Total powerspectrum mass: %g\n", sqrt(total_mass2));
+    kn_message(0, "total mass of the grid, |F(k = 0)|: %g\n", sqrt(total_mass2));
     for (int i = 0; i < nrbins; i++) {
         power[i] /= total_mass2;
         if (count[i]) {

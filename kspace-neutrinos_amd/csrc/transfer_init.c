@@ -25,7 +25,7 @@ void allocate_transfer_init_table(_transfer_init_table *t, const double BoxSize,
     double v[8];
     int n = 0;
     FILE *fd = fopen(file, "r");
-    if (!fd) kn_terminate(2019, "Can't read input transfer function in file '%s'\n", file);
+    if (!fd) kn_terminate(2019, "cannot open the transfer-function file %s\n", file);
     t->NPowerTable = 0;
     while (fgets(line, sizeof line, fd)) {
         if (line[0] == '#') continue;
@@ -33,11 +33,11 @@ void allocate_transfer_init_table(_transfer_init_table *t, const double BoxSize,
         if (v[0] > kmin) t->NPowerTable++;
     }
     fclose(fd);
-    kn_message(1, "Found transfer function, using %d rows. Min k used is %g.\n", t->NPowerTable, kmin);
+    kn_message(1, "transfer function: %d rows above k = %g\n", t->NPowerTable, kmin);
     t->logk = kn_malloc("Transfer_functions", 2 * (size_t)t->NPowerTable * sizeof(double));
     t->T_nu = t->logk + t->NPowerTable;
     fd = fopen(file, "r");
-    if (!fd) kn_terminate(2020, "Can't read input transfer function in file '%s'\n", file);
+    if (!fd) kn_terminate(2020, "cannot open the transfer-function file %s\n", file);
     while (n < t->NPowerTable && fgets(line, sizeof line, fd)) {
         if (line[0] == '#') continue;
         if (!parse8(line, v)) break;
@@ -49,7 +49,7 @@ void allocate_transfer_init_table(_transfer_init_table *t, const double BoxSize,
     }
     fclose(fd);
     if (n < t->NPowerTable)
-        kn_terminate(2021, "Expected %d rows in  file '%s' but only found %d\n", t->NPowerTable, file, n);
+        kn_terminate(2021, "%d rows expected in %s, %d found\n", t->NPowerTable, file, n);
 }
 
 void free_transfer_init_table(_transfer_init_table *t) { kn_free(t->logk); }

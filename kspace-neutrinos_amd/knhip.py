@@ -121,6 +121,10 @@ SIGNATURES = {
     "kn_bin_power_device_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                               C.c_void_p, C.c_void_p]),
     "kn_dnu_last_upload_columns": (C.c_int, []),
+    "kn_dnu_last_max_intervals": (C.c_int, []),
+    "kn_dnu_last_intervals": (C.c_longlong, [C.POINTER(C.c_int)]),
+    "kn_dnu_qag_timing": (C.c_int, [C.c_int]),
+    "kn_dnu_last_qag_ms": (C.c_double, []),
     "kn_rescale_green_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p,
                                           C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p]),
     "kn_last_error": (C.c_char_p, []),
@@ -136,6 +140,8 @@ SIGNATURES = {
     "kn_group_solve": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kn_group_adjoint": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kn_group_append": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kn_group_append_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kn_group_set_capacity_plan": (C.c_int, [C.c_void_p, C.c_int]),
     "kn_group_graph_capture": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kn_group_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kn_group_status": (C.c_int, [C.c_void_p]),
@@ -283,6 +289,49 @@ class Batch:
         self._put("mnu_bar", 0, la)
         _check(self._f("append")(self.h, C.c_void_p(self.ptr["mnu_bar"]), C.c_void_p(self.ptr["gbar"]), None),
                self.api + "_append", self.L)
+
+    def append_async(self, loga, rows):
+        """The same step with no host synchronisation (capacity plan): the
+        step is staged in the batch's own device buffers; a rejected history
+        shows in the next status()."""
+        import numpy as np
+        self._stage_step(np.ascontiguousarray(loga, dtype=np.float64), np.ascontiguousarray(rows, dtype=np.float64))
+        _check(self._f("append_async")(self.h, C.c_void_p(self.step_ptr[0]), C.c_void_p(self.step_ptr[1]), None),
+               self.api + "_append_async", self.L)
+
+    def _stage_step(self, la, rw):
+        """Copies a step into device buffers the batch owns (torch tensors,
+        allocated on first use; host tensors where there is no GPU, for the
+        CPU emulation build)."""
+        if not hasattr(self, "step_buf"):
+            import torch
+            dev = "cuda" if torch.cuda.is_available() else "cpu"
+            self.step_buf = (torch.zeros(self.n, dtype=torch.float64, device=dev),
+                             torch.zeros(int(self.off_k[self.n]), dtype=torch.float64, device=dev))
+            self.step_ptr = (self.step_buf[0].data_ptr(), self.step_buf[1].data_ptr())
+        self._mc(self.step_ptr[0], la.ctypes.data, la.nbytes, H2D)
+        self._mc(self.step_ptr[1], rw.ctypes.data, rw.nbytes, H2D)
+
+    def set_capacity_plan(self, on=True):
+        _check(self._f("set_capacity_plan")(self.h, 1 if on else 0), self.api + "_set_capacity_plan", self.L)
+
+    def capture(self, with_adjoint=True, with_append=False, stream=None):
+        """Capture [append +] solve [+ adjoint]; with_append stages each
+        replay's step with stage_step() before graph_launch()."""
+        if with_append and not hasattr(self, "step_buf"):
+            import numpy as np
+            self._stage_step(np.zeros(self.n), np.zeros(int(self.off_k[self.n])))
+        la = C.c_void_p(self.step_ptr[0]) if with_append else None
+        rw = C.c_void_p(self.step_ptr[1]) if with_append else None
+        _check(self._f("graph_capture")(self.h, 1 if with_adjoint else 0, la, rw, stream), self.api + "_graph_capture",
+               self.L)
+
+    def stage_step(self, loga, rows):
+        import numpy as np
+        self._stage_step(np.ascontiguousarray(loga, dtype=np.float64), np.ascontiguousarray(rows, dtype=np.float64))
+
+    def graph_launch(self, stream=None):
+        _check(self._f("graph_launch")(self.h, stream), self.api + "_graph_launch", self.L)
 
     def status(self):
         return self._f("status")(self.h)

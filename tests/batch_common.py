@@ -114,9 +114,18 @@ def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None, api="kn_g
 
 
 
-def append_flow(L=None, memcpy=None, nk=6, steps=6):
+def _sync(stream):
+    if stream is not None:
+        import torch
+        torch.cuda.synchronize()
+
+
+def append_flow(L=None, memcpy=None, nk=6, steps=6, mode="eager", stream=None):
     """Histories whose derived state changes while steps are appended, each
-    step compared with the oracle's get_delta_nu at the new a:
+    step compared with the oracle's get_delta_nu at the new a (mode "eager":
+    kn_group_append then solve; "async": on the capacity plan,
+    kn_group_append_async then solve; "graph": one captured graph of append +
+    solve + adjoint replayed per step, the step staged before each replay):
       0: created with one stored step (initial-condition piece only), then integrates;
       1: hybrid (vcrit 500 km/s, particles from a = 0.3333): qc and the tolerance switch on;
       2: hybrid with vcrit 1e5 km/s: past a = 0.3333 particles are nearly all
@@ -142,6 +151,11 @@ def append_flow(L=None, memcpy=None, nk=6, steps=6):
     B = knhip.Batch(hists, keep=cosmos, L=L, memcpy=memcpy)
     for i, s in enumerate(specs):
         B.load(i, s["logs"], s["rows"], s["k"], s["init"])
+    if mode != "eager":
+        B.set_capacity_plan(True)
+    if mode == "graph":
+        B._put("gbar", 0, np.zeros(B.off_k[B.n]))
+        B.capture(with_adjoint=True, with_append=True, stream=stream)
     worst = 0.0
     for step in range(steps + 1):
         if step:
@@ -153,11 +167,21 @@ def append_flow(L=None, memcpy=None, nk=6, steps=6):
                 s["rows"] = np.concatenate([s["rows"], r[:, None]], axis=1)
                 loga[i] = math.log(s["a"])
                 rows[B.off_k[i]:B.off_k[i] + nk] = r
-            B.append(loga, rows)
+            if mode == "eager":
+                B.append(loga, rows)
+            elif mode == "async":
+                B.append_async(loga, rows)
+            else:
+                B.stage_step(loga, rows)
         if step == steps:
             specs[3]["m"] = 0.15
             B.set_mass(3, 0.15)
-        B.solve()
+        if mode == "graph" and step:
+            B.graph_launch(stream)
+            _sync(stream)
+            assert B.status() == 0
+        else:
+            B.solve()
         for i, s in enumerate(specs):
             T = cosmos[s["c"]].oracle(nk, s["logs"], s["rows"], s["init"])
             rc, o = T.get_delta_nu(s["a"], s["k"], s["m"])
@@ -168,8 +192,17 @@ def append_flow(L=None, memcpy=None, nk=6, steps=6):
     # a step that does not follow the last stored one is rejected before anything changes
     sf = B._get("scalefact", 0, B.off_s[B.n])
     bad = np.array([s["logs"][-1] for s in specs])
-    with pytest_raises_code(3002):
-        B.append(bad, np.zeros(B.off_k[B.n]))
+    if mode == "eager":
+        with pytest_raises_code(3002):
+            B.append(bad, np.zeros(B.off_k[B.n]))
+    elif mode == "async":
+        B.append_async(bad, np.zeros(B.off_k[B.n]))
+        assert B.status() == 3002 and B.status() == 0  # reported once
+    else:
+        B.stage_step(bad, np.zeros(B.off_k[B.n]))
+        B.graph_launch(stream)
+        _sync(stream)
+        assert B.status() == 3002 and B.status() == 0
     assert np.array_equal(B._get("scalefact", 0, B.off_s[B.n]), sf)
     B.solve()
     for i, s in enumerate(specs):

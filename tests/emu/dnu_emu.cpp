@@ -67,12 +67,9 @@ extern "C" int emu_fs_two_pass_check(double (*hub)(double, void *), void *ctx, d
     const double w = 1.0 / 256.0;
     const int npanel = (int)ceil((xa - x0) / w) + 1;
     std::vector<double> coef;
-    h_table(x0, w, npanel, [hub, ctx](double a) { return hub(a, ctx); }, coef);
     HTab H;
+    h_table(x0, w, npanel, [hub, ctx](double a) { return hub(a, ctx); }, coef, H);
     H.coef = coef.data();
-    H.x0 = x0;
-    H.inv_w = 1.0 / w;
-    H.npanel = npanel;
     std::vector<double> fx(nfs), fy(nfs), gx(nfs);
     std::vector<int> list(nfs + 1, 0), status(2, 0);
     hipLaunchKernelGGL(k_fs_first, dim3(fs_first_blocks(nfs)), dim3(256), 0, 0, H, nfs, x0, xa, light, fx.data(),

@@ -148,7 +148,7 @@ def test_emu_group_argument_checks(emu):
     bad.a = bad.time_transfer
     assert emu.kn_group_create(C.byref(g), 1, C.byref(bad)) == 3002
     assert emu.kn_group_create(C.byref(g), 1, C.byref(hh)) == 0
-    # append is not capturable; a null stream cannot be captured; no graph yet
+    # a step needs both d_loga and d_rows; a null stream cannot be captured; no graph yet
     assert emu.kn_group_graph_capture(g, 0, C.c_void_p(8), None, C.c_void_p(1)) == 3002
     assert emu.kn_group_graph_capture(g, 0, None, None, None) == 3002
     assert emu.kn_group_graph_launch(g, None) == 3002
@@ -209,6 +209,46 @@ def test_emu_fslength_across_the_density_switch_matches_the_oracle():
     print("  fslength at %d nodes across ln a = %.4f: max rel err %.2e" % (nfs, x_switch, err))
     assert err < 1e-13
     assert np.array_equal(np.array(two[:]), np.array(full[:]))
+    c.free()
+
+
+def test_emu_group_capacity_plan_across_the_adjoint_part_boundary(emu):
+    """A history stepped from 1022 to 1026 stored steps on the capacity plan:
+    the adjoint's knot parts go from one of 1022 knots to two of 513 (the
+    part length is not monotone in the step count), and the launch sized for
+    na_max must still hold the longest part; every step's forward and
+    gradients against the stored-step plan (bit for bit) and the oracle."""
+    c = Cosmo([0.2, 0.0, 0.0])
+    na0, nk, steps = 1022, 3, 4
+    logs, rows, k, init = synthetic_history(na0 + steps, nk, 1.0, seed=31, noise=0.0)
+    outs = []
+    for cap in (False, True):
+        hh = c.history(nk, na0, na0 + steps, math.exp(logs[na0 - 1]), 1.0, 0.2)
+        B = knhip.Batch([hh], keep=(c,), L=emu, memcpy=host_memcpy, api="kn_group")
+        B.load(0, logs[:na0], rows[:, :na0], k, init)
+        if cap:
+            B.set_capacity_plan(True)
+        g = np.array([0.3, -1.0, 0.7])
+        res = []
+        for t in range(steps):
+            if cap:
+                B.append_async([logs[na0 + t]], rows[:, na0 + t])
+            else:
+                B.append([logs[na0 + t]], rows[:, na0 + t])
+            B.solve()
+            B.adjoint(g)
+            na = na0 + t + 1
+            res.append((B.delta_nu(0), B.dtot_bar(0, na), B.init_bar(0), B.mnu_bar()))
+            if cap:
+                T = c.oracle(nk, logs[:na], rows[:, :na], init)
+                orc = T.get_delta_nu_adjoint(math.exp(logs[na - 1]), k, 0.2, g)
+                compare_adjoint("step %d (Na=%d)" % (t, na), res[-1][:3] + (res[-1][3][0],), orc)
+        assert B.status() == 0
+        outs.append(res)
+        B.close()
+    for ra, rb in zip(*outs):
+        for a, b in zip(ra, rb):
+            assert np.array_equal(a, b)
     c.free()
 
 
@@ -305,15 +345,52 @@ def test_emu_wavefront_error_order_equals_quadpack(limit):
     assert sum(bad[:min(nsteps, limit - 1)]) == 0
 
 
-def test_emu_group_append_rederives_hybrid_and_ic_state(emu):
+@pytest.mark.parametrize("mode", ["eager", "async"])
+def test_emu_group_append_rederives_hybrid_and_ic_state(emu, mode):
     """Appends that change what get_delta_nu derives (delta_tot_table.c:513-555):
     a group created at one stored step, hybrid histories crossing nu_crit_time
     (one of them into initial-condition-only), a massless history given a mass,
-    and a non-increasing step rejected with nothing changed."""
+    and a non-increasing step rejected with nothing changed.  "async": the
+    group on its capacity plan (launches for na_max, live sizes read from
+    device memory) and the step appended by the device alone."""
     from batch_common import append_flow
-    worst = append_flow(L=emu, memcpy=host_memcpy)
+    worst = append_flow(L=emu, memcpy=host_memcpy, mode=mode)
     print("  worst rel err %.3e" % worst)
     assert worst < RTOL
+
+
+def test_emu_group_capacity_plan_equals_the_stored_step_plan(emu):
+    """The same histories solved and differentiated with launches for the
+    stored steps and with launches for na_max (capacity plan): every output
+    bit for bit, before and after a device append."""
+    outs = []
+    for cap in (False, True):
+        cosmos, specs, B = mixed_batch(3, seed=21, extra=3, L=emu, memcpy=host_memcpy, api="kn_group")
+        if cap:
+            B.set_capacity_plan(True)
+        rng = np.random.default_rng(22)
+        g = rng.uniform(-1, 1, B.off_k[B.n])
+        res = []
+        for step in range(2):
+            if step:
+                loga, rows = np.zeros(B.n), np.zeros(B.off_k[B.n])
+                for i, s in enumerate(specs):
+                    loga[i] = s["logs"][-1] + 0.02
+                    rows[B.off_k[i]:B.off_k[i] + s["nk"]] = s["rows"][:, -1] * 1.03
+                if cap:
+                    B.append_async(loga, rows)
+                else:
+                    B.append(loga, rows)
+            B.solve()
+            B.adjoint(g)
+            res += _group_arrays(B)
+        assert B.status() == 0
+        outs.append(res)
+        B.close()
+        for c in cosmos:
+            c.free()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
 
 
 def test_emu_group_subdivisions_equal_the_oracle(emu):

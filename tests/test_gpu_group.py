@@ -308,14 +308,18 @@ def test_group_merged_equals_streams_bit_for_bit(kn, torch_cuda):
         assert np.array_equal(a, b)
 
 
-def test_group_append_rederives_hybrid_and_ic_state(kn, torch_cuda):
+@pytest.mark.parametrize("mode", ["eager", "async", "graph"])
+def test_group_append_rederives_hybrid_and_ic_state(kn, torch_cuda, mode):
     """On the device: a group created at one stored step, hybrid histories
     crossing nu_crit_time (one into initial-condition-only), a massless
     history given a mass, a non-increasing step rejected with nothing changed;
-    every step against the oracle (delta_tot_table.c:513-555)."""
+    every step against the oracle (delta_tot_table.c:513-555).  "async": the
+    capacity plan and the device-only append; "graph": one captured graph of
+    append + solve + adjoint replayed for every step."""
     from batch_common import append_flow
-    print("group append flow", flush=True)
-    worst = append_flow()
+    print("group append flow (%s)" % mode, flush=True)
+    s = torch_cuda.cuda.Stream() if mode == "graph" else None
+    worst = append_flow(mode=mode, stream=C.c_void_p(s.cuda_stream) if s is not None else None)
     print("  worst rel err %.3e" % worst)
     assert worst < RTOL
 
