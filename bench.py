@@ -503,6 +503,8 @@ def main():
     ap.add_argument("--batch-hists", type=int, default=256, help="histories for --only-batch")
     ap.add_argument("--only-adjoint-batch", action="store_true",
                     help="only the 64-history 1024 k x 2000 step plain vs grouped forward + adjoint timing")
+    ap.add_argument("--only-ceiling", action="store_true",
+                    help="only the stream-ceiling kernels on a 1024^3 grid's bytes, for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 2 GiB (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -519,6 +521,12 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}))
+        return
+    if args.only_ceiling:
+        N = args.n
+        modes = N * N * (N // 2 + 1)
+        g = torch.zeros((N, N, N // 2 + 1), dtype=torch.complex128, device="cuda")
+        print(json.dumps({"stream_ceiling": stream_ceiling(g, modes)}))
         return
     import conftest
     conftest.load_testhost()

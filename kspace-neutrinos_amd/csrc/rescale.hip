@@ -142,8 +142,7 @@ extern "C" int kn_rescale_device(fftw_complex *d_field, int dims, int startslab,
     a.logkk = d_logkk;
     a.ratio = d_ratio;
     a.nan_flag = nan_flag_ptr();
-    a.iw = nullptr;
-    a.asmth2 = 0.0;
+    a.gw = nullptr;
     hipLaunchKernelGGL((kn_res::k_rescale_rows_pf<kResWaves, kResUnroll>), dim3(res_grid<false>(a.rows, smem)),
                        dim3(64 * kResWaves), smem, (hipStream_t)stream, a);
     KN_CHECK(hipGetLastError());
@@ -167,19 +166,28 @@ extern "C" int kn_rescale_green_device(fftw_complex *d_field, int dims, int star
         kn_set_last_error("kn_rescale_green_device: nbins=%d dims=%d exceed the LDS budget", nbins, dims);
         return KN_ERR_RANGE;
     }
-    // the per-axis inverse window of this dims, kept for the process
-    static std::vector<double> iw_h;
-    static double *iw_d = nullptr;
-    static int iw_dims = 0, iw_cap = 0;
-    if (iw_dims != dims) {
+    // the per-axis Green's factors of this (dims, r_s / BoxSize), kept for the
+    // process: gw[t] = exp(-kt^2 asmth2) W(t)^-4
+    static std::vector<double> iw_h, gw_h;
+    static double *gw_d = nullptr;
+    static int gw_dims = 0, gw_cap = 0;
+    static double gw_asmth2 = -1.0;
+    const double sc = 2 * M_PI * asmth / boxsize, asmth2 = sc * sc;
+    if (gw_dims != dims || gw_asmth2 != asmth2) {
         kn_bin::build_invwindow(dims, iw_h);
-        if (dims > iw_cap) {
-            if (iw_d) KN_CHECK(hipFree(iw_d));
-            KN_CHECK(hipMalloc(&iw_d, sizeof(double) * dims));
-            iw_cap = dims;
+        gw_h.resize(dims);
+        for (int t = 0; t < dims; t++) {
+            const double kt = kn_res::kval(t, dims), w2 = iw_h[t] * iw_h[t];
+            gw_h[t] = exp(-kt * kt * asmth2) * (w2 * w2);
         }
-        KN_CHECK(hipMemcpy(iw_d, iw_h.data(), sizeof(double) * dims, hipMemcpyHostToDevice));
-        iw_dims = dims;
+        if (dims > gw_cap) {
+            if (gw_d) KN_CHECK(hipFree(gw_d));
+            KN_CHECK(hipMalloc(&gw_d, sizeof(double) * dims));
+            gw_cap = dims;
+        }
+        KN_CHECK(hipMemcpy(gw_d, gw_h.data(), sizeof(double) * dims, hipMemcpyHostToDevice));
+        gw_dims = dims;
+        gw_asmth2 = asmth2;
     }
     kn_res::ResArgs a;
     a.F = (double2 *)d_field;
@@ -193,9 +201,7 @@ extern "C" int kn_rescale_green_device(fftw_complex *d_field, int dims, int star
     a.logkk = d_logkk;
     a.ratio = d_ratio;
     a.nan_flag = nan_flag_ptr();
-    a.iw = iw_d;
-    const double s = 2 * M_PI * asmth / boxsize;
-    a.asmth2 = s * s;
+    a.gw = gw_d;
     hipLaunchKernelGGL((kn_res::k_rescale_rows_pf<kResWaves, kResUnroll, true>), dim3(res_grid<true>(a.rows, smem)),
                        dim3(64 * kResWaves), smem, (hipStream_t)stream, a);
     KN_CHECK(hipGetLastError());

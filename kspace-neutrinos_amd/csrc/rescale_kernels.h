@@ -22,10 +22,10 @@ struct ResArgs {
     double boxsize, norm;
     const double *logkk, *ratio;
     int *nan_flag;
-    // Green's function of the PM potential (GREEN sweeps): the per-axis inverse
-    // CiC window (dims doubles) and (2 pi r_s / BoxSize)^2
-    const double *iw;
-    double asmth2;
+    // Green's function of the PM potential (GREEN sweeps): per axis index t,
+    // gw[t] = exp(-kt^2 asmth2) W(t)^-4 (dims doubles; kt the integer
+    // wavenumber, W the CiC window, asmth2 = (2 pi r_s / BoxSize)^2)
+    const double *gw;
 };
 
 constexpr int kBucketsPerBin = 4;
@@ -135,12 +135,16 @@ __device__ __forceinline__ Pos pos_of(long long off, long long row0, int rowlen,
 // pass of a PM step applies it after the neutrino rescale (Gadget-2's
 // pmforce_periodic, where the hook of 0002-Add-hooks-to-kspace-neutrino-code.patch:116-125
 // calls add_nu_power_to_rhogrid): -exp(-k2 asmth2) / k2 times ff^4, ff the
-// product of the three per-axis inverse windows.
-__device__ __forceinline__ double green_of(int k2, double iwx, double iwy, double iwz, double asmth2)
+// product of the three per-axis inverse windows.  exp(-k2 asmth2) factors
+// over the axes, so with gw (ResArgs) it is -gw[x] gw[y] gw[z] / k2: no
+// exponential per mode, and 1/k2 from the hardware reciprocal refined by one
+// Newton step (a few units in the last place from the direct form).
+__device__ __forceinline__ double green_of(int k2, double gx, double gy, double gz)
 {
-    const double ff = iwx * iwy * iwz;
-    double g = -exp(-k2 * asmth2) / k2;
-    return g * (ff * ff * ff * ff);
+    const double d = (double)k2;
+    const double r = __builtin_amdgcn_rcp(d);
+    const double inv = fma(r, fma(-d, r, 1.0), r);
+    return -((gx * gy) * gz) * inv;
 }
 
 // The rescale sweep: each block a contiguous row range, lane l of the block
@@ -155,15 +159,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_rescale_rows_pf(ResArgs a)
 {
     constexpr int B = 64 * WAVES;
     Tab T;
-    const double *iw = nullptr;
+    const double *gw = nullptr;
     if (!GREEN || a.nbins > 0) T = load_tab(a, B);
     if constexpr (GREEN) {
         extern __shared__ double sm[];
         int *after_buckets = (int *)(sm + 2 * (size_t)a.nbins) + a.nbuckets + 1 + ((a.nbuckets + 1) & 1);
-        double *iw_l = (double *)after_buckets;  // 8-byte aligned
-        for (int t = threadIdx.x; t < a.dims; t += B) iw_l[t] = a.iw[t];
+        double *gw_l = (double *)after_buckets;  // 8-byte aligned
+        for (int t = threadIdx.x; t < a.dims; t += B) gw_l[t] = a.gw[t];
         __syncthreads();
-        iw = iw_l;
+        gw = gw_l;
     }
     const int rowlen = a.dims / 2 + 1;
     const long long r0 = a.rows * blockIdx.x / gridDim.x, r1 = a.rows * (blockIdx.x + 1) / gridDim.x;
@@ -205,7 +209,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_rescale_rows_pf(ResArgs a)
                 f.y *= s;
             }
             if constexpr (GREEN) {
-                const double g = green_of(k2, iw[a.startslab + q.i], iw[q.j], iw[q.z], a.asmth2);
+                const double g = green_of(k2, gw[a.startslab + q.i], gw[q.j], gw[q.z]);
                 f.x *= g;
                 f.y *= g;
             }
