@@ -349,16 +349,10 @@ constexpr int kStageMax = 6144;
 
 // Item `item` (species sp, wavenumber k) by one wavefront; sxs: LDS for the
 // stored log a values (na <= kStageMax), or null to read them from memory.
-__device__ inline void dnu_qag_item(const QagArgs &a, const SpeciesDev &sp, int item, int k, QagWs &ws,
-                                    double *sxs)
+// The integrand of item (species sp, wavenumber index k), with the stored
+// log a values read from sx (device memory or their LDS copy).
+__device__ inline DnuIntegrand dnu_integrand(const QagArgs &a, const SpeciesDev &sp, int k, const double *sx)
 {
-    const int lane = threadIdx.x & 63;
-    const double *sx = a.sx;
-    if (sxs) {
-        for (int t = lane; t < a.ns; t += 64) sxs[t] = a.sx[t];
-        __builtin_amdgcn_wave_barrier();  // every lane's entries written before any lane reads
-        sx = sxs;
-    }
     DnuIntegrand f;
     f.H = a.H;
     f.fy = a.fy;
@@ -385,19 +379,60 @@ __device__ inline void dnu_qag_item(const QagArgs &a, const SpeciesDev &sp, int 
     f.k_by_m = f.k / sp.mnubykT;
     f.J = &sp.J;
     f.edom = a.status + 1;
-    double r;
-    int ni;
-    const int rc = wave_qag61(f, a.x0, a.xa, sp.relerr, kLimit, ws, &r, &ni);
-    if (a.ivl)  // the final subdivision, for the adjoint
-        for (int t = lane; t < ni; t += 64) {
+    return f;
+}
+
+// The item's result, interval count and failure flag; the final subdivision
+// (for the adjoint) by threads t0, t0 + stride, ...
+__device__ inline void dnu_qag_store(const QagArgs &a, const SpeciesDev &sp, int item, int k, const QagWs &ws,
+                                     double r, int ni, int rc, int t0, int stride)
+{
+    if (a.ivl)
+        for (int t = t0; t < ni; t += stride) {
             a.ivl[((size_t)item * kLimit + t) * 2] = ws.lo[t];
             a.ivl[((size_t)item * kLimit + t) * 2 + 1] = ws.hi[t];
         }
-    if (lane == 0) {
+    if (t0 == 0) {
         a.out[item] = a.ic.init ? ic_piece(a.wavenum[k], a.ic.init[k], sp, a.ic) + a.ic.prefac * r : r;
         a.nint_out[item] = ni;
         if (rc) atomicOr(a.status, 2);
     }
+}
+
+__device__ inline void dnu_qag_item(const QagArgs &a, const SpeciesDev &sp, int item, int k, QagWs &ws,
+                                    double *sxs)
+{
+    const int lane = threadIdx.x & 63;
+    const double *sx = a.sx;
+    if (sxs) {
+        for (int t = lane; t < a.ns; t += 64) sxs[t] = a.sx[t];
+        __builtin_amdgcn_wave_barrier();  // every lane's entries written before any lane reads
+        sx = sxs;
+    }
+    const DnuIntegrand f = dnu_integrand(a, sp, k, sx);
+    double r;
+    int ni;
+    const int rc = wave_qag61(f, a.x0, a.xa, sp.relerr, kLimit, ws, &r, &ni);
+    dnu_qag_store(a, sp, item, k, ws, r, ni, rc, lane, 64);
+}
+
+// One item by the W wavefronts of a block (block_qag61: speculative
+// bisections on the wavefronts other than the first).
+template <int W>
+__device__ inline void dnu_qag_item_block(const QagArgs &a, const SpeciesDev &sp, int item, int k, QagBlock<W> &S,
+                                          double *sxs)
+{
+    const double *sx = a.sx;
+    if (sxs) {
+        for (int t = threadIdx.x; t < a.ns; t += 64 * W) sxs[t] = a.sx[t];
+        __syncthreads();
+        sx = sxs;
+    }
+    const DnuIntegrand f = dnu_integrand(a, sp, k, sx);
+    double r;
+    int ni;
+    const int rc = block_qag61<W>(f, a.x0, a.xa, sp.relerr, kLimit, S, &r, &ni);
+    dnu_qag_store(a, sp, item, k, S.ws, r, ni, rc, threadIdx.x, 64 * W);
 }
 
 __global__ __launch_bounds__(64 * kWavesPerBlock) void k_dnu_qag(QagArgs a, const SpeciesDev *__restrict__ sp,
@@ -411,6 +446,54 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_dnu_qag(QagArgs a, cons
     if (item >= nitems) return;
     dnu_qag_item(a, sp[item_species[item / nk]], item, item % nk, ws[wave], a.ns <= kStageMax ? sxs : nullptr);
 }
+
+// The same items, one block of kQagSpec wavefronts each, the highest k first
+// (the longest bisection chains are at high k).
+#ifndef KN_QAG_SPEC
+#define KN_QAG_SPEC 4
+#endif
+constexpr int kQagSpec = KN_QAG_SPEC;
+__global__ __launch_bounds__(64 * kQagSpec) void k_dnu_qag_spec(QagArgs a, const SpeciesDev *__restrict__ sp,
+                                                                 const int *__restrict__ item_species, int nk,
+                                                                 int nitems)
+{
+    __shared__ QagBlock<kQagSpec> S;
+    HIP_DYNAMIC_SHARED(double, sxs)  // a.ns doubles when a.ns <= kStageMax
+    const int nsp = nitems / nk, b = (int)blockIdx.x;
+    const int k = nk - 1 - b / nsp, item = (b % nsp) * nk + k;
+    dnu_qag_item_block<kQagSpec>(a, sp[item_species[item / nk]], item, k, S, a.ns <= kStageMax ? sxs : nullptr);
+}
+
+// Items in a given order: the first nlong (expected to need long bisection
+// chains) one block of kQagSpec wavefronts each (dnu_qag_item_block), the
+// rest kQagSpec per block, one wavefront each (dnu_qag_item).  A QAG's
+// results do not depend on which of the two runs it.
+union QagMixLds {
+    QagBlock<kQagSpec> blk;
+    QagWs ws[kQagSpec];
+};
+__global__ __launch_bounds__(64 * kQagSpec) void k_dnu_qag_mix(QagArgs a, const SpeciesDev *__restrict__ sp,
+                                                                const int *__restrict__ item_species, int nk,
+                                                                int nitems, const int *__restrict__ order, int nlong)
+{
+    __shared__ QagMixLds U;
+    HIP_DYNAMIC_SHARED(double, sxs)  // a.ns doubles when a.ns <= kStageMax
+    double *st = a.ns <= kStageMax ? sxs : nullptr;
+    const int b = (int)blockIdx.x;
+    if (b < nlong) {  // block-uniform
+        const int item = order[b];
+        dnu_qag_item_block<kQagSpec>(a, sp[item_species[item / nk]], item, item % nk, U.blk, st);
+        return;
+    }
+    const int wave = threadIdx.x >> 6, j = nlong + (b - nlong) * kQagSpec + wave;
+    if (j >= nitems) return;
+    const int item = order[j];
+    dnu_qag_item(a, sp[item_species[item / nk]], item, item % nk, U.ws[wave], st);
+}
+
+// Interval counts at or above which an item is expected to gain from the
+// speculative block (about two bisections per round there against one).
+constexpr int kLongChain = 8;
 
 // Adjoint of the initial-condition piece at k: dL/d delta_nu_init[k] and the
 // piece's d/d(m/kT) term (zero when massless).
@@ -646,6 +729,10 @@ struct DnuDev {
     int ystride = 0, have_nk = 0, have_na = 0;
     std::vector<double> shadow;
     std::vector<int> first;
+    // each item's final interval count in the last call, which orders the
+    // next call's QAG items (the same k bins are integrated every step)
+    std::vector<int> prev_nint;
+    int *order = nullptr;
 };
 std::unordered_map<const void *, DnuDev *> g_dev;
 int g_last_max_intervals = 0, g_last_items = 0;
@@ -801,15 +888,18 @@ extern "C" void kn_htable_build_host(double origin, double xlo, double xhi)
 
 extern "C" double kn_htable_eval_host(double x);
 
-// Drop the table when the host's H no longer matches it (new cosmology).
+// Drop the table when the host's H no longer matches it (a new cosmology):
+// 17 points spread over [xlo, xhi], each within 1e-12 of the host's H (the
+// panels fit it to ~1e-14, split panels included).
 static void h_validate(double xlo, double xhi)
 {
-    const double xs[3] = {xlo, 0.5 * (xlo + xhi), xhi};
-    for (double x : xs) {
+    constexpr int kProbe = 17;
+    for (int i = 0; i < kProbe; i++) {
+        const double x = xlo + (xhi - xlo) * i / (kProbe - 1.0);
         const int p = (int)floor((x - g_h.x0) / g_h.width);
         if (p < 0 || p >= g_h.npanel || !g_h.built[p]) continue;
         const double h = kn_hubble(exp(x));
-        if (fabs(kn_htable_eval_host(x) / h - 1) > 1e-11) {
+        if (fabs(kn_htable_eval_host(x) / h - 1) > 1e-12) {
             h_reset(g_h.origin);
             return;
         }
@@ -886,7 +976,7 @@ extern "C" void kn_dnu_release(const _delta_tot_table *d)
     DnuDev *v = it->second;
     void *ptrs[] = {v->fs_list, v->fx,  v->fy,   v->fc,   v->sx,   v->Y,    v->C,    v->wavenum, v->bucket, v->out,
                     v->nint, v->item_species, v->sp, v->status, v->ivl, v->gbar, v->ybar, v->cbar,
-                    v->mubar, v->fac};
+                    v->mubar, v->fac, v->order};
     for (void *p : ptrs)
         if (p) KN_CHECK(hipFree(p));
     delete v;
@@ -967,6 +1057,8 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
         if (dev->nint) KN_CHECK(hipFree(dev->nint));
         KN_CHECK(hipMalloc(&dev->out, sizeof(double) * nitems));
         KN_CHECK(hipMalloc(&dev->nint, sizeof(int) * nitems));
+        if (dev->order) KN_CHECK(hipFree(dev->order));
+        KN_CHECK(hipMalloc(&dev->order, sizeof(int) * nitems));
         dev->cap_items = nitems;
     }
     if (!dev->sp) {
@@ -1011,11 +1103,12 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
     // pinned staging: inputs [Y columns col0..Na | scalefact | wavenum | species | species index], then outputs
     const size_t nY = (size_t)nk * ncol;
     const size_t b_sp = sizeof(double) * (nY + Na + nk);
-    const size_t b_in = align64(b_sp + sizeof(SpeciesDev) * NUSPECIES + sizeof(int) * NUSPECIES);
+    const size_t b_in = align64(b_sp + sizeof(SpeciesDev) * NUSPECIES + sizeof(int) * NUSPECIES + sizeof(int) * nitems);
     char *stage = pinned_stage(b_in + align64(sizeof(double) * nitems) + sizeof(int) * ((size_t)nitems + 2));
     double *stY = (double *)stage, *stX = stY + nY, *stK = stX + Na;
     SpeciesDev *sd = (SpeciesDev *)(stage + b_sp);
     int *ispec = (int *)(sd + NUSPECIES);
+    int *ord = ispec + NUSPECIES;
     double *res = (double *)(stage + b_in);
     int *nint = (int *)(stage + b_in + align64(sizeof(double) * nitems)), *status = nint + nitems;
 
@@ -1066,6 +1159,19 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
     KN_CHECK(hipMemcpyAsync(dev->wavenum, stK, sizeof(double) * nk, hipMemcpyHostToDevice, g_copy));
     KN_CHECK(hipMemcpyAsync(dev->sp, sd, sizeof(SpeciesDev) * need.size(), hipMemcpyHostToDevice, g_copy));
     KN_CHECK(hipMemcpyAsync(dev->item_species, ispec, sizeof(int) * need.size(), hipMemcpyHostToDevice, g_copy));
+    // the QAG items' order: those that needed kLongChain intervals or more in
+    // the last call first (longest first), then the rest in item order
+    int nlong = 0;
+    {
+        const bool pred = dev->prev_nint.size() == (size_t)nitems;
+        for (int i = 0; i < nitems; i++)
+            if (pred && dev->prev_nint[i] >= kLongChain) ord[nlong++] = i;
+        std::stable_sort(ord, ord + nlong, [&](int x, int y) { return dev->prev_nint[x] > dev->prev_nint[y]; });
+        int j = nlong;
+        for (int i = 0; i < nitems; i++)
+            if (!(pred && dev->prev_nint[i] >= kLongChain)) ord[j++] = i;
+    }
+    KN_CHECK(hipMemcpyAsync(dev->order, ord, sizeof(int) * nitems, hipMemcpyHostToDevice, g_copy));
     KN_CHECK(hipEventRecord(g_copied, g_copy));
     KN_CHECK(hipStreamWaitEvent(0, g_copied, 0));
 
@@ -1101,8 +1207,9 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
     qa.ic.init = nullptr;
     qa.ic.fsl0 = qa.ic.lin = qa.ic.prefac = 0.0;
     if (g_qag_timing) KN_CHECK(hipEventRecord(g_qag_ev[0], 0));
-    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nitems)), dim3(64 * kWavesPerBlock),
-                       Na <= kStageMax ? sizeof(double) * Na : 0, 0, qa, dev->sp, dev->item_species, nk, nitems);
+    hipLaunchKernelGGL(k_dnu_qag_mix, dim3(nlong + (nitems - nlong + kQagSpec - 1) / kQagSpec), dim3(64 * kQagSpec),
+                       Na <= kStageMax ? sizeof(double) * Na : 0, 0, qa, dev->sp, dev->item_species, nk, nitems,
+                       dev->order, nlong);
     KN_CHECK(hipGetLastError());
     if (g_qag_timing) KN_CHECK(hipEventRecord(g_qag_ev[1], 0));
     KN_CHECK(hipMemcpyAsync(res, dev->out, sizeof(double) * nitems, hipMemcpyDeviceToHost, 0));
@@ -1121,6 +1228,7 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
     g_last_max_intervals = 0;
     g_last_intervals = 0;
     g_last_items = nitems;
+    dev->prev_nint.assign(nint, nint + nitems);
     for (int i = 0; i < nitems; i++) {
         g_last_max_intervals = nint[i] > g_last_max_intervals ? nint[i] : g_last_max_intervals;
         g_last_intervals += nint[i];
@@ -1965,8 +2073,8 @@ void group_forward_one(kn_group *g, const GroupHist &q, int i, hipStream_t s)
     a.ic.fsl0 = q.fsl0;
     a.ic.lin = q.lin;
     a.ic.prefac = q.spec.delta_nu_prefac;
-    hipLaunchKernelGGL(k_dnu_qag, dim3(blocks_for(nk)), dim3(64 * kWavesPerBlock),
-                       na <= kStageMax ? sizeof(double) * na : 0, s, a, g->sp + i, g->spec0, nk, nk);
+    hipLaunchKernelGGL(k_dnu_qag_spec, dim3(nk), dim3(64 * kQagSpec), na <= kStageMax ? sizeof(double) * na : 0, s,
+                       a, g->sp + i, g->spec0, nk, nk);
 }
 
 // Adjoint of one history's last forward on stream s.

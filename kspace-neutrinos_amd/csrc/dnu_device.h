@@ -743,6 +743,204 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
     return 5;
 }
 
+// ---- QAG by W wavefronts with speculative bisections ----
+// wave_qag61's algorithm on one block of W wavefronts.  The chain of
+// bisections is serial (each choice waits for the previous estimates), so
+// the other wavefronts evaluate ahead: every round each wavefront evaluates
+// the two halves of one interval -- the interval to bisect now, then the two
+// halves it will be cut into (one of them is next when the cut leaves it the
+// largest error, as in the long chains of oscillatory integrands), then the
+// next interval(s) in QUADPACK's error order (ord[]) -- and wavefront 0 runs
+// wave_qag61's update for as long as the interval to bisect next has its
+// halves evaluated (an interval is known by its bounds: the list holds
+// disjoint intervals).  The evaluations are gk61_pair's and the update is
+// wave_qag61's, so the decisions and results are wave_qag61's bit for bit;
+// an evaluation that the update does not reach is only work lost.
+struct QagSlot {
+    int used, eval;    // holds an interval; evaluate it this round
+    double lo, hi;     // the interval
+    GK g1, g2;         // its halves
+};
+
+template <int W>
+struct QagBlock {
+    QagWs ws;
+    QagSlot slot[W];
+    int rc, n;         // return code (-1: running) and interval count, from wavefront 0
+};
+
+template <int W, class F>
+__device__ int block_qag61(const F &f, double a, double b, double epsrel, int limit, QagBlock<W> &S, double *result,
+                           int *nint)
+{
+    const int wave = threadIdx.x >> 6;
+    const bool lead = threadIdx.x == 0;  // lane 0 of wavefront 0 writes the list and the slots
+    QagWs &w = S.ws;
+    // round 0: wavefront 0 the first estimate; wavefront 1 the halves of [a, b] (the first bisection)
+    GK g0, gx;
+    if (wave == 0) {
+        gk61_pair(f, a, b, a, b, g0, gx);
+    } else if constexpr (W > 1) {
+        if (wave == 1) {
+            const double m = 0.5 * (a + b);
+            GK g1, g2;
+            gk61_pair(f, a, m, m, b, g1, g2);
+            if ((threadIdx.x & 63) == 0) {
+                S.slot[1].used = 1;
+                S.slot[1].lo = a;
+                S.slot[1].hi = b;
+                S.slot[1].g1 = g1;
+                S.slot[1].g2 = g2;
+            }
+        }
+    }
+    if (lead)
+        for (int q = 0; q < W; q++)
+            if (q != 1) S.slot[q].used = 0;
+    __syncthreads();
+    // wavefront 0's scalar state (identical in its 64 lanes)
+    double area = 0.0, errsum = 0.0, tol = 0.0;
+    int n = 1, it = 1, rt1 = 0, rt2 = 0, etype = 0, imax = 0, rc = -1;
+    if (wave == 0) {
+        if (lead) {
+            w.lo[0] = a;
+            w.hi[0] = b;
+            w.r[0] = g0.res;
+            w.e[0] = g0.err;
+            w.ord[0] = 0;
+            w.lev[0] = 0;
+        }
+        tol = epsrel * fabs(g0.res);
+        if (g0.err <= 50 * DBL_EPSILON * g0.resabs && g0.err > tol) rc = 18;
+        else if ((g0.err <= tol && g0.err != g0.resasc) || g0.err == 0.0) rc = 0;
+        else if (limit == 1) rc = 11;
+        area = g0.res;
+        errsum = g0.err;
+        imax = __shfl(0, 0, 64);
+    }
+    for (;;) {
+        if (wave == 0) {
+            // the bisections whose halves are at hand, exactly as wave_qag61's loop
+            while (rc < 0) {
+                const int im = imax;
+                const double ai = w.lo[im], bi = w.hi[im], ri = w.r[im], ei = w.e[im];
+                int s = -1;
+#pragma unroll
+                for (int q = 0; q < W; q++)
+                    if (S.slot[q].used && S.slot[q].lo == ai && S.slot[q].hi == bi) s = q;
+                // every lane has read the interval before lane 0 rewrites it below
+                __builtin_amdgcn_wave_barrier();
+                if (s < 0) break;
+                const double a1 = ai, b1 = 0.5 * (ai + bi), a2 = b1, b2 = bi;
+                const GK g1 = S.slot[s].g1, g2 = S.slot[s].g2;
+                const double r12 = g1.res + g2.res, e12 = g1.err + g2.err;
+                errsum += e12 - ei;
+                area += r12 - ri;
+                if (g1.resasc != g1.err && g2.resasc != g2.err) {
+                    if (fabs(ri - r12) <= 1.0e-5 * fabs(r12) && e12 >= 0.99 * ei) rt1++;
+                    if (it >= 10 && e12 > ei) rt2++;
+                }
+                tol = epsrel * fabs(area);
+                if (errsum > tol) {
+                    const double t = (1 + 100 * DBL_EPSILON) * (fabs(a2) + 1000 * DBL_MIN);
+                    if (rt1 >= 6 || rt2 >= 20) etype = 2;
+                    if (fabs(a1) <= t && fabs(b2) <= t) etype = 3;
+                }
+                if (lead) {
+                    const int lv = w.lev[im] + 1;
+                    if (g2.err > g1.err) {
+                        w.lo[im] = a2;
+                        w.r[im] = g2.res;
+                        w.e[im] = g2.err;
+                        w.lo[n] = a1;
+                        w.hi[n] = b1;
+                        w.r[n] = g1.res;
+                        w.e[n] = g1.err;
+                    } else {
+                        w.hi[im] = b1;
+                        w.r[im] = g1.res;
+                        w.e[im] = g1.err;
+                        w.lo[n] = a2;
+                        w.hi[n] = b2;
+                        w.r[n] = g2.res;
+                        w.e[n] = g2.err;
+                    }
+                    w.lev[im] = lv;
+                    w.lev[n] = lv;
+                }
+                imax = qag_sort_wave(w, n + 1, limit);
+                n++;
+                it++;
+                if (!(it < limit && !etype && errsum > tol)) {
+                    rc = errsum <= tol ? 0 : etype == 2 ? 18 : etype == 3 ? 21 : it == limit ? 11 : 5;
+                }
+            }
+            // the next round's intervals, one fixed slot each: 0 the one to
+            // bisect now (never evaluated yet: it would have been used above),
+            // 1 and 2 its halves (new), then the next in error order, kept in
+            // place when the slot already holds it
+            if (lead) {
+                S.rc = rc;
+                S.n = n;
+                if (rc < 0) {
+                    const double lo = w.lo[imax], hi = w.hi[imax], m = 0.5 * (lo + hi);
+                    const double ml = 0.5 * (lo + m), mr = 0.5 * (m + hi);
+                    // with 8 or more slots, 3..6 hold the interval's quarters
+                    constexpr int kQ = W >= 8 ? 4 : 0;
+#pragma unroll
+                    for (int q = 0; q < W; q++) {
+                        double ql = 0.0, qh = 0.0;
+                        bool any = true;
+                        const int r = q - 2 - kQ;  // rank in error order of the later slots
+                        if (q == 0) ql = lo, qh = hi;
+                        else if (q == 1) ql = lo, qh = m;
+                        else if (q == 2) ql = m, qh = hi;
+                        else if (kQ && q == 3) ql = lo, qh = ml;
+                        else if (kQ && q == 4) ql = ml, qh = m;
+                        else if (kQ && q == 5) ql = m, qh = mr;
+                        else if (kQ && q == 6) ql = mr, qh = hi;
+                        else if (r < n) ql = w.lo[w.ord[r]], qh = w.hi[w.ord[r]];
+                        else any = false;
+                        QagSlot &sl = S.slot[q];
+                        if (!any) {
+                            sl.used = 0;
+                            sl.eval = 0;
+                        } else if (q >= 3 + kQ && sl.used && sl.lo == ql && sl.hi == qh) {
+                            sl.eval = 0;
+                        } else {
+                            sl.used = 1;
+                            sl.eval = 1;
+                            sl.lo = ql;
+                            sl.hi = qh;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (S.rc >= 0) break;
+        {
+            QagSlot &sl = S.slot[wave];
+            if (sl.eval) {
+                const double lo = sl.lo, hi = sl.hi, m = 0.5 * (lo + hi);
+                GK g1, g2;
+                gk61_pair(f, lo, m, m, hi, g1, g2);
+                if ((threadIdx.x & 63) == 0) {
+                    sl.g1 = g1;
+                    sl.g2 = g2;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const int nn = S.n;
+    double s = 0.0;
+    for (int k = 0; k < nn; k++) s += w.r[k];
+    *result = s;
+    *nint = nn;
+    return S.rc;
+}
+
 // ---- natural cubic spline coefficients, windowed parallel solve ----
 // The interior system h_{i-1} c_{i-1} + 2(h_{i-1}+h_i) c_i + h_i c_{i+1} = g_i is
 // 2:1 diagonally dominant, so the influence of a row decays at least as

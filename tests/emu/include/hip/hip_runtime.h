@@ -4,12 +4,12 @@
 // source with g++ and runs every kernel on the host.  Each lane is a fiber
 // (its own stack, switched by emu_ctx_switch on one host thread); a wavefront
 // runs its 64 lanes round-robin up to the next wavefront operation
-// (__shfl and its variants, __ballot, readfirstlane, wave_barrier,
-// __syncthreads), where every lane must
-// arrive, as converged hardware code does; otherwise the emulation stops
-// with a message.  Waves and blocks run one after another, so __syncthreads
-// is supported in one-wave blocks only.  "Device" memory is host memory
-// filled with garbage when allocated; graphs are not emulated.
+// (__shfl and its variants, __ballot, readfirstlane, wave_barrier), where
+// every lane of the wave must arrive, as converged hardware code does;
+// otherwise the emulation stops with a message.  All waves of a block run
+// together, interleaved at those points, and __syncthreads waits for every
+// thread of the block.  Blocks run one after another.  "Device" memory is
+// host memory filled with garbage when allocated; graphs are not emulated.
 #pragma once
 #include <atomic>
 #include <cstdint>
@@ -50,13 +50,17 @@ inline emu_idx emu_gridDim() { return {emu_gdim, 1, 1}; }
 // current stack, stores its pointer in *save_sp and resumes load_sp.
 extern "C" void emu_ctx_switch(void **save_sp, void *load_sp);
 
-struct EmuWaveSched {
+constexpr int kEmuMaxThreads = 512;  // threads per block
+
+struct EmuWaveSched {  // the running block's threads
     void *main_sp;
-    void *sp[64];
-    bool done[64];
-    unsigned base;  // threadIdx.x of lane 0
-    int cur;
-    unsigned long long x[64];
+    void *sp[kEmuMaxThreads];
+    bool done[kEmuMaxThreads];
+    int n;    // threads in the block
+    int cur;  // threadIdx.x of the running thread
+    unsigned long long x[kEmuMaxThreads];
+    int bar_arrived;
+    unsigned bar_gen;
     void (*body)(void *);
     void *arg;
     std::vector<char> stacks;
@@ -74,9 +78,9 @@ inline unsigned long long emu_exchange(unsigned long long v, int src)
     EmuWaveSched &S = *emu_wave;
     const int lane = S.cur;
     S.x[lane] = v;
-    emu_yield();  // every lane has written
-    const unsigned long long r = S.x[src & 63];
-    emu_yield();  // every lane has read
+    emu_yield();  // every lane of the wave has written
+    const unsigned long long r = S.x[(lane & ~63) + (src & 63)];
+    emu_yield();  // every lane of the wave has read
     return r;
 }
 inline double __shfl(double v, int src, int = 64)
@@ -108,22 +112,27 @@ inline unsigned long long __ballot(int pred)
 {
     EmuWaveSched &S = *emu_wave;
     S.x[S.cur] = pred ? 1 : 0;
-    emu_yield();  // every lane has written
+    emu_yield();  // every lane of the wave has written
+    const int base = S.cur & ~63;
     unsigned long long m = 0;
     for (int l = 0; l < 64; l++)
-        if (S.x[l]) m |= 1ull << l;
-    emu_yield();  // every lane has read
+        if (S.x[base + l]) m |= 1ull << l;
+    emu_yield();  // every lane of the wave has read
     return m;
 }
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }  // the hardware estimate, exactly
+// every thread of the block arrives before any goes on
 inline void __syncthreads()
 {
-    if (emu_bdim > 64) {
-        fprintf(stderr, "emulation: __syncthreads in a block of more than one wave\n");
-        abort();
+    EmuWaveSched &S = *emu_wave;
+    const unsigned gen = S.bar_gen;
+    if (++S.bar_arrived == S.n) {
+        S.bar_arrived = 0;
+        S.bar_gen++;
     }
     emu_yield();
+    while (S.bar_gen == gen) emu_yield();
 }
 
 inline int atomicAdd(int *p, int v) { const int o = *p; *p = o + v; return o; }
@@ -200,15 +209,19 @@ constexpr size_t kEmuStack = 256 * 1024;
 
 extern "C" void emu_fiber_entry();
 
-// Runs body(arg) on the 64 lanes of one wave, lane l as threadIdx.x = base + l.
-inline void emu_run_wave(unsigned base, void (*body)(void *), void *arg)
+// Runs body(arg) on the n threads of one block (threadIdx.x = t), all waves
+// together: each pass gives every unfinished thread one turn up to its next
+// yield.  The lanes of a wave must yield and finish in the same passes.
+inline void emu_run_block(int n, void (*body)(void *), void *arg)
 {
     static EmuWaveSched S;
-    if (S.stacks.empty()) S.stacks.resize(64 * kEmuStack);
-    S.base = base;
+    if (S.stacks.empty()) S.stacks.resize(kEmuMaxThreads * kEmuStack);
+    S.n = n;
     S.body = body;
     S.arg = arg;
-    for (int l = 0; l < 64; l++) {
+    S.bar_arrived = 0;
+    S.bar_gen = 0;
+    for (int l = 0; l < n; l++) {
         S.done[l] = false;
         uintptr_t top = ((uintptr_t)(S.stacks.data() + (l + 1) * kEmuStack)) & ~(uintptr_t)15;
         void **sp = (void **)top;
@@ -219,25 +232,34 @@ inline void emu_run_wave(unsigned base, void (*body)(void *), void *arg)
     }
     emu_wave = &S;
     for (;;) {
-        int yielded = 0, finished = 0;
-        for (int l = 0; l < 64; l++) {
-            if (S.done[l]) continue;
-            S.cur = l;
-            emu_tid = base + l;
-            emu_ctx_switch(&S.main_sp, S.sp[l]);
-            if (S.done[l]) finished++;
-            else yielded++;
-        }
-        if (yielded == 0) break;
-        if (finished) {
-            fprintf(stderr, "emulation: lanes of wave %u diverged at a wavefront operation\n", base / 64);
-            abort();
-        }
-        for (int l = 0; l < 64; l++)
-            if (S.done[l]) {
-                fprintf(stderr, "emulation: lanes of wave %u diverged at a wavefront operation\n", base / 64);
+        int yielded = 0;
+        for (int w0 = 0; w0 < n; w0 += 64) {
+            int wy = 0, wf = 0, wdone = 0;
+            for (int l = w0; l < w0 + 64; l++) {
+                if (S.done[l]) {
+                    wdone++;
+                    continue;
+                }
+                S.cur = l;
+                emu_tid = l;
+                emu_ctx_switch(&S.main_sp, S.sp[l]);
+                if (S.done[l]) wf++;
+                else wy++;
+            }
+            if (wy && (wf || wdone)) {
+                fprintf(stderr, "emulation: lanes of wave %d diverged at a wavefront operation (%d yielded, %d "
+                                "finished now, %d before)\n", w0 / 64, wy, wf, wdone);
                 abort();
             }
+            yielded += wy;
+        }
+        if (yielded == 0) break;
+        int finished = 0;
+        for (int l = 0; l < n; l++) finished += S.done[l];
+        if (finished && S.bar_arrived > 0 && S.bar_arrived + finished == n) {
+            fprintf(stderr, "emulation: threads finished while others wait at __syncthreads\n");
+            abort();
+        }
     }
 }
 
@@ -251,12 +273,16 @@ void emu_launch(void (*k)(KA...), dim3 g, dim3 b, size_t, hipStream_t, A... args
         fprintf(stderr, "emulation: block size %u is not a multiple of 64\n", b.x);
         abort();
     }
+    if (b.x > (unsigned)kEmuMaxThreads) {
+        fprintf(stderr, "emulation: block size %u above %d\n", b.x, kEmuMaxThreads);
+        abort();
+    }
     emu_bdim = b.x;
     emu_gdim = g.x;
     auto body = [&]() { k(args...); };
     for (unsigned bx = 0; bx < g.x; bx++) {
         emu_bid = bx;
-        for (unsigned w = 0; w < b.x / 64; w++) emu_run_wave(w * 64, &emu_call<decltype(body)>, &body);
+        emu_run_block((int)b.x, &emu_call<decltype(body)>, &body);
     }
 }
 #define hipLaunchKernelGGL(k, g, b, sh, st, ...) emu_launch(k, dim3(g), dim3(b), sh, st, ##__VA_ARGS__)
