@@ -148,7 +148,7 @@ def dnu_bench(kn, nk, na=2000, calls=3, cpu_k=16, no_cpu=False):
     res = {"workload": "get_delta_nu_combined, hybrid 0.06/0.06/0.3 eV, %d k x %d steps, a=1" % (nk, na),
            "ms_per_call": round(tg * 1e3, 3), "kstep_per_s": round(ksteps / tg, 1),
            "note": "k.step = one (k bin, stored step) pair per distinct mass; plain C-ABI call, host in/out",
-           "roofline": {"bound": "fp64", "kernel": "k_dnu_qag", "achieved": round(flops / tqm / 1e12, 3),
+           "roofline": {"bound": "fp64", "kernel": "k_dnu_qag_mix", "achieved": round(flops / tqm / 1e12, 3),
                         "peak": 78.6, "unit": "TFLOP/s", "frac": round(flops / tqm / 1e12 / 78.6, 4),
                         "kernel_ms": round(tqm * 1e3, 4), "qag_items": items.value,
                         "intervals": int(total), "evaluations": int(evals),

@@ -492,7 +492,7 @@ __global__ __launch_bounds__(64 * kQagSpec) void k_dnu_qag_mix(QagArgs a, const 
 }
 
 // Interval counts at or above which an item is expected to gain from the
-// speculative block (about two bisections per round there against one).
+// speculative block (about two bisections per pass there against one).
 constexpr int kLongChain = 8;
 
 // Adjoint of the initial-condition piece at k: dL/d delta_nu_init[k] and the

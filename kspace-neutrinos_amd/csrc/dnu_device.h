@@ -746,7 +746,7 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
 // ---- QAG by W wavefronts with speculative bisections ----
 // wave_qag61's algorithm on one block of W wavefronts.  The chain of
 // bisections is serial (each choice waits for the previous estimates), so
-// the other wavefronts evaluate ahead: every round each wavefront evaluates
+// the other wavefronts evaluate ahead: every pass each wavefront evaluates
 // the two halves of one interval -- the interval to bisect now, then the two
 // halves it will be cut into (one of them is next when the cut leaves it the
 // largest error, as in the long chains of oscillatory integrands), then the
@@ -757,7 +757,7 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
 // wave_qag61's, so the decisions and results are wave_qag61's bit for bit;
 // an evaluation that the update does not reach is only work lost.
 struct QagSlot {
-    int used, eval;    // holds an interval; evaluate it this round
+    int used, eval;    // holds an interval; evaluate it this pass
     double lo, hi;     // the interval
     GK g1, g2;         // its halves
 };
@@ -776,7 +776,7 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
     const int wave = threadIdx.x >> 6;
     const bool lead = threadIdx.x == 0;  // lane 0 of wavefront 0 writes the list and the slots
     QagWs &w = S.ws;
-    // round 0: wavefront 0 the first estimate; wavefront 1 the halves of [a, b] (the first bisection)
+    // pass 0: wavefront 0 the first estimate; wavefront 1 the halves of [a, b] (the first bisection)
     GK g0, gx;
     if (wave == 0) {
         gk61_pair(f, a, b, a, b, g0, gx);
@@ -875,7 +875,7 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
                     rc = errsum <= tol ? 0 : etype == 2 ? 18 : etype == 3 ? 21 : it == limit ? 11 : 5;
                 }
             }
-            // the next round's intervals, one fixed slot each: 0 the one to
+            // the next pass's intervals, one fixed slot each: 0 the one to
             // bisect now (never evaluated yet: it would have been used above),
             // 1 and 2 its halves (new), then the next in error order, kept in
             // place when the slot already holds it
