@@ -344,11 +344,61 @@ typedef struct kn_batch_history {
 
 /* The history get_delta_nu(d, a, ..., mnu) integrates (delta_tot_table.c:507-555:
  * tolerance, hybrid qc, IC-only cases), with na = na_max = d->ia and the
- * process's hubble_function as its H (kn_hubble_ctx). */
+ * process's H: the background set with kn_set_background
+ * (kn_background_hubble_ctx), else its hubble_function (kn_hubble_ctx). */
 int kn_batch_history_from_table(const _delta_tot_table *d, double a, double mnu, kn_batch_history *out);
 /* The process's hubble_function (or the one registered with
  * kn_set_hubble_function) in the kn_batch_history callback form. */
 double kn_hubble_ctx(double a, void *ctx);
+
+/* ---- (f)4: the background, Omega_nu(a) and H(a), tabulated on the device ----
+ * The reference computes Omega_nu(a) on the host (omega_nu_single.c:16-205)
+ * and takes H from the N-body code's hubble_function (gadget-2 patch
+ * 0003:38-55), which the integrals sample on the host.  A kn_background holds
+ * both on the device:
+ *   - the density tables of rho_nu_init (omega_nu_single.c:117-153) for each
+ *     distinct mass: 200 log-a points from a0/1.2 to 120 kT/m, each a
+ *     QAG-GK61 Fermi-Dirac energy integral at relative error 1e-9 (one
+ *     wavefront each), and their natural cubic spline;
+ *   - Omega_nu(a) as get_omega_nu (:55-67), with the series and
+ *     relativistic limits of rho_nu (:156-205);
+ *   - H(a) = Hubble sqrt(Omega0/a^3 + OmegaLambda + (1 - OmegaLambda - Omega0)/a^2
+ *            + Omega_nu(a) - Omega_nu(1)/a^3 + Omega_g(a) if radiation);
+ * and builds the Chebyshev panels of 1/(a^2 H) and 1/(a H) that the device
+ * integrands read (split where H jumps) on the device.  A group history or
+ * the per-call path whose H is a background never calls the host's
+ * hubble_function.  Return codes: 0, 2001 (a density integral failed), 3002
+ * bad arguments. */
+typedef struct kn_background_params {
+    double Omega0;          /* All.Omega0: matter, neutrinos included */
+    double OmegaLambda;     /* All.OmegaLambda */
+    double Hubble;          /* All.Hubble: H0 in internal time units */
+    double HubbleParam;     /* h */
+    double tcmb0;           /* CMB temperature today, K */
+    double a0;              /* lowest a of the density tables (init_omega_nu's a0) */
+    double MNu[NUSPECIES];  /* neutrino masses, eV */
+    int radiation;          /* 1: photons, Omega_g(a) = get_omegag (omega_nu_single.c:77-81) */
+} kn_background_params;
+typedef struct kn_background kn_background;
+int kn_background_create(kn_background **out, const kn_background_params *p);
+void kn_background_destroy(kn_background *bg);
+/* Omega_nu(a) and H(a) at n scale factors: device arrays, on stream. */
+int kn_background_omega_nu(const kn_background *bg, int n, const double *d_a, double *d_out, void *stream);
+int kn_background_hubble(const kn_background *bg, int n, const double *d_a, double *d_out, void *stream);
+/* H(a) on the host from the same tables, in the kn_batch_history callback
+ * form (ctx: the kn_background).  A group history whose hubble is this
+ * function gets its H panels built on the device. */
+double kn_background_hubble_ctx(double a, void *ctx);
+/* Species i's density table: host copies of its log a and rho_nu values
+ * (200 each).  Returns the table length, 0 when species i has no table (a
+ * mass folded into an equal one, or one the limits cover), -3002 for bad
+ * arguments. */
+int kn_background_rho_table(const kn_background *bg, int i, double *loga, double *rhonu);
+/* The process's H becomes bg's (NULL: the host's hubble_function again):
+ * get_delta_nu's H panels are then built on the device, and the library's
+ * host uses of H (the initial-condition piece) take bg's host form.  bg must
+ * outlive its use. */
+int kn_set_background(const kn_background *bg);
 
 /* ---- (b) grouped: many get_delta_nu integrals on the per-call kernels ----
  * No reference counterpart (the reference evaluates one table per

@@ -867,6 +867,309 @@ inline size_t align64(size_t n) { return (n + 63) & ~(size_t)63; }
 
 }  // namespace
 
+// ------------------------------------------------------------ background
+
+// The background object: parameters, the device tables and host copies of
+// them, and the same numbers as a host BgDev (host pointers) and a device
+// BgDev (device pointers).
+struct kn_background {
+    kn_background_params p;
+    BgDev host, dev;
+    std::vector<double> tab;  // [species][loga, rho, c2][kRhoTab]
+    double *d_tab = nullptr;
+};
+
+namespace {
+
+struct BgBuild {
+    int nt;  // tables to build
+    int sp[kNuSpecies];
+    double lo[kNuSpecies], hi[kNuSpecies];
+    double mnu[kNuSpecies], kT, conv;
+    double *loga[kNuSpecies], *rho[kNuSpecies], *c2[kNuSpecies];
+};
+
+// rho_nu_init's integrals (omega_nu_single.c:141-150), one wavefront per
+// table point: QAG-GK61 of q^2 eps f0 over [0, 500 kT] at relative error
+// 1e-9, divided by a^4 and converted to g cm^-3.
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_bg_rho(BgBuild g, int *status)
+{
+    __shared__ QagWs ws[kWavesPerBlock];
+    const int wave = threadIdx.x >> 6;
+    const int item = blockIdx.x * kWavesPerBlock + wave;
+    if (item >= g.nt * kRhoTab) return;  // a whole wavefront; no block barrier follows
+    const int t = item / kRhoTab, i = item - t * kRhoTab;
+    const double la = g.lo[t] + i * (g.hi[t] - g.lo[t]) / (kRhoTab - 1);
+    FdEnergy f;
+    f.am = g.mnu[t] * exp(la);
+    f.kT = g.kT;
+    double r;
+    int n;
+    const int rc = wave_qag61(f, 0.0, 500 * g.kT, 1e-9, KN_GSL_VAL, ws[wave], &r, &n);
+    if ((threadIdx.x & 63) == 0) {
+        if (rc) atomicOr(status, 1);
+        g.loga[t][i] = la;
+        g.rho[t][i] = r / pow(exp(la), 4) * g.conv;
+    }
+}
+
+// The tables' natural cubic splines (gsl_interp_init of omega_nu_single.c:151),
+// one lane per table; work: 3 kRhoTab doubles per table.
+__global__ __launch_bounds__(64) void k_bg_spline(BgBuild g, double *work)
+{
+    const int t = threadIdx.x;
+    if (t < g.nt) bg_spline_init(g.loga[t], g.rho[t], kRhoTab, g.c2[t], work + (size_t)t * 3 * kRhoTab);
+}
+
+// Omega_nu(a) (which 0) or H(a) (which 1) at n points.
+__global__ __launch_bounds__(256) void k_bg_eval(BgDev b, int n, const double *__restrict__ a, double *__restrict__ out,
+                                                 int which)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = which ? bg_hubble(b, a[i]) : bg_omega_nu(b, a[i]);
+}
+
+// H panels from the background, one lane per panel: both functions' fits on
+// [x0 + p w, x0 + (p + 1) w] and their miss against H.
+__global__ __launch_bounds__(64) void k_bg_panels(BgDev b, double x0, double w, int npanel, double *__restrict__ coef,
+                                                  double *__restrict__ miss)
+{
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    if (p >= npanel) return;
+    const BgHub hub{b};
+    const double lo = x0 + p * w, hi = x0 + (p + 1) * w;
+    double *c = coef + (size_t)p * kHStride;
+    h_fit(lo, hi, hub, c);
+    miss[p] = h_fit_err(c, lo, hi, hub);
+}
+
+// The split records: lane 0 keeps the kHSplitMax panels from split_from on
+// that miss most, in panel order as the host's HSplits::offer does (panels
+// below split_from lie below the integrals' lower limit and are never read);
+// one lane per kept panel then finds its split point and fits both pieces
+// after the regular panels.  hdr: nsplit, then the split panels' indices.
+__global__ __launch_bounds__(64) void k_bg_splits(BgDev b, double x0, double w, int npanel, int split_from,
+                                                  double *__restrict__ coef, const double *__restrict__ miss,
+                                                  int *__restrict__ hdr)
+{
+    __shared__ int sp_n;
+    __shared__ int sp_p[kHSplitMax];
+    if (threadIdx.x == 0) {
+        int n = 0, p[kHSplitMax];
+        double err[kHSplitMax];
+        for (int q = split_from; q < npanel; q++) {
+            const double e = miss[q];
+            if (!(e > kHSplitTol)) continue;
+            int slot = n;
+            if (n == kHSplitMax) {
+                slot = 0;
+                for (int i = 1; i < n; i++)
+                    if (err[i] < err[slot]) slot = i;
+                if (err[slot] >= e) continue;
+            } else {
+                n++;
+            }
+            p[slot] = q;
+            err[slot] = e;
+        }
+        sp_n = n;
+        hdr[0] = n;
+        for (int i = 0; i < kHSplitMax; i++) {
+            if (i < n) sp_p[i] = p[i];
+            hdr[1 + i] = i < n ? p[i] : -1;
+        }
+    }
+    __syncthreads();
+    const int s = threadIdx.x;
+    if (s < sp_n) {
+        const int q = sp_p[s];
+        const BgHub hub{b};
+        double tmp[kHStride];
+        h_panel(x0 + q * w, x0 + (q + 1) * w, hub, tmp, coef + (size_t)npanel * kHStride + (size_t)s * kHSplitStride);
+    }
+}
+
+// H panels of the background on [x0, x0 + npanel w) into d (npanel *
+// kHStride + kHSplitMax * kHSplitStride doubles of device memory), split
+// where needed from panel split_from on; H gets the table's description.
+// Synchronous (the split list is read back).
+void bg_htable(const kn_background *bg, double x0, double w, int npanel, int split_from, double *d, HTab &H,
+               hipStream_t st)
+{
+    double *miss;
+    int *hdr;
+    KN_CHECK(hipMalloc(&miss, sizeof(double) * npanel));
+    KN_CHECK(hipMalloc(&hdr, sizeof(int) * (1 + kHSplitMax)));
+    hipLaunchKernelGGL(k_bg_panels, dim3((npanel + 63) / 64), dim3(64), 0, st, bg->dev, x0, w, npanel, d, miss);
+    KN_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_bg_splits, dim3(1), dim3(64), 0, st, bg->dev, x0, w, npanel, split_from, d,
+                       (const double *)miss, hdr);
+    KN_CHECK(hipGetLastError());
+    int h[1 + kHSplitMax];
+    KN_CHECK(hipMemcpyAsync(h, hdr, sizeof h, hipMemcpyDeviceToHost, st));
+    KN_CHECK(hipStreamSynchronize(st));
+    KN_CHECK(hipFree(miss));
+    KN_CHECK(hipFree(hdr));
+    H.coef = d;
+    H.x0 = x0;
+    H.inv_w = 1.0 / w;
+    H.npanel = npanel;
+    H.nsplit = h[0];
+    for (int i = 0; i < kHSplitMax; i++) H.split_p[i] = h[1 + i];
+}
+
+const kn_background *g_bg = nullptr;  // the process's H (kn_set_background)
+const kn_background *g_h_bg = nullptr;  // the background the per-call table was built from
+
+}  // namespace
+
+extern "C" double kn_background_hubble_ctx(double a, void *ctx)
+{
+    return bg_hubble(((const kn_background *)ctx)->host, a);
+}
+
+extern "C" int kn_background_create(kn_background **out, const kn_background_params *p)
+{
+    if (!out || !p || !(p->Hubble > 0) || !(p->HubbleParam > 0) || !(p->tcmb0 > 0) || !(p->a0 > 0) ||
+        !(p->a0 <= 1)) {
+        kn_set_last_error("kn_background_create: bad arguments");
+        return KN_ERR_ARG;
+    }
+    for (int i = 0; i < NUSPECIES; i++)
+        if (!(p->MNu[i] >= 0)) {
+            kn_set_last_error("kn_background_create: neutrino mass %d is %g", i, p->MNu[i]);
+            return KN_ERR_ARG;
+        }
+    upload_constants();
+    kn_background *bg = new kn_background();
+    bg->p = *p;
+    BgDev &b = bg->host;
+    memset(&b, 0, sizeof b);
+    b.Omega0 = p->Omega0;
+    b.OmegaLambda = p->OmegaLambda;
+    b.Hubble = p->Hubble;
+    b.kBtnu = kn_kbtnu(p->tcmb0);
+    b.rhocrit = kn_rhocrit(p->HubbleParam);
+    b.conv = kn_rho_conversion();
+    b.og = kn_omegag0(p->tcmb0, b.rhocrit);
+    b.radiation = p->radiation ? 1 : 0;
+    kn_nu_degeneracies(p->MNu, b.deg);
+    BgBuild g;
+    memset(&g, 0, sizeof g);
+    g.kT = b.kBtnu;
+    g.conv = b.conv;
+    for (int i = 0; i < kNuSpecies; i++) {
+        b.mnu[i] = p->MNu[i];
+        double lo, hi;
+        if (b.deg[i] && kn_rho_table_range(p->a0, p->MNu[i], b.kBtnu, &lo, &hi)) {
+            b.tab[i] = 1;
+            g.sp[g.nt] = i;
+            g.lo[g.nt] = lo;
+            g.hi[g.nt] = hi;
+            g.mnu[g.nt] = p->MNu[i];
+            g.nt++;
+        }
+    }
+    bg->tab.assign((size_t)kNuSpecies * 3 * kRhoTab, 0.0);
+    BgDev d = b;
+    if (g.nt) {
+        KN_CHECK(hipMalloc(&bg->d_tab, sizeof(double) * bg->tab.size()));
+        double *work;
+        int *status;
+        KN_CHECK(hipMalloc(&work, sizeof(double) * 3 * kRhoTab * g.nt));
+        KN_CHECK(hipMalloc(&status, sizeof(int)));
+        KN_CHECK(hipMemset(status, 0, sizeof(int)));
+        for (int t = 0; t < g.nt; t++) {
+            double *base = bg->d_tab + (size_t)g.sp[t] * 3 * kRhoTab;
+            g.loga[t] = base;
+            g.rho[t] = base + kRhoTab;
+            g.c2[t] = base + 2 * kRhoTab;
+        }
+        hipLaunchKernelGGL(k_bg_rho, dim3(blocks_for((long long)g.nt * kRhoTab)), dim3(64 * kWavesPerBlock), 0, 0, g,
+                           status);
+        KN_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_bg_spline, dim3(1), dim3(64), 0, 0, g, work);
+        KN_CHECK(hipGetLastError());
+        int st = 0;
+        KN_CHECK(hipMemcpy(&st, status, sizeof(int), hipMemcpyDeviceToHost));
+        KN_CHECK(hipMemcpy(bg->tab.data(), bg->d_tab, sizeof(double) * bg->tab.size(), hipMemcpyDeviceToHost));
+        KN_CHECK(hipFree(work));
+        KN_CHECK(hipFree(status));
+        if (st) {
+            KN_CHECK(hipFree(bg->d_tab));
+            delete bg;
+            kn_set_last_error("kn_background_create: quadrature failure in a neutrino density table");
+            return 2001;
+        }
+        for (int i = 0; i < kNuSpecies; i++)
+            if (b.tab[i]) {
+                const size_t o = (size_t)i * 3 * kRhoTab;
+                b.loga[i] = bg->tab.data() + o;
+                b.rho[i] = b.loga[i] + kRhoTab;
+                b.c2[i] = b.loga[i] + 2 * kRhoTab;
+                d.loga[i] = bg->d_tab + o;
+                d.rho[i] = d.loga[i] + kRhoTab;
+                d.c2[i] = d.loga[i] + 2 * kRhoTab;
+            }
+    }
+    // Omega_nu(1) once, on the host, for both copies
+    b.onu1 = bg_omega_nu(b, 1.0);
+    d.onu1 = b.onu1;
+    bg->dev = d;
+    *out = bg;
+    return 0;
+}
+
+extern "C" void kn_background_destroy(kn_background *bg)
+{
+    if (!bg) return;
+    if (g_bg == bg) kn_set_background(nullptr);
+    if (g_h_bg == bg) g_h_bg = nullptr;
+    if (bg->d_tab) KN_CHECK(hipFree(bg->d_tab));
+    delete bg;
+}
+
+static int bg_eval(const kn_background *bg, int n, const double *d_a, double *d_out, void *stream, int which)
+{
+    if (!bg || n < 0 || (n > 0 && (!d_a || !d_out))) {
+        kn_set_last_error("kn_background: bad arguments");
+        return KN_ERR_ARG;
+    }
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_bg_eval, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, bg->dev, n, d_a, d_out,
+                       which);
+    KN_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int kn_background_omega_nu(const kn_background *bg, int n, const double *d_a, double *d_out, void *stream)
+{
+    return bg_eval(bg, n, d_a, d_out, stream, 0);
+}
+
+extern "C" int kn_background_hubble(const kn_background *bg, int n, const double *d_a, double *d_out, void *stream)
+{
+    return bg_eval(bg, n, d_a, d_out, stream, 1);
+}
+
+extern "C" int kn_background_rho_table(const kn_background *bg, int i, double *loga, double *rhonu)
+{
+    if (!bg || i < 0 || i >= kNuSpecies) return -KN_ERR_ARG;
+    if (!bg->host.tab[i]) return 0;
+    const double *t = bg->tab.data() + (size_t)i * 3 * kRhoTab;
+    if (loga) memcpy(loga, t, sizeof(double) * kRhoTab);
+    if (rhonu) memcpy(rhonu, t + kRhoTab, sizeof(double) * kRhoTab);
+    return kRhoTab;
+}
+
+extern "C" int kn_set_background(const kn_background *bg)
+{
+    g_bg = bg;
+    kn_set_hubble_ctx(bg ? kn_background_hubble_ctx : nullptr, (void *)bg);
+    return 0;
+}
+
+
 // ------------------------------------------------------------ host API
 
 // Host part: build the panels covering [xlo, xhi] (no device work).  The
@@ -906,8 +1209,46 @@ static void h_validate(double xlo, double xhi)
     }
 }
 
+// The whole process table from the background, on the device, with host
+// copies of its panels and split records (kn_htable_eval_host reads them).
+static void h_from_background(double origin)
+{
+    h_reset(origin);
+    const size_t nreg = g_h.coef.size(), need = nreg + (size_t)kHSplitMax * kHSplitStride;
+    if (g_h.d_cap < need) {
+        if (g_h.d_coef) KN_CHECK(hipFree(g_h.d_coef));
+        KN_CHECK(hipMalloc(&g_h.d_coef, sizeof(double) * need));
+        g_h.d_cap = need;
+    }
+    HTab H;
+    const int p_origin = (int)llround((origin - g_h.x0) / g_h.width);
+    bg_htable(g_bg, g_h.x0, g_h.width, g_h.npanel, p_origin, g_h.d_coef, H, 0);
+    KN_CHECK(hipMemcpy(g_h.coef.data(), g_h.d_coef, sizeof(double) * nreg, hipMemcpyDeviceToHost));
+    g_h.splits.n = H.nsplit;
+    for (int i = 0; i < H.nsplit; i++) {
+        g_h.splits.p[i] = H.split_p[i];
+        g_h.splits.err[i] = 0.0;
+        KN_CHECK(hipMemcpy(g_h.splits.rec[i], g_h.d_coef + nreg + (size_t)i * kHSplitStride,
+                           sizeof(double) * kHSplitStride, hipMemcpyDeviceToHost));
+    }
+    g_h.built.assign(g_h.npanel, 1);
+    g_h.dirty = false;
+    g_h_bg = g_bg;
+}
+
 extern "C" void kn_htable_ensure(double origin, double xlo, double xhi)
 {
+    if (g_bg) {  // the H of a background: built whole on the device, no host sampling
+        if (g_h_bg != g_bg || g_h.coef.empty() || origin != g_h.origin) h_from_background(origin);
+        const int p0 = (int)floor((xlo - g_h.x0) / g_h.width), p1 = (int)floor((xhi - g_h.x0) / g_h.width);
+        if (p0 < 0 || p1 >= g_h.npanel)
+            kn_terminate(KN_ERR_RANGE, "scale factor range [%g, %g] outside the Hubble table\n", exp(xlo), exp(xhi));
+        return;
+    }
+    if (g_h_bg) {  // built from a background that is no longer the process's H
+        g_h.coef.clear();
+        g_h_bg = nullptr;
+    }
     if (!g_h.coef.empty() && origin == g_h.origin) h_validate(xlo, xhi);
     kn_htable_build_host(origin, xlo, xhi);
     const size_t nreg = g_h.coef.size(), need = nreg + (size_t)kHSplitMax * kHSplitStride;
@@ -2354,13 +2695,19 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
         }
         if (found < 0) {
             const int npanel = (int)ceil((q.xmax - q.x0) / kGroupHWidth) + 1;
-            std::vector<double> coef;
-            double (*hub)(double, void *) = q.spec.hubble;
-            void *ctx = q.spec.hubble_ctx;
             HTab H;
-            h_table(q.x0, kGroupHWidth, npanel, [hub, ctx](double a) { return hub(a, ctx); }, coef, H);
-            double *d = group_alloc<double>(coef.size());
-            KN_CHECK(hipMemcpy(d, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice));
+            double *d;
+            if (q.spec.hubble == kn_background_hubble_ctx) {  // built on the device from the background
+                d = group_alloc<double>((size_t)npanel * kHStride + (size_t)kHSplitMax * kHSplitStride);
+                bg_htable((const kn_background *)q.spec.hubble_ctx, q.x0, kGroupHWidth, npanel, 0, d, H, 0);
+            } else {
+                std::vector<double> coef;
+                double (*hub)(double, void *) = q.spec.hubble;
+                void *ctx = q.spec.hubble_ctx;
+                h_table(q.x0, kGroupHWidth, npanel, [hub, ctx](double a) { return hub(a, ctx); }, coef, H);
+                d = group_alloc<double>(coef.size());
+                KN_CHECK(hipMemcpy(d, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice));
+            }
             H.coef = d;
             g->hcoef.push_back(d);
             g->htabs.push_back(H);
@@ -2716,7 +3063,8 @@ extern "C" int kn_batch_history_from_table(const _delta_tot_table *d, double a, 
     hh.qc = s.qc;
     hh.relerr = s.relerr;
     hh.ic_only = s.ic_only;
-    hh.hubble = kn_hubble_ctx;
+    hh.hubble = g_bg ? kn_background_hubble_ctx : kn_hubble_ctx;
+    hh.hubble_ctx = (void *)g_bg;
     *out = hh;
     return 0;
 }

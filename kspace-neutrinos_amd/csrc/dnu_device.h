@@ -58,10 +58,10 @@ struct HTab {
     int split_p[kHSplitMax];
 };
 
-// Host: both functions' interpolants at the kHnc Chebyshev points of
-// [lo, hi]; hub(a) = H(a).
+// Both functions' interpolants at the kHnc Chebyshev points of [lo, hi];
+// hub(a) = H(a).  Host and device (the device background's panels).
 template <typename Hub>
-inline void h_fit(double lo, double hi, Hub hub, double *coef)
+__host__ __device__ inline void h_fit(double lo, double hi, Hub hub, double *coef)
 {
     const double half = 0.5 * (hi - lo), mid = lo + half;
     double g2[kHnc], g1[kHnc];
@@ -105,10 +105,10 @@ __host__ __device__ __forceinline__ double h_split_eval(P rec, double x, int whi
     return h_clenshaw(piece + 2 + which * kHnc, (x - piece[0]) * piece[1]);
 }
 
-// Host: the largest relative miss of the fits `coef` of [lo, hi], at the
-// ends and half-way between adjacent interpolation nodes.
+// The largest relative miss of the fits `coef` of [lo, hi], at the ends and
+// half-way between adjacent interpolation nodes.
 template <typename Hub>
-inline double h_fit_err(const double *coef, double lo, double hi, Hub hub)
+__host__ __device__ inline double h_fit_err(const double *coef, double lo, double hi, Hub hub)
 {
     const double half = 0.5 * (hi - lo), mid = lo + half;
     double worst = 0.0;
@@ -123,9 +123,9 @@ inline double h_fit_err(const double *coef, double lo, double hi, Hub hub)
     return worst;
 }
 
-// Host: one piece [lo, hi] of a split panel into piece[kHPiece].
+// One piece [lo, hi] of a split panel into piece[kHPiece].
 template <typename Hub>
-inline void h_piece(double lo, double hi, Hub hub, double *piece)
+__host__ __device__ inline void h_piece(double lo, double hi, Hub hub, double *piece)
 {
     const double half = 0.5 * (hi - lo);
     piece[0] = lo + half;
@@ -133,13 +133,13 @@ inline void h_piece(double lo, double hi, Hub hub, double *piece)
     h_fit(lo, hi, hub, piece + 2);
 }
 
-// Host: the panel [lo, hi] into coef[kHStride].  Where the fit misses H by
+// The panel [lo, hi] into coef[kHStride].  Where the fit misses H by
 // more than kHSplitTol the panel is also split, into rec[kHSplitStride], at
 // the point that bisection finds (the half whose own fit misses more is
 // kept, until both halves fit or the bracket is a few units in the last
 // place wide).  Returns the fit's miss (above kHSplitTol: rec was written).
 template <typename Hub>
-inline double h_panel(double lo, double hi, Hub hub, double *coef, double *rec)
+__host__ __device__ inline double h_panel(double lo, double hi, Hub hub, double *coef, double *rec)
 {
     h_fit(lo, hi, hub, coef);
     const double err = h_fit_err(coef, lo, hi, hub);
@@ -246,6 +246,136 @@ __device__ __forceinline__ double h_cheb_dev(const HTab &H, double x, int which)
 
 __device__ __forceinline__ double inv_a2h(const HTab &H, double x) { return h_cheb_dev(H, x, 0); }
 __device__ __forceinline__ double inv_ah(const HTab &H, double x) { return h_cheb_dev(H, x, 1); }
+
+// ---- the background: Omega_nu(a) and H(a), host and device ----
+// The neutrino density of omega_nu_single.c:55-205 (a table of kRhoTab log-a
+// points per distinct mass, each a Fermi-Dirac energy integral, interpolated
+// by a natural cubic spline; the relativistic and non-relativistic limits
+// outside it) and the Hubble function of the N-body code's hook (gadget-2
+// patch 0003:38-55, with the kspace neutrinos and, optionally, photons).
+constexpr int kRhoTab = 200;     // omega_nu_single.c:14
+constexpr double kNuSw = 100.0;  // omega_nu_single.h: the series holds for kT/(a m) < 1/100
+constexpr int kNuSpecies = 3;
+
+struct BgDev {
+    double Omega0, OmegaLambda, Hubble;  // Hubble: H0 in internal time units
+    double rhocrit, kBtnu;               // g cm^-3 (h included); eV
+    double conv;                         // (eV/c)^4 -> g cm^-3 for one species (omega_nu_single.c:100-114)
+    double og;                           // Omega_g at a = 1 (omega_nu_single.c:77-81)
+    double onu1;                         // Omega_nu(1)
+    int radiation;                       // add Omega_g(a) (INCLUDE_RADIATION)
+    int deg[kNuSpecies];                 // equal masses folded into the first (init_omega_nu)
+    int tab[kNuSpecies];                 // 1: species i has a density table
+    double mnu[kNuSpecies];
+    const double *loga[kNuSpecies], *rho[kNuSpecies], *c2[kNuSpecies];  // kRhoTab each
+};
+
+// omega_nu_single.c:89-96: the integrand of rho_nu_init, q^2 eps(q) f0(q).
+struct FdEnergy {
+    double am, kT;
+    __host__ __device__ double operator()(double q) const
+    {
+        const double eps = sqrt(q * q + am * am);
+        const double f0 = 1. / (exp(q / kT) + 1);
+        return q * q * eps * f0;
+    }
+};
+
+// omega_nu_single.c:156-160: heavily non-relativistic, zeta(3), zeta(5) and
+// zeta(7) terms.
+__host__ __device__ inline double bg_nonrel(double a, double kT, double am, double r2, double conv)
+{
+    return am * (kT * kT * kT) / (a * a * a * a) *
+           (1.5 * 1.202056903159594 + r2 * 45. / 4. * 1.0369277551433704 + 2835. / 32. * r2 * r2 * 1.0083492773819229 +
+            80325 / 32. * r2 * r2 * r2 * 1.0020083928260826) *
+           conv;
+}
+
+// omega_nu_single.c:164-167: heavily relativistic.
+__host__ __device__ inline double bg_rel(double a, double kT, double conv)
+{
+    return 7 * pow(M_PI * kT / a, 4) / 120. * conv;
+}
+
+// Natural cubic spline at v (knots x, values y, second-derivative halves c;
+// the interval by bisection, as gsl_interp_eval's cspline).
+__host__ __device__ inline double bg_spline(const double *x, const double *y, const double *c, int n, double v)
+{
+    int lo = 0, hi = n - 1;
+    while (hi > lo + 1) {
+        const int mid = (hi + lo) / 2;
+        if (x[mid] > v) hi = mid;
+        else lo = mid;
+    }
+    const double h = x[lo + 1] - x[lo], dy = y[lo + 1] - y[lo], dx = v - x[lo];
+    const double b = (dy / h) - h * (c[lo + 1] + 2.0 * c[lo]) / 3.0;
+    const double d = (c[lo + 1] - c[lo]) / (3.0 * h);
+    return y[lo] + dx * (b + dx * (c[lo] + dx * d));
+}
+
+// Natural cubic spline coefficients c[n] of (x, y): the tridiagonal system of
+// the interior second derivatives by forward elimination and back
+// substitution; work holds 3 (n - 2) doubles.
+__host__ __device__ inline void bg_spline_init(const double *x, const double *y, int n, double *c, double *work)
+{
+    const int m = n - 2;
+    for (int i = 0; i < n; i++) c[i] = 0.0;
+    if (m <= 0) return;
+    double *gam = work, *alp = work + m, *z = work + 2 * m;
+    for (int i = 0; i < m; i++) {
+        const double hl = x[i + 1] - x[i], hr = x[i + 2] - x[i + 1];
+        const double gl = hl != 0.0 ? 1.0 / hl : 0.0, gr = hr != 0.0 ? 1.0 / hr : 0.0;
+        const double diag = 2.0 * (hr + hl);
+        const double rhs = 3.0 * ((y[i + 2] - y[i + 1]) * gr - (y[i + 1] - y[i]) * gl);
+        if (m == 1) {
+            c[1] = rhs / diag;
+            return;
+        }
+        alp[i] = i == 0 ? diag : diag - (x[i + 1] - x[i]) * gam[i - 1];
+        if (i < m - 1) gam[i] = hr / alp[i];
+        z[i] = i == 0 ? rhs : rhs - gam[i - 1] * z[i - 1];
+    }
+    for (int i = 0; i < m; i++) z[i] = z[i] / alp[i];
+    c[m] = z[m - 1];
+    for (int i = m - 2; i >= 0; i--) c[i + 1] = z[i] - gam[i] * c[i + 2];
+}
+
+// omega_nu_single.c:171-205: the physical density of species s at a.
+__host__ __device__ inline double bg_rho_nu(const BgDev &b, int s, double a)
+{
+    const double kT = b.kBtnu, am = a * b.mnu[s];
+    const double r2 = kT * kT / am / am;
+    if (kNuSw * kNuSw * r2 < 1) return bg_nonrel(a, kT, am, r2, b.conv);
+    if (am < 1e-6 * kT) return bg_rel(a, kT, b.conv);
+    const double la = log(a);
+    if (!b.tab[s] || la < b.loga[s][0])
+        return am < 1e-4 * kT ? bg_rel(a, kT, b.conv) : bg_nonrel(a, kT, am, r2, b.conv);
+    return bg_spline(b.loga[s], b.rho[s], b.c2[s], kRhoTab, la);
+}
+
+// omega_nu_single.c:55-67 (get_omega_nu).
+__host__ __device__ inline double bg_omega_nu(const BgDev &b, double a)
+{
+    double rho = 0;
+    for (int i = 0; i < kNuSpecies; i++)
+        if (b.deg[i] > 0) rho += b.deg[i] * bg_rho_nu(b, i, a);
+    return rho / b.rhocrit;
+}
+
+// gadget-2 patch 0003:38-55: matter + Lambda + curvature, the neutrinos as
+// Omega_nu(a) in place of their share of Omega0/a^3, photons when enabled.
+__host__ __device__ inline double bg_hubble(const BgDev &b, double a)
+{
+    double omega_tot = b.Omega0 / pow(a, 3) + b.OmegaLambda + (1. - b.OmegaLambda - b.Omega0) / (a * a);
+    omega_tot += bg_omega_nu(b, a) - b.onu1 / pow(a, 3);
+    if (b.radiation) omega_tot += b.og / pow(a, 4);
+    return b.Hubble * sqrt(omega_tot);
+}
+
+struct BgHub {
+    BgDev b;
+    __host__ __device__ double operator()(double a) const { return bg_hubble(b, a); }
+};
 
 // 1/d from the hardware estimate refined by one Newton step (relative error
 // a few units in the last place, against ten instructions for a division);

@@ -5,7 +5,8 @@
  * a host that defines them (Gadget, or a test host library loaded with
  * RTLD_GLOBAL) is used; otherwise terminate prints and exits with the code,
  * message prints when KN_VERBOSE is set, allocation uses malloc, and the Hubble
- * function must be registered with kn_set_hubble_function.
+ * function must be registered with kn_set_hubble_function.  A background set
+ * with kn_set_background takes precedence over both forms of hubble_function.
  */
 #include <stdarg.h>
 #include <stdio.h>
@@ -22,12 +23,21 @@ extern void *mymalloc_fullinfo(const char *string, size_t size, const char *func
 extern void myfree_fullinfo(void *ptr, const char *func, const char *file, int line) __attribute__((weak));
 
 static double (*g_registered_hubble)(double);
+static double (*g_ctx_hubble)(double, void *);
+static void *g_ctx_hubble_ctx;
 static char g_last_error[512];
 
 void kn_set_hubble_function(double (*fn)(double a)) { g_registered_hubble = fn; }
 
+void kn_set_hubble_ctx(double (*fn)(double a, void *ctx), void *ctx)
+{
+    g_ctx_hubble = fn;
+    g_ctx_hubble_ctx = ctx;
+}
+
 double kn_hubble(double a)
 {
+    if (g_ctx_hubble) return g_ctx_hubble(a, g_ctx_hubble_ctx);
     if (g_registered_hubble) return g_registered_hubble(a);
     if (hubble_function) return hubble_function(a);
     kn_terminate(KN_ERR_NOHUBBLE, "no hubble_function: define it or call kn_set_hubble_function\n");

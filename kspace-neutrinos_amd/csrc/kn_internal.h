@@ -32,11 +32,24 @@ extern "C" {
 
 /* ---- host services (host_services.c) ---- */
 double kn_hubble(double a);
+/* The process's H in callback form (kn_set_background); NULL: the registered
+ * or linked hubble_function again. */
+void kn_set_hubble_ctx(double (*fn)(double a, void *ctx), void *ctx);
 void kn_terminate(int code, const char *fmt, ...) __attribute__((noreturn, format(printf, 2, 3)));
 void kn_message(int level, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void *kn_malloc(const char *name, size_t n);
 void kn_free(void *p);
 void kn_set_last_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+/* ---- background constants (omega_nu.c), shared with the device background ---- */
+double kn_kbtnu(double tcmb0);           /* k_B T_nu in eV (omega_nu_single.c:19-21) */
+double kn_rhocrit(double HubbleParam);   /* critical density, g cm^-3 */
+double kn_rho_conversion(void);          /* (eV/c)^4 -> g cm^-3, one species */
+double kn_omegag0(double tcmb0, double rhocrit); /* Omega_g at a = 1 */
+/* nu_degeneracies of init_omega_nu: equal masses folded into the first */
+void kn_nu_degeneracies(const double MNu[], int deg[]);
+/* rho_nu_init's table range [lo, hi] in log a; 0 when no table is built */
+int kn_rho_table_range(double a0, double mnu, double kBtnu, double *lo, double *hi);
 
 /* ---- host quadrature and interpolation (quad_host.c) ---- */
 typedef double (*kn_fn)(double x, void *p);

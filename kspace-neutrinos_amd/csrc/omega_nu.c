@@ -36,16 +36,39 @@ static double fd_energy_integrand(double q, void *vp)
     return q * q * eps * occ;
 }
 
+double kn_rho_conversion(void) { return rho_conversion(); }
+
+double kn_kbtnu(double tcmb0) { return KN_BOLEVK * tnu_over_tcmb() * tcmb0; }
+
+double kn_rhocrit(double HubbleParam)
+{
+    return (3 * KN_HUBBLE * HubbleParam * KN_HUBBLE * HubbleParam) / (8 * M_PI * KN_GRAVITY);
+}
+
+double kn_omegag0(double tcmb0, double rhocrit)
+{
+    return 4 * KN_STEFAN_BOLTZMANN / (KN_LIGHTCGS * KN_LIGHTCGS * KN_LIGHTCGS) * pow(tcmb0, 4) / rhocrit;
+}
+
+/* omega_nu_single.c:121-130: the table spans a0/1.2 to 1.2 NU_SW kT/m in a
+ * (the series takes over above NU_SW kT/m); none for a nearly massless
+ * species or an empty range. */
+int kn_rho_table_range(double a0, double mnu, double kBtnu, double *lo, double *hi)
+{
+    *lo = log(a0) - log(1.2);
+    *hi = log(NU_SW * kBtnu / mnu) + log(1.2);
+    return !(mnu < 1e-6 * kBtnu || *hi < *lo);
+}
+
 void rho_nu_init(_rho_nu_single *t, double a0, const double mnu, const double HubbleParam, const double kBtnu)
 {
-    const double lo = log(a0) - log(1.2);
-    const double hi = log(NU_SW * kBtnu / mnu) + log(1.2);
+    double lo, hi;
     int i;
     (void)HubbleParam;
     t->mnu = mnu;
     t->loga = t->rhonu = NULL;
     t->interp = t->acc = NULL;
-    if (mnu < 1e-6 * kBtnu || hi < lo) return;
+    if (!kn_rho_table_range(a0, mnu, kBtnu, &lo, &hi)) return;
     t->loga = kn_malloc("rho_nu_table", 2 * NRHOTAB * sizeof(double));
     t->rhonu = t->loga + NRHOTAB;
     t->interp = kn_malloc("rho_nu_spline", NRHOTAB * sizeof(double));
@@ -90,25 +113,32 @@ double rho_nu(_rho_nu_single *t, const double a, const double kT)
     }
 }
 
+/* species of equal mass (within FLOAT_ACC) are folded into the first one
+ * (omega_nu_single.c:27-40) */
+void kn_nu_degeneracies(const double MNu[], int deg[])
+{
+    int i;
+    for (i = 0; i < NUSPECIES; i++) {
+        int j;
+        deg[i] = 0;
+        for (j = 0; j < i; j++)
+            if (fabs(MNu[i] - MNu[j]) < KN_FLOAT_ACC) {
+                deg[j] += 1;
+                break;
+            }
+        if (j == i) deg[i] = 1;
+    }
+}
+
 void init_omega_nu(_omega_nu *om, const double MNu[], const double a0, const double HubbleParam,
                    const double tcmb0)
 {
     int i;
     memset(&om->hybnu, 0, sizeof om->hybnu);
     om->tcmb0 = tcmb0;
-    om->kBtnu = KN_BOLEVK * tnu_over_tcmb() * tcmb0;
-    om->rhocrit = (3 * KN_HUBBLE * HubbleParam * KN_HUBBLE * HubbleParam) / (8 * M_PI * KN_GRAVITY);
-    /* species of equal mass (within FLOAT_ACC) are folded into the first one */
-    for (i = 0; i < NUSPECIES; i++) {
-        int j;
-        om->nu_degeneracies[i] = 0;
-        for (j = 0; j < i; j++)
-            if (fabs(MNu[i] - MNu[j]) < KN_FLOAT_ACC) {
-                om->nu_degeneracies[j] += 1;
-                break;
-            }
-        if (j == i) om->nu_degeneracies[i] = 1;
-    }
+    om->kBtnu = kn_kbtnu(tcmb0);
+    om->rhocrit = kn_rhocrit(HubbleParam);
+    kn_nu_degeneracies(MNu, om->nu_degeneracies);
     for (i = 0; i < NUSPECIES; i++) {
         if (om->nu_degeneracies[i]) {
             om->RhoNuTab[i] = kn_malloc("RhoNuTab", sizeof(_rho_nu_single));
@@ -143,9 +173,7 @@ double get_omega_nu_nopart(const _omega_nu *om, const double a)
 
 double get_omegag(const _omega_nu *om, const double a)
 {
-    const double og = 4 * KN_STEFAN_BOLTZMANN / (KN_LIGHTCGS * KN_LIGHTCGS * KN_LIGHTCGS) * pow(om->tcmb0, 4) /
-                      om->rhocrit;
-    return og / pow(a, 4);
+    return kn_omegag0(om->tcmb0, om->rhocrit) / pow(a, 4);
 }
 
 double omega_nu_single(const _omega_nu *om, const double a, int i)

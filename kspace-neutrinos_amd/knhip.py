@@ -53,6 +53,13 @@ class KspaceParams(C.Structure):
                 ("hybrid_neutrinos_on", C.c_int), ("vcrit", C.c_double), ("nu_crit_time", C.c_double)]
 
 
+class BackgroundParams(C.Structure):
+    """kn_background_params: the background whose Omega_nu(a) and H(a) the device tabulates."""
+    _fields_ = [("Omega0", C.c_double), ("OmegaLambda", C.c_double), ("Hubble", C.c_double),
+                ("HubbleParam", C.c_double), ("tcmb0", C.c_double), ("a0", C.c_double),
+                ("MNu", C.c_double * NUSPECIES), ("radiation", C.c_int)]
+
+
 class BatchHistory(C.Structure):
     """kn_batch_history: one get_delta_nu integral of a group (hubble: C function address)."""
     _fields_ = [("nk", C.c_int), ("na", C.c_int), ("na_max", C.c_int), ("a", C.c_double), ("a_max", C.c_double),
@@ -133,6 +140,13 @@ SIGNATURES = {
     "kn_batch_history_from_table": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, C.c_double,
                                               C.POINTER(BatchHistory)]),
     "kn_hubble_ctx": (C.c_double, [C.c_double, C.c_void_p]),
+    "kn_background_create": (C.c_int, [vpp, C.POINTER(BackgroundParams)]),
+    "kn_background_destroy": (None, [C.c_void_p]),
+    "kn_background_omega_nu": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kn_background_hubble": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kn_background_hubble_ctx": (C.c_double, [C.c_double, C.c_void_p]),
+    "kn_background_rho_table": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
+    "kn_set_background": (C.c_int, [C.c_void_p]),
     "kn_group_create": (C.c_int, [vpp, C.c_int, C.POINTER(BatchHistory)]),
     "kn_group_destroy": (None, [C.c_void_p]),
     "kn_group_views": (C.c_int, [C.c_void_p] + [vpp] * 9),
@@ -211,6 +225,64 @@ def bind(L):
             f.restype = res
             f.argtypes = args
     return L
+
+
+class Background:
+    """kn_background: Omega_nu(a) and H(a) tabulated on the device from the
+    background's parameters (omega_nu_single.c:16-205; gadget-2 patch
+    0003:38-55).  hubble_fn / ctx go into a BatchHistory so that its H panels
+    are built on the device."""
+
+    def __init__(self, Omega0, HubbleParam, Hubble, masses, a0=0.01, tcmb0=2.7255, OmegaLambda=None,
+                 radiation=1, L=None):
+        self.L = L or lib()
+        p = BackgroundParams()
+        p.Omega0 = Omega0
+        p.OmegaLambda = 1.0 - Omega0 if OmegaLambda is None else OmegaLambda
+        p.Hubble, p.HubbleParam, p.tcmb0, p.a0, p.radiation = Hubble, HubbleParam, tcmb0, a0, radiation
+        for i in range(NUSPECIES):
+            p.MNu[i] = masses[i]
+        self.params = p
+        h = C.c_void_p()
+        _check(self.L.kn_background_create(C.byref(h), C.byref(p)), "kn_background_create", self.L)
+        self.ctx = h.value
+        self.hubble_fn = C.cast(self.L.kn_background_hubble_ctx, C.c_void_p).value
+
+    def hubble_host(self, a):
+        return self.L.kn_background_hubble_ctx(a, self.ctx)
+
+    def _eval(self, fn, a):
+        """a: float64 tensor on the device; returns the values in a new tensor."""
+        import torch
+        a = a.contiguous()
+        out = torch.empty_like(a)
+        _check(fn(self.ctx, a.numel(), a.data_ptr(), out.data_ptr(), None), "kn_background eval", self.L)
+        torch.cuda.synchronize()
+        return out
+
+    def omega_nu(self, a):
+        return self._eval(self.L.kn_background_omega_nu, a)
+
+    def hubble(self, a):
+        return self._eval(self.L.kn_background_hubble, a)
+
+    def rho_table(self, i):
+        """(loga, rhonu) of species i, or None when it has no table."""
+        import numpy as np
+        loga, rho = np.zeros(200), np.zeros(200)
+        n = self.L.kn_background_rho_table(self.ctx, i, loga.ctypes.data_as(dp), rho.ctypes.data_as(dp))
+        if n < 0:
+            raise RuntimeError("kn_background_rho_table: bad arguments")
+        return (loga, rho) if n else None
+
+    def set_process(self, on=True):
+        """Make this background the process's H (kn_set_background)."""
+        self.L.kn_set_background(self.ctx if on else None)
+
+    def close(self):
+        if self.ctx:
+            self.L.kn_background_destroy(self.ctx)
+            self.ctx = None
 
 
 class Batch:
@@ -502,7 +574,6 @@ class DeltaNuState:
         self._mc = memcpy or _memcpy
         hh = BatchHistory()
         _check(L.kn_batch_history_from_table(C.byref(table), a, mnu, C.byref(hh)), "history_from_table", L)
-        hh.hubble = C.cast(L.kn_hubble_ctx, C.c_void_p).value
         self.nk, self.na = hh.nk, hh.na
         self.batch = Batch([hh], L=L, memcpy=memcpy)
         logs = np.ctypeslib.as_array(table.scalefact, shape=(self.na,)).copy()

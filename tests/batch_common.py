@@ -24,9 +24,11 @@ class HCtx(C.Structure):
 
 
 class Cosmo:
-    """One cosmology on both sides: product omega_nu + per-history H context, oracle omega_nu."""
+    """One cosmology on both sides: product omega_nu + per-history H context, oracle omega_nu.
+    With background=L (a library build), the histories' H is a kn_background of L
+    (Omega_nu and the H panels tabulated on the device) instead of the host callback."""
 
-    def __init__(self, masses, hybrid=None, omega0=0.3):
+    def __init__(self, masses, hybrid=None, omega0=0.3, background=None):
         self.P = Pair(masses, omega0, 4, 2.0, hybrid=hybrid)
         self.masses = masses
         self.omega0 = omega0
@@ -35,6 +37,11 @@ class Cosmo:
         host.kn_test_hctx_init.argtypes = [C.POINTER(HCtx), C.c_void_p, C.c_double, C.c_double]
         host.kn_test_hctx_init(C.byref(self.ctx), C.addressof(self.P.om), omega0, UNIT_TIME)
         self.hfn = C.cast(host.kn_test_hubble_ctx, C.c_void_p).value
+        self.hctx = C.addressof(self.ctx)
+        self.bg = None
+        if background is not None:
+            self.bg = knhip.Background(omega0, 0.7, 3.24077929e-18 * UNIT_TIME, masses, a0=0.01, L=background)
+            self.hfn, self.hctx = self.bg.hubble_fn, self.bg.ctx
 
     def history(self, nk, na, na_max, a, a_max, mnu):
         """BatchHistory as get_delta_nu derives it (kn_batch_history_from_table on the product table)."""
@@ -44,7 +51,7 @@ class Cosmo:
         assert knhip.lib().kn_batch_history_from_table(C.byref(d), a, mnu, C.byref(hh)) == 0
         hh.nk, hh.na, hh.na_max, hh.a_max = nk, na, na_max, a_max
         hh.hubble = self.hfn
-        hh.hubble_ctx = C.addressof(self.ctx)
+        hh.hubble_ctx = self.hctx
         return hh
 
     def oracle(self, nk, logs, rows, init):
@@ -58,6 +65,8 @@ class Cosmo:
         return T
 
     def free(self):
+        if self.bg is not None:
+            self.bg.close()
         self.P.free()
 
 
@@ -89,10 +98,14 @@ def compare_adjoint(name, gpu, orc):
     assert e_f < RTOL and e_y < RTOL and e_i < RTOL and e_m < RTOL
 
 
-def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None, api="kn_group", streams=None):
-    """Histories over three cosmologies with mixed lengths and k counts, in one kn_group."""
+def mixed_batch(ncosmo_hists=16, seed=0, extra=0, L=None, memcpy=None, api="kn_group", streams=None,
+                background=False):
+    """Histories over three cosmologies with mixed lengths and k counts, in one kn_group
+    (background: their H from kn_background objects of the same library)."""
     rng = np.random.default_rng(seed)
-    cosmos = [Cosmo([0.15, 0, 0]), Cosmo([0.1, 0.1, 0.1]), Cosmo([0.3, 0.06, 0.06], hybrid=(500.0, 0.3333))]
+    bgl = (L or knhip.lib()) if background else None
+    cosmos = [Cosmo([0.15, 0, 0], background=bgl), Cosmo([0.1, 0.1, 0.1], background=bgl),
+              Cosmo([0.3, 0.06, 0.06], hybrid=(500.0, 0.3333), background=bgl)]
     specs = []
     for i in range(ncosmo_hists * len(cosmos)):
         c = i % len(cosmos)
@@ -120,7 +133,7 @@ def _sync(stream):
         torch.cuda.synchronize()
 
 
-def append_flow(L=None, memcpy=None, nk=6, steps=6, mode="eager", stream=None):
+def append_flow(L=None, memcpy=None, nk=6, steps=6, mode="eager", stream=None, background=False):
     """Histories whose derived state changes while steps are appended, each
     step compared with the oracle's get_delta_nu at the new a (mode "eager":
     kn_group_append then solve; "async": on the capacity plan,
@@ -131,9 +144,11 @@ def append_flow(L=None, memcpy=None, nk=6, steps=6, mode="eager", stream=None):
       2: hybrid with vcrit 1e5 km/s: past a = 0.3333 particles are nearly all
          the neutrinos and only the initial-condition piece remains;
       3: massless, then given a mass by set_mass before the last step.
+    background: the histories' H from kn_background objects of the same library.
     Returns the worst relative error over all steps."""
-    cosmos = [Cosmo([0.2, 0, 0]), Cosmo([0.3, 0, 0], hybrid=(500.0, 0.3333)),
-              Cosmo([0.3, 0, 0], hybrid=(1e5, 0.3333))]
+    bgl = (L or knhip.lib()) if background else None
+    cosmos = [Cosmo([0.2, 0, 0], background=bgl), Cosmo([0.3, 0, 0], hybrid=(500.0, 0.3333), background=bgl),
+              Cosmo([0.3, 0, 0], hybrid=(1e5, 0.3333), background=bgl)]
     rng = np.random.default_rng(17)
     a_start = [0.0101, 0.25, 0.25, 0.25]
     specs = []
