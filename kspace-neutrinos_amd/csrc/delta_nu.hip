@@ -929,36 +929,67 @@ __global__ __launch_bounds__(256) void k_bg_eval(BgDev b, int n, const double *_
     if (i < n) out[i] = which ? bg_hubble(b, a[i]) : bg_omega_nu(b, a[i]);
 }
 
-// H panels from the background, one lane per panel: both functions' fits on
+// One H table to build from a background: panels of width w from x0, split
+// where needed from panel split_from on (panels below it lie below the
+// integrals' lower limit and are never read).
+struct BgJob {
+    BgDev b;
+    double x0, w;
+    int npanel, split_from;
+    double *coef;  // npanel * kHStride, then kHSplitMax * kHSplitStride
+    double *miss;  // npanel
+    int *hdr;      // nsplit, then the split panels' indices
+};
+
+// One lane per panel (blockIdx.y: the table): both functions' fits on
 // [x0 + p w, x0 + (p + 1) w] and their miss against H.
-__global__ __launch_bounds__(64) void k_bg_panels(BgDev b, double x0, double w, int npanel, double *__restrict__ coef,
-                                                  double *__restrict__ miss)
+__global__ __launch_bounds__(64) void k_bg_panels(const BgJob *__restrict__ jobs)
 {
+    const BgJob &J = jobs[blockIdx.y];
     const int p = blockIdx.x * 64 + threadIdx.x;
-    if (p >= npanel) return;
-    const BgHub hub{b};
-    const double lo = x0 + p * w, hi = x0 + (p + 1) * w;
-    double *c = coef + (size_t)p * kHStride;
+    if (p >= J.npanel) return;
+    const BgHub hub{J.b};
+    const double lo = J.x0 + p * J.w, hi = J.x0 + (p + 1) * J.w;
+    double *c = J.coef + (size_t)p * kHStride;
     h_fit(lo, hi, hub, c);
-    miss[p] = h_fit_err(c, lo, hi, hub);
+    J.miss[p] = h_fit_err(c, lo, hi, hub);
 }
 
-// The split records: lane 0 keeps the kHSplitMax panels from split_from on
-// that miss most, in panel order as the host's HSplits::offer does (panels
-// below split_from lie below the integrals' lower limit and are never read);
-// one lane per kept panel then finds its split point and fits both pieces
-// after the regular panels.  hdr: nsplit, then the split panels' indices.
-__global__ __launch_bounds__(64) void k_bg_splits(BgDev b, double x0, double w, int npanel, int split_from,
-                                                  double *__restrict__ coef, const double *__restrict__ miss,
-                                                  int *__restrict__ hdr)
+// One wavefront per table: the panels that miss are gathered (a strided scan,
+// up to kCand of them); lane 0 keeps the kHSplitMax that miss most, offered
+// in panel order as the host's HSplits::offer does; the wavefront then splits
+// each kept panel (h_split_wave) into the records after the regular panels.
+constexpr int kCand = 64;
+__global__ __launch_bounds__(64) void k_bg_splits(const BgJob *__restrict__ jobs)
 {
-    __shared__ int sp_n;
-    __shared__ int sp_p[kHSplitMax];
-    if (threadIdx.x == 0) {
+    __shared__ int ncand, sp_n;
+    __shared__ int cand[kCand], sp_p[kHSplitMax];
+    __shared__ HFit2Lds L;
+    const BgJob &J = jobs[blockIdx.x];
+    const int lane = threadIdx.x;
+    if (lane == 0) ncand = 0;
+    __syncthreads();
+    for (int q = J.split_from + lane; q < J.npanel; q += 64)
+        if (J.miss[q] > kHSplitTol) {
+            const int i = atomicAdd(&ncand, 1);
+            if (i < kCand) cand[i] = q;
+        }
+    __syncthreads();
+    if (lane == 0) {
+        const bool all = ncand <= kCand;  // else every panel is offered again, in order
+        const int nc = all ? ncand : J.npanel - J.split_from;
+        if (all)
+            for (int i = 1; i < nc; i++)  // panel order
+                for (int j = i; j > 0 && cand[j - 1] > cand[j]; j--) {
+                    const int t = cand[j];
+                    cand[j] = cand[j - 1];
+                    cand[j - 1] = t;
+                }
         int n = 0, p[kHSplitMax];
         double err[kHSplitMax];
-        for (int q = split_from; q < npanel; q++) {
-            const double e = miss[q];
+        for (int c = 0; c < nc; c++) {
+            const int q = all ? cand[c] : J.split_from + c;
+            const double e = J.miss[q];
             if (!(e > kHSplitTol)) continue;
             int slot = n;
             if (n == kHSplitMax) {
@@ -973,49 +1004,86 @@ __global__ __launch_bounds__(64) void k_bg_splits(BgDev b, double x0, double w, 
             err[slot] = e;
         }
         sp_n = n;
-        hdr[0] = n;
+        J.hdr[0] = n;
         for (int i = 0; i < kHSplitMax; i++) {
             if (i < n) sp_p[i] = p[i];
-            hdr[1 + i] = i < n ? p[i] : -1;
+            J.hdr[1 + i] = i < n ? p[i] : -1;
         }
     }
     __syncthreads();
-    const int s = threadIdx.x;
-    if (s < sp_n) {
+    const BgHub hub{J.b};
+    for (int s = 0; s < sp_n; s++) {
         const int q = sp_p[s];
-        const BgHub hub{b};
-        double tmp[kHStride];
-        h_panel(x0 + q * w, x0 + (q + 1) * w, hub, tmp, coef + (size_t)npanel * kHStride + (size_t)s * kHSplitStride);
+        h_split_wave(J.x0 + q * J.w, J.x0 + (q + 1) * J.w, hub,
+                     J.coef + (size_t)J.npanel * kHStride + (size_t)s * kHSplitStride, L);
     }
 }
 
-// H panels of the background on [x0, x0 + npanel w) into d (npanel *
-// kHStride + kHSplitMax * kHSplitStride doubles of device memory), split
-// where needed from panel split_from on; H gets the table's description.
-// Synchronous (the split list is read back).
-void bg_htable(const kn_background *bg, double x0, double w, int npanel, int split_from, double *d, HTab &H,
-               hipStream_t st)
+// A table to build: the background, the panels, the device buffer (npanel *
+// kHStride + kHSplitMax * kHSplitStride doubles) and where its description goes.
+struct BgTableReq {
+    const kn_background *bg;
+    double x0, w;
+    int npanel, split_from;
+    double *coef;
+    HTab *out;
+};
+
+// All requested tables in two launches and one read-back (synchronous).
+void bg_htables(const std::vector<BgTableReq> &reqs, hipStream_t st)
 {
-    double *miss;
-    int *hdr;
-    KN_CHECK(hipMalloc(&miss, sizeof(double) * npanel));
-    KN_CHECK(hipMalloc(&hdr, sizeof(int) * (1 + kHSplitMax)));
-    hipLaunchKernelGGL(k_bg_panels, dim3((npanel + 63) / 64), dim3(64), 0, st, bg->dev, x0, w, npanel, d, miss);
-    KN_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_bg_splits, dim3(1), dim3(64), 0, st, bg->dev, x0, w, npanel, split_from, d,
-                       (const double *)miss, hdr);
-    KN_CHECK(hipGetLastError());
-    int h[1 + kHSplitMax];
-    KN_CHECK(hipMemcpyAsync(h, hdr, sizeof h, hipMemcpyDeviceToHost, st));
-    KN_CHECK(hipStreamSynchronize(st));
-    KN_CHECK(hipFree(miss));
-    KN_CHECK(hipFree(hdr));
-    H.coef = d;
-    H.x0 = x0;
-    H.inv_w = 1.0 / w;
-    H.npanel = npanel;
-    H.nsplit = h[0];
-    for (int i = 0; i < kHSplitMax; i++) H.split_p[i] = h[1 + i];
+    constexpr size_t kMaxY = 65535;  // tables per launch (grid y)
+    for (size_t base = 0; base < reqs.size(); base += kMaxY) {
+        const size_t n = std::min(kMaxY, reqs.size() - base);
+        size_t panels = 0;
+        int maxp = 0;
+        for (size_t i = 0; i < n; i++) {
+            panels += (size_t)reqs[base + i].npanel;
+            maxp = std::max(maxp, reqs[base + i].npanel);
+        }
+        BgJob *d_jobs;
+        double *miss;
+        int *hdr;
+        KN_CHECK(hipMalloc(&d_jobs, sizeof(BgJob) * n));
+        KN_CHECK(hipMalloc(&miss, sizeof(double) * std::max<size_t>(panels, 1)));
+        KN_CHECK(hipMalloc(&hdr, sizeof(int) * (1 + kHSplitMax) * n));
+        std::vector<BgJob> jobs(n);
+        size_t off = 0;
+        for (size_t i = 0; i < n; i++) {
+            const BgTableReq &r = reqs[base + i];
+            jobs[i].b = r.bg->dev;
+            jobs[i].x0 = r.x0;
+            jobs[i].w = r.w;
+            jobs[i].npanel = r.npanel;
+            jobs[i].split_from = r.split_from;
+            jobs[i].coef = r.coef;
+            jobs[i].miss = miss + off;
+            jobs[i].hdr = hdr + (1 + kHSplitMax) * i;
+            off += (size_t)r.npanel;
+        }
+        KN_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(BgJob) * n, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_bg_panels, dim3((maxp + 63) / 64, (unsigned)n), dim3(64), 0, st, (const BgJob *)d_jobs);
+        KN_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_bg_splits, dim3((unsigned)n), dim3(64), 0, st, (const BgJob *)d_jobs);
+        KN_CHECK(hipGetLastError());
+        std::vector<int> h((1 + kHSplitMax) * n);
+        KN_CHECK(hipMemcpyAsync(h.data(), hdr, sizeof(int) * h.size(), hipMemcpyDeviceToHost, st));
+        KN_CHECK(hipStreamSynchronize(st));
+        KN_CHECK(hipFree(d_jobs));
+        KN_CHECK(hipFree(miss));
+        KN_CHECK(hipFree(hdr));
+        for (size_t i = 0; i < n; i++) {
+            const BgTableReq &r = reqs[base + i];
+            HTab &H = *r.out;
+            const int *hi = h.data() + (1 + kHSplitMax) * i;
+            H.coef = r.coef;
+            H.x0 = r.x0;
+            H.inv_w = 1.0 / r.w;
+            H.npanel = r.npanel;
+            H.nsplit = hi[0];
+            for (int k = 0; k < kHSplitMax; k++) H.split_p[k] = hi[1 + k];
+        }
+    }
 }
 
 const kn_background *g_bg = nullptr;  // the process's H (kn_set_background)
@@ -1222,7 +1290,7 @@ static void h_from_background(double origin)
     }
     HTab H;
     const int p_origin = (int)llround((origin - g_h.x0) / g_h.width);
-    bg_htable(g_bg, g_h.x0, g_h.width, g_h.npanel, p_origin, g_h.d_coef, H, 0);
+    bg_htables({BgTableReq{g_bg, g_h.x0, g_h.width, g_h.npanel, p_origin, g_h.d_coef, &H}}, 0);
     KN_CHECK(hipMemcpy(g_h.coef.data(), g_h.d_coef, sizeof(double) * nreg, hipMemcpyDeviceToHost));
     g_h.splits.n = H.nsplit;
     for (int i = 0; i < H.nsplit; i++) {
@@ -2654,6 +2722,8 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
     kn_group *g = new kn_group();
     g->nhist = nhist;
     g->h.resize(nhist);
+    std::vector<BgTableReq> bg_reqs;  // H tables built from backgrounds, on the device
+    std::vector<int> bg_tab;          // their places in g->htabs
     for (int i = 0; i < nhist; i++) {
         GroupHist &q = g->h[i];
         q.spec = hist[i];
@@ -2697,9 +2767,11 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
             const int npanel = (int)ceil((q.xmax - q.x0) / kGroupHWidth) + 1;
             HTab H;
             double *d;
-            if (q.spec.hubble == kn_background_hubble_ctx) {  // built on the device from the background
+            if (q.spec.hubble == kn_background_hubble_ctx) {  // built on the device, all together below
                 d = group_alloc<double>((size_t)npanel * kHStride + (size_t)kHSplitMax * kHSplitStride);
-                bg_htable((const kn_background *)q.spec.hubble_ctx, q.x0, kGroupHWidth, npanel, 0, d, H, 0);
+                bg_reqs.push_back(BgTableReq{(const kn_background *)q.spec.hubble_ctx, q.x0, kGroupHWidth, npanel, 0,
+                                             d, nullptr});
+                bg_tab.push_back((int)g->htabs.size());
             } else {
                 std::vector<double> coef;
                 double (*hub)(double, void *) = q.spec.hubble;
@@ -2714,6 +2786,10 @@ extern "C" int kn_group_create(kn_group **out, int nhist, const kn_batch_history
             found = (int)g->htabs.size() - 1;
         }
         q.htab = found;
+    }
+    if (!bg_reqs.empty()) {
+        for (size_t j = 0; j < bg_reqs.size(); j++) bg_reqs[j].out = &g->htabs[bg_tab[j]];
+        bg_htables(bg_reqs, 0);
     }
     const size_t ny = (size_t)g->y_tot, nk = (size_t)g->nk_tot;
     g->sx = group_alloc<double>(g->s_tot);

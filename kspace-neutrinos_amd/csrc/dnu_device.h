@@ -377,6 +377,103 @@ struct BgHub {
     __host__ __device__ double operator()(double a) const { return bg_hubble(b, a); }
 };
 
+// ---- the split search of h_panel by one wavefront (device-built tables) ----
+// Two intervals' fits at once: lanes 0-19 sample H at the two intervals'
+// kHnc interpolation nodes while lanes 20-41 sample it at their 11 check
+// points (those of h_fit_err); lanes 0-39 then form one coefficient each
+// (interval, function, k) and lanes 20-41 the fits' misses.  Every lane gets
+// both intervals' largest misses (el, er; with check only).
+struct HFit2Lds {
+    double g[2][2][kHnc];  // samples: interval, function (1/(a^2 H), 1/(a H)), node
+    double c[2][2][kHnc];  // coefficients
+    double e[2][kHnc + 1];  // misses at the check points
+};
+
+template <typename Hub>
+__device__ void h_fit2_wave(double lo0, double hi0, double lo1, double hi1, const Hub &hub, HFit2Lds &L, bool check,
+                            double &el, double &er)
+{
+    const int lane = threadIdx.x & 63;
+    double f2 = 0.0, f1 = 0.0, t = 0.0;
+    if (lane < 2 * kHnc || (check && lane < 2 * kHnc + 2 * (kHnc + 1))) {
+        const int iv = lane < 2 * kHnc ? lane / kHnc : (lane - 2 * kHnc) / (kHnc + 1);
+        const double lo = iv ? lo1 : lo0, hi = iv ? hi1 : hi0;
+        const double half = 0.5 * (hi - lo), mid = lo + half;
+        if (lane < 2 * kHnc) {
+            t = cos(M_PI * (lane % kHnc + 0.5) / kHnc);
+        } else {
+            const int j = (lane - 2 * kHnc) % (kHnc + 1) - 1;
+            t = j < 0 ? -1.0 : (j == kHnc - 1 ? 1.0 : cos(M_PI * (j + 1) / kHnc));
+        }
+        const double x = mid + half * t;
+        const double h = hub(exp(x));
+        f2 = exp(-2.0 * x) / h;
+        f1 = exp(-x) / h;
+        if (lane < 2 * kHnc) {
+            L.g[iv][0][lane % kHnc] = f2;
+            L.g[iv][1][lane % kHnc] = f1;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < 4 * kHnc) {
+        const int iv = lane / (2 * kHnc), fn = (lane / kHnc) & 1, k = lane % kHnc;
+        double s = 0.0;
+        for (int j = 0; j < kHnc; j++) s += L.g[iv][fn][j] * cos(M_PI * k * (j + 0.5) / kHnc);
+        L.c[iv][fn][k] = (k == 0 ? 1.0 : 2.0) * s / kHnc;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (!check) return;
+    if (lane >= 2 * kHnc && lane < 2 * kHnc + 2 * (kHnc + 1)) {
+        const int iv = (lane - 2 * kHnc) / (kHnc + 1), jj = (lane - 2 * kHnc) % (kHnc + 1);
+        const double e2 = fabs(h_clenshaw(L.c[iv][0], t) / f2 - 1.0);
+        const double e1 = fabs(h_clenshaw(L.c[iv][1], t) / f1 - 1.0);
+        L.e[iv][jj] = fmax(e2, e1);
+    }
+    __builtin_amdgcn_wave_barrier();
+    el = 0.0;
+    er = 0.0;
+    for (int j = 0; j <= kHnc; j++) {
+        el = fmax(el, L.e[0][j]);
+        er = fmax(er, L.e[1][j]);
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// h_panel's split of [lo, hi] (bisection until both halves fit, then both
+// pieces' fits) by one wavefront into rec[kHSplitStride].  Every lane keeps
+// the same bracket, so the loop ends together.
+template <typename Hub>
+__device__ void h_split_wave(double lo, double hi, const Hub &hub, double *rec, HFit2Lds &L)
+{
+    double l = lo, r = hi, el, er;
+    for (;;) {
+        const double m = l + 0.5 * (r - l);
+        if (!(m > l && m < r)) break;
+        h_fit2_wave(l, m, m, r, hub, L, true, el, er);
+        if (el <= kHSplitTol && er <= kHSplitTol) {
+            l = r = m;
+            break;
+        }
+        if (el >= er) r = m;
+        else l = m;
+    }
+    const double xs = l + 0.5 * (r - l);
+    h_fit2_wave(lo, xs, xs, hi, hub, L, false, el, er);
+    const int lane = threadIdx.x & 63;
+    if (lane < 2 * kHStride) {
+        const int iv = lane / kHStride, i = lane % kHStride;
+        rec[1 + iv * kHPiece + 2 + i] = L.c[iv][i / kHnc][i % kHnc];
+    } else if (lane < 2 * kHStride + 2) {
+        const int iv = lane - 2 * kHStride;
+        const double plo = iv ? xs : lo, phi = iv ? hi : xs, half = 0.5 * (phi - plo);
+        rec[1 + iv * kHPiece] = plo + half;
+        rec[1 + iv * kHPiece + 1] = half > 0.0 ? 1.0 / half : 0.0;
+    } else if (lane == 2 * kHStride + 2) {
+        rec[0] = xs;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 // 1/d from the hardware estimate refined by one Newton step (relative error
 // a few units in the last place, against ten instructions for a division);
 // d finite and normal.

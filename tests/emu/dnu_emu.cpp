@@ -1,7 +1,7 @@
 // The library's per-call free-streaming kernels (kspace-neutrinos_amd/csrc/delta_nu.hip)
 // built for the CPU SIMT emulation (include/hip/hip_runtime.h).  Test infrastructure only.
 #include <hip/hip_runtime.h>
-unsigned emu_tid, emu_bid, emu_bdim, emu_gdim;
+unsigned emu_tid, emu_bid, emu_bdim, emu_gdim, emu_bid_y, emu_gdim_y = 1;
 EmuWaveSched *emu_wave;
 int emu_cus = 2, emu_per_cu = 1;
 double emu_dyn_lds[160 * 1024 / sizeof(double)];

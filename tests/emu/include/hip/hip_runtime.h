@@ -36,11 +36,11 @@ struct dim3 {
     dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
 };
 struct emu_idx { unsigned x, y, z; };
-extern unsigned emu_tid, emu_bid, emu_bdim, emu_gdim;
+extern unsigned emu_tid, emu_bid, emu_bdim, emu_gdim, emu_bid_y, emu_gdim_y;
 inline emu_idx emu_threadIdx() { return {emu_tid, 0, 0}; }
-inline emu_idx emu_blockIdx() { return {emu_bid, 0, 0}; }
+inline emu_idx emu_blockIdx() { return {emu_bid, emu_bid_y, 0}; }
 inline emu_idx emu_blockDim() { return {emu_bdim, 1, 1}; }
-inline emu_idx emu_gridDim() { return {emu_gdim, 1, 1}; }
+inline emu_idx emu_gridDim() { return {emu_gdim, emu_gdim_y, 1}; }
 #define threadIdx emu_threadIdx()
 #define blockIdx emu_blockIdx()
 #define blockDim emu_blockDim()
@@ -277,12 +277,19 @@ void emu_launch(void (*k)(KA...), dim3 g, dim3 b, size_t, hipStream_t, A... args
         fprintf(stderr, "emulation: block size %u above %d\n", b.x, kEmuMaxThreads);
         abort();
     }
+    if (b.y != 1 || b.z != 1 || g.z != 1) {
+        fprintf(stderr, "emulation: blocks must be one-dimensional and grids at most two-dimensional\n");
+        abort();
+    }
     emu_bdim = b.x;
     emu_gdim = g.x;
+    emu_gdim_y = g.y;
     auto body = [&]() { k(args...); };
-    for (unsigned bx = 0; bx < g.x; bx++) {
-        emu_bid = bx;
-        emu_run_block((int)b.x, &emu_call<decltype(body)>, &body);
-    }
+    for (unsigned by = 0; by < g.y; by++)
+        for (unsigned bx = 0; bx < g.x; bx++) {
+            emu_bid = bx;
+            emu_bid_y = by;
+            emu_run_block((int)b.x, &emu_call<decltype(body)>, &body);
+        }
 }
 #define hipLaunchKernelGGL(k, g, b, sh, st, ...) emu_launch(k, dim3(g), dim3(b), sh, st, ##__VA_ARGS__)

@@ -106,3 +106,28 @@ def test_emu_append_flow_on_a_background(emu):
     worst = append_flow(L=emu, memcpy=host_memcpy, mode="eager", background=True)
     print("  worst rel err over the steps: %.2e" % worst)
     assert worst < RTOL
+
+
+def test_emu_process_table_from_a_background(emu):
+    """The per-call H table (ln T0 - 20 to ln T0 + 8, 7168 panels) built from
+    a background on the device: the panels read back on the host within
+    1e-12 of the oracle's H over the integrals' range, across both masses'
+    switches to the non-relativistic series (split panels)."""
+    masses = [0.06, 0.06, 0.3]
+    bg = knhip.Background(0.3, 0.7, H0, masses, a0=0.01, L=emu)
+    om = O.omega_nu(masses, a0=0.01)
+    O.lib().orc_init_hubble(C.byref(om), 0.3, UNIT_TIME)
+    emu.kn_htable_eval_host.restype = C.c_double
+    emu.kn_htable_eval_host.argtypes = [C.c_double]
+    emu.kn_htable_ensure.argtypes = [C.c_double, C.c_double, C.c_double]
+    x0 = np.log(0.01)
+    bg.set_process(True)
+    try:
+        emu.kn_htable_ensure(x0, x0, 0.0)
+        xs = np.concatenate([np.random.default_rng(2).uniform(x0, 0.0, 3000), np.log(_grid()[_grid() >= 0.01])])
+        e = max(abs(emu.kn_htable_eval_host(x) / O.lib().orc_hubble(np.exp(x)) - 1) for x in xs)
+    finally:
+        bg.set_process(False)
+    print("  H panels from the background: max rel err %.2e" % e)
+    assert e < 1e-12
+    bg.close()
