@@ -221,7 +221,7 @@ def build_histories(kn, nhist, rng, draw):
     return keep, hists, data
 
 
-def plain_vs_group(kn, torch, keep, hists, data, reps=3, seed=5):
+def plain_vs_group(kn, torch, keep, hists, data, reps=3, seed=5, background=False):
     """The same histories through the per-history path (get_delta_nu, then
     kn_get_delta_nu_adjoint, per history; host tables) and through one kn_group
     (solve + adjoint, eager and as a replayed graph; inputs resident in HBM)."""
@@ -291,6 +291,31 @@ def plain_vs_group(kn, torch, keep, hists, data, reps=3, seed=5):
     dev = B.delta_nu()
     diff = max(float(np.max(np.abs(dev[B.off_k[i]:B.off_k[i + 1]] / plain_out[i] - 1))) for i in range(nhist))
     B.close()
+    bgres = None
+    if background:  # the same histories with H from device backgrounds (Omega_nu and H tabulated on the GPU)
+        t0 = time.perf_counter()
+        bgs, hb = [], []
+        for hh, x in zip(hists, data):
+            bg = knhip.Background(x["om0"], 0.7, 3.24077929e-18 * conftest.UNIT_TIME, [x["m"], 0.0, 0.0], a0=0.01)
+            h2 = knhip.BatchHistory.from_buffer_copy(hh)
+            h2.hubble, h2.hubble_ctx = bg.hubble_fn, bg.ctx
+            bgs.append(bg)
+            hb.append(h2)
+        t1 = time.perf_counter()
+        Bb = knhip.Batch(hb, keep=bgs, api="kn_group")
+        t2 = time.perf_counter()
+        for i, x in enumerate(data):
+            Bb.load(i, x["logs"], x["rows"], x["k"], x["init"])
+        Bb.solve()
+        db = Bb.delta_nu()
+        bgres = {"create_s": round(t2 - t0, 4), "backgrounds_s": round(t1 - t0, 4),
+                 "max_rel_diff_vs_host_h": float(np.max(np.abs(db / dev - 1))),
+                 "note": "one kn_background per history (its density table by 200 device QAGs) and the group's "
+                         "H tables built on the device in two launches, against create with the host's H "
+                         "sampled per history"}
+        Bb.close()
+        for bg in bgs:
+            bg.close()
     steps = pm_steps(kn, torch, keep, hists, data, plain_out, g_all)
     for om, d, ctx in keep:
         kn.free_delta_tot_table(C.byref(d))
@@ -311,6 +336,7 @@ def plain_vs_group(kn, torch, keep, hists, data, reps=3, seed=5):
             "speedup_forward_adjoint": round((t_fwd + t_adj) / min(tfa, tgr), 2),
             "speedup_forward": round(t_fwd / tf, 2),
             "max_rel_diff_group_vs_plain": diff,
+            "create_on_device_background": bgres,
             "pm_step": steps}
 
 
@@ -418,7 +444,7 @@ def c4_batching(kn, torch, nhist=256, seed=1000, reps=3, no_cpu=False):
     draw = lambda r: (int(r.integers(100, 3001)), int(r.integers(64, 513)), float(r.uniform(0.06, 0.6)),
                       float(r.uniform(0.25, 0.35)))
     keep, hists, data = build_histories(kn, nhist, np.random.default_rng(seed), draw)
-    res = plain_vs_group(kn, torch, keep, hists, data, reps)
+    res = plain_vs_group(kn, torch, keep, hists, data, reps, background=True)
     if not no_cpu:
         res["cpu_baseline"] = c4_cpu_baseline(data)
     res["workload"] = ("%d histories, Na~U[100,3000], nk~U[64,512], m~U[0.06,0.6] eV, Omega_m~U[0.25,0.35]; "
