@@ -117,27 +117,31 @@ __device__ inline void fs_node_wave(const HTab &H, int nfs, double x0, double xa
 }
 
 // The fslength nodes in two passes with the result of one QAG per node.
-// Pass 1 (k_fs_first, no LDS): four nodes per wavefront, 16 lanes each
-// forming the first 61-point Gauss-Kronrod estimate over [x_i, log a]
-// exactly as wave_qag61's first step does, and accepting it by the same test
-// (QUADPACK returns that estimate when it already meets the tolerance); a node
-// that needs bisection is appended to a list.  Pass 2 (k_fs_rest) runs the
-// full wavefront QAG for the listed nodes.  Node values are those of one
-// wave_qag61 per node, bit for bit.  `push(node)` lists a node for pass 2.
+// Pass 1 (k_fs_first, no LDS): sixteen nodes per wavefront, four lanes
+// each (gk61_q4), forming the first 61-point Gauss-Kronrod estimate over
+// [x_i, log a] exactly as wave_qag61's first step does, and accepting it by
+// the same test (QUADPACK returns that estimate when it already meets the
+// tolerance); a node that needs bisection is appended to a list.  Pass 2
+// (k_fs_rest) runs the full wavefront QAG for the listed nodes.  Node values
+// are those of one wave_qag61 per node, bit for bit.  The 16 lanes of a
+// quarter hold 16 neighbouring nodes ((log a - log T0) / (16 Na - 1) apart,
+// far below a panel's width), so their points at one abscissa lie in one or
+// two H panels and a coefficient load touches a few cache lines, not one per
+// lane.  `push(node)` lists a node for pass 2.
 template <class Push>
-__device__ inline void fs_first_wave(const HTab &H, int nfs, double x0, double xa, double light,
-                                     double *__restrict__ fx, double *__restrict__ fy, int *__restrict__ status,
-                                     int wave, const Push &push)
+__device__ inline void fs_first_q4(const HTab &H, int nfs, double x0, double xa, double light,
+                                   double *__restrict__ fx, double *__restrict__ fy, int *__restrict__ status,
+                                   int wave, const Push &push)
 {
-    if (4 * wave >= nfs) return;  // wave-uniform
+    if (16 * wave >= nfs) return;  // wave-uniform
     const int lane = threadIdx.x & 63;
-    const int node = 4 * wave + (lane >> 4);  // four nodes per wavefront, 16 lanes each
+    const int node = 16 * wave + (lane & 15);
     // delta_tot_table.c:582: log T0 + i (log a - log T0) / (Nfs - 1.)
     const double xi = node < nfs ? x0 + node * (xa - x0) / (nfs - 1.) : xa;
     FslIntegrand f;
     f.H = H;
-    const GK g = gk61_quad(f, xi < xa ? xi : xa, xa);
-    if ((lane & 15) == 0 && node < nfs) {
+    const GK g = gk61_q4(f, xi < xa ? xi : xa, xa);
+    if (lane < 16 && node < nfs) {
         double v = 0.0;
         if (xi < xa) {  // fslength returns 0 when logai >= logaf (:402)
             const double tol = 1e-6 * fabs(g.res);
@@ -167,7 +171,7 @@ __global__ __launch_bounds__(256) void k_fs_first(HTab H, int nfs, double x0, do
                                                   int *__restrict__ count)
 {
     const ListPush push = {list, count};
-    fs_first_wave(H, nfs, x0, xa, light, fx, fy, status, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), push);
+    fs_first_q4(H, nfs, x0, xa, light, fx, fy, status, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), push);
 }
 
 // Pass 2: full QAG for the listed nodes (a bounded loop over the list).
@@ -186,7 +190,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_fs_rest(HTab H, int nfs
 
 constexpr int kFsRestBlocks = 64;
 
-inline int fs_first_blocks(int nfs) { return (nfs + 15) / 16; }  // 4 waves x 4 nodes per block
+inline int fs_first_blocks(int nfs) { return (nfs + 63) / 64; }  // 4 waves x 16 nodes per block
 
 // Rows of natural-spline coefficients (row r: x[0..n), y + r*ystride,
 // c + r*cstride): one wavefront per (row, window of kKeep rows).
@@ -357,7 +361,7 @@ __device__ inline DnuIntegrand dnu_integrand(const QagArgs &a, const SpeciesDev 
     f.H = a.H;
     f.fy = a.fy;
     f.fc = a.fc;
-    f.fk.x0 = a.x0;  // the fslength knots' own expression (fs_first_wave)
+    f.fk.x0 = a.x0;  // the fslength knots' own expression (fs_first_q4)
     f.fk.span = a.xa - a.x0;
     f.fk.den = a.nfs - 1.;
     f.nfs = a.nfs;
@@ -609,7 +613,7 @@ __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
     const double kk = A.wavenum[k], mnubykT = A.sp->mnubykT, gb = A.gbar[item];
     const JPar &J = A.sp->J;
     const double fx0 = fx[0], finv_h = (nfs - 1) / (fx[nfs - 1] - fx[0]), sx0 = sx[0];
-    UniformKnots fk;  // the fslength knots (fs_first_wave's expression)
+    UniformKnots fk;  // the fslength knots (fs_first_q4's expression)
     fk.x0 = A.x0;
     fk.span = A.xa - A.x0;
     fk.den = nfs - 1.;
@@ -1902,6 +1906,21 @@ constexpr int kAdjClasses = 4;
 constexpr int kPhases = kPhAdj0 + kAdjClasses;
 constexpr int kMergedRestBlocks = 1280;
 
+// The hardware deals the blocks of a grid to the 8 XCDs in turn (block b to
+// XCD b mod 8), each XCD with its own L2.  Merged launches renumber their
+// blocks so that each XCD runs one contiguous range of logical blocks, and so
+// of histories: a history's tables are then read into one L2, not all eight.
+// A bijection of [0, gridDim.x): XCD x takes logical blocks
+// [x (q + 1), (x + 1)(q + 1)) for x < r, the next q-long ranges after, with
+// n = 8 q + r blocks.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block()
+{
+    const int n = gridDim.x, b = blockIdx.x;
+    const int x = b % kXcds, i = b / kXcds, q = n / kXcds, r = n % kXcds;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 __device__ __forceinline__ int hist_of(const int *__restrict__ pre, int nhist, int blk)
 {
     int lo = 0, hi = nhist;  // pre[lo] <= blk < pre[hi]
@@ -1924,12 +1943,13 @@ struct GroupPush {
 __global__ __launch_bounds__(256) void k_m_fs_first(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
                                                     int nhist, long long *__restrict__ list, int *__restrict__ count)
 {
-    const int h = hist_of(pre, nhist, blockIdx.x);
+    const int blk = xcd_block();
+    const int h = hist_of(pre, nhist, blk);
     const GroupDesc &d = D[h];
     if (!d.integ) return;
     const GroupPush push = {list, count, h};
-    fs_first_wave(d.H, d.nfs, d.x0, d.xa, d.light, d.fx, d.fy, d.status,
-                  (int)(((blockIdx.x - pre[h]) * blockDim.x + threadIdx.x) >> 6), push);
+    fs_first_q4(d.H, d.nfs, d.x0, d.xa, d.light, d.fx, d.fy, d.status,
+                (int)(((blk - pre[h]) * blockDim.x + threadIdx.x) >> 6), push);
 }
 
 __global__ __launch_bounds__(64 * kWavesPerBlock) void k_m_fs_rest(const GroupDesc *__restrict__ D,
@@ -1949,28 +1969,31 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_m_fs_rest(const GroupDe
 __global__ __launch_bounds__(256) void k_m_factor(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
                                                   int nhist)
 {
-    const int h = hist_of(pre, nhist, blockIdx.x);
+    const int blk = xcd_block();
+    const int h = hist_of(pre, nhist, blk);
     const GroupDesc &d = D[h];
     if (!d.integ) return;
-    factor_block(d.sx, d.na, d.fac, d.fac + (size_t)fac_windows(d.na) * kFacStride, blockIdx.x - pre[h]);
+    factor_block(d.sx, d.na, d.fac, d.fac + (size_t)fac_windows(d.na) * kFacStride, blk - pre[h]);
 }
 
 __global__ __launch_bounds__(256) void k_m_rows_prep(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
                                                      int nhist)
 {
-    const int h = hist_of(pre, nhist, blockIdx.x);
+    const int blk = xcd_block();
+    const int h = hist_of(pre, nhist, blk);
     const GroupDesc &d = D[h];
     if (!d.integ) return;
     rows_prep_block(d.sx, d.Y, d.C, d.na, d.nk, (long long)d.na_max, d.bsp, d.nsb, d.sinv_bw, d.bucket, d.fac,
-                    blockIdx.x - pre[h]);
+                    blk - pre[h]);
 }
 
 __global__ __launch_bounds__(256) void k_m_fs_spline(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
                                                      int nhist)
 {
-    const int h = hist_of(pre, nhist, blockIdx.x);
+    const int blk = xcd_block();
+    const int h = hist_of(pre, nhist, blk);
     const GroupDesc &d = D[h];
-    const long long item = (long long)(blockIdx.x - pre[h]) * 4 + (threadIdx.x >> 6);
+    const long long item = (long long)(blk - pre[h]) * 4 + (threadIdx.x >> 6);
     if (!d.integ || item >= (d.nfs + kKeep - 1) / kKeep) return;  // wave-uniform
     spline_item_wave(d.fx, d.fy, d.fc, d.nfs, 0LL, 0LL, item);
 }
@@ -1980,10 +2003,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_m_dnu_qag(const GroupDe
 {
     __shared__ QagWs ws[kWavesPerBlock];
     HIP_DYNAMIC_SHARED(double, sxs)  // min(largest na, kStageMax) doubles
-    const int h = hist_of(pre, nhist, blockIdx.x);
+    const int blk = xcd_block();
+    const int h = hist_of(pre, nhist, blk);
     const GroupDesc &d = D[h];
     const int wave = threadIdx.x >> 6;
-    const int k = (blockIdx.x - pre[h]) * kWavesPerBlock + wave;
+    const int k = (blk - pre[h]) * kWavesPerBlock + wave;
     if (!d.integ || k >= d.nk) return;
     QagArgs a;
     a.H = d.H;
@@ -2016,9 +2040,10 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_m_dnu_qag(const GroupDe
 // initial-condition-only histories, 64 k per block
 __global__ __launch_bounds__(64) void k_m_ic(const GroupDesc *__restrict__ D, const int *__restrict__ pre, int nhist)
 {
-    const int h = hist_of(pre, nhist, blockIdx.x);
+    const int blk = xcd_block();
+    const int h = hist_of(pre, nhist, blk);
     const GroupDesc &d = D[h];
-    const int k = (blockIdx.x - pre[h]) * 64 + threadIdx.x;
+    const int k = (blk - pre[h]) * 64 + threadIdx.x;
     if (d.integ || k >= d.nk) return;
     IcArgs ic;
     ic.init = d.init;
@@ -2044,9 +2069,10 @@ __device__ inline IcAdjArgs group_ica(const GroupDesc &d)
 __global__ __launch_bounds__(64) void k_m_ic_adj(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
                                                  int nhist)
 {
-    const int h = hist_of(pre, nhist, blockIdx.x);
+    const int blk = xcd_block();
+    const int h = hist_of(pre, nhist, blk);
     const GroupDesc &d = D[h];
-    const int k0 = (blockIdx.x - pre[h]) * 64;
+    const int k0 = (blk - pre[h]) * 64;
     if (d.integ) return;
     if (k0 + (int)threadIdx.x < d.nk) ic_adjoint_at(k0 + threadIdx.x, d.wavenum, d.gbar, *d.sp, group_ica(d));
     for (int r = 0; r < 64 && k0 + r < d.nk; r++) {
@@ -2059,7 +2085,8 @@ __global__ __launch_bounds__(64) void k_m_dnu_adj(const GroupDesc *__restrict__ 
                                                   int nhist)
 {
     HIP_DYNAMIC_SHARED(double, lds)  // adj_lds_bytes of the class's longest part
-    const int h = hist_of(pre, nhist, blockIdx.x);
+    const int blk = xcd_block();
+    const int h = hist_of(pre, nhist, blk);
     const GroupDesc &d = D[h];
     if (!d.integ) return;
     AdjArgs A;
@@ -2092,16 +2119,17 @@ __global__ __launch_bounds__(64) void k_m_dnu_adj(const GroupDesc *__restrict__ 
     A.nparts = adj_parts(d.na);
     A.plen = adj_plen(d.na);
     A.ica = group_ica(d);
-    dnu_adj_block(A, blockIdx.x - pre[h], lds);
+    dnu_adj_block(A, blk - pre[h], lds);
 }
 
 __global__ __launch_bounds__(256) void k_m_adj_solve(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
                                                      int nhist)
 {
-    const int h = hist_of(pre, nhist, blockIdx.x);
+    const int blk = xcd_block();
+    const int h = hist_of(pre, nhist, blk);
     const GroupDesc &d = D[h];
     if (!d.integ) return;
-    adj_solve_block(d.sx, d.na, d.nk, (long long)d.na_max, d.cbar, d.dbar, d.fac, blockIdx.x - pre[h]);
+    adj_solve_block(d.sx, d.na, d.nk, (long long)d.na_max, d.cbar, d.dbar, d.fac, blk - pre[h]);
 }
 
 // One more stored step per history, on the device (one wavefront per

@@ -229,7 +229,14 @@ __host__ __device__ __forceinline__ double h_cheb(const double *coef, double x0,
 // a split panel (index compared with the table's list, no load) takes the
 // out-of-line pieces.
 typedef const __attribute__((address_space(1))) double gconst_double;
-__device__ __forceinline__ double h_cheb_dev(const HTab &H, double x, int which)
+
+// x's panel p (clamped to the table), its split record s (-1: none) and the
+// local coordinate t in [-1, 1]; one expression for every evaluation form.
+struct HLoc {
+    int p, s;
+    double t;
+};
+__device__ __forceinline__ HLoc h_locate(const HTab &H, double x)
 {
     const double u = (x - H.x0) * H.inv_w;
     int p = (int)floor(u);
@@ -238,10 +245,23 @@ __device__ __forceinline__ double h_cheb_dev(const HTab &H, double x, int which)
 #pragma unroll
     for (int i = 0; i < kHSplitMax; i++)
         if (i < H.nsplit && p == H.split_p[i]) s = i;
-    if (s >= 0)
-        return h_split_eval((gconst_double *)(H.coef + (size_t)H.npanel * kHStride + (size_t)s * kHSplitStride), x,
-                            which);
-    return h_clenshaw((gconst_double *)(H.coef + (size_t)p * kHStride + which * kHnc), 2.0 * (u - p) - 1.0);
+    HLoc L;
+    L.p = p;
+    L.s = s;
+    L.t = 2.0 * (u - p) - 1.0;
+    return L;
+}
+
+__device__ __forceinline__ gconst_double *h_split_rec(const HTab &H, int s)
+{
+    return (gconst_double *)(H.coef + (size_t)H.npanel * kHStride + (size_t)s * kHSplitStride);
+}
+
+__device__ __forceinline__ double h_cheb_dev(const HTab &H, double x, int which)
+{
+    const HLoc L = h_locate(H, x);
+    if (L.s >= 0) return h_split_eval(h_split_rec(H, L.s), x, which);
+    return h_clenshaw((gconst_double *)(H.coef + (size_t)L.p * kHStride + which * kHnc), L.t);
 }
 
 __device__ __forceinline__ double inv_a2h(const HTab &H, double x) { return h_cheb_dev(H, x, 0); }
@@ -705,7 +725,7 @@ struct GK {
 template <class F>
 __device__ __forceinline__ void gk61_pair(const F &f, double a1, double b1, double a2, double b2, GK &g1, GK &g2)
 {
-#pragma clang fp contract(off)  // every product and sum rounded as written (gk61_quad's sums match bit for bit)
+#pragma clang fp contract(off)  // every product and sum rounded as written (gk61_q4's sums match bit for bit)
     const int lane = threadIdx.x & 63, hf = lane >> 5, j = lane & 31;
     const double a = hf ? a2 : a1, b = hf ? b2 : b1;
     const double c = 0.5 * (a + b), h = 0.5 * (b - a);
@@ -740,43 +760,61 @@ __device__ __forceinline__ void gk61_pair(const F &f, double a1, double b1, doub
     g2.resasc = __shfl(r.resasc, 32, 64);
 }
 
-// 61-point Gauss-Kronrod on [a, b] of this lane's 16-lane group (four
-// intervals per wavefront); the result is valid in every lane of the group.
-// Lane j of a group takes the nodes j and j + 16 of gk61_pair's 32-lane
-// layout and adds their terms itself, which is the xor-16 step of half_sum:
-// the sums, and so the results, are gk61_pair's bit for bit.
-__device__ __forceinline__ double quarter_sum(double v)
-{
-#pragma unroll
-    for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
-
+// The first 61-point Gauss-Kronrod estimate on [a, b] by four lanes of a
+// wavefront, lanes l, l ^ 16, l ^ 32, l ^ 48 (quarter q = (l >> 4) & 3; the
+// other 15 lanes of each quarter take 15 other intervals, so that lanes of
+// one load instruction evaluate nearby points of nearby intervals).  Quarter
+// q evaluates gk61_pair's lanes j and j + 16 for j = q, q + 4, q + 8, q + 12
+// and adds their terms as gk61_pair's butterfly does (xor 16, 8 and 4 within
+// the quarter), then xor 2 and xor 1 across quarters (lane xor 32, 16): the
+// sums, and so the estimate, are gk61_pair's bit for bit, in all four lanes.
 template <class F>
-__device__ __forceinline__ GK gk61_quad(const F &f, double a, double b)
+__device__ __forceinline__ GK gk61_q4(const F &f, double a, double b)
 {
 #pragma clang fp contract(off)  // as gk61_pair
-    const int j0 = threadIdx.x & 15, j1 = j0 + 16;
+    const int q = (threadIdx.x >> 4) & 3;
     const double c = 0.5 * (a + b), h = 0.5 * (b - a);
-    const double dx0 = h * c_xgk[j0];
-    const double f01 = f(c - dx0), f02 = f(c + dx0);
-    double f11 = 0.0, f12 = 0.0;
-    if (j1 <= 30) {
-        const double dx1 = h * c_xgk[j1];
-        f11 = f(c - dx1);
-        if (j1 < 30) f12 = f(c + dx1);
+    double f01[4], f02[4], f11[4], f12[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int j0 = q + 4 * r, j1 = j0 + 16;
+        const double dx0 = h * c_xgk[j0];
+        const double dx1 = j1 <= 30 ? h * c_xgk[j1] : 0.0;
+        f01[r] = f(c - dx0);
+        f02[r] = f(c + dx0);
+        f11[r] = j1 <= 30 ? f(c - dx1) : 0.0;
+        f12[r] = j1 < 30 ? f(c + dx1) : 0.0;
     }
-    const double wk0 = c_wgk[j0], wk1 = j1 <= 30 ? c_wgk[j1] : 0.0;
-    const double wg0 = (j0 & 1) ? c_wg[j0 >> 1] : 0.0;
-    const double wg1 = (j1 < 30 && (j1 & 1)) ? c_wg[j1 >> 1] : 0.0;
-    const double sk = quarter_sum(wk0 * (f01 + f02) + (j1 == 30 ? wk1 * f11 : wk1 * (f11 + f12)));
-    const double sg = quarter_sum(wg0 * (f01 + f02) + wg1 * (f11 + f12));
-    const double sa = quarter_sum(wk0 * (fabs(f01) + fabs(f02)) +
-                                  (j1 == 30 ? wk1 * fabs(f11) : wk1 * (fabs(f11) + fabs(f12))));
+    double tk[4], tg[4], ta[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int j0 = q + 4 * r, j1 = j0 + 16;
+        const double wk0 = c_wgk[j0], wk1 = j1 <= 30 ? c_wgk[j1] : 0.0;
+        const double wg0 = (j0 & 1) ? c_wg[j0 >> 1] : 0.0;
+        const double wg1 = (j1 < 30 && (j1 & 1)) ? c_wg[j1 >> 1] : 0.0;
+        tk[r] = wk0 * (f01[r] + f02[r]) + (j1 == 30 ? wk1 * f11[r] : wk1 * (f11[r] + f12[r]));
+        tg[r] = wg0 * (f01[r] + f02[r]) + wg1 * (f11[r] + f12[r]);
+        ta[r] = wk0 * (fabs(f01[r]) + fabs(f02[r])) +
+                (j1 == 30 ? wk1 * fabs(f11[r]) : wk1 * (fabs(f11[r]) + fabs(f12[r])));
+    }
+    // xor 8 (j, j + 8: r, r + 2), xor 4 (j, j + 4: r 0, 1), then across quarters
+    auto sum4 = [](const double(&t)[4]) {
+        double v = (t[0] + t[2]) + (t[1] + t[3]);
+        v += __shfl_xor(v, 32, 64);
+        v += __shfl_xor(v, 16, 64);
+        return v;
+    };
+    const double sk = sum4(tk), sg = sum4(tg), sa = sum4(ta);
     const double mean = sk * 0.5;
-    const double sasc = quarter_sum(wk0 * (fabs(f01 - mean) + fabs(f02 - mean)) +
-                                    (j1 == 30 ? wk1 * fabs(f11 - mean)
-                                              : (j1 < 30 ? wk1 * (fabs(f11 - mean) + fabs(f12 - mean)) : 0.0)));
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int j0 = q + 4 * r, j1 = j0 + 16;
+        const double wk0 = c_wgk[j0], wk1 = j1 <= 30 ? c_wgk[j1] : 0.0;
+        tk[r] = wk0 * (fabs(f01[r] - mean) + fabs(f02[r] - mean)) +
+                (j1 == 30 ? wk1 * fabs(f11[r] - mean)
+                          : (j1 < 30 ? wk1 * (fabs(f11[r] - mean) + fabs(f12[r] - mean)) : 0.0));
+    }
+    const double sasc = sum4(tk);
     GK r;
     r.res = sk * h;
     r.resabs = sa * fabs(h);
