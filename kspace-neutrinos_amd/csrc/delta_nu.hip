@@ -918,11 +918,20 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_bg_rho(BgBuild g, int *
 }
 
 // The tables' natural cubic splines (gsl_interp_init of omega_nu_single.c:151),
-// one lane per table; work: 3 kRhoTab doubles per table.
-__global__ __launch_bounds__(64) void k_bg_spline(BgBuild g, double *work)
+// one wavefront per table (block t): the wavefront stages the table in LDS,
+// lane 0 runs the elimination there, and the coefficients go out coalesced.
+__global__ __launch_bounds__(64) void k_bg_spline(BgBuild g)
 {
-    const int t = threadIdx.x;
-    if (t < g.nt) bg_spline_init(g.loga[t], g.rho[t], kRhoTab, g.c2[t], work + (size_t)t * 3 * kRhoTab);
+    __shared__ double xs[kRhoTab], ys[kRhoTab], cs[kRhoTab], work[3 * kRhoTab];
+    const int t = blockIdx.x;
+    for (int i = threadIdx.x; i < kRhoTab; i += 64) {
+        xs[i] = g.loga[t][i];
+        ys[i] = g.rho[t][i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) bg_spline_init(xs, ys, kRhoTab, cs, work);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kRhoTab; i += 64) g.c2[t][i] = cs[i];
 }
 
 // Omega_nu(a) (which 0) or H(a) (which 1) at n points.
@@ -1146,9 +1155,7 @@ extern "C" int kn_background_create(kn_background **out, const kn_background_par
     BgDev d = b;
     if (g.nt) {
         KN_CHECK(hipMalloc(&bg->d_tab, sizeof(double) * bg->tab.size()));
-        double *work;
         int *status;
-        KN_CHECK(hipMalloc(&work, sizeof(double) * 3 * kRhoTab * g.nt));
         KN_CHECK(hipMalloc(&status, sizeof(int)));
         KN_CHECK(hipMemset(status, 0, sizeof(int)));
         for (int t = 0; t < g.nt; t++) {
@@ -1160,12 +1167,11 @@ extern "C" int kn_background_create(kn_background **out, const kn_background_par
         hipLaunchKernelGGL(k_bg_rho, dim3(blocks_for((long long)g.nt * kRhoTab)), dim3(64 * kWavesPerBlock), 0, 0, g,
                            status);
         KN_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_bg_spline, dim3(1), dim3(64), 0, 0, g, work);
+        hipLaunchKernelGGL(k_bg_spline, dim3(g.nt), dim3(64), 0, 0, g);
         KN_CHECK(hipGetLastError());
         int st = 0;
         KN_CHECK(hipMemcpy(&st, status, sizeof(int), hipMemcpyDeviceToHost));
         KN_CHECK(hipMemcpy(bg->tab.data(), bg->d_tab, sizeof(double) * bg->tab.size(), hipMemcpyDeviceToHost));
-        KN_CHECK(hipFree(work));
         KN_CHECK(hipFree(status));
         if (st) {
             KN_CHECK(hipFree(bg->d_tab));
