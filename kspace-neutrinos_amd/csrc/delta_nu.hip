@@ -271,10 +271,11 @@ __device__ inline void bucket_at(const double *__restrict__ x, int n, double inv
 // factor_block) and the bucket index in one launch: blocks [0, b_sp) four
 // row windows each (none when na <= 2), the rest the bucket.  Rows of Y and
 // C are `ystride` doubles apart.
+// lds: kTriLds doubles per wavefront of the block
 __device__ inline void rows_prep_block(const double *__restrict__ sx, const double *__restrict__ Y,
                                        double *__restrict__ C, int na, int nk, long long ystride, int b_sp, int nsb,
                                        double sinv_bw, int *__restrict__ bucket, const double *__restrict__ fac,
-                                       int b)
+                                       int b, double *lds)
 {
     if (b < b_sp) {
         const int nwin = fac_windows(na);
@@ -282,10 +283,8 @@ __device__ inline void rows_prep_block(const double *__restrict__ sx, const doub
         if (item < (long long)nwin * nk) {  // wave-uniform
             const int r = (int)(item / nwin), w = (int)(item % nwin);
             const int lo = w * kKeep, hi = lo + kKeep < na ? lo + kKeep : na;
-            SplineRhsIh rhs;
-            rhs.ih = fac + (size_t)nwin * kFacStride;
-            rhs.y = Y + r * ystride;
-            tridiag_window_wave_fac(sx, na, rhs, lo, hi, C + r * ystride, fac);
+            tri_window_staged<false>(sx, na, Y + r * ystride, lo, hi, C + r * ystride, fac,
+                                     lds + (threadIdx.x >> 6) * kTriLds);
         }
         return;
     }
@@ -293,12 +292,15 @@ __device__ inline void rows_prep_block(const double *__restrict__ sx, const doub
     if (q <= nsb) bucket_at(sx, na, sinv_bw, bucket, q);
 }
 
-__global__ __launch_bounds__(256) void k_rows_prep(const double *__restrict__ sx, const double *__restrict__ Y,
+// three wavefronts per SIMD (168 registers, a 20-byte spill): 0.91 ms against
+// 1.06 ms at two for the 256-history group's rows
+__global__ __launch_bounds__(256, 3) void k_rows_prep(const double *__restrict__ sx, const double *__restrict__ Y,
                                                    double *__restrict__ C, int na, int nk, long long ystride,
                                                    int b_sp, int nsb, double sinv_bw, int *__restrict__ bucket,
                                                    const double *__restrict__ fac)
 {
-    rows_prep_block(sx, Y, C, na, nk, ystride, b_sp, nsb, sinv_bw, bucket, fac, blockIdx.x);
+    __shared__ double lds[4 * kTriLds];
+    rows_prep_block(sx, Y, C, na, nk, ystride, b_sp, nsb, sinv_bw, bucket, fac, blockIdx.x, lds);
 }
 
 inline int rows_prep_blocks(int b_sp, int nsb) { return b_sp + (nsb + 256) / 256; }
@@ -688,25 +690,26 @@ __global__ __launch_bounds__(64) void k_dnu_adj(AdjArgs A)
 // onto ybar in the same pass.
 // Rows `stride` doubles apart, on the stored factors of x; block `b` of
 // spline_blocks(n, nrows).
+// lds: kTriLds doubles per wavefront of the block
 __device__ inline void adj_solve_block(const double *__restrict__ x, int n, int nrows, long long stride,
                                        const double *__restrict__ cbar, double *__restrict__ ybar,
-                                       const double *__restrict__ fac, int b)
+                                       const double *__restrict__ fac, int b, double *lds)
 {
     const int nwin = (n + kKeep - 1) / kKeep;
     const long long item = (long long)b * 4 + (threadIdx.x >> 6);
     if (item >= (long long)nwin * nrows) return;  // wave-uniform
     const int r = (int)(item / nwin), w = (int)(item % nwin);
     const int lo = w * kKeep, hi = lo + kKeep < n ? lo + kKeep : n;
-    ArrayRhs rhs;
-    rhs.r = cbar + (size_t)r * stride;
-    tridiag_window_wave_fac<ArrayRhs, true>(x, n, rhs, lo, hi, ybar + (size_t)r * stride, fac);
+    tri_window_staged<true>(x, n, cbar + (size_t)r * stride, lo, hi, ybar + (size_t)r * stride, fac,
+                            lds + (threadIdx.x >> 6) * kTriLds);
 }
 
 __global__ __launch_bounds__(256) void k_adj_solve(const double *__restrict__ x, int n, int nrows, long long stride,
                                                    const double *__restrict__ cbar, double *__restrict__ ybar,
                                                    const double *__restrict__ fac)
 {
-    adj_solve_block(x, n, nrows, stride, cbar, ybar, fac, blockIdx.x);
+    __shared__ double lds[4 * kTriLds];
+    adj_solve_block(x, n, nrows, stride, cbar, ybar, fac, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------- state
@@ -1982,15 +1985,16 @@ __global__ __launch_bounds__(256) void k_m_factor(const GroupDesc *__restrict__ 
     factor_block(d.sx, d.na, d.fac, d.fac + (size_t)fac_windows(d.na) * kFacStride, blk - pre[h]);
 }
 
-__global__ __launch_bounds__(256) void k_m_rows_prep(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
+__global__ __launch_bounds__(256, 3) void k_m_rows_prep(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
                                                      int nhist)
 {
     const int blk = xcd_block();
     const int h = hist_of(pre, nhist, blk);
     const GroupDesc &d = D[h];
     if (!d.integ) return;
+    __shared__ double lds[4 * kTriLds];
     rows_prep_block(d.sx, d.Y, d.C, d.na, d.nk, (long long)d.na_max, d.bsp, d.nsb, d.sinv_bw, d.bucket, d.fac,
-                    blk - pre[h]);
+                    blk - pre[h], lds);
 }
 
 __global__ __launch_bounds__(256) void k_m_fs_spline(const GroupDesc *__restrict__ D, const int *__restrict__ pre,
@@ -2135,7 +2139,8 @@ __global__ __launch_bounds__(256) void k_m_adj_solve(const GroupDesc *__restrict
     const int h = hist_of(pre, nhist, blk);
     const GroupDesc &d = D[h];
     if (!d.integ) return;
-    adj_solve_block(d.sx, d.na, d.nk, (long long)d.na_max, d.cbar, d.dbar, d.fac, blk - pre[h]);
+    __shared__ double lds[4 * kTriLds];
+    adj_solve_block(d.sx, d.na, d.nk, (long long)d.na_max, d.cbar, d.dbar, d.fac, blk - pre[h], lds);
 }
 
 // One more stored step per history, on the device (one wavefront per

@@ -1378,10 +1378,22 @@ __device__ __forceinline__ void tri_factor(const double *x, const TriLane &t, do
 // transpose of the natural-spline right-hand side's y-derivative onto out
 // (the second half of the spline adjoint) for the kept rows, with the
 // reciprocal knot spacings ih (as SplineRhsIh's).
-template <class RHS, bool ADJ_RHS>
-__device__ __forceinline__ void tri_solve(const double *x, int n, const RHS &rhs, int lo, int hi, double *out,
+// OUT: where the kept rows go (set for the solution, add for ADJ_RHS); IH:
+// the reciprocal spacings.  Each kept row receives exactly one set or add.
+struct OutDirect {
+    double *o;
+    __device__ __forceinline__ void set(int i, double v) const { o[i] = v; }
+    __device__ __forceinline__ void add(int i, double v) const { o[i] += v; }
+};
+struct IhDirect {
+    const double *ih;
+    __device__ __forceinline__ double operator()(int i) const { return ih[i]; }
+};
+
+template <class RHS, bool ADJ_RHS, class OUT = OutDirect, class IH = IhDirect>
+__device__ __forceinline__ void tri_solve(const double *x, int n, const RHS &rhs, int lo, int hi, const OUT &out,
                                           const TriLane &t, const double (&g)[kRun], const double (&ial)[kRun],
-                                          double g_before, const double *ih)
+                                          double g_before, const IH &ih)
 {
     const int lane = threadIdx.x & 63;
     const int wa = t.wa, wb = t.wb, s = t.s, len = t.len;
@@ -1445,12 +1457,12 @@ __device__ __forceinline__ void tri_solve(const double *x, int n, const RHS &rhs
             if (j < len) {
                 const int i = s + j;
                 c = zz[j] - g[j] * c;
-                if (i >= lo && i < hi) out[i] = c;
+                if (i >= lo && i < hi) out.set(i, c);
             }
         }
         if (lane == 0) {
-            if (lo == 0) out[0] = 0.0;
-            if (lo <= n - 1 && n - 1 < hi) out[n - 1] = 0.0;
+            if (lo == 0) out.set(0, 0.0);
+            if (lo <= n - 1 && n - 1 < hi) out.set(n - 1, 0.0);
         }
         return;
     }
@@ -1476,19 +1488,19 @@ __device__ __forceinline__ void tri_solve(const double *x, int n, const RHS &rhs
         const int i = s + j;
         if (j < len && i >= lo && i < hi) {
             const double zl = j > 0 ? zv[j - 1] : zp, zr = j + 1 < len ? zv[j + 1] : zn;
-            const double wl = 3.0 * ih[i - 1], wr = 3.0 * ih[i];
+            const double wl = 3.0 * ih(i - 1), wr = 3.0 * ih(i);
             double acc = 0.0;
             if (i - 1 >= 1) acc += zl * wl;
             acc -= zv[j] * (wr + wl);
             if (i + 1 <= n - 2) acc += zr * wr;
-            out[i] += acc;
+            out.add(i, acc);
         }
     }
-    if (lane == 0 && lo == 0 && len > 0) out[0] += zv[0] * (3.0 * ih[0]);  // row 0: the z_1 term
+    if (lane == 0 && lo == 0 && len > 0) out.add(0, zv[0] * (3.0 * ih(0)));  // row 0: the z_1 term
     if (lo <= n - 1 && n - 1 < hi) {                                         // row n-1: the z_{n-2} term
 #pragma unroll
         for (int j = 0; j < kRun; j++)
-            if (j < len && s + j == n - 2) out[n - 1] += zv[j] * (3.0 * ih[n - 2]);
+            if (j < len && s + j == n - 2) out.add(n - 1, zv[j] * (3.0 * ih(n - 2)));
     }
 }
 
@@ -1507,7 +1519,7 @@ __device__ inline void tridiag_window_wave(const double *x, int n, const RHS &rh
     tri_factor(x, t, g, al, gb);
 #pragma unroll
     for (int j = 0; j < kRun; j++) al[j] = 1.0 / al[j];
-    tri_solve<RHS, false>(x, n, rhs, lo, hi, out, t, g, al, gb, nullptr);
+    tri_solve<RHS, false>(x, n, rhs, lo, hi, OutDirect{out}, t, g, al, gb, IhDirect{nullptr});
 }
 
 // Factors of the knots x[0..n) stored once for every row that shares them:
@@ -1533,11 +1545,80 @@ __device__ inline void tri_factor_store(const double *x, int n, int w, double *f
     f[2 * kRun * 64 + lane] = gb;
 }
 
+// A window's rows staged in LDS: the span [wa - 1, wb] of a window and its
+// two neighbours, twice (kTriLds doubles per wavefront).
+constexpr int kTriSpan = 64 * kRun + 2;
+constexpr int kTriLds = 2 * kTriSpan;
+constexpr int kTriPass = (kTriSpan + 63) / 64;  // 64-wide passes over a span
+
+// A wavefront's LDS writes ordered before its lanes' reads of them (LDS
+// operations of one wavefront complete in order; the fences keep the
+// compiler from moving accesses across).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// dst[k] = src[k] for k < cnt (cnt <= kTriSpan) by the 64 lanes: every load
+// issued before the first LDS store, so that the passes' loads are in flight
+// together.
+__device__ __forceinline__ void stage_span(double *dst, const double *__restrict__ src, int cnt)
+{
+    const int lane = threadIdx.x & 63;
+    double v[kTriPass];
+#pragma unroll
+    for (int p = 0; p < kTriPass; p++) {
+        const int k = p * 64 + lane;
+        v[p] = k < cnt ? src[k] : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < kTriPass; p++) {
+        const int k = p * 64 + lane;
+        if (k < cnt) dst[k] = v[p];
+    }
+}
+
+// SplineRhsIh's values, bit for bit, from staged y and ih (ys[k] = y[base + k]).
+struct SplineRhsLds {
+    const double *ys, *ihs;
+    int base;
+    __device__ __forceinline__ double operator()(int i) const
+    {
+        const int k = i - base;
+        return 3.0 * ((ys[k + 1] - ys[k]) * ihs[k] - (ys[k] - ys[k - 1]) * ihs[k - 1]);
+    }
+};
+struct ArrayRhsLds {
+    const double *rs;
+    int base;
+    __device__ __forceinline__ double operator()(int i) const { return rs[i - base]; }
+};
+struct OutLds {  // kept row i at o[i - lo]; add stores the row's one contribution
+    double *o;
+    int lo;
+    __device__ __forceinline__ void set(int i, double v) const { o[i - lo] = v; }
+    __device__ __forceinline__ void add(int i, double v) const { o[i - lo] = v; }
+};
+struct IhLds {
+    const double *ihs;
+    int base;
+    __device__ __forceinline__ double operator()(int i) const { return ihs[i - base]; }
+};
+
 // tridiag_window_wave for window w = lo / kKeep with the stored factors
-// (fac_windows(n) windows, then the reciprocal spacings).
-template <class RHS, bool ADJ_RHS = false>
-__device__ inline void tridiag_window_wave_fac(const double *x, int n, const RHS &rhs, int lo, int hi, double *out,
-                                               const double *fac)
+// (fac_windows(n) windows, then the reciprocal spacings), the window staged
+// through the wavefront's LDS (lds: kTriLds doubles): lane l solves rows
+// s .. s + kRun - 1, kRun rows from its neighbours', so read from global
+// memory directly each load's 64 lanes would touch ~40 cache lines (and a
+// CU's wavefronts together more lines than its L1 holds); staged, the rows
+// come in and go out by loads and stores of consecutive doubles across the
+// lanes.  src: the y row (natural-spline coefficients set on out) or, with
+// ADJ_RHS, the right-hand side (its adjoint added onto out).
+template <bool ADJ_RHS>
+__device__ inline void tri_window_staged(const double *x, int n, const double *__restrict__ src, int lo, int hi,
+                                         double *__restrict__ out, const double *__restrict__ fac, double *lds)
 {
     const int lane = threadIdx.x & 63;
     if (n < 3) {
@@ -1547,14 +1628,48 @@ __device__ inline void tridiag_window_wave_fac(const double *x, int n, const RHS
     }
     const TriLane t = tri_lane(n, lo, hi);
     const double *f = fac + (size_t)(lo / kKeep) * kFacStride;
+    const double *ih = fac + (size_t)fac_windows(n) * kFacStride;
+    double *sa = lds, *sb = lds + kTriSpan;
+    const int base = t.wa - 1;
+    // ih[wa - 1 .. wb - 1]; y[wa - 1 .. wb] or rhs[wa .. wb - 1]
+    stage_span(sb, ih + base, t.wb - base);
+    if (ADJ_RHS) stage_span(sa, src + t.wa, t.wb - t.wa);
+    else stage_span(sa, src + base, t.wb - base + 1);
+    wave_lds_sync();
     double g[kRun], ial[kRun];
 #pragma unroll
     for (int j = 0; j < kRun; j++) {
         g[j] = f[j * 64 + lane];
         ial[j] = f[(kRun + j) * 64 + lane];
     }
-    tri_solve<RHS, ADJ_RHS>(x, n, rhs, lo, hi, out, t, g, ial, f[2 * kRun * 64 + lane],
-                            fac + (size_t)fac_windows(n) * kFacStride);
+    const double gb = f[2 * kRun * 64 + lane];
+    const OutLds o{sa, lo};  // the staged rows are all read before the first kept row is written
+    if (ADJ_RHS)
+        tri_solve<ArrayRhsLds, true>(x, n, ArrayRhsLds{sa, t.wa}, lo, hi, o, t, g, ial, gb, IhLds{sb, base});
+    else
+        tri_solve<SplineRhsLds, false>(x, n, SplineRhsLds{sa, sb, base}, lo, hi, o, t, g, ial, gb,
+                                       IhLds{sb, base});
+    wave_lds_sync();
+    const int cnt = hi - lo;
+    if (ADJ_RHS) {  // out += the staged contributions, every load issued first
+        double v[kTriPass];
+#pragma unroll
+        for (int p = 0; p < kTriPass; p++) {
+            const int k = p * 64 + lane;
+            v[p] = k < cnt ? out[lo + k] : 0.0;
+        }
+#pragma unroll
+        for (int p = 0; p < kTriPass; p++) {
+            const int k = p * 64 + lane;
+            if (k < cnt) out[lo + k] = v[p] + sa[k];
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < kTriPass; p++) {
+            const int k = p * 64 + lane;
+            if (k < cnt) out[lo + k] = sa[k];
+        }
+    }
 }
 
 // Right-hand side of the natural-spline system with the reciprocal knot

@@ -108,6 +108,8 @@ inline double __shfl_down(double v, unsigned d, int = 64)
 }
 // a wavefront sync point here (every lane arrives before any goes on)
 inline void __builtin_amdgcn_wave_barrier() { emu_yield(); }
+// memory fences: lanes run one at a time on one host thread, so ordering holds already
+inline void __builtin_amdgcn_fence(int, const char *) {}
 inline unsigned long long __ballot(int pred)
 {
     EmuWaveSched &S = *emu_wave;
