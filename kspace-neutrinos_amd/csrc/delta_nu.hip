@@ -1493,6 +1493,19 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
         KN_CHECK(hipMalloc(&dev->ivl, sizeof(double) * (size_t)nitems * kLimit * 2));
         dev->cap_ivl = nitems;
     }
+    // the fslength nodes and their spline need only (T0, a): they start
+    // first, and run while the host finds and packs the missing columns
+    const HTab H = device_htab();
+    KN_CHECK(hipMemsetAsync(dev->status, 0, sizeof(int) * 2, 0));
+    int *fs_count = dev->fs_list + dev->cap_fs;
+    KN_CHECK(hipMemsetAsync(fs_count, 0, sizeof(int), 0));
+    hipLaunchKernelGGL(k_fs_first, dim3(fs_first_blocks(nfs)), dim3(256), 0, 0, H, nfs, x0, xa, d->light, dev->fx,
+                       dev->fy, dev->status, dev->fs_list, fs_count);
+    hipLaunchKernelGGL(k_fs_rest, dim3(kFsRestBlocks), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, d->light,
+                       dev->fx, dev->fy, dev->status, dev->fs_list, fs_count);
+    hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, 0, dev->fx, dev->fy, dev->fc, nfs, 1, 0LL,
+                       0LL);
+    KN_CHECK(hipGetLastError());
     // which columns of the history the device lacks: per row, the first entry
     // that differs from the shadow (entries past the shadow's count differ)
     {
@@ -1534,18 +1547,6 @@ static int dnu_solve(const _delta_tot_table *d, double a, const double *wavenum,
     double *res = (double *)(stage + b_in);
     int *nint = (int *)(stage + b_in + align64(sizeof(double) * nitems)), *status = nint + nitems;
 
-    // the fslength nodes and their spline need only (T0, a): they start now
-    const HTab H = device_htab();
-    KN_CHECK(hipMemsetAsync(dev->status, 0, sizeof(int) * 2, 0));
-    int *fs_count = dev->fs_list + dev->cap_fs;
-    KN_CHECK(hipMemsetAsync(fs_count, 0, sizeof(int), 0));
-    hipLaunchKernelGGL(k_fs_first, dim3(fs_first_blocks(nfs)), dim3(256), 0, 0, H, nfs, x0, xa, d->light, dev->fx,
-                       dev->fy, dev->status, dev->fs_list, fs_count);
-    hipLaunchKernelGGL(k_fs_rest, dim3(kFsRestBlocks), dim3(64 * kWavesPerBlock), 0, 0, H, nfs, x0, xa, d->light,
-                       dev->fx, dev->fy, dev->status, dev->fs_list, fs_count);
-    hipLaunchKernelGGL(k_spline, dim3(spline_blocks(nfs, 1)), dim3(256), 0, 0, dev->fx, dev->fy, dev->fc, nfs, 1, 0LL,
-                       0LL);
-    KN_CHECK(hipGetLastError());
     // meanwhile the missing columns are packed (ncol per row) and uploaded on
     // the copy stream into rows of stride namax, in chunks of rows: a chunk's
     // upload runs while the next is packed
