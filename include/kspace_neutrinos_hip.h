@@ -456,6 +456,17 @@ int kn_group_status(kn_group *g);
  * nk * 400 doubles and nk ints. */
 int kn_group_subdivision(kn_group *g, int h, double *ivl, int *nint);
 int kn_group_set_mass(kn_group *g, int h, double mnu);
+/* (f)2: history h's stored steps to a host table (delta_tot rows, scalefact,
+ * delta_nu_init, wavenum, nk and ia set; delta_nu_last gets the last solve's
+ * delta_nu when allocated) and from one (same nk, at most na_max steps,
+ * integrated to a >= the last stored step from then on, re-derived as after
+ * an append), so that the reference's state files (save_all_nu_state /
+ * read_all_nu_state, delta_tot_table.c:254-352) and get_nu_state /
+ * set_nu_state (interface_common.c:150-180) checkpoint a device-resident
+ * group.  3002 for a table that does not fit or a non-increasing history,
+ * 3004 for a outside [last stored step, the H table's end]. */
+int kn_group_export(kn_group *g, int h, _delta_tot_table *d);
+int kn_group_import(kn_group *g, int h, const _delta_tot_table *d, double a);
 /* Pick the launch strategy by timing the forward: merged launches (0) or
  * per-history sequences on 1, 2, 4 or 8 streams. */
 int kn_group_autotune(kn_group *g, void *stream);

@@ -3103,6 +3103,94 @@ extern "C" int kn_group_set_mass(kn_group *g, int h, double mnu)
     return 0;
 }
 
+// History h's stored steps to a host table and back (the table's own layout:
+// rows delta_tot[k], scalefact, delta_nu_init, wavenum), so that the
+// reference's state files (save_all_nu_state / read_all_nu_state,
+// delta_tot_table.c:254-352) and flat blocks (get_nu_state / set_nu_state,
+// interface_common.c:150-180) checkpoint a device-resident group.  Export
+// also gives delta_nu_last the last solve's delta_nu.
+extern "C" int kn_group_export(kn_group *g, int h, _delta_tot_table *d)
+{
+    if (!g || !d || h < 0 || h >= g->nhist) return KN_ERR_ARG;
+    group_pull(g);
+    const GroupHist &q = g->h[h];
+    const int nk = q.spec.nk, na = q.na, na_max = q.spec.na_max;
+    if (d->nk_allocated < nk || d->namax < na || !d->scalefact || !d->delta_tot || !d->delta_nu_init ||
+        !d->wavenum) {
+        kn_set_last_error("kn_group_export: the table holds fewer than %d k or %d steps", nk, na);
+        return KN_ERR_ARG;
+    }
+    KN_CHECK(hipDeviceSynchronize());
+    std::vector<double> rows((size_t)nk * na);
+    KN_CHECK(hipMemcpy2D(rows.data(), sizeof(double) * na, g->Y + q.off_y, sizeof(double) * na_max,
+                         sizeof(double) * na, nk, hipMemcpyDeviceToHost));
+    for (int k = 0; k < nk; k++) memcpy(d->delta_tot[k], rows.data() + (size_t)k * na, sizeof(double) * na);
+    KN_CHECK(hipMemcpy(d->scalefact, g->sx + q.off_s, sizeof(double) * na, hipMemcpyDeviceToHost));
+    KN_CHECK(hipMemcpy(d->delta_nu_init, g->init + q.off_k, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    KN_CHECK(hipMemcpy(d->wavenum, g->wavenum + q.off_k, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    if (d->delta_nu_last)
+        KN_CHECK(hipMemcpy(d->delta_nu_last, g->out + q.off_k, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    d->nk = nk;
+    d->ia = na;
+    return 0;
+}
+
+// A host table's stored steps into history h (same nk, at most na_max steps),
+// integrated to a from then on: the history's derived state (tolerance, qc,
+// IC-only, the initial-condition factors) follows as after an append.
+extern "C" int kn_group_import(kn_group *g, int h, const _delta_tot_table *d, double a)
+{
+    if (!g || !d || h < 0 || h >= g->nhist) return KN_ERR_ARG;
+    group_pull(g);
+    GroupHist &q = g->h[h];
+    const int nk = q.spec.nk, na = d->ia, na_max = q.spec.na_max;
+    if (d->nk != nk || na < 1 || na > na_max || !d->scalefact || !d->delta_tot || !d->delta_nu_init ||
+        !d->wavenum) {
+        kn_set_last_error("kn_group_import: history %d: the table has %d k and %d steps (the history %d k, at most "
+                          "%d steps)", h, d->nk, na, nk, na_max);
+        return KN_ERR_ARG;
+    }
+    const double xa = log(a);
+    for (int i = 1; i < na; i++)
+        if (!(d->scalefact[i] > d->scalefact[i - 1])) {
+            kn_set_last_error("kn_group_import: history %d: stored log a not increasing at step %d", h, i);
+            return KN_ERR_ARG;
+        }
+    if (!(a > q.spec.time_transfer) || !(xa >= d->scalefact[na - 1]) || !(xa <= q.xmax)) {
+        kn_set_last_error("kn_group_import: history %d: a = %g before the last stored step or outside the H table",
+                          h, a);
+        return KN_ERR_RANGE;
+    }
+    KN_CHECK(hipDeviceSynchronize());
+    std::vector<double> rows((size_t)nk * na);
+    for (int k = 0; k < nk; k++) memcpy(rows.data() + (size_t)k * na, d->delta_tot[k], sizeof(double) * na);
+    KN_CHECK(hipMemcpy2D(g->Y + q.off_y, sizeof(double) * na_max, rows.data(), sizeof(double) * na,
+                         sizeof(double) * na, nk, hipMemcpyHostToDevice));
+    KN_CHECK(hipMemcpy(g->sx + q.off_s, d->scalefact, sizeof(double) * na, hipMemcpyHostToDevice));
+    KN_CHECK(hipMemcpy(g->init + q.off_k, d->delta_nu_init, sizeof(double) * nk, hipMemcpyHostToDevice));
+    KN_CHECK(hipMemcpy(g->wavenum + q.off_k, d->wavenum, sizeof(double) * nk, hipMemcpyHostToDevice));
+    const GroupHist keep = q;
+    q.na = na;
+    q.spec.na = na;
+    q.spec.a = a;
+    q.xa = xa;
+    group_derive(q);
+    group_set_integ(q);
+    const int rc = group_refresh_ic(q);
+    if (rc) {
+        q = keep;
+        return rc;
+    }
+    const SpeciesDev sd = species_dev(q.spec);
+    KN_CHECK(hipMemcpy(g->sp + h, &sd, sizeof sd, hipMemcpyHostToDevice));
+    group_plan(g);
+    if (!g->cap_plan) {
+        group_schedule(g);
+        g->gen++;
+    }
+    return 0;
+}
+
 // Pick the launch strategy by timing the forward: merged launches (0) or
 // per-history launch sequences on 1, 2, 4 or 8 streams.
 extern "C" int kn_group_autotune(kn_group *g, void *stream)

@@ -140,6 +140,8 @@ SIGNATURES = {
     "kn_batch_history_from_table": (C.c_int, [C.POINTER(DeltaTotTable), C.c_double, C.c_double,
                                               C.POINTER(BatchHistory)]),
     "kn_hubble_ctx": (C.c_double, [C.c_double, C.c_void_p]),
+    "kn_group_export": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(DeltaTotTable)]),
+    "kn_group_import": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(DeltaTotTable), C.c_double]),
     "kn_background_create": (C.c_int, [vpp, C.POINTER(BackgroundParams)]),
     "kn_background_destroy": (None, [C.c_void_p]),
     "kn_background_omega_nu": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -253,3 +253,83 @@ def compare_subdivisions(B, cosmos, specs):
             items += n > 0
             total += n
     return items, total
+
+
+def checkpoint_flow(tmp_path, L=None, memcpy=None):
+    """(f)2 on a group: history states out to host tables (kn_group_export),
+    through the reference's state file (save_all_nu_state / read_all_nu_state,
+    7 significant digits) and back into another group (kn_group_import).
+    Returns (worst error of the file round trip against the oracle on the
+    reloaded values, whether the direct export -> import round trip solved to
+    the same delta_nu bit for bit)."""
+    L = L or knhip.lib()
+    kn = knhip.lib()
+    cosmos, specs, B = mixed_batch(2, seed=4, extra=3, L=L, memcpy=memcpy)
+    B.solve()
+    # a second group on the same histories, loaded with other rows
+    hists = [cosmos[s["c"]].history(s["nk"], s["na"], s["na"] + 3, s["amax"], 1.0, s["m"]) for s in specs]
+    B2 = knhip.Batch(hists, keep=cosmos, L=L, memcpy=memcpy)
+    for i, s in enumerate(specs):
+        B2.load(i, s["logs"], s["rows"] * 1.5, s["k"], s["init"])
+    worst, same = 0.0, True
+    for i, s in enumerate(specs):
+        c = cosmos[s["c"]]
+        d = knhip.DeltaTotTable()
+        kn.allocate_delta_tot_table(C.byref(d), s["nk"], 0.01, 2.0, c.omega0, C.byref(c.P.om), UNIT_TIME,
+                                    UNIT_LENGTH, 0)
+        assert L.kn_group_export(B.h, i, C.byref(d)) == 0
+        sf, dt, init = knhip.table_views(d)
+        na = len(s["logs"])
+        assert d.ia == na and d.nk == s["nk"]
+        assert np.array_equal(sf[:na], s["logs"]) and np.array_equal(dt[:s["nk"], :na], s["rows"])
+        assert np.array_equal(init, s["init"])
+        assert np.array_equal(np.ctypeslib.as_array(d.wavenum, shape=(s["nk"],)), s["k"])
+        assert np.array_equal(np.ctypeslib.as_array(d.delta_nu_last, shape=(s["nk"],)), B.delta_nu(i))
+        # straight back into the other group: the same solve
+        assert L.kn_group_import(B2.h, i, C.byref(d), s["amax"]) == 0
+        # through the state file into a fresh table, then into the other group
+        path = str(tmp_path / ("state%d.txt" % i))
+        kn.save_all_nu_state(C.byref(d), path.encode())
+        d2 = knhip.DeltaTotTable()
+        kn.allocate_delta_tot_table(C.byref(d2), s["nk"], 0.01, 2.0, c.omega0, C.byref(c.P.om), UNIT_TIME,
+                                    UNIT_LENGTH, 0)
+        kn.read_all_nu_state(C.byref(d2), path.encode())
+        assert d2.ia == na
+        sf2, dt2, init2 = knhip.table_views(d2)
+        init2[:] = s["init"]
+        np.ctypeslib.as_array(d2.wavenum, shape=(s["nk"],))[:] = s["k"]
+        s["file"] = (sf2[:na].copy(), dt2[:s["nk"], :na].copy())
+        s["d2"] = d2
+        kn.free_delta_tot_table(C.byref(d))
+    B2.solve()
+    for i, s in enumerate(specs):
+        same = same and np.array_equal(B2.delta_nu(i), B.delta_nu(i))
+    for i, s in enumerate(specs):
+        # the file's last log a (7 digits) may round past log amax: integrate to the later of the two
+        a2 = math.exp(s["file"][0][-1])
+        while math.log(a2) < s["file"][0][-1]:  # exp and log round: a2 no earlier than the stored step
+            a2 = float(np.nextafter(a2, 2.0))
+        s["a2"] = max(s["amax"], a2)
+        assert L.kn_group_import(B2.h, i, C.byref(s["d2"]), s["a2"]) == 0
+    B2.solve()
+    for i, s in enumerate(specs):
+        logs2, rows2 = s["file"]
+        T = cosmos[s["c"]].oracle(s["nk"], logs2, rows2, s["init"])
+        rc, o = T.get_delta_nu(s["a2"], s["k"], s["m"])
+        assert rc == 0
+        worst = max(worst, rel(B2.delta_nu(i), o))
+    # a before the last stored step, and a table too small for the history
+    d = specs[0]["d2"]
+    assert L.kn_group_import(B2.h, 0, C.byref(d), math.exp(specs[0]["file"][0][-1]) * 0.99) == 3004
+    small = knhip.DeltaTotTable()
+    kn.allocate_delta_tot_table(C.byref(small), 2, 0.01, 2.0, 0.3, C.byref(cosmos[0].P.om), UNIT_TIME,
+                                UNIT_LENGTH, 0)
+    assert L.kn_group_export(B2.h, 0, C.byref(small)) == 3002
+    kn.free_delta_tot_table(C.byref(small))
+    for s in specs:
+        kn.free_delta_tot_table(C.byref(s["d2"]))
+    B.close()
+    B2.close()
+    for c in cosmos:
+        c.free()
+    return worst, same

@@ -406,3 +406,13 @@ def test_emu_group_subdivisions_equal_the_oracle(emu):
     B.close()
     for c in cosmos:
         c.free()
+
+
+def test_emu_group_checkpoint_through_host_tables(emu, tmp_path):
+    """kn_group_export / kn_group_import with the reference's state file in
+    between: the direct round trip solves bit for bit as before; the file's
+    values (7 significant digits) solve to the oracle's delta_nu on them."""
+    from batch_common import checkpoint_flow
+    worst, same = checkpoint_flow(tmp_path, L=emu, memcpy=host_memcpy)
+    print("  state file round trip: worst rel err %.2e vs the oracle; direct round trip identical: %s" % (worst, same))
+    assert same and worst < RTOL

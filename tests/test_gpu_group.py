@@ -337,3 +337,15 @@ def test_group_subdivisions_equal_the_oracle(kn, torch_cuda):
     assert items > 0
     B.close()
     _free(cosmos)
+
+
+def test_group_checkpoint_through_host_tables(kn, torch_cuda, tmp_path):
+    """kn_group_export / kn_group_import on the device, with the reference's
+    state file in between (save_all_nu_state / read_all_nu_state): the
+    direct round trip solves bit for bit as before, the file's values solve to
+    the oracle's delta_nu on them."""
+    from batch_common import checkpoint_flow
+    print("group checkpoint through host tables", flush=True)
+    worst, same = checkpoint_flow(tmp_path)
+    print("  state file round trip: worst rel err %.2e vs the oracle; direct round trip identical: %s" % (worst, same))
+    assert same and worst < RTOL
