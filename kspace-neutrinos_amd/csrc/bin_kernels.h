@@ -260,15 +260,26 @@ __global__ __launch_bounds__(64 * kSweepWaves) void k_bin_sweep(BinArgs a)
         const long long mrun = c + (long long)RUN * lane;
         Pos q = p;
         int curb = -1, b = -1, bnext = 0;
-        double acc = 0.0, iwxy = 0.0;
+        double acc = 0.0;
+        // the row's kx^2 + ky^2 and window product, at the run start and at a new row
+        int kx = kval(a.startslab + q.i, a.dims), ky = kval(q.j, a.dims);
+        int kxy2 = kx * kx + ky * ky;
+        double iwxy = __dmul_rn(iw[a.startslab + q.i], iw[q.j]);
 #pragma unroll
         for (int r = 0; r < RUN; r++) {
             if (mrun + r >= m1) break;
-            if (r) advance(q, 1, rowlen, a.dims);
-            // the row's window product, at the run start and at a new row
-            if (r == 0 || q.z == 0) iwxy = __dmul_rn(iw[a.startslab + q.i], iw[q.j]);
-            const int kx = kval(a.startslab + q.i, a.dims), ky = kval(q.j, a.dims);
-            const int k2 = kx * kx + ky * ky + q.z * q.z;
+            if (r && ++q.z == rowlen) {
+                q.z = 0;
+                if (++q.j == a.dims) {
+                    q.j = 0;
+                    q.i++;
+                }
+                kx = kval(a.startslab + q.i, a.dims);
+                ky = kval(q.j, a.dims);
+                kxy2 = kx * kx + ky * ky;
+                iwxy = __dmul_rn(iw[a.startslab + q.i], iw[q.j]);
+            }
+            const int k2 = kxy2 + q.z * q.z;
             if (k2 == 0) continue;
             if (b < 0 || q.z == 0) {
                 b = find_bin(k2, a.half_bpu, thr, a.nrbins);
