@@ -36,7 +36,7 @@ DP = C.POINTER(C.c_double)
 # WRITE_SIZE (KiB per dispatch) scaled by what the same counters read for a 2 GiB
 # device copy (FETCH_SIZE x2.0000, WRITE_SIZE x1.0000); commands and tables in
 # profiles/bin_rescale_1024_hbm_counters.txt.
-RESCALE_1024_FETCH_KIB, RESCALE_1024_WRITE_KIB = 4202615.8, 8405013.0
+RESCALE_1024_FETCH_KIB, RESCALE_1024_WRITE_KIB = 4202615.6, 8405006.9
 FETCH_SCALE, WRITE_SCALE = 2.0000, 1.0000
 
 
