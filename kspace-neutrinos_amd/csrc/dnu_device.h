@@ -1021,6 +1021,11 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
 // disjoint intervals).  The evaluations are gk61_pair's and the update is
 // wave_qag61's, so the decisions and results are wave_qag61's bit for bit;
 // an evaluation that the update does not reach is only work lost.
+// A half's error, as a fraction of its parent's, when the halves of the
+// interval bisected now compete for a slot with the next intervals in error
+// order (the chains of the oscillatory hybrid integrand alternate between
+// the next interval in order and a half of the one just bisected).
+constexpr double kHalfErr = 0.1;
 struct QagSlot {
     int used, eval;    // holds an interval; evaluate it this pass
     double lo, hi;     // the interval
@@ -1140,45 +1145,84 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
                     rc = errsum <= tol ? 0 : etype == 2 ? 18 : etype == 3 ? 21 : it == limit ? 11 : 5;
                 }
             }
-            // the next pass's intervals, one fixed slot each: 0 the one to
-            // bisect now (never evaluated yet: it would have been used above),
-            // 1 and 2 its halves (new), then the next in error order, kept in
-            // place when the slot already holds it
+            // the next pass's intervals: slot 0 the one to bisect now (never
+            // evaluated yet: it would have been used above); slots 1.. the
+            // W - 1 largest by error of the next W - 1 intervals in error
+            // order and the two halves of the one bisected now, a half's error
+            // taken as kHalfErr of its parent's.  An interval a slot already
+            // holds stays there unevaluated; a slot not chosen keeps what it
+            // holds (the list only ever splits its intervals, so a held
+            // interval is either still listed with those halves or gone).
+            // Lane c of wavefront 0 weighs candidate c.
             if (lead) {
                 S.rc = rc;
                 S.n = n;
-                if (rc < 0) {
-                    const double lo = w.lo[imax], hi = w.hi[imax], m = 0.5 * (lo + hi);
-                    const double ml = 0.5 * (lo + m), mr = 0.5 * (m + hi);
-                    // with 8 or more slots, 3..6 hold the interval's quarters
-                    constexpr int kQ = W >= 8 ? 4 : 0;
-#pragma unroll
-                    for (int q = 0; q < W; q++) {
-                        double ql = 0.0, qh = 0.0;
-                        bool any = true;
-                        const int r = q - 2 - kQ;  // rank in error order of the later slots
-                        if (q == 0) ql = lo, qh = hi;
-                        else if (q == 1) ql = lo, qh = m;
-                        else if (q == 2) ql = m, qh = hi;
-                        else if (kQ && q == 3) ql = lo, qh = ml;
-                        else if (kQ && q == 4) ql = ml, qh = m;
-                        else if (kQ && q == 5) ql = m, qh = mr;
-                        else if (kQ && q == 6) ql = mr, qh = hi;
-                        else if (r < n) ql = w.lo[w.ord[r]], qh = w.hi[w.ord[r]];
-                        else any = false;
-                        QagSlot &sl = S.slot[q];
-                        if (!any) {
-                            sl.used = 0;
-                            sl.eval = 0;
-                        } else if (q >= 3 + kQ && sl.used && sl.lo == ql && sl.hi == qh) {
-                            sl.eval = 0;
-                        } else {
-                            sl.used = 1;
-                            sl.eval = 1;
-                            sl.lo = ql;
-                            sl.hi = qh;
-                        }
+            }
+            if (rc < 0) {  // wavefront 0's state: uniform
+                const int lane = threadIdx.x & 63;
+                const double lo = w.lo[imax], hi = w.hi[imax], m = 0.5 * (lo + hi), eh = kHalfErr * w.e[imax];
+                double cl = 0.0, ch = 0.0, ce = 0.0;
+                bool has = false;
+                if (lane < W - 1) {
+                    if (lane + 1 < n) {
+                        const int i = w.ord[lane + 1];
+                        cl = w.lo[i];
+                        ch = w.hi[i];
+                        ce = w.e[i];
+                        has = true;
                     }
+                } else if (lane < W + 1) {
+                    cl = lane == W - 1 ? lo : m;
+                    ch = lane == W - 1 ? m : hi;
+                    ce = eh;
+                    has = true;
+                }
+                // rank by error among the candidates, the lower lane first on ties
+                int rank = 0;
+#pragma unroll
+                for (int c = 0; c < W + 1; c++) {
+                    const double oe = __shfl(ce, c, 64);
+                    const int oh = __shfl((int)has, c, 64);
+                    if (oh && (oe > ce || (oe == ce && c < lane))) rank++;
+                }
+                const bool take = has && rank < W - 1;
+                int held = 0;  // the slot (1..) that already holds this candidate
+#pragma unroll
+                for (int q = 1; q < W; q++)
+                    if (S.slot[q].used && S.slot[q].lo == cl && S.slot[q].hi == ch) held = q;
+                const bool keep = take && held > 0, place = take && held == 0;
+                unsigned kept = 0;  // bit q: slot q keeps a chosen interval
+#pragma unroll
+                for (int q = 1; q < W; q++)
+                    if (__ballot(keep && held == q)) kept |= 1u << q;
+                const unsigned long long pl = __ballot(place);
+                // the j-th candidate to place (in lane order) goes to the j-th slot not kept
+                const int j = __popcll(pl & ((1ull << lane) - 1));
+                int slot = 0, free = 0;
+#pragma unroll
+                for (int q = 1; q < W; q++)
+                    if (!(kept >> q & 1)) {
+                        if (free == j) slot = q;
+                        free++;
+                    }
+                unsigned fresh = 0;
+#pragma unroll
+                for (int q = 1; q < W; q++)
+                    if (__ballot(place && slot == q)) fresh |= 1u << q;
+                __builtin_amdgcn_wave_barrier();  // every lane has read the slots before any lane writes
+                if (place && slot > 0) {
+                    QagSlot &sl = S.slot[slot];
+                    sl.used = 1;
+                    sl.lo = cl;
+                    sl.hi = ch;
+                }
+                if (lane == 0) {
+                    S.slot[0].used = 1;
+                    S.slot[0].lo = lo;
+                    S.slot[0].hi = hi;
+                    S.slot[0].eval = 1;
+                } else if (lane < W) {
+                    S.slot[lane].eval = (int)(fresh >> lane & 1);
                 }
             }
         }
@@ -1711,7 +1755,7 @@ __device__ inline double specialJ_d(double x, const JPar &P, double &dJdx, doubl
         const double j0p = sph_j0_prime(z);
         double J = 0.0, Jx = 0.0, Jq = 0.0;
         for (int n = 1; n < 20; n++) {
-            const double nn = (double)(n * n), d = nn + x * x;
+            const double nn = (double)(n * n), d = nn + x * x, rd = rcp_fast(d);
             const double A = (n * n + n * n * n * qc + n * qc * x * x - x * x);
             const double B = (2 * n + n * n * qc + qc * x * x);
             const double ii = A * qc * j0 + B * c;
@@ -1720,9 +1764,9 @@ __device__ inline double specialJ_d(double x, const JPar &P, double &dJdx, doubl
             const double iix = Ax * qc * j0 + A * qc * qc * j0p + Bx * c - B * qc * s;
             const double iiq = Aq * qc * j0 + A * (j0 + z * j0p) + Bq * c - B * x * s;
             const double sgn = (n & 1) ? 1.0 : -1.0;
-            const double w = sgn * P.en[n] / (d * d);
+            const double w = sgn * P.en[n] * (rd * rd);
             J += w * ii;
-            Jx += w * (iix - 4.0 * x / d * ii);
+            Jx += w * (iix - 4.0 * x * rd * ii);
             Jq += w * (iiq - n * ii);
         }
         const double Z = 1.5 * 1.202056903159594 * (1 - P.nufrac_low);
@@ -1749,12 +1793,12 @@ __device__ inline double specialJ_d(double x, const JPar &P, double &dJdx, doubl
 __device__ __forceinline__ void cspline_grad(const double *xa, int i, double x, double &wy0, double &wy1, double &wc0,
                                              double &wc1)
 {
-    const double h = xa[i + 1] - xa[i], dx = x - xa[i];
-    const double t = dx / h;
+    const double h = xa[i + 1] - xa[i], dx = x - xa[i], ih = rcp_fast(h);
+    const double t = dx * ih, dx3 = dx * dx * dx * ih * (1.0 / 3.0);
     wy0 = 1.0 - t;
     wy1 = t;
-    wc0 = -dx * h * (2.0 / 3.0) + dx * dx - dx * dx * dx / (3.0 * h);
-    wc1 = -dx * h / 3.0 + dx * dx * dx / (3.0 * h);
+    wc0 = -dx * h * (2.0 / 3.0) + dx * dx - dx3;
+    wc1 = -dx * h * (1.0 / 3.0) + dx3;
 }
 
 // Windowed solve of the natural-spline system M z = rhs (interior rows

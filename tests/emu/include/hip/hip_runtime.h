@@ -123,6 +123,7 @@ inline unsigned long long __ballot(int pred)
     return m;
 }
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }
+inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }  // the hardware estimate, exactly
 // every thread of the block arrives before any goes on
 inline void __syncthreads()
