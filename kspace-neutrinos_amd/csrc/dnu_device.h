@@ -922,6 +922,81 @@ __device__ __forceinline__ int qag_sort_wave(QagWs &w, int n, int limit)
     return w.ord[0];
 }
 
+// The same error order held by the wavefront in registers while the list
+// is short: lane p keeps ord[p] (os) and that interval's error (es).  While
+// fewer than limit / 2 + 2 intervals are listed, dqpsrt's top is the last
+// index, so the whole order is kept sorted; the two scans are wavefront
+// searches on es and the two shifts are moves between neighbouring lanes.
+// ord[0] (im) was just bisected: emax is the error of the half it kept,
+// emin that of the half appended at index last.  Returns the new ord[0].
+#ifndef KN_WAVE_ORD
+#define KN_WAVE_ORD 64
+#endif
+constexpr int kWaveOrd = KN_WAVE_ORD;  // most intervals held in registers
+__device__ __forceinline__ int wave_ord_insert(double &es, int &os, int im, double emax, double emin, int last)
+{
+    const int p = threadIdx.x & 63;
+    if (last < 2) {
+        es = p == 0 ? emax : emin;
+        os = p;
+        return 0;
+    }
+    const int top = last;
+    // i = first p in [1, top) where !(emax < e[ord[p]]), else top
+    const unsigned long long m1 = __ballot(p >= 1 && p < top && !(emax < es));
+    const int i = m1 ? __ffsll((long long)m1) - 1 : top;
+    // ord[p - 1] = ord[p] for p in [1, i), then ord[i - 1] = im
+    const int dn = p < 63 ? p + 1 : 63;
+    const double es_dn = __shfl(es, dn, 64);
+    const int os_dn = __shfl(os, dn, 64);
+    if (p < i - 1) {
+        es = es_dn;
+        os = os_dn;
+    } else if (p == i - 1) {
+        es = emax;
+        os = im;
+    }
+    // k = largest p in [i - 1, top - 1] where !(emin >= e[ord[p]]), else i - 2
+    const unsigned long long m2 = __ballot(p >= i - 1 && p <= top - 1 && !(emin >= es));
+    const int k = m2 ? 63 - __builtin_clzll(m2) : i - 2;
+    // ord[p + 1] = ord[p] for p in [k + 1, top - 1], then ord[k + 1] = last
+    const int up = p > 0 ? p - 1 : 0;
+    const double es_up = __shfl(es, up, 64);
+    const int os_up = __shfl(os, up, 64);
+    if (p > k + 1 && p <= top) {
+        es = es_up;
+        os = os_up;
+    } else if (p == k + 1) {
+        es = emin;
+        os = last;
+    }
+    return __shfl(os, 0, 64);
+}
+
+// The order after appending interval n - 1: in registers while it fits,
+// else (once, when it outgrows them) copied to w.ord and kept there by
+// qag_sort_wave.  w holds the bisected interval's new values already.
+__device__ __forceinline__ int ord_insert(QagWs &w, double &es, int &os, bool &in_lds, int im, double emax,
+                                          double emin, int n, int limit)
+{
+    if (!in_lds && n <= kWaveOrd && n - 1 < limit / 2 + 2) return wave_ord_insert(es, os, im, emax, emin, n - 1);
+    if (!in_lds) {
+        const int p = threadIdx.x & 63;
+        if (p < n - 1) w.ord[p] = os;
+        __builtin_amdgcn_wave_barrier();
+        in_lds = true;
+    }
+    return qag_sort_wave(w, n, limit);
+}
+
+// ord[r] (r < the interval count), from wherever the order is kept; every
+// lane of the wavefront calls it.
+__device__ __forceinline__ int ord_rank(const QagWs &w, int os, bool in_lds, int r)
+{
+    const int v = __shfl(os, r & 63, 64);
+    return in_lds ? w.ord[r] : v;
+}
+
 // QAG (QUADPACK dqage semantics, GSL's qag.c control flow) by one wavefront.
 // Returns 0 on success or the GSL errno (11 iteration limit, 18 roundoff,
 // 21 bad integrand); *result is the sum over the final subintervals.  Every
@@ -946,6 +1021,9 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
         w.lev[0] = 0;
     }
     int imax = __shfl(0, 0, 64);
+    double es = g0.err;  // the error order in registers (ord_insert)
+    int os = 0;
+    bool ord_lds = false;
     double tol = epsrel * fabs(g0.res);
     if (g0.err <= 50 * DBL_EPSILON * g0.resabs && g0.err > tol) return 18;
     if ((g0.err <= tol && g0.err != g0.resasc) || g0.err == 0.0) return 0;
@@ -993,7 +1071,8 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
             w.lev[im] = lv;
             w.lev[n] = lv;
         }
-        imax = qag_sort_wave(w, n + 1, limit);
+        const bool hi2 = g2.err > g1.err;
+        imax = ord_insert(w, es, os, ord_lds, im, hi2 ? g2.err : g1.err, hi2 ? g1.err : g2.err, n + 1, limit);
         n++;
         it++;
     } while (it < limit && !etype && errsum > tol);
@@ -1071,6 +1150,9 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
     // wavefront 0's scalar state (identical in its 64 lanes)
     double area = 0.0, errsum = 0.0, tol = 0.0;
     int n = 1, it = 1, rt1 = 0, rt2 = 0, etype = 0, imax = 0, rc = -1;
+    double es = g0.err;  // the error order in registers (ord_insert)
+    int os = 0;
+    bool ord_lds = false;
     if (wave == 0) {
         if (lead) {
             w.lo[0] = a;
@@ -1138,7 +1220,8 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
                     w.lev[im] = lv;
                     w.lev[n] = lv;
                 }
-                imax = qag_sort_wave(w, n + 1, limit);
+                const bool hi2 = g2.err > g1.err;
+                imax = ord_insert(w, es, os, ord_lds, im, hi2 ? g2.err : g1.err, hi2 ? g1.err : g2.err, n + 1, limit);
                 n++;
                 it++;
                 if (!(it < limit && !etype && errsum > tol)) {
@@ -1163,9 +1246,10 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
                 const double lo = w.lo[imax], hi = w.hi[imax], m = 0.5 * (lo + hi), eh = kHalfErr * w.e[imax];
                 double cl = 0.0, ch = 0.0, ce = 0.0;
                 bool has = false;
+                const int ri = ord_rank(w, os, ord_lds, lane + 1 < n ? lane + 1 : 0);
                 if (lane < W - 1) {
                     if (lane + 1 < n) {
-                        const int i = w.ord[lane + 1];
+                        const int i = ri;
                         cl = w.lo[i];
                         ch = w.hi[i];
                         ce = w.e[i];
