@@ -519,26 +519,69 @@ __device__ __forceinline__ double sph_j0(double x)
     return sin(x) / x;
 }
 
-__device__ __forceinline__ double specialJ(double x, const JPar &P)
+// The hybrid series of specialJ (delta_tot_table.c:435-463),
+//   sum_{n=1}^{19} (-1)^{n+1} e^{-n qc} [A_n qc j0 + B_n cos(qc x)] / (n^2 + x^2)^2,
+//   A_n = n^2 + n^3 qc + n qc x^2 - x^2,  B_n = 2n + n^2 qc + qc x^2,
+// with each bracket regrouped by powers of n,
+//   A_n qc j0 + B_n c = n^2 al + n^3 be + n ga + de,
+//   al = qc j0 + qc c, be = qc^2 j0, ga = qc^2 x^2 j0 + 2c, de = qc x^2 c - qc x^2 j0,
+// so that a term is three fused multiply-adds, the squared denominator's
+// reciprocal and one more fused multiply-add.  SJTerms holds one point's
+// per-x factors; specialJ and specialJ2 run the same operations per point.
+struct SJTerms {
+    double y, al, be, ga, de, integ;
+};
+__device__ __forceinline__ SJTerms sj_terms(double x, double qc)
 {
-    if (P.qc > 0) {
-        const double qc = P.qc;
-        double s, c;
-        sincos(qc * x, &s, &c);
-        const double j0 = fabs(qc * x) < 0.5 ? sph_j0(qc * x) : s / (qc * x);
-        double integ = 0.0;
-        for (int n = 1; n < 20; n++) {
-            const double nn = (double)(n * n);
-            const double ii = (n * n + n * n * n * qc + n * qc * x * x - x * x) * qc * j0 + (2 * n + n * n * qc + qc * x * x) * c;
-            const double sgn = (n & 1) ? 1.0 : -1.0;   // -1 * pow(-1, n)
-            const double d = nn + x * x;
-            integ += sgn * P.en[n] * rcp_fast(d * d) * ii;
-        }
-        return integ / (1.5 * 1.202056903159594 * (1 - P.nufrac_low));
-    }
+    double s, c;
+    sincos(qc * x, &s, &c);
+    const double j0 = fabs(qc * x) < 0.5 ? sph_j0(qc * x) : s / (qc * x);
+    const double y = x * x, qj = qc * j0;
+    return {y, qj + qc * c, qc * qj, qc * (y * qj) + 2.0 * c, qc * y * c - y * qj, 0.0};
+}
+__device__ __forceinline__ void sj_term(SJTerms &t, int n, double w)
+{
+    const double fn = (double)n, n2 = fn * fn, n3 = n2 * fn;
+    const double ii = fma(n2, t.al, fma(n3, t.be, fma(fn, t.ga, t.de)));
+    const double d = n2 + t.y;
+    t.integ = fma(w * rcp_fast(d * d), ii, t.integ);
+}
+__device__ __forceinline__ double sj_fit(double x)
+{
     if (x <= 0.) return 1.;
     const double x2 = x * x, x4 = x2 * x2, x8 = x4 * x4;
     return (1. + 0.0168 * x2 + 0.0407 * x4) / (1. + 2.1734 * x2 + 1.6787 * exp(4.1811 * log(x)) + 0.1467 * x8);
+}
+
+__device__ __forceinline__ double specialJ(double x, const JPar &P)
+{
+    if (P.qc > 0) {
+        SJTerms t = sj_terms(x, P.qc);
+#pragma unroll 2
+        for (int n = 1; n < 20; n++) sj_term(t, n, ((n & 1) ? 1.0 : -1.0) * P.en[n]);
+        return t.integ / (1.5 * 1.202056903159594 * (1 - P.nufrac_low));
+    }
+    return sj_fit(x);
+}
+
+// specialJ at two points, the two series in one loop (two independent
+// chains per term); each point's operations are specialJ's.
+__device__ __forceinline__ void specialJ2(double x1, double x2, const JPar &P, double &j1, double &j2)
+{
+    if (P.qc > 0) {
+        SJTerms t1 = sj_terms(x1, P.qc), t2 = sj_terms(x2, P.qc);
+#pragma unroll 2
+        for (int n = 1; n < 20; n++) {
+            const double w = ((n & 1) ? 1.0 : -1.0) * P.en[n];
+            sj_term(t1, n, w);
+            sj_term(t2, n, w);
+        }
+        j1 = t1.integ / (1.5 * 1.202056903159594 * (1 - P.nufrac_low));
+        j2 = t2.integ / (1.5 * 1.202056903159594 * (1 - P.nufrac_low));
+        return;
+    }
+    j1 = sj_fit(x1);
+    j2 = sj_fit(x2);
 }
 
 // ---- natural cubic spline evaluation, GSL cspline/linear semantics ----
@@ -654,7 +697,8 @@ struct DnuIntegrand {  // delta_tot_table.c:492-500
         const int si1 = settle(sx, ns, b1, x1), si2 = settle(sx, ns, b2, x2);
         const double dt1 = linear ? linear_at(sx, sy, si1, x1) : cspline_at(sx, sy, sc, si1, x1);
         const double dt2 = linear ? linear_at(sx, sy, si2, x2) : cspline_at(sx, sy, sc, si2, x2);
-        const double j1 = specialJ(k_by_m * fsl1, *J), j2 = specialJ(k_by_m * fsl2, *J);
+        double j1, j2;
+        specialJ2(k_by_m * fsl1, k_by_m * fsl2, *J, j1, j2);
         r1 = in1 ? fsl1 * ih1 * j1 * dt1 : 0.0;
         r2 = in2 ? fsl2 * ih2 * j2 * dt2 : 0.0;
     }
