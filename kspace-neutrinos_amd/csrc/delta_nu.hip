@@ -557,15 +557,23 @@ struct AdjArgs {
 constexpr int kAdjPartKnots = 1024;
 __host__ __device__ inline int adj_parts(int ns) { return (ns + kAdjPartKnots - 1) / kAdjPartKnots; }
 __host__ __device__ inline int adj_plen(int ns) { return (ns + adj_parts(ns) - 1) / adj_parts(ns); }
-// LDS of one block: ybar and cbar rows of plen doubles, the selected interval list
-inline size_t adj_lds_bytes(int ns) { return sizeof(double) * 2 * (size_t)adj_plen(ns) + sizeof(int) * kLimit; }
+// LDS of one block: ybar and cbar rows of plen doubles, the selected interval
+// list and the queue of points to evaluate (kAdjQueue)
+constexpr int kAdjQueue = 128;
+inline size_t adj_lds_bytes(int ns)
+{
+    return sizeof(double) * 2 * (size_t)adj_plen(ns) + sizeof(int) * (kLimit + kAdjQueue);
+}
 
 // Block `blk` of one history's adjoint: (item, part) blocks [0, nk nparts),
 // then the initial-condition piece, 64 k per block.  The part owning knots
 // [K0, K1) evaluates the points of the recorded intervals that can reach
-// them (a point in knot interval si feeds knots si and si+1) and keeps the
-// terms that land in [K0, K1); a point's d/d(m/kT) term goes to the part
-// owning its si, so every term is counted once.  `lds`: adj_lds_bytes(ns).
+// them (a point in knot interval si feeds knots si and si+1: x within
+// [sx[K0 - 1], sx[K1]]) and keeps the terms that land in [K0, K1); a point's
+// d/d(m/kT) term goes to the part owning its si, so every term is counted
+// once.  The points that pass are queued in LDS and evaluated 64 at a time,
+// one per lane, so a part evaluates its own share of a long interval's
+// points, not all of them.  `lds`: adj_lds_bytes(ns).
 __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
 {
     const int nk = A.nk, ns = A.ns, nfs = A.nfs;
@@ -581,7 +589,6 @@ __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
         return;
     }
     const int lane = threadIdx.x & 63, item = blk / P, part = blk % P, k = item % nk;
-    const int hf = lane >> 5, j = lane & 31;
     const int K0 = part * L, K1 = K0 + L < ns ? K0 + L : ns;
     int *sel = (int *)(lds + 2 * L);
     for (int t = lane; t < 2 * L; t += 64) lds[t] = 0.0;
@@ -590,8 +597,8 @@ __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
     // the recorded intervals that reach this part's knots, in their order:
     // x in [sx[K0 - 1], sx[K1]] (clamped to the table)
     int nsel = 0;
+    const double xlo = sx[K0 > 0 ? K0 - 1 : 0], xhi = sx[K1 < ns ? K1 : ns - 1];
     {
-        const double xlo = sx[K0 > 0 ? K0 - 1 : 0], xhi = sx[K1 < ns ? K1 : ns - 1];
         for (int t0 = 0; t0 < ni; t0 += 64) {
             const int t = t0 + lane;
             int hit = 0;
@@ -620,51 +627,77 @@ __device__ inline void dnu_adj_block(const AdjArgs &A, int blk, double *lds)
     fk.span = A.xa - A.x0;
     fk.den = nfs - 1.;
     double mu_acc = 0.0;
-    for (int t0 = 0; t0 < nsel; t0 += 2) {
-        const int ts = t0 + hf;
-        if (ts < nsel && j <= 30) {
-            const int t = sel[ts];
-            const double a = ivl[((size_t)item * kLimit + t) * 2], b = ivl[((size_t)item * kLimit + t) * 2 + 1];
-            const double c = 0.5 * (a + b), hl = 0.5 * (b - a);
-            const int npt = j < 30 ? 2 : 1;
-            for (int side = 0; side < npt; side++) {
-                const double x = j < 30 ? (side ? c + hl * c_xgk[j] : c - hl * c_xgk[j]) : c;
-                const double w = c_wgk[j] * hl * prefac * gb;
-                // the pieces of delta_tot_table.c:492-500
-                double xl, xr;
-                const int fi = settle_uniform(fk, nfs, (int)((x - fx0) * finv_h), x, xl, xr);
-                const double fsl = cspline_val(xl, xr, fy[fi], fy[fi + 1], fc[fi], fc[fi + 1], x);
-                int q = (int)((x - sx0) * sinv_bw);
-                q = q < 0 ? 0 : (q > nsb ? nsb : q);
-                const int si = settle(sx, ns, sb[q], x);
-                const double kx = kk * fsl / mnubykT;
-                double Jx, Jq;
-                const double Jv = specialJ_d(kx, J, Jx, Jq);
-                const double base = fsl * inv_ah(H, x);  // fsl / (a H), as the forward
-                const double S = linear ? linear_at(sx, syk, si, x) : cspline_at(sx, syk, sck, si, x);
-                const bool own0 = si >= K0 && si < K1, own1 = si + 1 >= K0 && si + 1 < K1;
-                if (own0)  // d/d(m/kT), qc = vcrit m/kT
-                    mu_acc += w * base * S * (Jx * (-kx / mnubykT) + Jq * vcrit);
-                const double wj = w * base * Jv;
-                if (linear) {
-                    const double t1 = (x - sx[si]) / (sx[si + 1] - sx[si]);
-                    if (own0) atomicAdd(&lds[si - K0], wj * (1.0 - t1));
-                    if (own1) atomicAdd(&lds[si + 1 - K0], wj * t1);
-                } else {
-                    double wy0, wy1, wc0, wc1;
-                    cspline_grad(sx, si, x, wy0, wy1, wc0, wc1);
-                    if (own0) {
-                        atomicAdd(&lds[si - K0], wj * wy0);
-                        atomicAdd(&lds[L + si - K0], wj * wc0);
-                    }
-                    if (own1) {
-                        atomicAdd(&lds[si + 1 - K0], wj * wy1);
-                        atomicAdd(&lds[L + si + 1 - K0], wj * wc1);
-                    }
-                }
+    // point g of the selected intervals: interval sel[g / 61], node jj = g % 61
+    // (jj < 30: c - h xgk[jj]; 30: c; jj > 30: c + h xgk[60 - jj])
+    auto point = [&](int g, double &x, double &w) {
+        const int t = sel[g / 61], jj = g % 61, jn = jj <= 30 ? jj : 60 - jj;
+        const double a = ivl[((size_t)item * kLimit + t) * 2], b = ivl[((size_t)item * kLimit + t) * 2 + 1];
+        const double c = 0.5 * (a + b), hl = 0.5 * (b - a);
+        x = jj < 30 ? c - hl * c_xgk[jn] : (jj == 30 ? c : c + hl * c_xgk[jn]);
+        w = c_wgk[jn] * hl * prefac * gb;
+    };
+    auto eval = [&](int g) {
+        double x, w;
+        point(g, x, w);
+        // the pieces of delta_tot_table.c:492-500
+        double xl, xr;
+        const int fi = settle_uniform(fk, nfs, (int)((x - fx0) * finv_h), x, xl, xr);
+        const double fsl = cspline_val(xl, xr, fy[fi], fy[fi + 1], fc[fi], fc[fi + 1], x);
+        int q = (int)((x - sx0) * sinv_bw);
+        q = q < 0 ? 0 : (q > nsb ? nsb : q);
+        const int si = settle(sx, ns, sb[q], x);
+        const double kx = kk * fsl / mnubykT;
+        double Jx, Jq;
+        const double Jv = specialJ_d(kx, J, Jx, Jq);
+        const double base = fsl * inv_ah(H, x);  // fsl / (a H), as the forward
+        const double S = linear ? linear_at(sx, syk, si, x) : cspline_at(sx, syk, sck, si, x);
+        const bool own0 = si >= K0 && si < K1, own1 = si + 1 >= K0 && si + 1 < K1;
+        if (own0)  // d/d(m/kT), qc = vcrit m/kT
+            mu_acc += w * base * S * (Jx * (-kx / mnubykT) + Jq * vcrit);
+        const double wj = w * base * Jv;
+        if (linear) {
+            const double t1 = (x - sx[si]) / (sx[si + 1] - sx[si]);
+            if (own0) atomicAdd(&lds[si - K0], wj * (1.0 - t1));
+            if (own1) atomicAdd(&lds[si + 1 - K0], wj * t1);
+        } else {
+            double wy0, wy1, wc0, wc1;
+            cspline_grad(sx, si, x, wy0, wy1, wc0, wc1);
+            if (own0) {
+                atomicAdd(&lds[si - K0], wj * wy0);
+                atomicAdd(&lds[L + si - K0], wj * wc0);
+            }
+            if (own1) {
+                atomicAdd(&lds[si + 1 - K0], wj * wy1);
+                atomicAdd(&lds[L + si + 1 - K0], wj * wc1);
             }
         }
+    };
+    int *queue = sel + kLimit;  // kAdjQueue point indices
+    const int npts = nsel * 61;
+    int nq = 0;  // queued points (wavefront-uniform), below 64 between rounds
+    for (int g0 = 0; g0 < npts; g0 += 64) {
+        const int g = g0 + lane;
+        bool in = false;
+        if (g < npts) {
+            double x, w;
+            point(g, x, w);
+            in = x >= xlo && x <= xhi;
+        }
+        const unsigned long long m = __ballot(in);
+        if (in) queue[nq + __popcll(m & ((1ull << lane) - 1))] = g;
+        nq += __popcll(m);
+        __builtin_amdgcn_wave_barrier();  // the queue is written before any lane reads it
+        if (nq >= 64) {
+            eval(queue[lane]);
+            const int rest = nq - 64;
+            const int v = lane < rest ? queue[64 + lane] : 0;
+            __builtin_amdgcn_wave_barrier();  // every lane has read before any lane moves
+            if (lane < rest) queue[lane] = v;
+            __builtin_amdgcn_wave_barrier();
+            nq = rest;
+        }
     }
+    if (lane < nq) eval(queue[lane]);
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) mu_acc += __shfl_xor(mu_acc, m, 64);
     __syncthreads();
