@@ -1305,14 +1305,12 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
                     ce = eh;
                     has = true;
                 }
-                // rank by error among the candidates, the lower lane first on ties
-                int rank = 0;
-#pragma unroll
-                for (int c = 0; c < W + 1; c++) {
-                    const double oe = __shfl(ce, c, 64);
-                    const int oh = __shfl((int)has, c, 64);
-                    if (oh && (oe > ce || (oe == ce && c < lane))) rank++;
-                }
+                // rank by error among the candidates, the lower lane first on
+                // ties: the intervals in error order come sorted, the halves
+                // share one error, so the kge intervals at or above it come
+                // first, then the halves, then the rest
+                const int kge = __popcll(__ballot(lane < W - 1 && has && ce >= eh));
+                const int rank = lane < W - 1 ? (lane < kge ? lane : lane + 2) : kge + (lane - (W - 1));
                 const bool take = has && rank < W - 1;
                 int held = 0;  // the slot (1..) that already holds this candidate
 #pragma unroll
