@@ -691,12 +691,33 @@ struct DnuIntegrand {  // delta_tot_table.c:492-500
         q1 = q1 < 0 ? 0 : (q1 > nsb ? nsb : q1);
         q2 = q2 < 0 ? 0 : (q2 > nsb ? nsb : q2);
         const int b1 = sbucket[q1], b2 = sbucket[q2];
+        // the delta_tot spline's values at the bucket's knot and the next two,
+        // loaded before the knot walk (which moves at most one knot up but
+        // rarely); the walk's result picks the pair
+        const int i1 = b1 < 0 ? 0 : (b1 > ns - 2 ? ns - 2 : b1), i2 = b2 < 0 ? 0 : (b2 > ns - 2 ? ns - 2 : b2);
+        const int u1 = i1 + 2 < ns ? i1 + 2 : ns - 1, u2 = i2 + 2 < ns ? i2 + 2 : ns - 1;
+        const double y10 = sy[i1], y11 = sy[i1 + 1], y12 = sy[u1], y20 = sy[i2], y21 = sy[i2 + 1], y22 = sy[u2];
+        double c10 = 0.0, c11 = 0.0, c12 = 0.0, c20 = 0.0, c21 = 0.0, c22 = 0.0;
+        if (!linear) {
+            c10 = sc[i1], c11 = sc[i1 + 1], c12 = sc[u1];
+            c20 = sc[i2], c21 = sc[i2 + 1], c22 = sc[u2];
+        }
         const double fsl1 = cspline_val(xl1, xr1, fy[fi1], fy[fi1 + 1], fc[fi1], fc[fi1 + 1], x1);
         const double fsl2 = cspline_val(xl2, xr2, fy[fi2], fy[fi2 + 1], fc[fi2], fc[fi2 + 1], x2);
         const double ih1 = inv_ah(H, x1), ih2 = inv_ah(H, x2);
         const int si1 = settle(sx, ns, b1, x1), si2 = settle(sx, ns, b2, x2);
-        const double dt1 = linear ? linear_at(sx, sy, si1, x1) : cspline_at(sx, sy, sc, si1, x1);
-        const double dt2 = linear ? linear_at(sx, sy, si2, x2) : cspline_at(sx, sy, sc, si2, x2);
+        double dt1, dt2;
+        if (linear) {
+            dt1 = linear_at(sx, sy, si1, x1);
+            dt2 = linear_at(sx, sy, si2, x2);
+        } else {
+            dt1 = si1 == i1       ? cspline_val(sx[si1], sx[si1 + 1], y10, y11, c10, c11, x1)
+                  : si1 == i1 + 1 ? cspline_val(sx[si1], sx[si1 + 1], y11, y12, c11, c12, x1)
+                                  : cspline_at(sx, sy, sc, si1, x1);
+            dt2 = si2 == i2       ? cspline_val(sx[si2], sx[si2 + 1], y20, y21, c20, c21, x2)
+                  : si2 == i2 + 1 ? cspline_val(sx[si2], sx[si2 + 1], y21, y22, c21, c22, x2)
+                                  : cspline_at(sx, sy, sc, si2, x2);
+        }
         double j1, j2;
         specialJ2(k_by_m * fsl1, k_by_m * fsl2, *J, j1, j2);
         r1 = in1 ? fsl1 * ih1 * j1 * dt1 : 0.0;
