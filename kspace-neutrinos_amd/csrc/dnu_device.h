@@ -1165,11 +1165,12 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
 // disjoint intervals).  The evaluations are gk61_pair's and the update is
 // wave_qag61's, so the decisions and results are wave_qag61's bit for bit;
 // an evaluation that the update does not reach is only work lost.
-// A half's error, as a fraction of its parent's, when the halves of the
-// interval bisected now compete for a slot with the next intervals in error
-// order (the chains of the oscillatory hybrid integrand alternate between
-// the next interval in order and a half of the one just bisected).
-constexpr double kHalfErr = 0.1;
+// Estimated errors of the intervals the bisection now in hand will create,
+// as fractions of its error, when they compete for a slot with the next
+// intervals in error order.  The chains of the oscillatory hybrid integrand
+// alternate between the next interval in order and a half of the one just
+// bisected, mostly the right half, sometimes twice running.
+constexpr double kRightErr = 0.1, kLeftErr = 0.05, kRightRightErr = 0.05;
 struct QagSlot {
     int used, eval;    // holds an interval; evaluate it this pass
     double lo, hi;     // the interval
@@ -1296,11 +1297,12 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
             // the next pass's intervals: slot 0 the one to bisect now (never
             // evaluated yet: it would have been used above); slots 1.. the
             // W - 1 largest by error of the next W - 1 intervals in error
-            // order and the two halves of the one bisected now, a half's error
-            // taken as kHalfErr of its parent's.  An interval a slot already
-            // holds stays there unevaluated; a slot not chosen keeps what it
-            // holds (the list only ever splits its intervals, so a held
-            // interval is either still listed with those halves or gone).
+            // order, the two halves of the one bisected now and its right
+            // half's right half, the new intervals' errors estimated as
+            // fractions of its error.  An interval a slot already holds stays
+            // there unevaluated; a slot not chosen keeps what it holds (the
+            // list only ever splits its intervals, so a held interval is
+            // either still listed with those halves or gone).
             // Lane c of wavefront 0 weighs candidate c.
             if (lead) {
                 S.rc = rc;
@@ -1308,7 +1310,10 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
             }
             if (rc < 0) {  // wavefront 0's state: uniform
                 const int lane = threadIdx.x & 63;
-                const double lo = w.lo[imax], hi = w.hi[imax], m = 0.5 * (lo + hi), eh = kHalfErr * w.e[imax];
+                const double lo = w.lo[imax], hi = w.hi[imax], m = 0.5 * (lo + hi), e0 = w.e[imax];
+                // the extra candidates, in lane order W - 1.. and by estimated
+                // error: the right half, the left half, the right half's right half
+                const double eR = kRightErr * e0, eL = kLeftErr * e0, eRR = kRightRightErr * e0;
                 double cl = 0.0, ch = 0.0, ce = 0.0;
                 bool has = false;
                 const int ri = ord_rank(w, os, ord_lds, lane + 1 < n ? lane + 1 : 0);
@@ -1320,18 +1325,24 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
                         ce = w.e[i];
                         has = true;
                     }
-                } else if (lane < W + 1) {
-                    cl = lane == W - 1 ? lo : m;
-                    ch = lane == W - 1 ? m : hi;
-                    ce = eh;
-                    has = true;
+                } else if (lane == W - 1) {
+                    cl = m, ch = hi, ce = eR, has = true;
+                } else if (lane == W) {
+                    cl = lo, ch = m, ce = eL, has = true;
+                } else if (lane == W + 1) {
+                    cl = 0.5 * (m + hi), ch = hi, ce = eRR, has = true;
                 }
                 // rank by error among the candidates, the lower lane first on
-                // ties: the intervals in error order come sorted, the halves
-                // share one error, so the kge intervals at or above it come
-                // first, then the halves, then the rest
-                const int kge = __popcll(__ballot(lane < W - 1 && has && ce >= eh));
-                const int rank = lane < W - 1 ? (lane < kge ? lane : lane + 2) : kge + (lane - (W - 1));
+                // ties: the intervals in error order come sorted and so do the
+                // extras, so an extra's rank counts the intervals at or above
+                // its error, and an interval's the extras above its error
+                const bool ordc = lane < W - 1 && has;
+                const int nR = __popcll(__ballot(ordc && ce >= eR)), nL = __popcll(__ballot(ordc && ce >= eL));
+                const int nRR = __popcll(__ballot(ordc && ce >= eRR));
+                const int rank = lane < W - 1 ? lane + (eR > ce) + (eL > ce) + (eRR > ce)
+                                 : lane == W - 1 ? nR
+                                 : lane == W     ? nL + 1
+                                                 : nRR + 2;
                 const bool take = has && rank < W - 1;
                 int held = 0;  // the slot (1..) that already holds this candidate
 #pragma unroll
