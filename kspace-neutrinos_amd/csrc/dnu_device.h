@@ -888,10 +888,11 @@ __device__ __forceinline__ GK gk61_q4(const F &f, double a, double b)
     return r;
 }
 
-// Interval list of one wavefront (QUADPACK workspace), in LDS.
+// Interval list of one wavefront (QUADPACK workspace), in LDS.  QUADPACK's
+// level counts are left out: only QAGS's extrapolation reads them.
 struct QagWs {
     double lo[kLimit], hi[kLimit], r[kLimit], e[kLimit];
-    int ord[kLimit], lev[kLimit];
+    int ord[kLimit];
 };
 
 // Keep ord[] partially sorted by error, as QUADPACK's dqpsrt.  All lanes run
@@ -1083,7 +1084,6 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
         w.r[0] = g0.res;
         w.e[0] = g0.err;
         w.ord[0] = 0;
-        w.lev[0] = 0;
     }
     int imax = __shfl(0, 0, 64);
     double es = g0.err;  // the error order in registers (ord_insert)
@@ -1115,7 +1115,6 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
             if (fabs(a1) <= t && fabs(b2) <= t) etype = 3;
         }
         if (lead) {
-            const int lv = w.lev[im] + 1;
             if (g2.err > g1.err) {
                 w.lo[im] = a2;
                 w.r[im] = g2.res;
@@ -1133,8 +1132,6 @@ __device__ int wave_qag61(const F &f, double a, double b, double epsrel, int lim
                 w.r[n] = g2.res;
                 w.e[n] = g2.err;
             }
-            w.lev[im] = lv;
-            w.lev[n] = lv;
         }
         const bool hi2 = g2.err > g1.err;
         imax = ord_insert(w, es, os, ord_lds, im, hi2 ? g2.err : g1.err, hi2 ? g1.err : g2.err, n + 1, limit);
@@ -1226,8 +1223,7 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
             w.r[0] = g0.res;
             w.e[0] = g0.err;
             w.ord[0] = 0;
-            w.lev[0] = 0;
-        }
+            }
         tol = epsrel * fabs(g0.res);
         if (g0.err <= 50 * DBL_EPSILON * g0.resabs && g0.err > tol) rc = 18;
         else if ((g0.err <= tol && g0.err != g0.resasc) || g0.err == 0.0) rc = 0;
@@ -1265,8 +1261,7 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
                     if (fabs(a1) <= t && fabs(b2) <= t) etype = 3;
                 }
                 if (lead) {
-                    const int lv = w.lev[im] + 1;
-                    if (g2.err > g1.err) {
+                            if (g2.err > g1.err) {
                         w.lo[im] = a2;
                         w.r[im] = g2.res;
                         w.e[im] = g2.err;
@@ -1283,8 +1278,6 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
                         w.r[n] = g2.res;
                         w.e[n] = g2.err;
                     }
-                    w.lev[im] = lv;
-                    w.lev[n] = lv;
                 }
                 const bool hi2 = g2.err > g1.err;
                 imax = ord_insert(w, es, os, ord_lds, im, hi2 ? g2.err : g1.err, hi2 ? g1.err : g2.err, n + 1, limit);
