@@ -1235,9 +1235,11 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
     for (;;) {
         if (wave == 0) {
             // the bisections whose halves are at hand, exactly as wave_qag61's loop
+            double pa = 0.0, pb = 0.0, pe = 0.0;  // the last interval looked at: the next to bisect
             while (rc < 0) {
                 const int im = imax;
                 const double ai = w.lo[im], bi = w.hi[im], ri = w.r[im], ei = w.e[im];
+                pa = ai, pb = bi, pe = ei;
                 int s = -1;
 #pragma unroll
                 for (int q = 0; q < W; q++)
@@ -1303,7 +1305,7 @@ __device__ int block_qag61(const F &f, double a, double b, double epsrel, int li
             }
             if (rc < 0) {  // wavefront 0's state: uniform
                 const int lane = threadIdx.x & 63;
-                const double lo = w.lo[imax], hi = w.hi[imax], m = 0.5 * (lo + hi), e0 = w.e[imax];
+                const double lo = pa, hi = pb, m = 0.5 * (lo + hi), e0 = pe;  // w's values at imax
                 // the extra candidates, in lane order W - 1.. and by estimated
                 // error: the right half, the left half, the right half's right half
                 const double eR = kRightErr * e0, eL = kLeftErr * e0, eRR = kRightRightErr * e0;
