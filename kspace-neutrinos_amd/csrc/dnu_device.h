@@ -242,9 +242,10 @@ __device__ __forceinline__ HLoc h_locate(const HTab &H, double x)
     int p = (int)floor(u);
     p = p < 0 ? 0 : (p >= H.npanel ? H.npanel - 1 : p);
     int s = -1;
+    if (H.nsplit > 0)  // the table's (uniform): most tables have no split panel
 #pragma unroll
-    for (int i = 0; i < kHSplitMax; i++)
-        if (i < H.nsplit && p == H.split_p[i]) s = i;
+        for (int i = 0; i < kHSplitMax; i++)
+            if (i < H.nsplit && p == H.split_p[i]) s = i;
     HLoc L;
     L.p = p;
     L.s = s;
@@ -260,7 +261,7 @@ __device__ __forceinline__ gconst_double *h_split_rec(const HTab &H, int s)
 __device__ __forceinline__ double h_cheb_dev(const HTab &H, double x, int which)
 {
     const HLoc L = h_locate(H, x);
-    if (L.s >= 0) return h_split_eval(h_split_rec(H, L.s), x, which);
+    if (H.nsplit > 0 && L.s >= 0) return h_split_eval(h_split_rec(H, L.s), x, which);
     return h_clenshaw((gconst_double *)(H.coef + (size_t)L.p * kHStride + which * kHnc), L.t);
 }
 
