@@ -317,3 +317,25 @@ def test_history_stays_on_the_device_and_sees_host_edits(kn, torch_cuda):
     check(n0, 0, "last step dropped")
     check(n0 + 2, 2, "two steps appended")
     P.free()
+
+
+@pytest.mark.parametrize("noise", [0.1, 1.0], ids=["108 intervals", "171 intervals"])
+def test_long_interval_lists_match_oracle(kn, torch_cuda, noise):
+    """Histories whose QAGs need 108 and 171 subintervals (per-entry noise on
+    300 stored steps): past 64 intervals the error order leaves registers for
+    the LDS list, and past limit / 2 + 2 = 102 QUADPACK keeps it only
+    partially sorted.  Two calls, the first with one wavefront per item, the
+    second with the long items on speculative blocks, each against the oracle
+    (a different subdivision would move the result by far more than 1e-9)."""
+    P = Pair([0.3, 0, 0], 0.3, 8, 0.01 + 300 / 100.0 + 0.02)
+    logs, rows, k, init = synthetic_history(300, 8, 1.0, noise=noise)
+    P.set_history(logs, rows, init)
+    rc, o = P.T.get_delta_nu(1.0, k, 0.3)
+    assert rc == 0 and P.T.d.qag_max_intervals > 102
+    for call in range(2):
+        g = P.gpu_get_delta_nu(1.0, k, 0.3)
+        e = rel(g, o)
+        print("  noise %.1f call %d: %d intervals (oracle max %d), rel err %.2e"
+              % (noise, call, kn.kn_dnu_last_intervals(C.byref(C.c_int())), P.T.d.qag_max_intervals, e))
+        assert e < RTOL
+    P.free()
